@@ -1,0 +1,190 @@
+"""Benchmark: three-species backward-Euler steps/s on the ~800k-tet graded disc, rtol 1e-8.
+
+One step = RHS assembly + batched solve of O2, Cl-, OH- to ||b - A x|| <= 1e-8 ||b|| (FP64
+true residual).  Two GPU paths run on the same trajectory from the same initial state:
+  plain  -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64)
+  chosen -- FP32 matrix-free fused 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64)
+`value` is the chosen path's steps/s.  The roofline entry is the fused FP32 3-species CG apply
+kernel (the solve's dominant kernel), timed with HIP events on its own stream; the CPU baseline
+is the oracle (numpy assembly + C OpenMP Jacobi-PCG) on one 3-species step of the same mesh.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload disc3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "dissolve-2.0_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def apply_bytes(C, V, species):
+    """Algorithmic bytes of one fused FP32 apply: conn 16C + coords 12V + per species
+    x in / y out 8V + coefficient bytes (4C per-cell, 4V per-vertex, 0 scalar)."""
+    b = 16 * C + 12 * V
+    for s in species:
+        b += 8 * V
+        for key in ("D", "alpha"):
+            a = s.get(key)
+            if a is not None:
+                b += 4 * len(a)
+    return b
+
+
+def load_workload(name):
+    from dsv import workloads
+    cache = os.path.join("/tmp", f"dsv_{name}.npz")
+    if name == "disc3":
+        return workloads.disc3()
+    if name == "cube_jitter":
+        return workloads.cube_jitter()
+    if name == "gyroid3":
+        return workloads.gyroid3()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_path(mesh, w, method, steps, warmup, rtol):
+    import torch
+    from dsv.transport import Stepper
+    st = Stepper(mesh, w["species"], w["dt"], method=method, rtol=rtol)
+    for i in range(warmup):
+        st.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for i in range(steps):
+        d = st.step()
+        its.append(d)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    conv = all(all(h["converged"]) for h in its)
+    return {
+        "steps_per_s": steps / t, "ms_per_step": 1e3 * t / steps, "converged": conv,
+        "iterations_per_step": [h["iterations"] for h in its],
+        "outer_per_step": [h["outer"] for h in its],
+        "rel_residual_last": its[-1]["rel_residual"],
+    }, st
+
+
+def cpu_baseline(w, threads):
+    """Oracle CPU path on one 3-species step of the same mesh (bounded sample)."""
+    import numpy as np
+    from oracle import assembly as OA
+    from oracle import solve as OS
+    conn, xyz = w["conn"], w["xyz"]
+    inv_dt = 1.0 / w["dt"]
+    t0 = time.perf_counter()
+    for s in w["species"]:
+        De = s["D"] if s.get("D") is not None else s["D_value"]
+        ip, ix, data = OA.assemble_values(conn, xyz, De, inv_dt, s.get("alpha"), 1, s.get("bc_dofs"))
+        b = OA.assemble_rhs(conn, xyz, s["u0"], inv_dt, s.get("f"), 1, s.get("bc_dofs"),
+                            s.get("bc_value", 0.0))
+        x0 = np.array(s["u0"])
+        if s.get("bc_dofs") is not None and len(s["bc_dofs"]):
+            x0[s["bc_dofs"]] = s["bc_value"]
+        OS.pcg_jacobi_c(ip, ix, data, b, x0, w["rtol"], 20000, threads)
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="disc3")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--apply-reps", type=int, default=50)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from dsv import _lib
+    from dsv.core import Mesh, species_array, species_struct, stream_ptr
+
+    t = time.perf_counter()
+    w = load_workload(a.workload)
+    C, V = len(w["conn"]), len(w["xyz"])
+    log(f"workload {w['name']}: {C} tets, {V} vertices, built in {time.perf_counter() - t:.1f}s")
+    t = time.perf_counter()
+    mesh = Mesh(w["conn"], w["xyz"])
+    log(f"device mesh: nnz={mesh.nnz} blocks={mesh.n_blocks} entries={mesh.n_entries} "
+        f"({time.perf_counter() - t:.1f}s)")
+    rtol = w["rtol"]
+    res = {}
+    for method in ("csr64", "mf32_ir64"):
+        r, _ = run_path(mesh, w, method, a.steps, a.warmup, rtol)
+        res[method] = r
+        log(f"{method}: {r['steps_per_s']:.2f} steps/s, iterations/step {r['iterations_per_step'][-1]}, "
+            f"converged={r['converged']}")
+
+    # roofline of the fused FP32 3-species CG apply (dominant kernel of the chosen path)
+    keep = []
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), bc_dofs=s.get("bc_dofs"),
+                              bc_value=s.get("bc_value", 0.0), keep=keep) for s in w["species"]]
+    arr = species_array(structs)
+    stream = torch.cuda.Stream()
+    import ctypes
+    ms = ctypes.c_double()
+    _lib.call("dsv_time_apply", mesh.handle, len(structs), arr, 1.0 / w["dt"], 1, 32, 1,
+              a.apply_reps, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ms))
+    ms_apply = ms.value
+    _lib.call("dsv_time_apply", mesh.handle, len(structs), arr, 1.0 / w["dt"], 1, 32, 0,
+              a.apply_reps, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ms))
+    ms_plain_apply = ms.value
+    nbytes = apply_bytes(C, V, w["species"])
+    achieved = nbytes / (ms_apply * 1e-3) / 1e9
+    log(f"fused FP32 CG apply: {ms_apply * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
+        f"(plain apply {ms_plain_apply * 1e3:.1f} us)")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        t_cpu = cpu_baseline(w, threads)
+        cpu = {"value": 1.0 / t_cpu, "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"one 3-species BE step of {w['name']} (assembly numpy + C OpenMP "
+                         f"Jacobi-PCG to rtol {rtol:g}), {t_cpu:.1f}s"}
+        log(f"cpu baseline: {t_cpu:.1f}s per step on {threads} threads")
+
+    chosen = res["mf32_ir64"]
+    plain = res["csr64"]
+    out = {
+        "metric": "3-species BE steps/s at 800k tets, rtol 1e-8: chosen vs plain FP64; HBM %",
+        "value": round(chosen["steps_per_s"], 3), "unit": "steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(chosen["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w['name']}: graded Delaunay disc r=1 h=0.2, {C} tets, {V} "
+                               f"vertices, O2/Cl/OH D=2.0/2.0/5.3, dt={w['dt']}, rtol={rtol:g}",
+                   "method": "FP32 matrix-free fused 3-species Jacobi-PCG in FP64 refinement, "
+                             "previous-solution warm start, hipGraph chunks",
+                   "parallelism": "1 GPU"},
+        "plain_fp64": {"steps_per_s": round(plain["steps_per_s"], 3),
+                       "iterations_last_step": plain["iterations_per_step"][-1]},
+        "chosen_vs_plain": round(chosen["steps_per_s"] / plain["steps_per_s"], 3),
+        "iterations_last_step": chosen["iterations_per_step"][-1],
+        "outer_last_step": chosen["outer_per_step"][-1],
+        "converged": chosen["converged"] and plain["converged"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_mf<float,3,MF_CG> (fused 3-species FP32 CG apply)",
+                     "bytes_per_launch": nbytes, "us_per_launch": round(ms_apply * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+// Internal declarations of libdsv (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/dsv.h"
+
+namespace dsv {
+
+void set_error(const std::string& msg);
+
+struct HipError {
+    hipError_t code;
+    std::string where;
+};
+
+#define DSV_HIP(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) throw ::dsv::HipError{_e, std::string(#expr) + " at " + \
+                                                            __FILE__ + ":" +           \
+                                                            std::to_string(__LINE__)}; \
+    } while (0)
+
+struct ArgError {
+    std::string msg;
+};
+
+// Runs f(), turning exceptions into a status code + dsv_last_error message.
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return DSV_OK;
+    } catch (const ArgError& e) {
+        set_error(e.msg);
+        return DSV_E_ARG;
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return e.code == hipErrorOutOfMemory ? DSV_E_NOMEM : DSV_E_HIP;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return DSV_E_HIP;
+    }
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    DSV_HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Coefficient access modes inside kernels.
+enum CoefMode : int { COEF_SCALAR = 0, COEF_CELL = 1, COEF_VERT = 2 };
+
+// Matrix-free partition: vertices (in internal Morton order) are cut into contiguous blocks
+// of NV; block b owns vertices [b*NV, (b+1)*NV) and lists every cell touching one of them.
+struct Partition {
+    int NV = 0;
+    int nb = 0;
+    int threads = 256;
+    int64_t n_entries = 0;
+    int32_t* bptr = nullptr;   // nb+1
+    int4* bconn = nullptr;     // entry -> 4 internal vertex ids
+    int32_t* belem = nullptr;  // entry -> caller cell id
+};
+
+struct Workspace;  // solver buffers, defined in solve.hip
+
+}  // namespace dsv
+
+struct dsv_mesh {
+    int device = 0;
+    int32_t C = 0, V = 0;
+    int64_t nnz = 0;
+    // caller numbering
+    int32_t* conn = nullptr;      // 4C
+    double* xyz = nullptr;        // 3V SoA (x[V], y[V], z[V])
+    double* detJ = nullptr;       // C  |detJ|
+    double* Gflat = nullptr;      // 9C (lazy)
+    double* JinvT = nullptr;      // 9C (lazy)
+    int32_t* indptr = nullptr;    // V+1
+    int32_t* indices = nullptr;   // nnz
+    int32_t* diag_pos = nullptr;  // V
+    int32_t* smap = nullptr;      // 16C scatter map (cell*16+i*4+j -> CSR position)
+    int32_t* cptr = nullptr;      // nnz+1 contribution ranges per CSR entry
+    int32_t* cslot = nullptr;     // 16C slots sorted by (row, col, slot)
+    double* lumped_m = nullptr;   // V  sum over incident cells of |detJ|/24
+    double* Ae = nullptr;         // 16C cell matrices (assembly scratch)
+    // internal (Morton) numbering for the matrix-free path
+    int32_t* perm = nullptr;      // internal -> caller
+    int32_t* iperm = nullptr;     // caller -> internal
+    float* xyz32 = nullptr;       // 3V SoA internal
+    double* xyz64 = nullptr;      // 3V SoA internal
+    dsv::Partition part;
+    std::vector<int32_t> h_conn;  // caller numbering
+    std::vector<int32_t> h_iperm;
+    // solver options
+    double inner_rtol = 1e-3;
+    int max_outer = 12;
+    int graph_chunk = 8;
+    int use_graphs = 1;
+    dsv::Workspace* ws[3][4] = {};  // [method][n_species]
+};
+
+namespace dsv {
+void build_partition(dsv_mesh* m, int NV);
+void free_partition(Partition& p);
+void free_workspaces(dsv_mesh* m);
+void ensure_geometry_views(dsv_mesh* m);
+}  // namespace dsv

@@ -1,0 +1,123 @@
+"""ctypes binding of libdsv.so (declared in include/dsv.h).
+
+The library is loaded from this package directory.  There is no fallback: if libdsv.so is
+missing or fails to load, every entry point raises ``DsvLibraryError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdsv.so")
+
+DSV_OK = 0
+DSV_E_ARG = -1
+DSV_E_MESH = -2
+DSV_E_HIP = -3
+DSV_E_NOMEM = -4
+DSV_E_BREAKDOWN = -5
+
+METHOD_CSR64 = 0
+METHOD_MF32_IR64 = 1
+METHOD_MF64 = 2
+METHODS = {"csr64": METHOD_CSR64, "mf32_ir64": METHOD_MF32_IR64, "mf64": METHOD_MF64}
+
+# every symbol include/dsv.h declares
+EXPORTS = (
+    "dsv_last_error", "dsv_device_count", "dsv_mesh_create", "dsv_mesh_destroy", "dsv_mesh_info",
+    "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs", "dsv_apply",
+    "dsv_solve", "dsv_set_solver_options", "dsv_time_apply",
+)
+
+
+class DsvLibraryError(RuntimeError):
+    pass
+
+
+class DsvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libdsv error {code}: {msg}")
+        self.code = code
+
+
+class Species(ctypes.Structure):
+    _fields_ = [
+        ("D", ctypes.c_void_p), ("D_len", ctypes.c_int32), ("D_value", ctypes.c_double),
+        ("alpha", ctypes.c_void_p), ("alpha_len", ctypes.c_int32), ("alpha_value", ctypes.c_double),
+        ("bc_dofs", ctypes.c_void_p), ("n_bc", ctypes.c_int32), ("bc_value", ctypes.c_double),
+        ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32 * 3), ("outer", ctypes.c_int32 * 3),
+        ("converged", ctypes.c_int32 * 3), ("rel_residual", ctypes.c_double * 3),
+        ("initial_rel_residual", ctypes.c_double * 3), ("solve_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self, n):
+        return {
+            "iterations": list(self.iterations[:n]), "outer": list(self.outer[:n]),
+            "converged": [bool(c) for c in self.converged[:n]],
+            "rel_residual": list(self.rel_residual[:n]),
+            "initial_rel_residual": list(self.initial_rel_residual[:n]),
+            "solve_ms": self.solve_ms,
+        }
+
+
+_lib = None
+
+
+def load():
+    """Load libdsv.so once and declare the signatures; raises DsvLibraryError if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DsvLibraryError(f"{LIB_PATH} not built (run __graft_entry__.build() or make -C csrc)")
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        raise DsvLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+    P = ctypes.c_void_p
+    I32 = ctypes.c_int32
+    D = ctypes.c_double
+    sig = {
+        "dsv_last_error": (ctypes.c_char_p, []),
+        "dsv_device_count": (ctypes.c_int, [ctypes.POINTER(I32)]),
+        "dsv_mesh_create": (ctypes.c_int, [ctypes.POINTER(P), I32, I32, P, P, I32]),
+        "dsv_mesh_destroy": (ctypes.c_int, [P]),
+        "dsv_mesh_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64)]),
+        "dsv_mesh_view": (ctypes.c_int, [P, I32, ctypes.POINTER(P)]),
+        "dsv_mesh_copy_view": (ctypes.c_int, [P, I32, P, P]),
+        "dsv_mesh_set_partition": (ctypes.c_int, [P, I32, I32]),
+        "dsv_assemble": (ctypes.c_int, [P, ctypes.POINTER(Species), D, I32, P, P]),
+        "dsv_rhs": (ctypes.c_int, [P, P, P, I32, D, I32, P, I32, D, P, P]),
+        "dsv_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, ctypes.POINTER(P),
+                                     ctypes.POINTER(P), I32, P]),
+        "dsv_solve": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, D, I32, P,
+                                     ctypes.POINTER(Stats)]),
+        "dsv_set_solver_options": (ctypes.c_int, [P, D, I32, I32, I32]),
+        "dsv_time_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, I32, I32,
+                                          P, ctypes.POINTER(D)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != DSV_OK:
+        msg = load().dsv_last_error().decode(errors="replace")
+        if rc == DSV_E_ARG:
+            raise ValueError(msg)
+        raise DsvError(rc, msg)
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args))

@@ -1,0 +1,156 @@
+"""solve_transport with the call surface of the reference's `numerics/solvers.py:206-372`.
+
+The GPU branch of the reference (`_solve_gpu_asm`, taken when use_gpu and de_field are given
+and pc_type is None / bjacobi / jacobi) assembles the operator on the GPU and solves it with
+PETSc GMRES(50) + bjacobi, warm-started from the current solution.  Here the same operator
+(same assembly, same RHS, same row-only Dirichlet rows) is solved by libdsv's Jacobi-PCG,
+all on the GPU: the matrix is symmetric positive definite on the free rows once the
+Dirichlet values are placed in the initial guess, so CG replaces GMRES; `ksp_type` and
+`pc_type` are validated for compatibility but the Krylov method is always Jacobi-PCG.
+Convergence: ||b - A x||_2 <= rtol ||b||_2 measured in FP64 (PETSc's default test is relative
+to the preconditioned ||b||).
+
+Every other branch of the reference (libCEED MatShell, DOLFINx CPU/GPU standard path) needs
+DOLFINx forms; they are not part of this library and raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib
+from .core import ptr, species_array, species_struct, stream_ptr, to_device
+from .gpu_assembler import GPUSparseAssembler
+
+_SHARED_GEOM = None
+_GPU_ASM_CACHE = {}
+
+
+def set_shared_geometry(asm) -> None:
+    """Register the primary GPUSparseAssembler as the shared geometry source
+    (`numerics/solvers.py:44-47`)."""
+    global _SHARED_GEOM
+    _SHARED_GEOM = asm
+
+
+def _mesh_of(de_field):
+    V = de_field.function_space
+    m = V.mesh
+    return getattr(m, "dsv_mesh", m)
+
+
+def _arr(f):
+    if f is None:
+        return None
+    return f.x.array if hasattr(f, "x") else f
+
+
+def solve_transport(name: str, a_form, L_form, u, bcs: list = None, ksp_type: str = "gmres",
+                    pc_type: str = "bjacobi", rtol: float = 1e-8, max_it: int = 1000,
+                    under_relax: float = None, comm=None, extra_petsc_opts: dict = None,
+                    ceed_op=None, use_gpu: bool = False, de_field=None, alpha_field=None,
+                    u_old=None, source_field=None, bc_value: float = 0.0,
+                    method: str = "csr64") -> dict:
+    """Solve one species' backward-Euler step a(u, v) = L(v) on the GPU.
+
+    Returns {"converged", "iterations", "time_s"} like the reference, plus "rel_residual".
+    `method` selects the libdsv solver: "csr64" (assembled FP64 CSR, the reference's
+    operator path), "mf64" or "mf32_ir64" (matrix-free)."""
+    t0 = time.time()
+    if bcs is None:
+        bcs = []
+    u_prev = None
+    if under_relax is not None and under_relax < 1.0:
+        u_prev = u.x.array.copy()
+        u_old = u_prev   # `numerics/solvers.py:246-247` rebinds u_old to the current u
+    if ceed_op is not None:
+        raise NotImplementedError("the libCEED MatShell path needs DOLFINx forms; use de_field")
+    if not (use_gpu and de_field is not None and pc_type in (None, "bjacobi", "jacobi")):
+        raise NotImplementedError("only the GPU assembled path (use_gpu=True, de_field given, "
+                                  "pc_type in {None, 'bjacobi', 'jacobi'}) is provided")
+    if ksp_type not in ("gmres", "fgmres", "cg", "bcgs"):
+        raise ValueError(f"unsupported ksp_type {ksp_type!r}")
+    converged, iterations, rel = _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type,
+                                                pc_type or "bjacobi", rtol, max_it, comm,
+                                                alpha_field=alpha_field, u_old=u_old,
+                                                source_field=source_field, bc_value=bc_value,
+                                                method=method)
+    if u_prev is not None:
+        u.x.array[:] = under_relax * u.x.array[:] + (1.0 - under_relax) * u_prev
+    u.x.scatter_forward()
+    return {"converged": converged, "iterations": iterations, "time_s": time.time() - t0,
+            "rel_residual": rel}
+
+
+def _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type, pc_type, rtol, max_it, comm,
+                   alpha_field=None, u_old=None, source_field=None, bc_value: float = 0.0,
+                   method: str = "csr64"):
+    """Per-name cached assembler; assemble, RHS, warm start from u, solve, read back
+    (`numerics/solvers.py:287-372`)."""
+    import torch
+    mesh = _mesh_of(de_field)
+    inv_dt = float(getattr(de_field, "_solver_inv_dt", 1.0))
+    bc_dofs = getattr(de_field, "_solver_bc_dofs", None)
+    alpha_arr = _arr(alpha_field)
+    cache = _GPU_ASM_CACHE.get(name)
+    if cache is None:
+        asm = GPUSparseAssembler(mesh, comm=comm, ref_asm=_SHARED_GEOM)
+        cache = {"asm": asm, "x": torch.zeros(asm.n_dofs, dtype=torch.float64, device=mesh.device)}
+        _GPU_ASM_CACHE[name] = cache
+    asm = cache["asm"]
+    if u_old is None:
+        raise NotImplementedError("the DOLFINx RHS branch needs L_form; pass u_old")
+    d_b = asm.assemble_rhs(_arr(u_old), inv_dt, f=_arr(source_field), bc_dofs=bc_dofs,
+                           bc_value=bc_value)
+    x = cache["x"]
+    x.copy_(torch.as_tensor(np.asarray(u.x.array), dtype=torch.float64))
+    keep = []
+    s = asm.species(_arr(de_field), alpha_arr, bc_dofs, bc_value, keep=keep)
+    s.b = ptr(d_b).value
+    s.x = ptr(x).value
+    st = _lib.Stats()
+    _lib.call("dsv_solve", mesh.handle, 1, species_array([s]), inv_dt, int(asm.lumped),
+              _lib.METHODS[method], float(rtol), int(max_it), stream_ptr(mesh.device), st)
+    if method == "csr64":
+        # keep the assembled operator visible as asm.mat, like the reference's cached Mat
+        asm.assemble(_arr(de_field), inv_dt, alpha=alpha_arr, bc_dofs=bc_dofs)
+    u.x.array[:] = x.cpu().numpy()
+    return bool(st.converged[0]), int(st.iterations[0]), float(st.rel_residual[0])
+
+
+def solve_batched(mesh, species, inv_dt, rtol=1e-8, max_it=5000, method="mf32_ir64",
+                  lumped=1, stream=None):
+    """Solve up to three species sharing `mesh` in the same kernels.
+
+    species: list of dicts with keys D / D_value, alpha / alpha_value, bc_dofs, bc_value,
+    b (torch f64 device), x (torch f64 device, warm start in, solution out).
+    Returns the libdsv statistics as a dict."""
+    keep = []
+    structs = []
+    for sp in species:
+        s = species_struct(mesh, D=sp.get("D"), D_value=sp.get("D_value", 0.0),
+                           alpha=sp.get("alpha"), alpha_value=sp.get("alpha_value", 0.0),
+                           bc_dofs=sp.get("bc_dofs"), bc_value=sp.get("bc_value", 0.0),
+                           b=sp["b"], x=sp["x"], keep=keep)
+        structs.append(s)
+    st = _lib.Stats()
+    _lib.call("dsv_solve", mesh.handle, len(structs), species_array(structs), float(inv_dt),
+              int(lumped), _lib.METHODS[method], float(rtol), int(max_it),
+              stream if stream is not None else stream_ptr(mesh.device), st)
+    return st.as_dict(len(structs))
+
+
+class MatrixFreeOperator:
+    """Matrix-free operator handle with the reference's `mult(A, x, y)` hook
+    (`numerics/solvers.py:172-203`); x and y are host or device arrays of length n_dofs."""
+
+    def __init__(self, ceed_op, ndofs: int, comm=None):
+        self.ceed_op = ceed_op
+        self.ndofs = ndofs
+        self.comm = comm
+        self.mat = self
+
+    def mult(self, A, x, y):
+        out = self.ceed_op.apply(x, None)
+        y[:] = out.cpu().numpy() if hasattr(y, "shape") and not hasattr(y, "device") else out

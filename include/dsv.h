@@ -1,0 +1,144 @@
+/*
+ * dsv.h -- C-ABI of libdsv.so, the MI355X (gfx950) transport hot path of Dissolve 2.0:
+ * P1-tet backward-Euler diffusion-reaction assembly into CSR, the matrix-free apply of the
+ * same operator, and the Jacobi-PCG solve for up to three species at once.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative DSV_E* code on failure; the message
+ *     of the last failure on the calling thread is returned by dsv_last_error().  Nothing is
+ *     thrown across the ABI.
+ *   - Mesh arrays given to dsv_mesh_create are HOST pointers and are copied.  Every other
+ *     array argument is a DEVICE pointer (memory of the device the mesh was created on),
+ *     in the caller's vertex / cell numbering.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are stream-ordered;
+ *     dsv_solve and dsv_pattern synchronise the stream before returning.
+ *   - Cell / vertex coefficient arrays follow the reference's _to_cell_avg rule: length C is
+ *     taken per cell, length V is averaged over the 4 vertices of each cell.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * Dissolve 2.0 repository).
+ */
+#ifndef DSV_H
+#define DSV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSV_OK 0
+#define DSV_E_ARG (-1)      /* invalid argument (length, index, NULL, bad mode) */
+#define DSV_E_MESH (-2)     /* degenerate cell (detJ == 0) or unused vertex */
+#define DSV_E_HIP (-3)      /* HIP runtime error */
+#define DSV_E_NOMEM (-4)    /* device allocation failed */
+#define DSV_E_BREAKDOWN (-5) /* CG breakdown (p.Ap <= 0 or non-finite) */
+
+/* solve methods */
+#define DSV_METHOD_CSR64 0    /* assembled FP64 CSR + FP64 Jacobi-PCG ("plain FP64 path") */
+#define DSV_METHOD_MF32_IR64 1 /* FP32 matrix-free Jacobi-PCG inside FP64 iterative refinement */
+#define DSV_METHOD_MF64 2     /* FP64 matrix-free Jacobi-PCG */
+
+typedef struct dsv_mesh dsv_mesh;
+
+/* One species of a batched operator / solve.  Coefficient pointers are device arrays of
+ * length `*_len` (C or V); a NULL pointer means the scalar `*_value` everywhere (the
+ * reference passes None for alpha -> 0, `gpu/gpu_assembler.py:459-460`). */
+typedef struct dsv_species {
+    const double* D;        /* diffusion coefficient De */
+    int32_t D_len;
+    double D_value;
+    const double* alpha;    /* linear reaction coefficient */
+    int32_t alpha_len;
+    double alpha_value;
+    const int32_t* bc_dofs; /* Dirichlet vertices (device, caller numbering) or NULL */
+    int32_t n_bc;
+    double bc_value;
+    const double* b;        /* right-hand side (dsv_solve) */
+    double* x;              /* initial guess in, solution out (dsv_solve) */
+} dsv_species;
+
+typedef struct dsv_stats {
+    int32_t iterations[3];       /* CG iterations (inner FP32 iterations for MF32_IR64) */
+    int32_t outer[3];            /* refinement sweeps (MF32_IR64), 1 otherwise */
+    int32_t converged[3];        /* ||b - A x||_2 <= rtol ||b||_2 in FP64 */
+    double rel_residual[3];      /* final true FP64 ||b - A x|| / ||b|| */
+    double initial_rel_residual[3];
+    double solve_ms;             /* wall time of the solve on the stream */
+} dsv_stats;
+
+const char* dsv_last_error(void);
+int dsv_device_count(int32_t* n);
+
+/* --- mesh / geometry / pattern: GPUSparseAssembler.__init__ (`gpu/gpu_assembler.py:369-450`) --- */
+
+/* Copies conn (C x 4, caller numbering) and xyz (V x 3) to the current device, checks the
+ * indices and that no cell is degenerate, and builds the CSR pattern, the scatter map, the
+ * per-cell geometry and the matrix-free partition.  vertices_per_block <= 0 picks the default. */
+int dsv_mesh_create(dsv_mesh** out, int32_t n_cells, int32_t n_verts, const int32_t* conn,
+                    const double* xyz, int32_t vertices_per_block);
+int dsv_mesh_destroy(dsv_mesh* m);
+/* sizes: n_cells, n_verts, nnz, number of matrix-free blocks, vertices per block, the
+ * number of (block, cell) entries of the partition */
+int dsv_mesh_info(const dsv_mesh* m, int64_t* info6);
+
+/* Device views owned by the mesh (valid until dsv_mesh_destroy): the attributes
+ * d_indptr, d_indices, d_scatter_map, d_detJ, d_Gflat, d_JinvT, d_elem_nodes of
+ * GPUSparseAssembler (`gpu/gpu_assembler.py:388-438`). which: 0 indptr (int32 V+1),
+ * 1 indices (int32 nnz), 2 scatter_map (int32 16C), 3 detJ (f64 C), 4 Gflat (f64 9C),
+ * 5 JinvT (f64 9C), 6 elem_nodes (int32 4C). */
+int dsv_mesh_view(const dsv_mesh* m, int32_t which, void** ptr);
+/* Copies view `which` (see dsv_mesh_view) into the caller's device buffer dst. */
+int dsv_mesh_copy_view(const dsv_mesh* m, int32_t which, void* dst, void* stream);
+
+/* Repartitions the matrix-free blocks (vertices_per_block) and sets the workgroup size of
+ * the matrix-free kernels; used by the autotuner. */
+int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t block_threads);
+
+/* --- assembly: GPUSparseAssembler.assemble (`gpu/gpu_assembler.py:472-532`) ---
+ * values (device, nnz, pattern order) = A = (inv_dt + alpha_e) M_e + De_e K_e summed over
+ * cells, lumped (M = dJ/24 I) when lumped != 0 else consistent, then Dirichlet rows
+ * (row zeroed, diagonal 1; `gpu/gpu_assembler.py:147-165`).  Deterministic (no atomics). */
+int dsv_assemble(dsv_mesh* m, const dsv_species* sp, double inv_dt, int32_t lumped,
+                 double* values, void* stream);
+
+/* --- RHS: GPUSparseAssembler.assemble_rhs (`gpu/gpu_assembler.py:534-603`) ---
+ * b = M (inv_dt u_old + f) with f per vertex (f_len == V) or absent (f == NULL), then
+ * b[bc] = bc_value. */
+int dsv_rhs(dsv_mesh* m, const double* u_old, const double* f, int32_t f_len, double inv_dt,
+            int32_t lumped, const int32_t* bc_dofs, int32_t n_bc, double bc_value, double* b,
+            void* stream);
+
+/* --- matrix-free apply: CeedDiffusionMassOp.apply (`gpu/ceed_operators.py:341-343`) ---
+ * y_s = A_s x_s for n_species species sharing the mesh, the SAME operator as dsv_assemble
+ * (including the Dirichlet identity rows).  precision: 64 -> FP64 arithmetic, 32 -> FP32
+ * geometry and arithmetic (x, y stay FP64 at the boundary).  x[s], y[s] are device arrays. */
+int dsv_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+              int32_t lumped, const double* const* x, double* const* y, int32_t precision,
+              void* stream);
+
+/* --- solve: _solve_gpu_asm's KSP solve (`numerics/solvers.py:287-372`) ---
+ * For each species s: solve A_s x_s = b_s with Jacobi-PCG to ||b - A x||_2 <= rtol ||b||_2
+ * (FP64 true residual), starting from sp[s].x (Dirichlet entries are set to bc_value first).
+ * All species advance in the same kernels. */
+int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+              int32_t lumped, int32_t method, double rtol, int32_t max_it, void* stream,
+              dsv_stats* stats);
+
+/* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
+ * sweeps) and the number of CG iterations per captured hipGraph chunk. */
+int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
+                           int32_t graph_chunk, int32_t use_graphs);
+
+/* Average device duration (ms) of one launch of the fused n_species matrix-free kernel,
+ * measured with HIP events around n_rep back-to-back launches on `stream`.  mode 0 times the
+ * plain apply y = A x (x = sp[s].x, or the lumped mass when NULL); mode 1 times the CG
+ * iteration form (p = z + beta p_old on the fly, q = A p, p.q partials) used inside dsv_solve. */
+int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                   int32_t lumped, int32_t precision, int32_t mode, int32_t n_rep, void* stream,
+                   double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSV_H */

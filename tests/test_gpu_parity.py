@@ -1,0 +1,246 @@
+"""GPU parity of libdsv against the oracle, through the C-ABI (ctypes).
+
+Bars: CSR pattern and scatter map bit-exact; FP64 matrix / RHS entries within 1e-12
+relative (to the largest entry of the row set); FP64 matrix-free apply within 1e-12; FP32
+apply within 2e-5 relative to max|y|; every solve within 1e-8 relative of scipy splu.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from dsv import _lib, meshes, workloads
+from oracle import assembly as OA
+from oracle import solve as OS
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh_cases():
+    out = {}
+    conn, xyz = meshes.kuhn_cube(6)
+    out["kuhn6"] = (conn, xyz)
+    rng = np.random.default_rng(11)
+    c2, x2 = meshes.kuhn_cube(7)
+    x2 = meshes.jitter(x2, 1 / 7, 0.2, rng, ~meshes.boundary_vertex_mask(x2))
+    out["kuhn7_jitter_morton"] = meshes.locality_order(c2, x2)
+    out["disc_small"] = meshes.graded_disc(seed=1, scale=6.0)
+    # random cell order + random vertex numbering (no locality at all)
+    c3, x3 = meshes.kuhn_cube(5)
+    p = rng.permutation(len(x3))
+    inv = np.empty_like(p)
+    inv[p] = np.arange(len(p))
+    out["kuhn5_shuffled"] = (inv[c3][rng.permutation(len(c3))].astype(np.int32), x3[p])
+    return out
+
+
+CASES = _mesh_cases()
+
+
+@pytest.fixture(scope="module")
+def torch_dev(gpu):
+    return gpu
+
+
+def _m(name, **kw):
+    from dsv.core import Mesh
+    conn, xyz = CASES[name]
+    return Mesh(conn, xyz, **kw), conn, xyz
+
+
+def _relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_pattern_and_scatter_map_bit_exact(torch_dev, name):
+    from dsv.core import VIEW_INDICES, VIEW_INDPTR, VIEW_SCATTER_MAP, VIEW_DETJ
+    m, conn, xyz = _m(name)
+    ip, ix = OA.pattern(conn, len(xyz))
+    assert m.nnz == len(ix)
+    np.testing.assert_array_equal(m.view(VIEW_INDPTR).cpu().numpy(), ip)
+    np.testing.assert_array_equal(m.view(VIEW_INDICES).cpu().numpy(), ix)
+    np.testing.assert_array_equal(m.view(VIEW_SCATTER_MAP).cpu().numpy(), OA.scatter_map(conn, ip, ix))
+    detJ, G, JinvT = OA.geometry(conn, xyz)
+    assert _relerr(m.view(VIEW_DETJ).cpu().numpy(), detJ) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["kuhn7_jitter_morton", "disc_small", "kuhn5_shuffled"])
+@pytest.mark.parametrize("lumped", [1, 0])
+@pytest.mark.parametrize("coef", ["cell", "vertex", "scalar"])
+def test_assembly_values(torch_dev, name, lumped, coef):
+    from dsv.gpu_assembler import GPUSparseAssembler
+    m, conn, xyz = _m(name)
+    rng = np.random.default_rng(3)
+    C, V = len(conn), len(xyz)
+    if coef == "scalar":
+        D, al = np.full(V, 1.7), None
+    else:
+        n = C if coef == "cell" else V
+        D, al = rng.lognormal(0, 0.5, n), rng.uniform(0, 2, n)
+    bc = np.sort(rng.choice(V, size=max(1, V // 20), replace=False)).astype(np.int32)
+    asm = GPUSparseAssembler(m)
+    asm.lumped = lumped
+    asm.assemble(D, 200.0, alpha=al, bc_dofs=bc)
+    ip, ix, ref = OA.assemble_values(conn, xyz, D, 200.0, al, lumped, bc)
+    got = asm.d_data.cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the exported CSR has the reference layout
+    ip2, ix2, v2 = asm.mat.getValuesCSR()
+    np.testing.assert_array_equal(ip2, ip)
+    np.testing.assert_array_equal(ix2, ix)
+
+
+@pytest.mark.parametrize("lumped", [1, 0])
+def test_rhs(torch_dev, lumped):
+    from dsv.gpu_assembler import GPUSparseAssembler
+    m, conn, xyz = _m("disc_small")
+    rng = np.random.default_rng(4)
+    V = len(xyz)
+    u, f = rng.uniform(0, 1, V), rng.uniform(-1, 1, V)
+    bc = np.arange(0, V, 17, dtype=np.int32)
+    asm = GPUSparseAssembler(m)
+    asm.lumped = lumped
+    got = asm.assemble_rhs(u, 200.0, f=f, bc_dofs=bc, bc_value=0.3).cpu().numpy()
+    ref = OA.assemble_rhs(conn, xyz, u, 200.0, f, lumped, bc, 0.3)
+    assert _relerr(got, ref) < 1e-12
+    # per-cell f: the reference's lossy branch
+    fc = rng.uniform(0, 1, len(conn))
+    got = asm.assemble_rhs(u, 200.0, f=fc).cpu().numpy()
+    assert _relerr(got, OA.assemble_rhs(conn, xyz, u, 200.0, fc, lumped)) < 1e-12
+    with pytest.raises(ValueError):
+        asm.assemble_rhs(u[:-1], 200.0)
+    with pytest.raises(ValueError):
+        asm.assemble_rhs(u, 200.0, f=np.ones(V + 3))
+
+
+def _species_csr(conn, xyz, sp, inv_dt, lumped):
+    ip, ix, data = OA.assemble_values(conn, xyz, sp.get("D", sp.get("D_value")), inv_dt,
+                                      sp.get("alpha"), lumped, sp.get("bc_dofs"))
+    return OA.to_scipy(ip, ix, data, len(xyz))
+
+
+@pytest.mark.parametrize("name", ["kuhn7_jitter_morton", "disc_small", "kuhn5_shuffled"])
+@pytest.mark.parametrize("lumped", [1, 0])
+def test_matrix_free_apply_three_species(torch_dev, name, lumped):
+    import torch
+    from dsv.ceed_operators import apply_batched
+    m, conn, xyz = _m(name, vertices_per_block=64)
+    rng = np.random.default_rng(5)
+    C, V = len(conn), len(xyz)
+    bc = np.arange(0, V, 11, dtype=np.int32)
+    species = [dict(D=rng.lognormal(0, 0.5, C), alpha=rng.uniform(0, 3, V), bc_dofs=bc),
+               dict(D_value=2.0),
+               dict(D=rng.uniform(1, 5, V), alpha_value=0.5)]
+    xs = [rng.uniform(-1, 1, V) for _ in species]
+    for prec, tol in ((64, 1e-12), (32, 2e-5)):
+        ys = apply_batched(m, species, 200.0, xs, lumped=lumped, precision=prec)
+        for s, x, y in zip(species, xs, ys):
+            A = _species_csr(conn, xyz, s, 200.0, lumped)
+            ref = A @ x
+            assert _relerr(y.cpu().numpy(), ref) < tol, (prec, name)
+
+
+def _solve_case(name, lumped=1, seed=6):
+    conn, xyz = CASES[name]
+    rng = np.random.default_rng(seed)
+    C, V = len(conn), len(xyz)
+    bc = np.nonzero(xyz[:, 2] <= xyz[:, 2].min() + 1e-9)[0].astype(np.int32)
+    dt = 0.005
+    sps = [dict(D=rng.lognormal(0, 0.5, C), alpha=rng.uniform(0, 50, V), bc_dofs=bc, bc_value=1.0),
+           dict(D_value=2.0),
+           dict(D_value=5.3, f=rng.uniform(0, 1, V))]
+    u0 = [rng.uniform(0, 1, V) for _ in sps]
+    refs, bs = [], []
+    for s, u in zip(sps, u0):
+        A = _species_csr(conn, xyz, s, 1 / dt, lumped)
+        b = OA.assemble_rhs(conn, xyz, u, 1 / dt, s.get("f"), lumped, s.get("bc_dofs"),
+                            s.get("bc_value", 0.0))
+        refs.append(OS.direct(A, b))
+        bs.append(b)
+    return conn, xyz, sps, u0, bs, refs, 1 / dt
+
+
+@pytest.mark.parametrize("method", ["csr64", "mf64", "mf32_ir64"])
+@pytest.mark.parametrize("name", ["kuhn7_jitter_morton", "disc_small"])
+@pytest.mark.parametrize("lumped", [1, 0])
+def test_batched_solve_matches_direct(torch_dev, method, name, lumped):
+    import torch
+    from dsv.core import Mesh
+    from dsv.solvers import solve_batched
+    conn, xyz, sps, u0, bs, refs, inv_dt = _solve_case(name, lumped)
+    m = Mesh(conn, xyz, vertices_per_block=64)
+    sp = []
+    for s, u, b in zip(sps, u0, bs):
+        d = dict(s)
+        d.pop("f", None)
+        d["b"] = torch.tensor(b, dtype=torch.float64, device=torch_dev)
+        d["x"] = torch.tensor(u, dtype=torch.float64, device=torch_dev)
+        sp.append(d)
+    st = solve_batched(m, sp, inv_dt, rtol=1e-11, method=method, lumped=lumped)
+    for k in range(3):
+        assert st["converged"][k], st
+        assert st["rel_residual"][k] <= 1e-11
+        err = _relerr(sp[k]["x"].cpu().numpy(), refs[k])
+        assert err < 1e-8, (method, k, err, st)
+
+
+def test_single_species_and_warm_start(torch_dev):
+    import torch
+    from dsv.core import Mesh
+    from dsv.solvers import solve_batched
+    conn, xyz, sps, u0, bs, refs, inv_dt = _solve_case("kuhn7_jitter_morton")
+    m = Mesh(conn, xyz)
+    s = dict(sps[0])
+    s["b"] = torch.tensor(bs[0], device=torch_dev)
+    s["x"] = torch.tensor(u0[0], device=torch_dev)
+    st1 = solve_batched(m, [s], inv_dt, rtol=1e-10, method="csr64")
+    assert st1["converged"][0]
+    # warm start from the solution: converged before any iteration
+    st2 = solve_batched(m, [s], inv_dt, rtol=1e-10, method="csr64")
+    assert st2["converged"][0] and st2["iterations"][0] == 0
+    # iteration cap reports non-convergence instead of raising
+    s["x"] = torch.tensor(u0[0], device=torch_dev)
+    st3 = solve_batched(m, [s], inv_dt, rtol=1e-12, method="mf64", max_it=2)
+    assert not st3["converged"][0]
+
+
+def test_fixed_points_on_gpu(torch_dev):
+    import torch
+    from dsv.core import Mesh
+    from dsv.solvers import solve_batched
+    conn, xyz = CASES["kuhn7_jitter_morton"]
+    V = len(xyz)
+    m = Mesh(conn, xyz)
+    dt, al = 0.01, 3.0
+    u = np.full(V, 0.8)
+    b = OA.assemble_rhs(conn, xyz, u, 1 / dt, None, 1)
+    for method in ("csr64", "mf64", "mf32_ir64"):
+        s = dict(D_value=2.0, alpha_value=al, b=torch.tensor(b, device=torch_dev),
+                 x=torch.zeros(V, dtype=torch.float64, device=torch_dev))
+        st = solve_batched(m, [s], 1 / dt, rtol=1e-12, method=method)
+        assert _relerr(s["x"].cpu().numpy(), u / (1 + al * dt)) < 1e-10, (method, st)
+
+
+def test_argument_errors(torch_dev):
+    import torch
+    from dsv.core import Mesh
+    from dsv.gpu_assembler import GPUSparseAssembler
+    from dsv.solvers import solve_batched
+    conn, xyz = CASES["kuhn6"]
+    m = Mesh(conn, xyz)
+    asm = GPUSparseAssembler(m)
+    with pytest.raises(ValueError, match="matches neither"):
+        asm.assemble(np.ones(len(xyz) + 1), 1.0)
+    V = len(xyz)
+    s = dict(D_value=1.0, bc_dofs=np.array([V + 5], dtype=np.int32),
+             b=torch.zeros(V, dtype=torch.float64, device=torch_dev),
+             x=torch.zeros(V, dtype=torch.float64, device=torch_dev))
+    with pytest.raises(ValueError, match="out of range"):
+        solve_batched(m, [s], 1.0)
+    bad = conn.copy()
+    bad[0, 1] = bad[0, 0]
+    with pytest.raises(_lib.DsvError):
+        Mesh(bad, xyz)
+    with pytest.raises(ValueError):
+        Mesh(np.array([[0, 1, 2, 99]], dtype=np.int32), xyz[:4])
