@@ -103,6 +103,7 @@ def main():
     ap.add_argument("--workload", default="disc3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-reps", type=int, default=50)
+    ap.add_argument("--no-tune", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -122,6 +123,14 @@ def main():
     log(f"device mesh: nnz={mesh.nnz} blocks={mesh.n_blocks} entries={mesh.n_entries} "
         f"({time.perf_counter() - t:.1f}s)")
     rtol = w["rtol"]
+    tuned = None
+    if not a.no_tune:
+        from dsv.autotune import tune
+        best, table = tune(mesh, w["species"], 1.0 / w["dt"])
+        for row in table:
+            log(f"tune nv={row[0]} threads={row[1]} entries={row[2]} {row[3] * 1e3:.1f} us")
+        tuned = {"vertices_per_block": best[0], "threads": best[1]}
+        log(f"tuned: {tuned}")
     res = {}
     for method in ("csr64", "mf32_ir64"):
         r, _ = run_path(mesh, w, method, a.steps, a.warmup, rtol)
@@ -170,7 +179,7 @@ def main():
                                f"vertices, O2/Cl/OH D=2.0/2.0/5.3, dt={w['dt']}, rtol={rtol:g}",
                    "method": "FP32 matrix-free fused 3-species Jacobi-PCG in FP64 refinement, "
                              "previous-solution warm start, hipGraph chunks",
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU", "launch_shape": tuned},
         "plain_fp64": {"steps_per_s": round(plain["steps_per_s"], 3),
                        "iterations_last_step": plain["iterations_per_step"][-1]},
         "chosen_vs_plain": round(chosen["steps_per_s"] / plain["steps_per_s"], 3),

@@ -1,0 +1,52 @@
+"""Autotuner of the matrix-free launch shape.
+
+The fused apply kernel runs one workgroup per block of `vertices_per_block` consecutive
+(Morton-ordered) vertices; the workgroup size sets how many cells each lane processes.  Small
+blocks give more workgroups but duplicate more boundary cells; large workgroups hide more
+latency per block.  The tuner times the CG form of the fused kernel (HIP events, libdsv
+dsv_time_apply) for each candidate on the real coefficients and keeps the fastest.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+from .core import species_array, species_struct
+
+DEFAULT_NV = (64, 128, 256, 512, 1024)
+DEFAULT_THREADS = (256, 512, 1024)
+
+
+def time_apply(mesh, species, inv_dt, precision=32, mode=1, reps=30, lumped=1, stream=None):
+    """Average ms per launch of the fused kernel for `species` (list of dicts)."""
+    import torch
+    keep = []
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                              bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                              keep=keep) for s in species]
+    arr = species_array(structs)
+    if stream is None:
+        stream = torch.cuda.Stream(device=mesh.device)
+    ms = ctypes.c_double()
+    _lib.call("dsv_time_apply", mesh.handle, len(structs), arr, float(inv_dt), int(lumped),
+              int(precision), int(mode), int(reps), ctypes.c_void_p(stream.cuda_stream),
+              ctypes.byref(ms))
+    return ms.value
+
+
+def tune(mesh, species, inv_dt, nv_list=DEFAULT_NV, threads_list=DEFAULT_THREADS, reps=30,
+         precision=32):
+    """Try every (vertices_per_block, threads) pair, leave the mesh on the fastest, and
+    return (best, table) with table entries (nv, threads, entries, ms)."""
+    table = []
+    best = None
+    for nv in nv_list:
+        for th in threads_list:
+            mesh.set_partition(nv, th)
+            ms = time_apply(mesh, species, inv_dt, precision=precision, reps=reps)
+            table.append((nv, th, mesh.n_entries, ms))
+            if best is None or ms < best[3]:
+                best = (nv, th, mesh.n_entries, ms)
+    mesh.set_partition(best[0], best[1])
+    return best, table

@@ -125,7 +125,7 @@ def tet_quality(conn, xyz):
 
 
 def graded_disc(seed=0, h_top=0.005, h_bot=0.04, radius=1.0, height=0.2, scale=1.0,
-                q_min=0.05):
+                q_min=0.05, xy_factor=1.0):
     """Seeded graded Delaunay fill of the cylinder r <= radius, 0 <= z <= height.
 
     Points lie on horizontal layers; the layer spacing and the in-layer spacing both follow
@@ -133,7 +133,9 @@ def graded_disc(seed=0, h_top=0.005, h_bot=0.04, radius=1.0, height=0.2, scale=1
     hexagonal lattice clipped to the disc plus a ring of points on the rim.  The Delaunay
     tets (scipy.spatial.Delaunay) with quality below ``q_min`` are removed (slivers), then
     unused vertices are dropped.  ``scale`` sets the density: scale=1 gives ~2.4M tets and
-    455k vertices (seed 0); the ~800k-tet transport workload uses scale=1.47.
+    455k vertices (seed 0); ``xy_factor`` shrinks the in-layer spacing relative to the layer
+    spacing.  The count moves in steps (the top layers hold most points), so the ~800k-tet
+    transport workload uses scale=1.47, xy_factor=0.94 (805,903 tets for seed 0).
     """
     from scipy.spatial import Delaunay
 
@@ -153,7 +155,7 @@ def graded_disc(seed=0, h_top=0.005, h_bot=0.04, radius=1.0, height=0.2, scale=1
     pts = []
     last = len(zs) - 1
     for li, z in enumerate(zs):
-        h = h_of(z)
+        h = xy_factor * h_of(z)
         # rim ring: random phase per layer; interior layers get a z jitter so that no four
         # rim points are co-circular (co-spherical input makes the Delaunay fill very slow)
         nr = max(8, int(np.ceil(2 * np.pi * radius / h)))

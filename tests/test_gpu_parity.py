@@ -115,8 +115,9 @@ def test_rhs(torch_dev, lumped):
 
 
 def _species_csr(conn, xyz, sp, inv_dt, lumped):
-    ip, ix, data = OA.assemble_values(conn, xyz, sp.get("D", sp.get("D_value")), inv_dt,
-                                      sp.get("alpha"), lumped, sp.get("bc_dofs"))
+    D = sp["D"] if sp.get("D") is not None else sp.get("D_value", 0.0)
+    al = sp["alpha"] if sp.get("alpha") is not None else sp.get("alpha_value", 0.0)
+    ip, ix, data = OA.assemble_values(conn, xyz, D, inv_dt, al, lumped, sp.get("bc_dofs"))
     return OA.to_scipy(ip, ix, data, len(xyz))
 
 
