@@ -66,14 +66,28 @@ enum CoefMode : int { COEF_SCALAR = 0, COEF_CELL = 1, COEF_VERT = 2 };
 
 // Matrix-free partition: vertices (in internal Morton order) are cut into contiguous blocks
 // of NV; block b owns vertices [b*NV, (b+1)*NV) and lists every cell touching one of them.
+// A block's local vertex numbering: owned vertices first (local = internal - b*NV), then its
+// halo (the other vertices of its cells, ascending internal id).  Cells are stored with
+// local 16-bit indices, so the kernels read vertex data from LDS after staging it once.
 struct Partition {
     int NV = 0;
     int nb = 0;
     int threads = 256;
+    int max_local = 0;          // max over blocks of owned + halo vertices
+    int max_entries = 0;        // max over blocks of listed cells
+    int max_inc = 0;            // max over blocks of owned-vertex incidences
     int64_t n_entries = 0;
-    int32_t* bptr = nullptr;   // nb+1
-    int4* bconn = nullptr;     // entry -> 4 internal vertex ids
-    int32_t* belem = nullptr;  // entry -> caller cell id
+    int64_t n_halo = 0;
+    int32_t* bptr = nullptr;    // nb+1 entry ranges
+    ushort4* bloc = nullptr;    // entry -> 4 local vertex indices
+    int32_t* belem = nullptr;   // entry -> caller cell id
+    int32_t* hptr = nullptr;    // nb+1 halo ranges
+    int32_t* hlist = nullptr;   // halo vertices (internal ids)
+    // Owned-corner slots: the 4C (cell, owned corner) pairs are numbered per block, grouped
+    // by owning vertex (vertex v's slots are [vptr[v], vptr[v+1]) minus the block's first
+    // slot).  bslot gives, per entry, the block-local slot of each corner (0xFFFF: halo).
+    int32_t* vptr = nullptr;    // V+1
+    ushort4* bslot = nullptr;   // per entry
 };
 
 struct Workspace;  // solver buffers, defined in solve.hip

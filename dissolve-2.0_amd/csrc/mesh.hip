@@ -155,14 +155,21 @@ static void morton_order(const double* xyz_aos, int32_t V, std::vector<int32_t>&
 
 void free_partition(Partition& p) {
     dfree(p.bptr);
-    dfree(p.bconn);
+    dfree(p.bloc);
     dfree(p.belem);
+    dfree(p.hptr);
+    dfree(p.hlist);
+    dfree(p.vptr);
+    dfree(p.bslot);
     p.nb = 0;
     p.n_entries = 0;
+    p.n_halo = 0;
+    p.max_local = 0;
 }
 
 // Block b owns internal vertices [b*NV, min((b+1)*NV, V)); its entry list holds every cell
-// with at least one owned vertex, in ascending caller cell order.
+// with at least one owned vertex, in ascending caller cell order, with the cell's vertices
+// renumbered locally (owned first, then the sorted halo).
 void build_partition(dsv_mesh* m, int NV) {
     if (NV < 32 || NV > 2048) throw ArgError{"vertices_per_block must be in [32, 2048]"};
     const int32_t C = m->C, V = m->V;
@@ -190,7 +197,7 @@ void build_partition(dsv_mesh* m, int NV) {
     if (ne >= (int64_t)INT32_MAX) throw ArgError{"partition too large for int32 entry indices"};
     std::vector<int32_t> bptr(nb + 1);
     for (int b = 0; b <= nb; ++b) bptr[b] = (int32_t)cnt[b];
-    std::vector<int4> bconn(ne);
+    std::vector<int4> gconn(ne);
     std::vector<int32_t> belem(ne);
     std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
     for (int64_t e = 0; e < C; ++e) {
@@ -199,21 +206,99 @@ void build_partition(dsv_mesh* m, int NV) {
         int4 c4 = make_int4(ip[cn[4 * e]], ip[cn[4 * e + 1]], ip[cn[4 * e + 2]], ip[cn[4 * e + 3]]);
         for (int q = 0; q < n; ++q) {
             int64_t k = fill[bl[q]]++;
-            bconn[k] = c4;
+            gconn[k] = c4;
             belem[k] = (int32_t)e;
         }
+    }
+    // halo lists and local numbering
+    std::vector<int32_t> hptr(nb + 1, 0);
+    std::vector<int32_t> vptr((size_t)V + 1, 0);
+    std::vector<ushort4> bslot(ne);
+    std::vector<int32_t> vcount;
+    int max_entries = 0, max_inc = 0;
+    std::vector<int32_t> hlist;
+    std::vector<ushort4> bloc(ne);
+    std::vector<int32_t> halo;
+    int max_local = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int v0 = b * NV, nv = std::min(NV, V - v0);
+        halo.clear();
+        for (int64_t k = bptr[b]; k < bptr[b + 1]; ++k) {
+            const int32_t g4[4] = {gconn[k].x, gconn[k].y, gconn[k].z, gconn[k].w};
+            for (int j = 0; j < 4; ++j)
+                if (g4[j] < v0 || g4[j] >= v0 + nv) halo.push_back(g4[j]);
+        }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        const int nloc = nv + (int)halo.size();
+        if (nloc > 65535) throw ArgError{"partition block has more than 65535 local vertices"};
+        max_local = std::max(max_local, nloc);
+        for (int64_t k = bptr[b]; k < bptr[b + 1]; ++k) {
+            const int32_t g4[4] = {gconn[k].x, gconn[k].y, gconn[k].z, gconn[k].w};
+            unsigned short l4[4];
+            for (int j = 0; j < 4; ++j) {
+                const int32_t g = g4[j];
+                if (g >= v0 && g < v0 + nv)
+                    l4[j] = (unsigned short)(g - v0);
+                else
+                    l4[j] = (unsigned short)(nv + (std::lower_bound(halo.begin(), halo.end(), g) -
+                                                   halo.begin()));
+            }
+            bloc[k] = make_ushort4(l4[0], l4[1], l4[2], l4[3]);
+        }
+        // owned-vertex incidence: every corner of every cell is owned by exactly one block
+        const int64_t nbe = bptr[b + 1] - bptr[b];
+        if (nbe > 65535) throw ArgError{"partition block lists more than 65535 cells"};
+        max_entries = std::max(max_entries, (int)nbe);
+        vcount.assign(nv + 1, 0);
+        for (int64_t e = 0; e < nbe; ++e) {
+            const ushort4 l = bloc[bptr[b] + e];
+            const unsigned short l4[4] = {l.x, l.y, l.z, l.w};
+            for (int j = 0; j < 4; ++j)
+                if (l4[j] < nv) vcount[l4[j] + 1]++;
+        }
+        for (int i = 0; i < nv; ++i) vcount[i + 1] += vcount[i];
+        const int32_t base = vptr[v0];
+        for (int i = 0; i <= nv; ++i) vptr[v0 + i] = base + vcount[i];
+        max_inc = std::max(max_inc, vcount[nv]);
+        if (vcount[nv] > 65535) throw ArgError{"partition block has more than 65535 owned corners"};
+        for (int64_t e = 0; e < nbe; ++e) {
+            const ushort4 l = bloc[bptr[b] + e];
+            const unsigned short l4[4] = {l.x, l.y, l.z, l.w};
+            unsigned short s4[4];
+            for (int j = 0; j < 4; ++j)
+                s4[j] = l4[j] < nv ? (unsigned short)(vcount[l4[j]]++) : (unsigned short)0xFFFF;
+            bslot[bptr[b] + e] = make_ushort4(s4[0], s4[1], s4[2], s4[3]);
+        }
+        hlist.insert(hlist.end(), halo.begin(), halo.end());
+        if ((int64_t)hlist.size() >= (int64_t)INT32_MAX) throw ArgError{"halo lists too large"};
+        hptr[b + 1] = (int32_t)hlist.size();
     }
     Partition p;
     p.NV = NV;
     p.nb = nb;
     p.threads = m->part.threads ? m->part.threads : 256;
     p.n_entries = ne;
+    p.n_halo = (int64_t)hlist.size();
+    p.max_local = max_local;
+    p.max_entries = max_entries;
+    p.max_inc = max_inc;
+    if ((int64_t)vptr[V] != 4 * (int64_t)C) throw ArgError{"internal error: incidence count"};
+    p.vptr = dalloc<int32_t>((size_t)V + 1);
+    p.bslot = dalloc<ushort4>(ne);
+    DSV_HIP(hipMemcpy(p.vptr, vptr.data(), ((size_t)V + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    DSV_HIP(hipMemcpy(p.bslot, bslot.data(), ne * sizeof(ushort4), hipMemcpyHostToDevice));
     p.bptr = dalloc<int32_t>(nb + 1);
-    p.bconn = dalloc<int4>(ne);
+    p.bloc = dalloc<ushort4>(ne);
     p.belem = dalloc<int32_t>(ne);
+    p.hptr = dalloc<int32_t>(nb + 1);
+    p.hlist = dalloc<int32_t>(std::max<size_t>(1, hlist.size()));
     DSV_HIP(hipMemcpy(p.bptr, bptr.data(), (nb + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    DSV_HIP(hipMemcpy(p.bconn, bconn.data(), ne * sizeof(int4), hipMemcpyHostToDevice));
+    DSV_HIP(hipMemcpy(p.bloc, bloc.data(), ne * sizeof(ushort4), hipMemcpyHostToDevice));
     DSV_HIP(hipMemcpy(p.belem, belem.data(), ne * sizeof(int32_t), hipMemcpyHostToDevice));
+    DSV_HIP(hipMemcpy(p.hptr, hptr.data(), (nb + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!hlist.empty())
+        DSV_HIP(hipMemcpy(p.hlist, hlist.data(), hlist.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     free_partition(m->part);
     m->part = p;
 }
@@ -370,7 +455,7 @@ static void create_mesh(dsv_mesh** out, int32_t C, int32_t V, const int32_t* con
         hipLaunchKernelGGL(k_soa_internal<double>, dim3(grid_for(V, 256)), dim3(256), 0, 0, V,
                            m->perm, m->xyz, m->xyz64);
         DSV_HIP(hipGetLastError());
-        build_partition(m, NV > 0 ? NV : 512);
+        build_partition(m, NV > 0 ? NV : 128);
         DSV_HIP(hipDeviceSynchronize());
     } catch (...) {
         destroy_mesh(m);
@@ -418,6 +503,8 @@ int dsv_mesh_info(const dsv_mesh* m, int64_t* info) {
         info[3] = m->part.nb;
         info[4] = m->part.NV;
         info[5] = m->part.n_entries;
+        info[6] = m->part.n_halo;
+        info[7] = m->part.max_local;
     });
 }
 

@@ -13,8 +13,8 @@ import ctypes
 from . import _lib
 from .core import species_array, species_struct
 
-DEFAULT_NV = (64, 128, 256, 512, 1024)
-DEFAULT_THREADS = (256, 512, 1024)
+DEFAULT_NV = (64, 128, 256, 512)
+DEFAULT_THREADS = (64, 128, 256, 512)
 
 
 def time_apply(mesh, species, inv_dt, precision=32, mode=1, reps=30, lumped=1, stream=None):
@@ -44,9 +44,16 @@ def tune(mesh, species, inv_dt, nv_list=DEFAULT_NV, threads_list=DEFAULT_THREADS
     for nv in nv_list:
         for th in threads_list:
             mesh.set_partition(nv, th)
-            ms = time_apply(mesh, species, inv_dt, precision=precision, reps=reps)
+            try:
+                # the FP64 residual kernel must fit the same partition (LDS per workgroup)
+                time_apply(mesh, species, inv_dt, precision=64, mode=0, reps=1)
+                ms = time_apply(mesh, species, inv_dt, precision=precision, reps=reps)
+            except ValueError:
+                continue
             table.append((nv, th, mesh.n_entries, ms))
             if best is None or ms < best[3]:
                 best = (nv, th, mesh.n_entries, ms)
+    if best is None:
+        raise RuntimeError("no launch shape fits this device")
     mesh.set_partition(best[0], best[1])
     return best, table

@@ -72,10 +72,11 @@ class Mesh:
         self.refresh_info()
 
     def refresh_info(self):
-        info = (ctypes.c_int64 * 6)()
+        info = (ctypes.c_int64 * 8)()
         _lib.call("dsv_mesh_info", self._h, info)
         self.n_cells, self.n_dofs, self.nnz = int(info[0]), int(info[1]), int(info[2])
         self.n_blocks, self.vertices_per_block, self.n_entries = int(info[3]), int(info[4]), int(info[5])
+        self.n_halo, self.max_local = int(info[6]), int(info[7])
 
     @property
     def handle(self):

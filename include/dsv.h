@@ -78,9 +78,10 @@ int dsv_device_count(int32_t* n);
 int dsv_mesh_create(dsv_mesh** out, int32_t n_cells, int32_t n_verts, const int32_t* conn,
                     const double* xyz, int32_t vertices_per_block);
 int dsv_mesh_destroy(dsv_mesh* m);
-/* sizes: n_cells, n_verts, nnz, number of matrix-free blocks, vertices per block, the
- * number of (block, cell) entries of the partition */
-int dsv_mesh_info(const dsv_mesh* m, int64_t* info6);
+/* sizes (8 values): n_cells, n_verts, nnz, number of matrix-free blocks, vertices per block,
+ * number of (block, cell) entries of the partition, total halo vertices, max local vertices
+ * of a block */
+int dsv_mesh_info(const dsv_mesh* m, int64_t* info8);
 
 /* Device views owned by the mesh (valid until dsv_mesh_destroy): the attributes
  * d_indptr, d_indices, d_scatter_map, d_detJ, d_Gflat, d_JinvT, d_elem_nodes of
