@@ -128,8 +128,9 @@ def main():
         from dsv.autotune import tune
         best, table = tune(mesh, w["species"], 1.0 / w["dt"])
         for row in table:
-            log(f"tune nv={row[0]} threads={row[1]} entries={row[2]} {row[3] * 1e3:.1f} us")
-        tuned = {"vertices_per_block": best[0], "threads": best[1]}
+            log(f"tune nv={row[0]} threads={row[1]} lanes={row[2]} entries={row[3]} "
+                f"{row[4] * 1e3:.1f} us")
+        tuned = {"vertices_per_block": best[0], "threads": best[1], "lanes_per_vertex": best[2]}
         log(f"tuned: {tuned}")
     res = {}
     for method in ("csr64", "mf32_ir64"):

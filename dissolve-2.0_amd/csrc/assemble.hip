@@ -9,15 +9,6 @@
 
 namespace dsv {
 
-// coefficient of cell e for a caller-order coefficient array (length C or V) or a scalar
-__device__ __forceinline__ double cell_coef(const double* p, int mode, double val, int e,
-                                            const int32_t* __restrict__ conn) {
-    if (mode == COEF_SCALAR) return val;
-    if (mode == COEF_CELL) return p[e];
-    const int n0 = conn[4 * e], n1 = conn[4 * e + 1], n2 = conn[4 * e + 2], n3 = conn[4 * e + 3];
-    return (((p[n0] + p[n1]) + p[n2]) + p[n3]) * 0.25;
-}
-
 __global__ void k_elem_A(int32_t C, int32_t V, const int32_t* __restrict__ conn,
                          const double* __restrict__ xyz, const double* __restrict__ D, int Dmode,
                          double Dval, const double* __restrict__ A, int Amode, double Aval,

@@ -65,29 +65,34 @@ void dfree(T*& p) {
 enum CoefMode : int { COEF_SCALAR = 0, COEF_CELL = 1, COEF_VERT = 2 };
 
 // Matrix-free partition: vertices (in internal Morton order) are cut into contiguous blocks
-// of NV; block b owns vertices [b*NV, (b+1)*NV) and lists every cell touching one of them.
-// A block's local vertex numbering: owned vertices first (local = internal - b*NV), then its
-// halo (the other vertices of its cells, ascending internal id).  Cells are stored with
-// local 16-bit indices, so the kernels read vertex data from LDS after staging it once.
+// of NV (a multiple of 64); block b owns vertices [b*NV, (b+1)*NV).  A block's local vertex
+// numbering: owned vertices first (local = internal - b*NV), then its halo (the other
+// vertices of the cells touching an owned vertex, ascending internal id), at most
+// ELL_MAX_LOCAL in all.
+//
+// Vertex-centric incidence lists in sliced ELL: the owned vertices of a block are taken in
+// order of decreasing degree (`lperm`: rank -> owned local index) and each gets L lanes
+// (L = 1, 2 or 4); lane k of a vertex takes its incident cells k, k+L, k+2L, ...  Every 64
+// consecutive lanes (64/L vertices) form a slice whose width is the longest lane list in it.
+// Entry (t, lane) of slice s sits at ell[sptr[s] + 64 t + lane] and packs the local indices
+// of the other three corners of that cell (10 bits each); padding entries are ELL_PAD.
+constexpr int ELL_MAX_LOCAL = 1024;
+constexpr uint32_t ELL_PAD = 0xFFFFFFFFu;
+
 struct Partition {
     int NV = 0;
+    int L = 4;                    // lanes per owned vertex
     int nb = 0;
     int threads = 256;
-    int max_local = 0;          // max over blocks of owned + halo vertices
-    int max_entries = 0;        // max over blocks of listed cells
-    int max_inc = 0;            // max over blocks of owned-vertex incidences
-    int64_t n_entries = 0;
+    int max_local = 0;            // max over blocks of owned + halo vertices
+    int64_t n_entries = 0;        // ELL entries including padding
     int64_t n_halo = 0;
-    int32_t* bptr = nullptr;    // nb+1 entry ranges
-    ushort4* bloc = nullptr;    // entry -> 4 local vertex indices
-    int32_t* belem = nullptr;   // entry -> caller cell id
-    int32_t* hptr = nullptr;    // nb+1 halo ranges
-    int32_t* hlist = nullptr;   // halo vertices (internal ids)
-    // Owned-corner slots: the 4C (cell, owned corner) pairs are numbered per block, grouped
-    // by owning vertex (vertex v's slots are [vptr[v], vptr[v+1]) minus the block's first
-    // slot).  bslot gives, per entry, the block-local slot of each corner (0xFFFF: halo).
-    int32_t* vptr = nullptr;    // V+1
-    ushort4* bslot = nullptr;   // per entry
+    int32_t* hptr = nullptr;      // nb+1 halo ranges
+    int32_t* hlist = nullptr;     // halo vertices (internal ids)
+    uint16_t* lperm = nullptr;    // nb*NV: degree rank -> owned local index
+    int32_t* sptr = nullptr;      // nb*(NV*L/64)+1 slice offsets (multiples of 64)
+    uint32_t* ell = nullptr;      // packed corner triples
+    int32_t* ell_cell = nullptr;  // caller cell id per entry, -1 for padding
 };
 
 struct Workspace;  // solver buffers, defined in solve.hip
@@ -115,9 +120,9 @@ struct dsv_mesh {
     // internal (Morton) numbering for the matrix-free path
     int32_t* perm = nullptr;      // internal -> caller
     int32_t* iperm = nullptr;     // caller -> internal
-    float* xyz32 = nullptr;       // 3V SoA internal
-    double* xyz64 = nullptr;      // 3V SoA internal
-    dsv::Partition part;
+    float* xyz32 = nullptr;       // 4V AoS (x, y, z, 0) internal
+    double* xyz64 = nullptr;      // 4V AoS (x, y, z, 0) internal
+    dsv::Partition part;          // matrix-free kernels (autotuned)
     std::vector<int32_t> h_conn;  // caller numbering
     std::vector<int32_t> h_iperm;
     // solver options
@@ -129,7 +134,7 @@ struct dsv_mesh {
 };
 
 namespace dsv {
-void build_partition(dsv_mesh* m, int NV);
+void build_partition(dsv_mesh* m, int NV, int L);
 void free_partition(Partition& p);
 void free_workspaces(dsv_mesh* m);
 void ensure_geometry_views(dsv_mesh* m);

@@ -108,13 +108,15 @@ __global__ void k_lumped_mass(int32_t V, const int32_t* __restrict__ diag_pos,
     m[r] = s;
 }
 
+// internal-order AoS coordinates (x, y, z, 0): one 16/32-byte load per vertex
 template <class T>
-__global__ void k_soa_internal(int32_t V, const int32_t* __restrict__ perm,
-                               const double* __restrict__ xyz, T* __restrict__ out) {
+__global__ void k_aos4_internal(int32_t V, const int32_t* __restrict__ perm,
+                                const double* __restrict__ xyz, T* __restrict__ out) {
     int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     int32_t c = perm[v];
-    for (int d = 0; d < 3; ++d) out[(int64_t)d * V + v] = (T)xyz[(int64_t)d * V + c];
+    for (int d = 0; d < 3; ++d) out[4 * (int64_t)v + d] = (T)xyz[(int64_t)d * V + c];
+    out[4 * (int64_t)v + 3] = T(0);
 }
 
 static inline unsigned grid_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -154,121 +156,113 @@ static void morton_order(const double* xyz_aos, int32_t V, std::vector<int32_t>&
 }
 
 void free_partition(Partition& p) {
-    dfree(p.bptr);
-    dfree(p.bloc);
-    dfree(p.belem);
     dfree(p.hptr);
     dfree(p.hlist);
-    dfree(p.vptr);
-    dfree(p.bslot);
+    dfree(p.lperm);
+    dfree(p.sptr);
+    dfree(p.ell);
+    dfree(p.ell_cell);
     p.nb = 0;
     p.n_entries = 0;
     p.n_halo = 0;
     p.max_local = 0;
 }
 
-// Block b owns internal vertices [b*NV, min((b+1)*NV, V)); its entry list holds every cell
-// with at least one owned vertex, in ascending caller cell order, with the cell's vertices
-// renumbered locally (owned first, then the sorted halo).
-void build_partition(dsv_mesh* m, int NV) {
-    if (NV < 32 || NV > 2048) throw ArgError{"vertices_per_block must be in [32, 2048]"};
+template <class T>
+static T* upload(const std::vector<T>& h) {
+    T* d = dalloc<T>(std::max<size_t>(1, h.size()));
+    if (!h.empty()) DSV_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+// Block b owns internal vertices [b*NV, min((b+1)*NV, V)).  Every owned vertex gets the
+// cells incident to it (ascending caller cell id) with the other three corners renumbered
+// locally (owned first, then the sorted halo), dealt round-robin to its L lanes; vertices are
+// ranked by decreasing degree so that the lanes of a slice have similar list lengths.
+void build_partition(dsv_mesh* m, int NV, int L) {
+    if (NV < 64 || NV > 1024 || NV % 64 != 0)
+        throw ArgError{"vertices_per_block must be a multiple of 64 in [64, 1024]"};
+    if (L != 1 && L != 2 && L != 4) throw ArgError{"lanes_per_vertex must be 1, 2 or 4"};
     const int32_t C = m->C, V = m->V;
     const int nb = (V + NV - 1) / NV;
-    std::vector<int64_t> cnt(nb + 1, 0);
+    const int vps = 64 / L;        // vertices per slice
+    const int spb = NV / vps;      // slices per block
     const int32_t* cn = m->h_conn.data();
     const int32_t* ip = m->h_iperm.data();
-    auto blocks_of = [&](int64_t e, int* bl) {
-        int n = 0;
-        for (int k = 0; k < 4; ++k) {
-            int b = ip[cn[4 * e + k]] / NV;
-            bool seen = false;
-            for (int q = 0; q < n; ++q) seen |= (bl[q] == b);
-            if (!seen) bl[n++] = b;
-        }
-        return n;
-    };
-    for (int64_t e = 0; e < C; ++e) {
-        int bl[4];
-        int n = blocks_of(e, bl);
-        for (int q = 0; q < n; ++q) cnt[bl[q] + 1]++;
+    // vertex -> incident cells, internal vertex order, ascending caller cell id
+    std::vector<int32_t> vinc((size_t)V + 1, 0);
+    for (int64_t k = 0; k < 4 * (int64_t)C; ++k) vinc[ip[cn[k]] + 1]++;
+    for (int32_t v = 0; v < V; ++v) vinc[v + 1] += vinc[v];
+    std::vector<int32_t> inc_cell(4 * (size_t)C);
+    {
+        std::vector<int32_t> fill(vinc.begin(), vinc.end() - 1);
+        for (int64_t e = 0; e < C; ++e)
+            for (int k = 0; k < 4; ++k) inc_cell[fill[ip[cn[4 * e + k]]]++] = (int32_t)e;
     }
-    for (int b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
-    const int64_t ne = cnt[nb];
-    if (ne >= (int64_t)INT32_MAX) throw ArgError{"partition too large for int32 entry indices"};
-    std::vector<int32_t> bptr(nb + 1);
-    for (int b = 0; b <= nb; ++b) bptr[b] = (int32_t)cnt[b];
-    std::vector<int4> gconn(ne);
-    std::vector<int32_t> belem(ne);
-    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-    for (int64_t e = 0; e < C; ++e) {
-        int bl[4];
-        int n = blocks_of(e, bl);
-        int4 c4 = make_int4(ip[cn[4 * e]], ip[cn[4 * e + 1]], ip[cn[4 * e + 2]], ip[cn[4 * e + 3]]);
-        for (int q = 0; q < n; ++q) {
-            int64_t k = fill[bl[q]]++;
-            gconn[k] = c4;
-            belem[k] = (int32_t)e;
-        }
-    }
-    // halo lists and local numbering
-    std::vector<int32_t> hptr(nb + 1, 0);
-    std::vector<int32_t> vptr((size_t)V + 1, 0);
-    std::vector<ushort4> bslot(ne);
-    std::vector<int32_t> vcount;
-    int max_entries = 0, max_inc = 0;
-    std::vector<int32_t> hlist;
-    std::vector<ushort4> bloc(ne);
-    std::vector<int32_t> halo;
+    std::vector<int32_t> hptr(nb + 1, 0), hlist, halo;
+    std::vector<uint16_t> lperm((size_t)nb * NV, 0);
+    std::vector<int32_t> sptr((size_t)nb * spb + 1, 0);
+    std::vector<uint32_t> ell;
+    std::vector<int32_t> ell_cell;
+    std::vector<int> order;
     int max_local = 0;
     for (int b = 0; b < nb; ++b) {
         const int v0 = b * NV, nv = std::min(NV, V - v0);
         halo.clear();
-        for (int64_t k = bptr[b]; k < bptr[b + 1]; ++k) {
-            const int32_t g4[4] = {gconn[k].x, gconn[k].y, gconn[k].z, gconn[k].w};
-            for (int j = 0; j < 4; ++j)
-                if (g4[j] < v0 || g4[j] >= v0 + nv) halo.push_back(g4[j]);
-        }
+        for (int i = 0; i < nv; ++i)
+            for (int32_t j = vinc[v0 + i]; j < vinc[v0 + i + 1]; ++j)
+                for (int k = 0; k < 4; ++k) {
+                    const int32_t g = ip[cn[4 * (int64_t)inc_cell[j] + k]];
+                    if (g < v0 || g >= v0 + nv) halo.push_back(g);
+                }
         std::sort(halo.begin(), halo.end());
         halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
         const int nloc = nv + (int)halo.size();
-        if (nloc > 65535) throw ArgError{"partition block has more than 65535 local vertices"};
+        if (nloc > ELL_MAX_LOCAL)
+            throw ArgError{"partition block has " + std::to_string(nloc) + " local vertices (max " +
+                           std::to_string(ELL_MAX_LOCAL) + "); use fewer vertices_per_block"};
         max_local = std::max(max_local, nloc);
-        for (int64_t k = bptr[b]; k < bptr[b + 1]; ++k) {
-            const int32_t g4[4] = {gconn[k].x, gconn[k].y, gconn[k].z, gconn[k].w};
-            unsigned short l4[4];
-            for (int j = 0; j < 4; ++j) {
-                const int32_t g = g4[j];
-                if (g >= v0 && g < v0 + nv)
-                    l4[j] = (unsigned short)(g - v0);
-                else
-                    l4[j] = (unsigned short)(nv + (std::lower_bound(halo.begin(), halo.end(), g) -
-                                                   halo.begin()));
+        auto local = [&](int32_t g) -> uint32_t {
+            if (g >= v0 && g < v0 + nv) return (uint32_t)(g - v0);
+            return (uint32_t)(nv + (std::lower_bound(halo.begin(), halo.end(), g) - halo.begin()));
+        };
+        order.resize(nv);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+            return vinc[v0 + a + 1] - vinc[v0 + a] > vinc[v0 + c + 1] - vinc[v0 + c];
+        });
+        for (int r = 0; r < nv; ++r) lperm[(size_t)b * NV + r] = (uint16_t)order[r];
+        for (int s = 0; s < spb; ++s) {
+            const size_t sidx = (size_t)b * spb + s;
+            const int r0 = vps * s, r1 = std::min(nv, vps * s + vps);  // degree ranks
+            int width = 0;
+            for (int r = r0; r < r1; ++r) {
+                const int v = v0 + order[r];
+                width = std::max(width, (vinc[v + 1] - vinc[v] + L - 1) / L);
             }
-            bloc[k] = make_ushort4(l4[0], l4[1], l4[2], l4[3]);
-        }
-        // owned-vertex incidence: every corner of every cell is owned by exactly one block
-        const int64_t nbe = bptr[b + 1] - bptr[b];
-        if (nbe > 65535) throw ArgError{"partition block lists more than 65535 cells"};
-        max_entries = std::max(max_entries, (int)nbe);
-        vcount.assign(nv + 1, 0);
-        for (int64_t e = 0; e < nbe; ++e) {
-            const ushort4 l = bloc[bptr[b] + e];
-            const unsigned short l4[4] = {l.x, l.y, l.z, l.w};
-            for (int j = 0; j < 4; ++j)
-                if (l4[j] < nv) vcount[l4[j] + 1]++;
-        }
-        for (int i = 0; i < nv; ++i) vcount[i + 1] += vcount[i];
-        const int32_t base = vptr[v0];
-        for (int i = 0; i <= nv; ++i) vptr[v0 + i] = base + vcount[i];
-        max_inc = std::max(max_inc, vcount[nv]);
-        if (vcount[nv] > 65535) throw ArgError{"partition block has more than 65535 owned corners"};
-        for (int64_t e = 0; e < nbe; ++e) {
-            const ushort4 l = bloc[bptr[b] + e];
-            const unsigned short l4[4] = {l.x, l.y, l.z, l.w};
-            unsigned short s4[4];
-            for (int j = 0; j < 4; ++j)
-                s4[j] = l4[j] < nv ? (unsigned short)(vcount[l4[j]]++) : (unsigned short)0xFFFF;
-            bslot[bptr[b] + e] = make_ushort4(s4[0], s4[1], s4[2], s4[3]);
+            const size_t base = ell.size();
+            if (base + 64 * (size_t)width >= (size_t)INT32_MAX)
+                throw ArgError{"matrix-free incidence lists too large for int32 offsets"};
+            sptr[sidx + 1] = (int32_t)(base + 64 * (size_t)width);
+            ell.resize(base + 64 * (size_t)width, ELL_PAD);
+            ell_cell.resize(base + 64 * (size_t)width, -1);
+            for (int r = r0; r < r1; ++r) {
+                const int v = v0 + order[r];
+                for (int32_t i = 0; i < vinc[v + 1] - vinc[v]; ++i) {
+                    const int32_t e = inc_cell[vinc[v] + i];
+                    uint32_t pk = 0;
+                    int f = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        const int32_t g = ip[cn[4 * (int64_t)e + k]];
+                        if (g == v) continue;
+                        pk |= local(g) << (10 * f++);
+                    }
+                    const int lane = (r - r0) * L + i % L;
+                    const size_t at = base + 64 * (size_t)(i / L) + lane;
+                    ell[at] = pk;
+                    ell_cell[at] = e;
+                }
+            }
         }
         hlist.insert(hlist.end(), halo.begin(), halo.end());
         if ((int64_t)hlist.size() >= (int64_t)INT32_MAX) throw ArgError{"halo lists too large"};
@@ -276,29 +270,23 @@ void build_partition(dsv_mesh* m, int NV) {
     }
     Partition p;
     p.NV = NV;
+    p.L = L;
     p.nb = nb;
     p.threads = m->part.threads ? m->part.threads : 256;
-    p.n_entries = ne;
+    p.n_entries = (int64_t)ell.size();
     p.n_halo = (int64_t)hlist.size();
     p.max_local = max_local;
-    p.max_entries = max_entries;
-    p.max_inc = max_inc;
-    if ((int64_t)vptr[V] != 4 * (int64_t)C) throw ArgError{"internal error: incidence count"};
-    p.vptr = dalloc<int32_t>((size_t)V + 1);
-    p.bslot = dalloc<ushort4>(ne);
-    DSV_HIP(hipMemcpy(p.vptr, vptr.data(), ((size_t)V + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    DSV_HIP(hipMemcpy(p.bslot, bslot.data(), ne * sizeof(ushort4), hipMemcpyHostToDevice));
-    p.bptr = dalloc<int32_t>(nb + 1);
-    p.bloc = dalloc<ushort4>(ne);
-    p.belem = dalloc<int32_t>(ne);
-    p.hptr = dalloc<int32_t>(nb + 1);
-    p.hlist = dalloc<int32_t>(std::max<size_t>(1, hlist.size()));
-    DSV_HIP(hipMemcpy(p.bptr, bptr.data(), (nb + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    DSV_HIP(hipMemcpy(p.bloc, bloc.data(), ne * sizeof(ushort4), hipMemcpyHostToDevice));
-    DSV_HIP(hipMemcpy(p.belem, belem.data(), ne * sizeof(int32_t), hipMemcpyHostToDevice));
-    DSV_HIP(hipMemcpy(p.hptr, hptr.data(), (nb + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-    if (!hlist.empty())
-        DSV_HIP(hipMemcpy(p.hlist, hlist.data(), hlist.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    try {
+        p.hptr = upload(hptr);
+        p.hlist = upload(hlist);
+        p.lperm = upload(lperm);
+        p.sptr = upload(sptr);
+        p.ell = upload(ell);
+        p.ell_cell = upload(ell_cell);
+    } catch (...) {
+        free_partition(p);
+        throw;
+    }
     free_partition(m->part);
     m->part = p;
 }
@@ -448,14 +436,26 @@ static void create_mesh(dsv_mesh** out, int32_t C, int32_t V, const int32_t* con
         m->iperm = dalloc<int32_t>(V);
         DSV_HIP(hipMemcpy(m->perm, perm.data(), V * sizeof(int32_t), hipMemcpyHostToDevice));
         DSV_HIP(hipMemcpy(m->iperm, m->h_iperm.data(), V * sizeof(int32_t), hipMemcpyHostToDevice));
-        m->xyz32 = dalloc<float>(3 * (size_t)V);
-        m->xyz64 = dalloc<double>(3 * (size_t)V);
-        hipLaunchKernelGGL(k_soa_internal<float>, dim3(grid_for(V, 256)), dim3(256), 0, 0, V,
+        m->xyz32 = dalloc<float>(4 * (size_t)V);
+        m->xyz64 = dalloc<double>(4 * (size_t)V);
+        hipLaunchKernelGGL(k_aos4_internal<float>, dim3(grid_for(V, 256)), dim3(256), 0, 0, V,
                            m->perm, m->xyz, m->xyz32);
-        hipLaunchKernelGGL(k_soa_internal<double>, dim3(grid_for(V, 256)), dim3(256), 0, 0, V,
+        hipLaunchKernelGGL(k_aos4_internal<double>, dim3(grid_for(V, 256)), dim3(256), 0, 0, V,
                            m->perm, m->xyz, m->xyz64);
         DSV_HIP(hipGetLastError());
-        build_partition(m, NV > 0 ? NV : 128);
+        if (NV > 0) {
+            build_partition(m, NV, 4);
+        } else {
+            // default: the largest block size whose blocks fit the 10-bit local indices
+            for (int nv = 256;; nv /= 2) {
+                try {
+                    build_partition(m, nv, 4);
+                    break;
+                } catch (const ArgError&) {
+                    if (nv == 64) throw;
+                }
+            }
+        }
         DSV_HIP(hipDeviceSynchronize());
     } catch (...) {
         destroy_mesh(m);
@@ -547,15 +547,17 @@ int dsv_mesh_copy_view(const dsv_mesh* mc, int32_t which, void* dst, void* strea
     });
 }
 
-int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t block_threads) {
+int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t block_threads,
+                           int32_t lanes_per_vertex) {
     return guard([&] {
         if (!m) throw ArgError{"dsv_mesh_set_partition: NULL mesh"};
         if (block_threads != 64 && block_threads != 128 && block_threads != 256 &&
             block_threads != 512 && block_threads != 1024)
             throw ArgError{"block_threads must be 64, 128, 256, 512 or 1024"};
         free_workspaces(m);  // workspaces and captured graphs depend on the launch shape
+        if (vertices_per_block != m->part.NV || lanes_per_vertex != m->part.L)
+            build_partition(m, vertices_per_block, lanes_per_vertex);
         m->part.threads = block_threads;
-        if (vertices_per_block != m->part.NV) build_partition(m, vertices_per_block);
     });
 }
 

@@ -6,22 +6,26 @@
 // (`numerics/solvers.py:322-372`, GMRES(50)+bjacobi on a CSR that is copied through the host
 // every step).
 //
-// Matrix-free layout (internal Morton vertex numbering):
-//   partition block b owns vertices [b*NV, b*NV+NV) and lists every cell touching them; a
-//   workgroup recomputes each listed cell's Jacobian from SoA coordinates, applies the 4x4
-//   cell operator of all species at once and accumulates the owned rows in LDS (no global
-//   atomics, no colouring); owned rows are then written with coalesced stores.  The 8 XCDs
-//   each get a contiguous (spatially compact) run of partition blocks, which keeps the halo
-//   vertex gathers in the XCD's own L2.
+// Data layout: every solver vector is species-interleaved -- one record SV<T,S> per vertex
+// holding the S species (padded to 4 when S = 3) -- so one wide load fetches every species of
+// a vertex.  Coordinates are (x, y, z, 0) records.
 //
-// CG: per iteration two kernels -- (A) the apply: p = z + beta p_old computed on the fly for
-// gathered vertices, p and q = A p written for owned vertices, block partials of p.q;
-// (U) the vertex update: x += alpha p, r -= alpha q, z = r/diag, block partials of r.z and
-// r.r.  Scalars never go through the host and no kernel uses a fence or a global atomic:
-// every workgroup of a consumer kernel reduces ALL partials of the producer kernel in the
-// same fixed order, so all workgroups derive identical alpha / beta / convergence decisions;
-// workgroup 0 records them in the state block for the host.  Chunks of iterations are
-// captured in a hipGraph and replayed until every species has converged.
+// Matrix-free apply (internal Morton vertex numbering), one workgroup per partition block:
+//   (1) stage the block's local vertices (owned range, then halo) in LDS: coordinates, the
+//       species input (x, or p = z + beta p_old in CG form) and per-vertex coefficients;
+//   (2) one lane per owned vertex walks the vertex's incident cells (sliced-ELL list of the
+//       other three corners' local indices), recomputes each cell's geometry from LDS and
+//       accumulates the vertex's row of the cell operator for every species in registers --
+//       no atomics, no second barrier, a fixed summation order -- and writes the row
+//       (identity rows at Dirichlet vertices).  The 8 XCDs each take a contiguous run of
+//       partition blocks, which keeps halo reads in the XCD's own L2.
+//
+// CG: per iteration two kernels -- (A) the apply in CG form, writing p and q = A p and block
+// partials of p.q; (U) the vertex update x += alpha p, r -= alpha q, z = r/diag with partials
+// of r.z and r.r.  No kernel uses a fence or a global atomic: every workgroup of a consumer
+// kernel reduces ALL partials of the producer kernel in the same fixed order, so all derive
+// identical alpha / beta / convergence decisions; workgroup 0 records them for the host.
+// Chunks of iterations are captured in a hipGraph and replayed until every species is done.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -38,9 +42,9 @@ void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped,
                    hipStream_t st);
 
 constexpr int MAXS = 3;
-constexpr int MAX_PROD = 1024;  // max workgroups of a partial-producing kernel
+constexpr int MAX_PROD = 2048;  // max workgroups of a partial-producing kernel
 
-enum MFMode { MF_APPLY = 0, MF_DIAG = 1, MF_CG = 2, MF_RESID = 3 };
+enum MFMode { MF_APPLY = 0, MF_DIAG = 1, MF_CG = 2, MF_RESID = 3, MF_MASS = 4 };
 
 // status codes in CGState::status
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXIT = 3, ST_IDLE = 4 };
@@ -60,77 +64,92 @@ struct CGState {
     int pad;
 };
 
+// species-interleaved vertex record (S = 3 padded to 4 for aligned wide accesses)
+template <class T, int S>
+struct SVs;
 template <class T>
-struct SpeciesP {
-    const T* D;  // COEF_CELL: partition-entry order; COEF_VERT: vertex order of the kernel
-    const T* A;
-    const uint8_t* bc;  // per-vertex Dirichlet mask (vertex order of the kernel) or null
-    T Dval, Aval;
-    int Dmode, Amode;
-    int Dslot, Aslot;   // LDS slot of a per-vertex coefficient staged by the MF kernel
+struct alignas(sizeof(T)) SVs<T, 1> {
+    T v[1];
 };
-
 template <class T>
-struct OpParams {
-    SpeciesP<T> sp[MAXS];
-    T inv_dt;
-    int lumped;
-    int n_vslots;  // per-vertex coefficient arrays staged in LDS
+struct alignas(2 * sizeof(T)) SVs<T, 2> {
+    T v[2];
 };
+template <class T>
+struct alignas(4 * sizeof(T)) SVs<T, 3> {
+    T v[4];
+};
+template <class T, int S>
+using SV = SVs<T, S>;
 
 template <class T>
 struct alignas(4 * sizeof(T)) Vec4 {
     T x, y, z, w;
 };
-template <class T>
-__device__ __forceinline__ T comp(const Vec4<T>& a, int i) {
-    return i == 0 ? a.x : (i == 1 ? a.y : (i == 2 ? a.z : a.w));
-}
 
-// vectors of the kernels, per species (vertex order of the kernel)
+// Operator coefficients of one species.  Generic form: the cell's diffusivity D_e and
+// reaction coefficient alpha_e per ELL entry (cell-averaged as in assembly).  Fast form
+// (lumped mass and a scalar D for every species): D/6 as a scalar, the whole mass term folded
+// into the per-vertex diagonal `mdiag`.
 template <class T>
-struct Vecs {
-    T* x[MAXS];             // APPLY/RESID input x ; CG: z ; UPDATE: z out
-    T* y[MAXS];             // APPLY output y ; DIAG output dinv ; CG: q ; RESID: r
-    T* pold[MAXS];          // CG
-    T* pnew[MAXS];          // CG
-    const double* b[MAXS];  // RESID right-hand side (FP64)
+struct SpeciesP {
+    const T* D;
+    const T* A;
+    T D6;
 };
 
-// vectors of the update kernel
 template <class T>
-struct UVecs {
-    T* x[MAXS];      // correction d
-    T* r[MAXS];
-    T* z[MAXS];
-    const T* dinv[MAXS];
-    const T* p[MAXS];     // p_k (written by A_k)
-    const T* q[MAXS];
-    T* pzero[MAXS];       // INIT: p_old buffer to clear
-    const double* rin[MAXS];  // INIT: FP64 residual
+struct OpParams {
+    SpeciesP<T> sp[MAXS];
+    const uint8_t* bcbits;  // per vertex: bit s set at the Dirichlet vertices of species s
+    const void* mdiag;      // fast form: sum over cells of (inv_dt + alpha_e) |det J_e| / 24
+    T inv_dt;
+    int lumped;
+    int fast;
+};
+
+// vectors of the apply-type kernels (vertex order of the kernel)
+template <class T, int S>
+struct MFVec {
+    const SV<T, S>* x;       // APPLY / RESID: x ; CG: z
+    SV<T, S>* y;             // APPLY: y ; DIAG: dinv ; CG: q ; RESID: r ; MASS: mass diagonal
+    const SV<T, S>* pold;    // CG
+    SV<T, S>* pnew;          // CG
+    const SV<double, S>* b;  // RESID
+};
+
+template <class T>
+__device__ __forceinline__ T recip(T x);
+template <>
+__device__ __forceinline__ float recip<float>(float x) {
+    return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp: the FP32 inner operator only
+}
+template <>
+__device__ __forceinline__ double recip<double>(double x) {
+    return 1.0 / x;
+}
+
+// vectors of the update kernel
+template <class T, int S>
+struct UVec {
+    SV<T, S>* x;  // correction d
+    SV<T, S>* r;
+    SV<T, S>* z;
+    const SV<T, S>* dinv;
+    const SV<T, S>* p;         // p_k (written by A_k)
+    const SV<T, S>* q;
+    SV<T, S>* pzero;           // INIT: p_old of A_0, cleared
+    const SV<double, S>* rin;  // INIT: FP64 residual
 };
 
 struct Geo {
-    int V, NV, nb, per_xcd, max_local, max_entries, max_inc;
-    const int32_t* bptr;
-    const ushort4* bloc;
+    int V, NV, L, nb, per_xcd, max_local;
     const int32_t* hptr;
     const int32_t* hlist;
-    const int32_t* vptr;
-    const ushort4* bslot;
+    const uint16_t* lperm;
+    const int32_t* sptr;
+    const uint32_t* ell;
 };
-
-// coefficient of partition entry k: scalar, per cell (entry order) or the mean of the four
-// vertex values staged in LDS slot `slot`
-template <class T>
-__device__ __forceinline__ T coef_at(const T* __restrict__ p, int mode, T val, int k,
-                                     const T* __restrict__ scv, int slot, int ml,
-                                     const ushort4& l) {
-    if (mode == COEF_SCALAR) return val;
-    if (mode == COEF_CELL) return p[k];
-    const T* c = scv + slot * ml;
-    return (((c[l.x] + c[l.y]) + c[l.z]) + c[l.w]) * T(0.25);
-}
 
 // Block sums of N values, result in EVERY thread (fixed order: lanes, then waves).
 template <int N>
@@ -167,76 +186,89 @@ __device__ __forceinline__ void consume_partials(const double* __restrict__ part
     block_sum_all<NQ>(out, scratch);
 }
 
+// Preamble of A_k shared by the matrix-free and CSR kernels: reduce U_{k-1}'s partials and
+// decide which species continue and their beta.  Returns false when no species is active.
+template <int S>
+__device__ __forceinline__ bool cg_begin(CGState* __restrict__ st, const double* __restrict__ pin,
+                                         int nin, int par, bool (&act)[S], double (&beta)[S],
+                                         double* red) {
+    bool prev[S];
+    bool anyprev = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        prev[s] = st->act[par ^ 1][s] != 0;
+        anyprev |= prev[s];
+    }
+    if (!anyprev) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            for (int s = 0; s < S; ++s) st->act[par][s] = 0;
+            st->any_active = 0;
+        }
+        return false;
+    }
+    double sums[2 * S];
+    consume_partials<2 * S>(pin, nin, sums, red);  // U_{k-1}: r.z, r.r per species
+    const int k = st->k;
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double rz = sums[s], rr = sums[S + s];
+        act[s] = prev[s] && st->dead[s] == 0 && (rr > st->tol2[s]) && (k < st->max_it) &&
+                 isfinite(rr);
+        beta[s] = (k == 0 || !act[s]) ? 0.0 : rz / st->rz[par ^ 1][s];
+        any |= act[s];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int s = 0; s < S; ++s) {
+            st->act[par][s] = act[s] ? 1 : 0;
+            if (!prev[s]) continue;
+            st->rz[par][s] = sums[s];
+            st->rr[s] = sums[S + s];
+            if (!act[s] && st->dead[s] == 0)
+                st->status[s] = (sums[S + s] <= st->tol2[s]) ? ST_CONVERGED : ST_MAXIT;
+        }
+        st->any_active = any ? 1 : 0;
+    }
+    return any;
+}
+
 // ---------------------------------------------------------------------------------------
-// matrix-free kernel, all modes.  CG mode = A_k with parity `par` = k & 1.
+// matrix-free kernel.  MODE: APPLY y = A x; DIAG y = 1 / diag(A); CG = A_k of the CG loop
+// with parity `par` = k & 1; RESID r = b - A x (FP64 partials of r.r and b.b); MASS (generic
+// form only) y = lumped (inv_dt + alpha) mass diagonal, the `mdiag` of the fast form.
 //
-// Per partition block: (1) stage the block's local vertices (owned range, then halo) into
-// LDS -- coordinates, the species input (x, or p = z + beta p_old in CG mode) and any
-// per-vertex coefficient; (2) every lane takes cells of the block's entry list, reads its
-// four vertices from LDS, recomputes the Jacobian, applies the 4x4 operator of all species
-// and adds the rows of owned vertices into LDS accumulators; (3) owned rows are written
-// out with coalesced stores (identity rows at Dirichlet vertices).
-template <class T, int S, int MODE>
+// Row of owned vertex 0 of a cell with edges e_k = x_k - x_0 (k = 1..3), c1 = e2 x e3,
+// c2 = e3 x e1, c3 = e1 x e2, |det| = |e1 . c1|, cs = c1 + c2 + c3: the stiffness row is
+// K_0k = D c0.c_k / (6 |det|) with c0 = -cs, so (K u)_0 = -(D/6) sum_k g_k (u_k - u_0) with
+// g_k = cs.c_k / |det|; the diagonal is (D/6) cs.cs / |det|.  Lumped mass |det|/24 on the
+// diagonal, consistent mass |det| (1 + delta_0k) / 120.
+template <class T, int S, int MODE, bool FAST>
 __global__ void __launch_bounds__(1024)
-k_mf(Geo g, const T* __restrict__ X, const T* __restrict__ Y, const T* __restrict__ Z,
-     OpParams<T> prm, Vecs<T> v, CGState* __restrict__ st, const double* __restrict__ pin,
-     int nin, double* __restrict__ pout, int par) {
+k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
+     CGState* __restrict__ st, const double* __restrict__ pin, int nin,
+     double* __restrict__ pout, int par) {
+    static_assert(!(FAST && MODE == MF_MASS), "the mass diagonal is built by the generic form");
     extern __shared__ __align__(32) unsigned char smem_raw[];
     const int ml = g.max_local;
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
-    Vec4<T>* spv = sgeo + ml;
-    Vec4<T>* ys = spv + ml;                               // owned-corner rows
-    T* scv = reinterpret_cast<T*>(ys + g.max_inc);        // per-vertex coefficients
+    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + ml);  // species input per local vertex
     __shared__ double red[16 * 2 * MAXS];
     const int nwg = gridDim.x;
+    constexpr bool HAS_X = MODE != MF_DIAG && MODE != MF_MASS;
 
     bool act[S];
-    T beta[S];
+    double betad[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         act[s] = true;
-        beta[s] = T(0);
+        betad[s] = 0.0;
     }
     if constexpr (MODE == MF_CG) {
-        bool prev[S];
-        bool anyprev = false;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            prev[s] = st->act[par ^ 1][s] != 0;
-            anyprev |= prev[s];
-        }
-        if (!anyprev) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                for (int s = 0; s < S; ++s) st->act[par][s] = 0;
-                st->any_active = 0;
-            }
-            return;
-        }
-        double sums[2 * S];
-        consume_partials<2 * S>(pin, nin, sums, red);  // U_{k-1}: r.z, r.r per species
-        const int k = st->k;
-        bool any = false;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const double rz = sums[s], rr = sums[S + s];
-            const bool dead = st->dead[s] != 0;
-            act[s] = prev[s] && !dead && (rr > st->tol2[s]) && (k < st->max_it) && isfinite(rr);
-            beta[s] = (k == 0 || !act[s]) ? T(0) : (T)(rz / st->rz[par ^ 1][s]);
-            any |= act[s];
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            for (int s = 0; s < S; ++s) {
-                st->act[par][s] = act[s] ? 1 : 0;
-                if (!prev[s]) continue;
-                st->rz[par][s] = sums[s];
-                st->rr[s] = sums[S + s];
-                if (!act[s] && st->dead[s] == 0)
-                    st->status[s] = (sums[S + s] <= st->tol2[s]) ? ST_CONVERGED : ST_MAXIT;
-            }
-            st->any_active = any ? 1 : 0;
-        }
-        if (!any) return;
+        if (!cg_begin<S>(st, pin, nin, par, act, betad, red)) return;
     }
+    T beta[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) beta[s] = (T)betad[s];
 
     double part[S];
     double part2[S];
@@ -245,6 +277,7 @@ k_mf(Geo g, const T* __restrict__ X, const T* __restrict__ Y, const T* __restric
 
     const T inv_dt = prm.inv_dt;
     const int lumped = prm.lumped;
+    const SV<T, S>* __restrict__ mdiag = reinterpret_cast<const SV<T, S>*>(prm.mdiag);
     // XCD x (= workgroup id mod 8) owns partition blocks [x*per_xcd, (x+1)*per_xcd)
     const int xcd = blockIdx.x & 7;
     const int wl = blockIdx.x >> 3;
@@ -259,138 +292,151 @@ k_mf(Geo g, const T* __restrict__ X, const T* __restrict__ Y, const T* __restric
         // (1) stage local vertices
         for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
             const int gv = i < nv ? v0 + i : g.hlist[h0 + i - nv];
-            sgeo[i] = Vec4<T>{X[gv], Y[gv], Z[gv], T(0)};
-            if constexpr (MODE != MF_DIAG) {
-                T pv[4] = {T(0), T(0), T(0), T(0)};
+            sgeo[i] = xyz[gv];
+            if constexpr (MODE == MF_CG) {
+                const SV<T, S> z = v.x[gv];
+                const SV<T, S> po = v.pold[gv];
+                SV<T, S> p{};
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    if (!act[s]) continue;
-                    if constexpr (MODE == MF_CG)
-                        pv[s] = v.x[s][gv] + beta[s] * v.pold[s][gv];
-                    else
-                        pv[s] = v.x[s][gv];
-                }
-                spv[i] = Vec4<T>{pv[0], pv[1], pv[2], pv[3]};
-            }
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (prm.sp[s].Dslot >= 0) scv[prm.sp[s].Dslot * ml + i] = prm.sp[s].D[gv];
-                if (prm.sp[s].Aslot >= 0) scv[prm.sp[s].Aslot * ml + i] = prm.sp[s].A[gv];
+                for (int s = 0; s < S; ++s) p.v[s] = z.v[s] + beta[s] * po.v[s];
+                spv[i] = p;
+            } else if constexpr (HAS_X) {
+                spv[i] = v.x[gv];
             }
         }
-        const int i0 = g.vptr[v0];
         __syncthreads();
-        // (2) cells: one lane per listed cell computes the rows of its four corners for every
-        // species and stores the rows of owned corners in their own LDS slots (no atomics)
-        const int kend = g.bptr[b + 1];
-        for (int k = g.bptr[b] + threadIdx.x; k < kend; k += blockDim.x) {
-            const ushort4 l = g.bloc[k];
-            const ushort4 sl4 = g.bslot[k];
-            const Vec4<T> q0 = sgeo[l.x], q1 = sgeo[l.y], q2 = sgeo[l.z], q3 = sgeo[l.w];
-            Vec4<T> w0{}, w1{}, w2{}, w3{};
-            if constexpr (MODE != MF_DIAG) {
-                w0 = spv[l.x];
-                w1 = spv[l.y];
-                w2 = spv[l.z];
-                w3 = spv[l.w];
-            }
-            const Vec3<T> e1{q1.x - q0.x, q1.y - q0.y, q1.z - q0.z};
-            const Vec3<T> e2{q2.x - q0.x, q2.y - q0.y, q2.z - q0.z};
-            const Vec3<T> e3{q3.x - q0.x, q3.y - q0.y, q3.z - q0.z};
-            const Vec3<T> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
-            const T adet = fabs(dot(e1, c1));
-            const T r6 = T(1) / (T(6) * adet);
-            T y[4][4];
+        // (2) L lanes per owned vertex (vertices in decreasing-degree order); whole waves step
+        // through the lanes of the block's slices
+        const int L = g.L;
+        const int vps = 64 / L;
+        const int spb = g.NV / vps;
+        const int nls = ((nv + vps - 1) / vps) * 64;
+        for (int ls = threadIdx.x; ls < nls; ls += blockDim.x) {
+            const int lane = ls & 63;
+            const int rank = ls / L;
+            const int* sp0 = g.sptr + b * spb + (ls >> 6);
+            const int jbeg = sp0[0] + lane;
+            const int width = (sp0[1] - sp0[0]) >> 6;
+            const bool live = rank < nv;
+            const int ov = live ? (int)g.lperm[b * g.NV + rank] : 0;
+            const Vec4<T> q0 = sgeo[ov];
+            SV<T, S> own{};
+            if constexpr (HAS_X) own = spv[ov];
+            T acc[S];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int s = 0; s < S; ++s) acc[s] = T(0);
+            for (int t0 = 0; t0 < width; t0 += 4) {
+              // the list words of 4 steps are loaded together (clamped in-bounds addresses,
+              // steps past the list read as padding), off the dependent chain
+              uint32_t pk4[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) y[i][s] = T(0);
+              for (int u = 0; u < 4; ++u) {
+                  const uint32_t wv = g.ell[jbeg + 64 * min(t0 + u, width - 1)];
+                  pk4[u] = t0 + u < width ? wv : ELL_PAD;
+              }
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (!act[s]) continue;
-                const T De = coef_at(prm.sp[s].D, prm.sp[s].Dmode, prm.sp[s].Dval, k, scv,
-                                     prm.sp[s].Dslot, ml, l);
-                const T al = coef_at(prm.sp[s].A, prm.sp[s].Amode, prm.sp[s].Aval, k, scv,
-                                     prm.sp[s].Aslot, ml, l);
-                const T w = De * r6;
-                const T mf = inv_dt + al;
-                if constexpr (MODE == MF_DIAG) {
-                    const Vec3<T> c0{-(c1.x + c2.x + c3.x), -(c1.y + c2.y + c3.y),
-                                     -(c1.z + c2.z + c3.z)};
-                    const T md = lumped ? mf * adet * T(5.0 / 120.0) : mf * adet * T(2.0 / 120.0);
-                    y[0][s] = w * dot(c0, c0) + md;
-                    y[1][s] = w * dot(c1, c1) + md;
-                    y[2][s] = w * dot(c2, c2) + md;
-                    y[3][s] = w * dot(c3, c3) + md;
+              for (int u = 0; u < 4; ++u) {
+                const uint32_t pk = pk4[u];
+                if (pk == ELL_PAD) continue;  // padding, also every entry of a lane past nv
+                const int j = jbeg + 64 * (t0 + u);
+                const int la = pk & 1023, lb = (pk >> 10) & 1023, lc = pk >> 20;
+                const Vec4<T> q1 = sgeo[la], q2 = sgeo[lb], q3 = sgeo[lc];
+                const Vec3<T> e1{q1.x - q0.x, q1.y - q0.y, q1.z - q0.z};
+                const Vec3<T> e2{q2.x - q0.x, q2.y - q0.y, q2.z - q0.z};
+                const Vec3<T> e3{q3.x - q0.x, q3.y - q0.y, q3.z - q0.z};
+                const Vec3<T> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
+                const T adet = fabs(dot(e1, c1));
+                const T ri = recip(adet);
+                const Vec3<T> cs{c1.x + c2.x + c3.x, c1.y + c2.y + c3.y, c1.z + c2.z + c3.z};
+                if constexpr (MODE == MF_DIAG && FAST) {
+                    acc[0] += dot(cs, cs) * ri;  // species-independent; D/6 and mdiag at the end
+                } else if constexpr (MODE == MF_DIAG || MODE == MF_MASS) {
+                    const T gd = dot(cs, cs) * ri;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const T mf = inv_dt + prm.sp[s].A[j];
+                        if constexpr (MODE == MF_MASS)
+                            acc[s] += mf * adet * T(5.0 / 120.0);
+                        else
+                            acc[s] += prm.sp[s].D[j] * T(1.0 / 6.0) * gd +
+                                      (lumped ? mf * adet * T(5.0 / 120.0) : mf * adet * T(2.0 / 120.0));
+                    }
                 } else {
-                    const T u0 = comp(w0, s), u1 = comp(w1, s), u2 = comp(w2, s), u3 = comp(w3, s);
-                    const T d1 = u1 - u0, d2 = u2 - u0, d3 = u3 - u0;
-                    const Vec3<T> gr{d1 * c1.x + d2 * c2.x + d3 * c3.x,
-                                     d1 * c1.y + d2 * c2.y + d3 * c3.y,
-                                     d1 * c1.z + d2 * c2.z + d3 * c3.z};
-                    const T y1 = w * dot(c1, gr), y2 = w * dot(c2, gr), y3 = w * dot(c3, gr);
-                    const T y0 = -(y1 + y2 + y3);
-                    if (lumped) {
-                        const T mlp = mf * adet * T(5.0 / 120.0);
-                        y[0][s] = y0 + mlp * u0;
-                        y[1][s] = y1 + mlp * u1;
-                        y[2][s] = y2 + mlp * u2;
-                        y[3][s] = y3 + mlp * u3;
-                    } else {
-                        const T mc = mf * adet * T(1.0 / 120.0);
-                        const T su = ((u0 + u1) + u2) + u3;
-                        y[0][s] = y0 + mc * (su + u0);
-                        y[1][s] = y1 + mc * (su + u1);
-                        y[2][s] = y2 + mc * (su + u2);
-                        y[3][s] = y3 + mc * (su + u3);
+                    const T g1 = dot(cs, c1) * ri, g2 = dot(cs, c2) * ri, g3 = dot(cs, c3) * ri;
+                    const SV<T, S> w1 = spv[la], w2 = spv[lb], w3 = spv[lc];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const T u0 = own.v[s], u1 = w1.v[s], u2 = w2.v[s], u3 = w3.v[s];
+                        const T gsum = g1 * (u1 - u0) + g2 * (u2 - u0) + g3 * (u3 - u0);
+                        if constexpr (FAST) {
+                            acc[s] += gsum;
+                        } else {
+                            const T mf = inv_dt + prm.sp[s].A[j];
+                            const T kr = -prm.sp[s].D[j] * T(1.0 / 6.0) * gsum;
+                            if (lumped)
+                                acc[s] += kr + mf * adet * T(5.0 / 120.0) * u0;
+                            else
+                                acc[s] += kr + mf * adet * T(1.0 / 120.0) *
+                                                   ((((u0 + u1) + u2) + u3) + u0);
+                        }
                     }
                 }
+              }
             }
-            if (sl4.x != 0xFFFF) ys[sl4.x] = Vec4<T>{y[0][0], y[0][1], y[0][2], y[0][3]};
-            if (sl4.y != 0xFFFF) ys[sl4.y] = Vec4<T>{y[1][0], y[1][1], y[1][2], y[1][3]};
-            if (sl4.z != 0xFFFF) ys[sl4.z] = Vec4<T>{y[2][0], y[2][1], y[2][2], y[2][3]};
-            if (sl4.w != 0xFFFF) ys[sl4.w] = Vec4<T>{y[3][0], y[3][1], y[3][2], y[3][3]};
-        }
-        __syncthreads();
-        // (3) owned rows: sum the vertex's slots (fixed order), identity rows at Dirichlet
-        // vertices, coalesced stores
-        for (int lv = threadIdx.x; lv < nv; lv += blockDim.x) {
-            const int vtx = v0 + lv;
-            T acc[4] = {T(0), T(0), T(0), T(0)};
-            const int jb = g.vptr[vtx] - i0, je = g.vptr[vtx + 1] - i0;
-            for (int j = jb; j < je; ++j) {
-                const Vec4<T> r = ys[j];
-                acc[0] += r.x;
-                acc[1] += r.y;
-                acc[2] += r.z;
-            }
-            Vec4<T> own{};
-            if constexpr (MODE != MF_DIAG) own = spv[lv];
+            // sum the partial rows of the L adjacent lanes of a vertex (pairwise, so every
+            // lane holds the same bits)
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                if (!act[s]) continue;
-                const uint8_t* __restrict__ bc = prm.sp[s].bc;
-                const bool isbc = bc && bc[vtx];
-                T val = acc[s];
-                if constexpr (MODE == MF_APPLY) {
-                    if (isbc) val = comp(own, s);
-                    v.y[s][vtx] = val;
-                } else if constexpr (MODE == MF_DIAG) {
-                    v.y[s][vtx] = isbc ? T(1) : T(1) / val;
-                } else if constexpr (MODE == MF_CG) {
-                    const T p = comp(own, s);
-                    if (isbc) val = p;
-                    v.pnew[s][vtx] = p;
-                    v.y[s][vtx] = val;
-                    part[s] += (double)p * (double)val;
-                } else {  // MF_RESID
-                    if (isbc) val = comp(own, s);
-                    const double bv = v.b[s][vtx];
-                    const double r = bv - (double)val;
-                    v.y[s][vtx] = (T)r;
-                    part[s] += r * r;
-                    part2[s] += bv * bv;
+                if (L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
+                if (L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+            }
+            if (!live || (lane & (L - 1)) != 0) continue;
+            const int vtx = v0 + ov;
+            if constexpr (FAST) {
+                // rows from the geometric sums: -(D/6) sum + mdiag u (diag: (D/6) sum + mdiag)
+                const SV<T, S> md = mdiag[vtx];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if constexpr (MODE == MF_DIAG)
+                        acc[s] = prm.sp[s].D6 * acc[0] + md.v[s];
+                    else
+                        acc[s] = -prm.sp[s].D6 * acc[s] + md.v[s] * own.v[s];
                 }
+            }
+            const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
+            SV<T, S> out{};
+            if constexpr (MODE == MF_RESID) {
+                const SV<double, S> bv = v.b[vtx];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const T val = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
+                    const double r = bv.v[s] - (double)val;
+                    out.v[s] = (T)r;
+                    part[s] += r * r;
+                    part2[s] += bv.v[s] * bv.v[s];
+                }
+                v.y[vtx] = out;
+            } else if constexpr (MODE == MF_APPLY) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) out.v[s] = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
+                v.y[vtx] = out;
+            } else if constexpr (MODE == MF_DIAG) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) out.v[s] = ((bcb >> s) & 1u) ? T(1) : T(1) / acc[s];
+                v.y[vtx] = out;
+            } else if constexpr (MODE == MF_MASS) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) out.v[s] = acc[s];
+                v.y[vtx] = out;
+            } else {  // MF_CG
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const T val = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
+                    out.v[s] = val;
+                    if (act[s]) part[s] += (double)own.v[s] * (double)val;
+                }
+                v.pnew[vtx] = own;
+                v.y[vtx] = out;
             }
         }
         __syncthreads();  // LDS is reused by the next partition block
@@ -420,7 +466,7 @@ template <int S, int MODE>
 __global__ void __launch_bounds__(CSR_THREADS)
 k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
       const double* __restrict__ A0, const double* __restrict__ A1, const double* __restrict__ A2,
-      Vecs<double> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
+      MFVec<double, S> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
       double* __restrict__ pout, int par) {
     __shared__ double red[16 * 2 * MAXS];
     const int nwg = gridDim.x;
@@ -432,44 +478,7 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
         beta[s] = 0.0;
     }
     if constexpr (MODE == MF_CG) {
-        bool prev[S];
-        bool anyprev = false;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            prev[s] = st->act[par ^ 1][s] != 0;
-            anyprev |= prev[s];
-        }
-        if (!anyprev) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                for (int s = 0; s < S; ++s) st->act[par][s] = 0;
-                st->any_active = 0;
-            }
-            return;
-        }
-        double sums[2 * S];
-        consume_partials<2 * S>(pin, nin, sums, red);
-        const int k = st->k;
-        bool any = false;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const double rz = sums[s], rr = sums[S + s];
-            act[s] = prev[s] && st->dead[s] == 0 && (rr > st->tol2[s]) && (k < st->max_it) &&
-                     isfinite(rr);
-            beta[s] = (k == 0 || !act[s]) ? 0.0 : rz / st->rz[par ^ 1][s];
-            any |= act[s];
-        }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            for (int s = 0; s < S; ++s) {
-                st->act[par][s] = act[s] ? 1 : 0;
-                if (!prev[s]) continue;
-                st->rz[par][s] = sums[s];
-                st->rr[s] = sums[S + s];
-                if (!act[s] && st->dead[s] == 0)
-                    st->status[s] = (sums[S + s] <= st->tol2[s]) ? ST_CONVERGED : ST_MAXIT;
-            }
-            st->any_active = any ? 1 : 0;
-        }
-        if (!any) return;
+        if (!cg_begin<S>(st, pin, nin, par, act, beta, red)) return;
     }
     const double* __restrict__ Av[3] = {A0, A1, A2};
     double part[S], part2[S];
@@ -484,38 +493,43 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
         for (int s = 0; s < S; ++s) acc[s] = 0.0;
         for (int j = js + lane; j < je; j += CSR_LANES) {
             const int c = indices[j];
-            double a[S], u[S];
+            const SV<double, S> z = v.x[c];
+            SV<double, S> po{};
+            if constexpr (MODE == MF_CG) po = v.pold[c];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                a[s] = act[s] ? Av[s][j] : 0.0;
-                if constexpr (MODE == MF_CG)
-                    u[s] = act[s] ? v.x[s][c] + beta[s] * v.pold[s][c] : 0.0;
-                else
-                    u[s] = v.x[s][c];
+                if (!act[s]) continue;
+                const double u = MODE == MF_CG ? z.v[s] + beta[s] * po.v[s] : z.v[s];
+                acc[s] += Av[s][j] * u;
             }
-#pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] += a[s] * u[s];
         }
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int off = CSR_LANES / 2; off > 0; off >>= 1) acc[s] += __shfl_xor(acc[s], off, 64);
         if (lane == 0) {
+            if constexpr (MODE == MF_CG) {
+                const SV<double, S> z = v.x[row];
+                const SV<double, S> po = v.pold[row];
+                SV<double, S> p{}, q{};
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (!act[s]) continue;
-                if constexpr (MODE == MF_CG) {
-                    const double p = v.x[s][row] + beta[s] * v.pold[s][row];
-                    v.pnew[s][row] = p;
-                    v.y[s][row] = acc[s];
-                    part[s] += p * acc[s];
-                } else {
-                    const double bv = v.b[s][row];
-                    const double r = bv - acc[s];
-                    v.y[s][row] = r;
-                    part[s] += r * r;
-                    part2[s] += bv * bv;
+                for (int s = 0; s < S; ++s) {
+                    p.v[s] = z.v[s] + beta[s] * po.v[s];
+                    q.v[s] = acc[s];
+                    if (act[s]) part[s] += p.v[s] * acc[s];
                 }
+                v.pnew[row] = p;
+                v.y[row] = q;
+            } else {
+                const SV<double, S> bv = v.b[row];
+                SV<double, S> r{};
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    r.v[s] = bv.v[s] - acc[s];
+                    part[s] += r.v[s] * r.v[s];
+                    part2[s] += bv.v[s] * bv.v[s];
+                }
+                v.y[row] = r;
             }
         }
     }
@@ -533,10 +547,10 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
 }
 
 // U_k: x += alpha p; r -= alpha q; z = r * dinv; partials r.z, r.r (parity par = k & 1).
-// INIT (U_{-1}): r = (T) rin; x = 0; p_old = 0; z = r * dinv; partials.
+// INIT (U_{-1}): r = (T) rin (0 for species not solved); x = 0; p_old = 0; z = r * dinv.
 template <class T, int S, bool INIT>
 __global__ void __launch_bounds__(256)
-k_update(int V, UVecs<T> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
+k_update(int V, UVec<T, S> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
          double* __restrict__ pout, int par) {
     __shared__ double red[16 * 2 * MAXS];
     const int nwg = gridDim.x;
@@ -577,32 +591,35 @@ k_update(int V, UVecs<T> v, CGState* __restrict__ st, const double* __restrict__
     for (int s = 0; s < S; ++s) prz[s] = prr[s] = 0.0;
     const int stride = nwg * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        T r[S], d[S];
+        const SV<T, S> d = v.dinv[i];
+        SV<T, S> r{}, z{}, x{};
+        if constexpr (INIT) {
+            const SV<double, S> ri = v.rin[i];
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (!act[s]) continue;
-            d[s] = v.dinv[s][i];
-            if constexpr (INIT) {
-                r[s] = (T)v.rin[s][i];
-            } else {
-                const T pi = v.p[s][i], qi = v.q[s][i], xi = v.x[s][i], ri = v.r[s][i];
-                v.x[s][i] = xi + alpha[s] * pi;
-                r[s] = ri - alpha[s] * qi;
+            for (int s = 0; s < S; ++s) r.v[s] = act[s] ? (T)ri.v[s] : T(0);
+            v.pzero[i] = SV<T, S>{};
+        } else {
+            const SV<T, S> p = v.p[i], q = v.q[i];
+            x = v.x[i];
+            r = v.r[i];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (!act[s]) continue;
+                x.v[s] += alpha[s] * p.v[s];
+                r.v[s] -= alpha[s] * q.v[s];
             }
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (!act[s]) continue;
-            if constexpr (INIT) {
-                v.x[s][i] = T(0);
-                v.pzero[s][i] = T(0);
+            z.v[s] = r.v[s] * d.v[s];
+            if (act[s]) {
+                prz[s] += (double)r.v[s] * (double)z.v[s];
+                prr[s] += (double)r.v[s] * (double)r.v[s];
             }
-            const T z = r[s] * d[s];
-            v.r[s][i] = r[s];
-            v.z[s][i] = z;
-            prz[s] += (double)r[s] * (double)z;
-            prr[s] += (double)r[s] * (double)r[s];
         }
+        v.x[i] = x;
+        v.r[i] = r;
+        v.z[i] = z;
     }
     double tot[2 * S];
 #pragma unroll
@@ -618,57 +635,88 @@ k_update(int V, UVecs<T> v, CGState* __restrict__ st, const double* __restrict__
 
 // x64 += (double) d over the species in `mask` (refinement correction)
 template <class T, int S>
-__global__ void k_add_correction(int V, double* __restrict__ x0, double* __restrict__ x1,
-                                 double* __restrict__ x2, const T* __restrict__ d0,
-                                 const T* __restrict__ d1, const T* __restrict__ d2, int mask) {
+__global__ void k_add_correction(int V, SV<double, S>* __restrict__ x,
+                                 const SV<T, S>* __restrict__ d, int mask) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    double* xs[3] = {x0, x1, x2};
-    const T* ds[3] = {d0, d1, d2};
+    SV<double, S> a = x[i];
+    const SV<T, S> c = d[i];
 #pragma unroll
     for (int s = 0; s < S; ++s)
-        if (mask & (1 << s)) xs[s][i] += (double)ds[s][i];
+        if (mask & (1 << s)) a.v[s] += (double)c.v[s];
+    x[i] = a;
 }
 
-// caller order -> internal order (and back), with conversion
+// caller-order species arrays <-> species-interleaved records (internal order when perm)
+template <class To, int S>
+__global__ void k_pack(int V, const int32_t* __restrict__ perm, const double* __restrict__ s0,
+                       const double* __restrict__ s1, const double* __restrict__ s2,
+                       SV<To, S>* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int c = perm ? perm[i] : i;
+    const double* src[3] = {s0, s1, s2};
+    SV<To, S> r{};
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.v[s] = src[s] ? (To)src[s][c] : To(0);
+    out[i] = r;
+}
+template <class Ti, int S>
+__global__ void k_unpack(int V, const int32_t* __restrict__ perm, const SV<Ti, S>* __restrict__ in,
+                         double* __restrict__ d0, double* __restrict__ d1, double* __restrict__ d2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int c = perm ? perm[i] : i;
+    double* dst[3] = {d0, d1, d2};
+    const SV<Ti, S> r = in[i];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (dst[s]) dst[s][c] = (double)r.v[s];
+}
+// per-ELL-entry coefficient of the entry's cell (0 for padding entries)
+template <class T>
+__global__ void k_inc_coef(int64_t n, const int32_t* __restrict__ ell_cell,
+                           const int32_t* __restrict__ conn, const double* __restrict__ p, int mode,
+                           double val, T* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t e = ell_cell[i];
+    out[i] = e >= 0 ? (T)cell_coef(p, mode, val, e, conn) : T(0);
+}
+
+// out[i] = in[perm[i]], 0 where perm[i] < 0
 template <class Ti, class To>
-__global__ void k_gather_perm(int V, const int32_t* __restrict__ perm, const Ti* __restrict__ in,
+__global__ void k_gather_perm(int n, const int32_t* __restrict__ perm, const Ti* __restrict__ in,
                               To* __restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= V) return;
-    out[i] = (To)in[perm[i]];
-}
-template <class Ti, class To>
-__global__ void k_scatter_perm(int V, const int32_t* __restrict__ perm, const Ti* __restrict__ in,
-                               To* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= V) return;
-    out[perm[i]] = (To)in[i];
-}
-template <class Ti, class To>
-__global__ void k_convert(int64_t n, const Ti* __restrict__ in, To* __restrict__ out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = (To)in[i];
+    const int32_t p = perm[i];
+    out[i] = p >= 0 ? (To)in[p] : To(0);
 }
-__global__ void k_set_bc_mask(int n_bc, const int32_t* __restrict__ bc,
-                              const int32_t* __restrict__ iperm, uint8_t* __restrict__ mask) {
+__global__ void k_set_bc_bits(int n_bc, const int32_t* __restrict__ bc,
+                              const int32_t* __restrict__ iperm, int s,
+                              uint8_t* __restrict__ bits) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bc) return;
-    mask[iperm ? iperm[bc[k]] : bc[k]] = 1;
+    const int v = iperm ? iperm[bc[k]] : bc[k];
+    // one launch per species: concurrent writers of a byte set the same bit
+    bits[v] = (uint8_t)(bits[v] | (1u << s));
 }
+template <int S>
 __global__ void k_set_bc_values(int n_bc, const int32_t* __restrict__ bc,
-                                const int32_t* __restrict__ iperm, double value,
-                                double* __restrict__ x) {
+                                const int32_t* __restrict__ iperm, int s, double value,
+                                SV<double, S>* __restrict__ x) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bc) return;
-    x[iperm ? iperm[bc[k]] : bc[k]] = value;
+    x[iperm ? iperm[bc[k]] : bc[k]].v[s] = value;
 }
+template <int S>
 __global__ void k_diag_inv_csr(int V, const int32_t* __restrict__ diag_pos,
-                               const double* __restrict__ A, double* __restrict__ dinv) {
+                               const double* __restrict__ A, int s,
+                               SV<double, S>* __restrict__ dinv) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    dinv[i] = 1.0 / A[diag_pos[i]];
+    dinv[i].v[s] = 1.0 / A[diag_pos[i]];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -683,27 +731,28 @@ struct Workspace {
     double* pU[2] = {};       // partials of U kernels by parity (2S per workgroup)
     double* pR = nullptr;     // partials of the residual kernel (2S per workgroup)
     double* h_pR = nullptr;   // pinned copy
-    // coefficient copies (target precision) and masks
-    void* Dc[MAXS] = {};
-    void* Ac[MAXS] = {};
-    void* Dc2[MAXS] = {};
-    void* Ac2[MAXS] = {};
-    uint8_t* bcmask[MAXS] = {};
+    // operator coefficients: [0] kernels of the CG precision, [1] FP64 residual of the
+    // refinement path; per-ELL-entry D then alpha per species, and the fast form's mass
+    // diagonal (species records)
+    void* coef[2][2][MAXS] = {};
+    size_t coef_cap[2][2][MAXS] = {};
+    void* mdiag[2] = {};
+    uint8_t* bcbits = nullptr;
     unsigned char prm[sizeof(OpParams<double>)] = {};   // OpParams of the CG precision
     unsigned char prm2[sizeof(OpParams<double>)] = {};  // OpParams<double> of the residual
-    // FP64 vectors
-    double* b64[MAXS] = {};
-    double* x64[MAXS] = {};
-    double* r64[MAXS] = {};
-    // CG vectors in the CG precision
-    void* xcg[MAXS] = {};
-    void* rcg[MAXS] = {};
-    void* zcg[MAXS] = {};
-    void* qcg[MAXS] = {};
-    void* pa[MAXS] = {};
-    void* pb[MAXS] = {};
-    void* dinv[MAXS] = {};
-    double* csr[MAXS] = {};  // CSR64 values
+    // FP64 records
+    void* b64 = nullptr;
+    void* x64 = nullptr;
+    void* r64 = nullptr;
+    // CG records in the CG precision
+    void* xcg = nullptr;
+    void* rcg = nullptr;
+    void* zcg = nullptr;
+    void* qcg = nullptr;
+    void* pa = nullptr;
+    void* pb = nullptr;
+    void* dinv = nullptr;
+    double* csr[MAXS] = {};  // CSR64 values per species
     // captured CG chunk and the parameters it was captured with
     hipGraphExec_t graph = nullptr;
     int graph_chunk = 0;
@@ -711,7 +760,7 @@ struct Workspace {
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
 };
 
-static size_t tsize(int method) { return method == DSV_METHOD_MF32_IR64 ? sizeof(float) : sizeof(double); }
+static size_t rec_bytes(size_t t, int S) { return t * (S == 3 ? 4 : S); }
 
 static void free_ws(Workspace* w) {
     if (!w) return;
@@ -723,24 +772,23 @@ static void free_ws(Workspace* w) {
     dfree(w->pU[0]);
     dfree(w->pU[1]);
     dfree(w->pR);
-    for (int s = 0; s < MAXS; ++s) {
-        dfree(w->Dc[s]);
-        dfree(w->Ac[s]);
-        dfree(w->Dc2[s]);
-        dfree(w->Ac2[s]);
-        dfree(w->bcmask[s]);
-        dfree(w->b64[s]);
-        dfree(w->x64[s]);
-        dfree(w->r64[s]);
-        dfree(w->xcg[s]);
-        dfree(w->rcg[s]);
-        dfree(w->zcg[s]);
-        dfree(w->qcg[s]);
-        dfree(w->pa[s]);
-        dfree(w->pb[s]);
-        dfree(w->dinv[s]);
-        dfree(w->csr[s]);
-    }
+    for (int a = 0; a < 2; ++a)
+        for (int c = 0; c < 2; ++c)
+            for (int s = 0; s < MAXS; ++s) dfree(w->coef[a][c][s]);
+    dfree(w->mdiag[0]);
+    dfree(w->mdiag[1]);
+    for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
+    dfree(w->bcbits);
+    dfree(w->b64);
+    dfree(w->x64);
+    dfree(w->r64);
+    dfree(w->xcg);
+    dfree(w->rcg);
+    dfree(w->zcg);
+    dfree(w->qcg);
+    dfree(w->pa);
+    dfree(w->pb);
+    dfree(w->dinv);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
 }
@@ -754,13 +802,32 @@ void free_workspaces(dsv_mesh* m) {
     dfree(m->Ae);
 }
 
+static int max_lds_per_block(int device) {
+    static int cached[64] = {};
+    if (device >= 0 && device < 64 && cached[device]) return cached[device];
+    int v = 0;
+    DSV_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    if (device >= 0 && device < 64) cached[device] = v;
+    return v;
+}
+
+// LDS bytes of the MF kernel: coordinates and species input of every local vertex
+template <class T>
+static size_t mf_lds_bytes(const Partition& P, int S) {
+    return (size_t)P.max_local * (sizeof(Vec4<T>) + rec_bytes(sizeof(T), S));
+}
+
 // launch grids (every partial-producing kernel has at most MAX_PROD workgroups)
-static int mf_grid(const dsv_mesh* m) {
-    const int per = (m->part.nb + 7) / 8;
+static int mf_grid(const Partition& P) {
+    const int per = (P.nb + 7) / 8;
     return 8 * std::min(per, MAX_PROD / 8);
 }
 static int csr_grid(const dsv_mesh* m) { return std::min((int)cdiv(m->V, CSR_ROWS), MAX_PROD); }
-static int upd_grid(const dsv_mesh* m) { return std::max(1, std::min((int)cdiv(m->V, 256 * 4), MAX_PROD)); }
+static int upd_grid(const dsv_mesh* m) {
+    return std::max(1, std::min((int)cdiv(m->V, 256), MAX_PROD));
+}
+
+static bool is_fp32(int method) { return method == DSV_METHOD_MF32_IR64; }
 
 static Workspace* get_ws(dsv_mesh* m, int method, int S) {
     Workspace*& w = m->ws[method][S];
@@ -780,22 +847,22 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S) {
         w->pU[0] = dalloc<double>((size_t)2 * MAXS * MAX_PROD);
         w->pU[1] = dalloc<double>((size_t)2 * MAXS * MAX_PROD);
         w->pR = dalloc<double>((size_t)2 * MAXS * MAX_PROD);
-        const size_t ts = tsize(method);
         const size_t V = m->V;
-        for (int s = 0; s < S; ++s) {
-            w->bcmask[s] = dalloc<uint8_t>(V);
-            w->b64[s] = dalloc<double>(V);
-            w->x64[s] = dalloc<double>(V);
-            w->r64[s] = dalloc<double>(V);
-            DSV_HIP(hipMalloc(&w->xcg[s], V * ts));
-            DSV_HIP(hipMalloc(&w->rcg[s], V * ts));
-            DSV_HIP(hipMalloc(&w->zcg[s], V * ts));
-            DSV_HIP(hipMalloc(&w->qcg[s], V * ts));
-            DSV_HIP(hipMalloc(&w->pa[s], V * ts));
-            DSV_HIP(hipMalloc(&w->pb[s], V * ts));
-            DSV_HIP(hipMalloc(&w->dinv[s], V * ts));
-            if (method == DSV_METHOD_CSR64) w->csr[s] = dalloc<double>(m->nnz);
-        }
+        const size_t rt = rec_bytes(is_fp32(method) ? sizeof(float) : sizeof(double), S);
+        const size_t r64 = rec_bytes(sizeof(double), S);
+        w->bcbits = dalloc<uint8_t>(V);
+        DSV_HIP(hipMalloc(&w->b64, V * r64));
+        DSV_HIP(hipMalloc(&w->x64, V * r64));
+        DSV_HIP(hipMalloc(&w->r64, V * r64));
+        DSV_HIP(hipMalloc(&w->xcg, V * rt));
+        DSV_HIP(hipMalloc(&w->rcg, V * rt));
+        DSV_HIP(hipMalloc(&w->zcg, V * rt));
+        DSV_HIP(hipMalloc(&w->qcg, V * rt));
+        DSV_HIP(hipMalloc(&w->pa, V * rt));
+        DSV_HIP(hipMalloc(&w->pb, V * rt));
+        DSV_HIP(hipMalloc(&w->dinv, V * rt));
+        if (method == DSV_METHOD_CSR64)
+            for (int s = 0; s < S; ++s) w->csr[s] = dalloc<double>(m->nnz);
     } catch (...) {
         free_ws(w);
         w = nullptr;
@@ -804,215 +871,172 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S) {
     return w;
 }
 
-// Copies / converts the coefficients of every species into the workspace (precision T):
-// per-vertex arrays in the kernel's vertex order, per-cell arrays in partition-entry order
-// (matrix-free) so that the MF kernels read them with coalesced loads.  Fills the kernel
-// parameter block, including the LDS slots of staged per-vertex coefficients.
-template <class T>
-static void prepare_op(dsv_mesh* m, Workspace* w, int S, const dsv_species* sp, double inv_dt,
-                       int lumped, bool internal, hipStream_t st, bool second = false) {
-    OpParams<T> h{};
-    h.inv_dt = (T)inv_dt;
-    h.lumped = lumped ? 1 : 0;
-    h.n_vslots = 0;
-    const size_t cap = std::max({(size_t)m->C, (size_t)m->V, (size_t)m->part.n_entries});
-    for (int s = 0; s < S; ++s) {
-        SpeciesP<T>& q = h.sp[s];
-        q.Dmode = coef_mode(sp[s].D, sp[s].D_len, m);
-        q.Amode = coef_mode(sp[s].alpha, sp[s].alpha_len, m);
-        q.Dval = (T)sp[s].D_value;
-        q.Aval = sp[s].alpha ? T(0) : (T)sp[s].alpha_value;
-        q.Dslot = q.Dmode == COEF_VERT ? h.n_vslots++ : -1;
-        q.Aslot = q.Amode == COEF_VERT ? h.n_vslots++ : -1;
-        const double* src[2] = {sp[s].D, sp[s].alpha};
-        const int32_t lens[2] = {sp[s].D_len, sp[s].alpha_len};
-        void** dst[2] = {second ? &w->Dc2[s] : &w->Dc[s], second ? &w->Ac2[s] : &w->Ac[s]};
-        const T** out[2] = {&q.D, &q.A};
-        for (int c = 0; c < 2; ++c) {
-            *out[c] = nullptr;
-            if (!src[c]) continue;
-            if (!*dst[c]) DSV_HIP(hipMalloc(dst[c], cap * sizeof(T)));
-            T* d = (T*)*dst[c];
-            const int mode = c == 0 ? q.Dmode : q.Amode;
-            if (mode == COEF_CELL && internal)
-                hipLaunchKernelGGL((k_gather_perm<double, T>), dim3(cdiv(m->part.n_entries, 256)), dim3(256),
-                                   0, st, (int)m->part.n_entries, m->part.belem, src[c], d);
-            else if (mode == COEF_VERT && internal)
-                hipLaunchKernelGGL((k_gather_perm<double, T>), dim3(cdiv(m->V, 256)), dim3(256), 0, st,
-                                   m->V, m->perm, src[c], d);
-            else
-                hipLaunchKernelGGL((k_convert<double, T>), dim3(cdiv(lens[c], 256)), dim3(256), 0, st,
-                                   (int64_t)lens[c], src[c], d);
-            *out[c] = d;
-        }
-        if (!second) {
-            DSV_HIP(hipMemsetAsync(w->bcmask[s], 0, m->V, st));
-            if (sp[s].n_bc > 0)
-                hipLaunchKernelGGL(k_set_bc_mask, dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
-                                   sp[s].n_bc, sp[s].bc_dofs,
-                                   internal ? m->iperm : (const int32_t*)nullptr, w->bcmask[s]);
-        }
-        q.bc = sp[s].n_bc > 0 ? w->bcmask[s] : nullptr;
-    }
-    std::memcpy(second ? w->prm2 : w->prm, &h, sizeof(h));
+// Dirichlet bits (internal vertex order): bit s at the Dirichlet vertices of species s
+static void prepare_bc(dsv_mesh* m, Workspace* w, int S, const dsv_species* sp, hipStream_t st) {
+    DSV_HIP(hipMemsetAsync(w->bcbits, 0, m->V, st));
+    for (int s = 0; s < S; ++s)
+        if (sp[s].n_bc > 0)
+            hipLaunchKernelGGL(k_set_bc_bits, dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
+                               sp[s].n_bc, sp[s].bc_dofs, m->iperm, s, w->bcbits);
     DSV_HIP(hipGetLastError());
 }
 
-static Geo make_geo(const dsv_mesh* m) {
+static Geo make_geo(const dsv_mesh* m, const Partition& P) {
     Geo g;
     g.V = m->V;
-    g.NV = m->part.NV;
-    g.nb = m->part.nb;
-    g.per_xcd = (m->part.nb + 7) / 8;
-    g.max_local = m->part.max_local;
-    g.max_entries = m->part.max_entries;
-    g.max_inc = m->part.max_inc;
-    g.bptr = m->part.bptr;
-    g.bloc = m->part.bloc;
-    g.hptr = m->part.hptr;
-    g.hlist = m->part.hlist;
-    g.vptr = m->part.vptr;
-    g.bslot = m->part.bslot;
+    g.NV = P.NV;
+    g.L = P.L;
+    g.nb = P.nb;
+    g.per_xcd = (P.nb + 7) / 8;
+    g.max_local = P.max_local;
+    g.hptr = P.hptr;
+    g.hlist = P.hlist;
+    g.lperm = P.lperm;
+    g.sptr = P.sptr;
+    g.ell = P.ell;
     return g;
 }
 
-static int max_lds_per_block(int device) {
-    static int cached[64] = {};
-    if (device >= 0 && device < 64 && cached[device]) return cached[device];
-    int v = 0;
-    DSV_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    if (device >= 0 && device < 64) cached[device] = v;
-    return v;
-}
-
-// LDS bytes of the MF kernel: staged geometry + species values (Vec4 each) and per-vertex
-// coefficient slots of every local vertex, and one Vec4 row per owned cell corner
-template <class T>
-static size_t mf_lds_bytes(const dsv_mesh* m, int S, int n_vslots) {
-    (void)S;
-    return (size_t)m->part.max_local * (2 * sizeof(Vec4<T>) + n_vslots * sizeof(T)) +
-           (size_t)m->part.max_inc * sizeof(Vec4<T>);
-}
-
 template <class T, int S, int MODE>
-static void launch_mf(const dsv_mesh* m, const unsigned char* prm, const Vecs<T>& v, CGState* st,
-                      const double* pin, int nin, double* pout, int par, hipStream_t stream) {
-    const Geo g = make_geo(m);
-    const T* X = sizeof(T) == 4 ? (const T*)m->xyz32 : (const T*)m->xyz64;
+static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char* prm,
+                      const MFVec<T, S>& v, CGState* st, const double* pin, int nin, double* pout,
+                      int par, hipStream_t stream) {
+    const Geo g = make_geo(m, P);
+    const Vec4<T>* X = sizeof(T) == 4 ? (const Vec4<T>*)(const void*)m->xyz32
+                                      : (const Vec4<T>*)(const void*)m->xyz64;
     OpParams<T> p;
     std::memcpy(&p, prm, sizeof(p));
-    const size_t lds = mf_lds_bytes<T>(m, S, p.n_vslots);
+    const size_t lds = mf_lds_bytes<T>(P, S);
     if (lds + 1024 > (size_t)max_lds_per_block(m->device))
         throw ArgError{"matrix-free partition needs " + std::to_string(lds) +
                        " B of LDS per workgroup; use fewer vertices_per_block"};
-    hipLaunchKernelGGL((k_mf<T, S, MODE>), dim3(mf_grid(m)), dim3(m->part.threads), lds, stream, g, X,
-                       X + m->V, X + 2 * (size_t)m->V, p, v, st, pin, nin, pout, par);
-}
-
-template <class T, int MODE>
-static void launch_mf_s(int S, const dsv_mesh* m, const unsigned char* prm, const Vecs<T>& v,
-                        CGState* st, const double* pin, int nin, double* pout, int par,
-                        hipStream_t stream) {
-    switch (S) {
-        case 1: launch_mf<T, 1, MODE>(m, prm, v, st, pin, nin, pout, par, stream); break;
-        case 2: launch_mf<T, 2, MODE>(m, prm, v, st, pin, nin, pout, par, stream); break;
-        default: launch_mf<T, 3, MODE>(m, prm, v, st, pin, nin, pout, par, stream); break;
+    if constexpr (MODE != MF_MASS) {
+        if (p.fast) {
+            hipLaunchKernelGGL((k_mf<T, S, MODE, true>), dim3(mf_grid(P)), dim3(P.threads), lds,
+                               stream, g, X, p, v, st, pin, nin, pout, par);
+            return;
+        }
     }
+    hipLaunchKernelGGL((k_mf<T, S, MODE, false>), dim3(mf_grid(P)), dim3(P.threads), lds, stream, g,
+                       X, p, v, st, pin, nin, pout, par);
 }
 
-template <int MODE>
-static void launch_csr_s(int S, const dsv_mesh* m, Workspace* w, const Vecs<double>& v,
-                         const double* pin, int nin, double* pout, int par, hipStream_t stream) {
-    const int grid = csr_grid(m);
+// Builds the operator parameters of precision T in coefficient set `set`: per-ELL-entry cell
+// coefficients (D and alpha, cell-averaged like the assembly) and, for the fast form (lumped
+// mass, scalar D for every species), the mass diagonal computed by the generic MASS kernel.
+template <class T, int S>
+static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_species* sp,
+                       double inv_dt, int lumped, hipStream_t st, int set) {
+    OpParams<T> h{};
+    h.inv_dt = (T)inv_dt;
+    h.lumped = lumped ? 1 : 0;
+    h.bcbits = w->bcbits;
+    bool fast = lumped != 0;
+    for (int s = 0; s < S; ++s) fast = fast && sp[s].D == nullptr;
+    const size_t n = (size_t)P.n_entries;
+    for (int s = 0; s < S; ++s) {
+        const double* src[2] = {sp[s].D, sp[s].alpha};
+        const double val[2] = {sp[s].D_value, sp[s].alpha ? 0.0 : sp[s].alpha_value};
+        const T** out[2] = {&h.sp[s].D, &h.sp[s].A};
+        for (int c = 0; c < 2; ++c) {
+            *out[c] = nullptr;
+            if (c == 0 && fast) continue;  // the fast form takes D as a scalar
+            void*& buf = w->coef[set][c][s];
+            size_t& cap = w->coef_cap[set][c][s];
+            if (cap < n * sizeof(T)) {
+                dfree(buf);
+                cap = 0;
+                DSV_HIP(hipMalloc(&buf, std::max<size_t>(1, n) * sizeof(T)));
+                cap = n * sizeof(T);
+            }
+            hipLaunchKernelGGL((k_inc_coef<T>), dim3(cdiv(n, 256)), dim3(256), 0, st, (int64_t)n,
+                               P.ell_cell, m->conn, src[c], coef_mode(src[c], c ? sp[s].alpha_len : sp[s].D_len, m),
+                               val[c], (T*)buf);
+            *out[c] = (const T*)buf;
+        }
+        h.sp[s].D6 = (T)(sp[s].D_value / 6.0);
+    }
+    h.fast = 0;
+    h.mdiag = nullptr;
+    if (fast) {
+        if (!w->mdiag[set]) DSV_HIP(hipMalloc(&w->mdiag[set], (size_t)m->V * rec_bytes(sizeof(T), S)));
+        MFVec<T, S> v{};
+        v.y = (SV<T, S>*)w->mdiag[set];
+        unsigned char hb[sizeof(OpParams<double>)] = {};
+        std::memcpy(hb, &h, sizeof(h));
+        launch_mf<T, S, MF_MASS>(m, P, hb, v, w->st, nullptr, 0, nullptr, 0, st);
+        h.fast = 1;
+        h.mdiag = w->mdiag[set];
+    }
+    std::memcpy(set ? w->prm2 : w->prm, &h, sizeof(h));
+    DSV_HIP(hipGetLastError());
+}
+
+template <int S, int MODE>
+static void launch_csr(const dsv_mesh* m, Workspace* w, const MFVec<double, S>& v,
+                       const double* pin, int nin, double* pout, int par, hipStream_t stream) {
     const double* A0 = w->csr[0];
     const double* A1 = S > 1 ? w->csr[1] : A0;
     const double* A2 = S > 2 ? w->csr[2] : A0;
-#define DSV_CSR_LAUNCH(SS)                                                                        \
-    hipLaunchKernelGGL((k_csr<SS, MODE>), dim3(grid), dim3(CSR_THREADS), 0, stream, m->V, m->indptr, \
-                       m->indices, A0, A1, A2, v, w->st, pin, nin, pout, par)
-    switch (S) {
-        case 1: DSV_CSR_LAUNCH(1); break;
-        case 2: DSV_CSR_LAUNCH(2); break;
-        default: DSV_CSR_LAUNCH(3); break;
-    }
-#undef DSV_CSR_LAUNCH
+    hipLaunchKernelGGL((k_csr<S, MODE>), dim3(csr_grid(m)), dim3(CSR_THREADS), 0, stream, m->V,
+                       m->indptr, m->indices, A0, A1, A2, v, w->st, pin, nin, pout, par);
 }
 
-template <class T>
-static UVecs<T> upd_vecs(Workspace* w, int S, int parity) {
-    UVecs<T> u{};
-    for (int s = 0; s < MAXS; ++s) {
-        const int q = s < S ? s : 0;
-        u.x[s] = (T*)w->xcg[q];
-        u.r[s] = (T*)w->rcg[q];
-        u.z[s] = (T*)w->zcg[q];
-        u.dinv[s] = (const T*)w->dinv[q];
-        u.p[s] = (const T*)(parity ? w->pa[q] : w->pb[q]);  // p_new of A_k with this parity
-        u.q[s] = (const T*)w->qcg[q];
-        u.pzero[s] = (T*)w->pa[q];  // INIT clears the p_old of A_0 (parity 0 reads pa)
-        u.rin[s] = w->r64[q];
-    }
-    return u;
+// producer grid of the A kernels of this workspace
+static int a_grid(dsv_mesh* m, const Workspace* w) {
+    if (w->method == DSV_METHOD_CSR64) return csr_grid(m);
+    return mf_grid(m->part);
 }
 
-template <class T, bool INIT>
-static void launch_update(int S, const dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
-    const UVecs<T> u = upd_vecs<T>(w, S, parity);
-    const int grid = upd_grid(m);
-    const int nin = w->method == DSV_METHOD_CSR64 ? csr_grid(m) : mf_grid(m);
-    double* pout = w->pU[parity];
-    switch (S) {
-        case 1:
-            hipLaunchKernelGGL((k_update<T, 1, INIT>), dim3(grid), dim3(256), 0, stream, m->V, u, w->st,
-                               w->pA, nin, pout, parity);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_update<T, 2, INIT>), dim3(grid), dim3(256), 0, stream, m->V, u, w->st,
-                               w->pA, nin, pout, parity);
-            break;
-        default:
-            hipLaunchKernelGGL((k_update<T, 3, INIT>), dim3(grid), dim3(256), 0, stream, m->V, u, w->st,
-                               w->pA, nin, pout, parity);
-            break;
-    }
+template <class T, int S, bool INIT>
+static void launch_update(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
+    UVec<T, S> u{};
+    u.x = (SV<T, S>*)w->xcg;
+    u.r = (SV<T, S>*)w->rcg;
+    u.z = (SV<T, S>*)w->zcg;
+    u.dinv = (const SV<T, S>*)w->dinv;
+    u.p = (const SV<T, S>*)(parity ? w->pa : w->pb);  // p_new of A_k with this parity
+    u.q = (const SV<T, S>*)w->qcg;
+    u.pzero = (SV<T, S>*)w->pa;  // INIT clears the p_old of A_0 (parity 0 reads pa)
+    u.rin = (const SV<double, S>*)w->r64;
+    hipLaunchKernelGGL((k_update<T, S, INIT>), dim3(upd_grid(m)), dim3(256), 0, stream, m->V, u,
+                       w->st, w->pA, a_grid(m, w), w->pU[parity], parity);
 }
 
-// vectors of A_k: z in x[], q in y[], p_old / p_new alternate between pa / pb by parity
-template <class T>
-static Vecs<T> cg_vecs(Workspace* w, int S, int parity) {
-    Vecs<T> v{};
-    for (int s = 0; s < MAXS; ++s) {
-        const int q = s < S ? s : 0;
-        v.x[s] = (T*)w->zcg[q];
-        v.y[s] = (T*)w->qcg[q];
-        v.pold[s] = (T*)(parity ? w->pb[q] : w->pa[q]);
-        v.pnew[s] = (T*)(parity ? w->pa[q] : w->pb[q]);
-        v.b[s] = nullptr;
-    }
+// vectors of A_k: z in x, q in y, p_old / p_new alternate between pa / pb by parity
+template <class T, int S>
+static MFVec<T, S> cg_vecs(Workspace* w, int parity) {
+    MFVec<T, S> v{};
+    v.x = (const SV<T, S>*)w->zcg;
+    v.y = (SV<T, S>*)w->qcg;
+    v.pold = (const SV<T, S>*)(parity ? w->pb : w->pa);
+    v.pnew = (SV<T, S>*)(parity ? w->pa : w->pb);
+    v.b = nullptr;
     return v;
 }
 
 // one CG iteration (A_k + U_k) with parity k & 1
-template <class T>
-static void cg_iteration(dsv_mesh* m, Workspace* w, int S, int parity, hipStream_t stream) {
-    const Vecs<T> v = cg_vecs<T>(w, S, parity);
+template <class T, int S>
+static void cg_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
+    const MFVec<T, S> v = cg_vecs<T, S>(w, parity);
     const int nU = upd_grid(m);
     if constexpr (std::is_same<T, double>::value) {
         if (w->method == DSV_METHOD_CSR64) {
-            launch_csr_s<MF_CG>(S, m, w, v, w->pU[parity ^ 1], nU, w->pA, parity, stream);
-            launch_update<T, false>(S, m, w, parity, stream);
+            launch_csr<S, MF_CG>(m, w, v, w->pU[parity ^ 1], nU, w->pA, parity, stream);
+            launch_update<T, S, false>(m, w, parity, stream);
             return;
         }
     }
-    launch_mf_s<T, MF_CG>(S, m, w->prm, v, w->st, w->pU[parity ^ 1], nU, w->pA, parity, stream);
-    launch_update<T, false>(S, m, w, parity, stream);
+    launch_mf<T, S, MF_CG>(m, m->part, w->prm, v, w->st, w->pU[parity ^ 1], nU,
+                           w->pA, parity, stream);
+    launch_update<T, S, false>(m, w, parity, stream);
 }
 
 // Runs CG iterations until no species is active.  Chunks of an even number of iterations
 // are captured once into a hipGraph (re-captured when the operator parameters change) and
 // replayed; the host polls one state word per chunk.
-template <class T>
-static int run_cg(dsv_mesh* m, Workspace* w, int S, int max_it, hipStream_t stream) {
+template <class T, int S>
+static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream) {
     const int chunk = std::max(2, m->graph_chunk & ~1);
     int launched = 0;
     if (m->use_graphs && (!w->graph_valid || w->graph_chunk != chunk ||
@@ -1022,7 +1046,7 @@ static int run_cg(dsv_mesh* m, Workspace* w, int S, int max_it, hipStream_t stre
         w->graph_valid = 0;
         hipGraph_t gr;
         DSV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < chunk; ++i) cg_iteration<T>(m, w, S, i & 1, stream);
+        for (int i = 0; i < chunk; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
         DSV_HIP(hipStreamEndCapture(stream, &gr));
         DSV_HIP(hipGraphInstantiate(&w->graph, gr, nullptr, nullptr, 0));
         DSV_HIP(hipGraphDestroy(gr));
@@ -1034,7 +1058,7 @@ static int run_cg(dsv_mesh* m, Workspace* w, int S, int max_it, hipStream_t stre
         if (m->use_graphs) {
             DSV_HIP(hipGraphLaunch(w->graph, stream));
         } else {
-            for (int i = 0; i < chunk; ++i) cg_iteration<T>(m, w, S, i & 1, stream);
+            for (int i = 0; i < chunk; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
         }
         launched += chunk;
         DSV_HIP(hipMemcpyAsync(&w->h_st->any_active, &w->st->any_active, sizeof(int),
@@ -1059,23 +1083,21 @@ static void write_state(Workspace* w, const CGState& h, hipStream_t stream) {
 
 // ---------------------------------------------------------------------------------------
 // residual r64 = b - A x64 (FP64, vertex order of the method); returns rr and bb per species
-static void residual64(dsv_mesh* m, Workspace* w, int S, hipStream_t stream, double* rr,
-                       double* bb) {
-    Vecs<double> v{};
-    for (int s = 0; s < MAXS; ++s) {
-        const int q = s < S ? s : 0;
-        v.x[s] = w->x64[q];
-        v.y[s] = w->r64[q];
-        v.b[s] = w->b64[q];
-    }
+template <int S>
+static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb) {
+    MFVec<double, S> v{};
+    v.x = (const SV<double, S>*)w->x64;
+    v.y = (SV<double, S>*)w->r64;
+    v.b = (const SV<double, S>*)w->b64;
     int nprod;
     if (w->method == DSV_METHOD_CSR64) {
-        launch_csr_s<MF_RESID>(S, m, w, v, nullptr, 0, w->pR, 0, stream);
+        launch_csr<S, MF_RESID>(m, w, v, nullptr, 0, w->pR, 0, stream);
         nprod = csr_grid(m);
     } else {
-        launch_mf_s<double, MF_RESID>(S, m, w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm, v,
-                                      w->st, nullptr, 0, w->pR, 0, stream);
-        nprod = mf_grid(m);
+        const Partition& P = m->part;
+        launch_mf<double, S, MF_RESID>(m, P, w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm,
+                                       v, w->st, nullptr, 0, w->pR, 0, stream);
+        nprod = mf_grid(P);
     }
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * 2 * S * nprod, hipMemcpyDeviceToHost,
@@ -1101,27 +1123,10 @@ static void check_solve_args(const dsv_mesh* m, int S, const dsv_species* sp, bo
     }
 }
 
-template <class T>
-static void add_correction(int S, dsv_mesh* m, Workspace* w, int mask, hipStream_t st) {
-    double* x[3];
-    const T* d[3];
-    for (int s = 0; s < 3; ++s) {
-        const int q = s < S ? s : 0;
-        x[s] = w->x64[q];
-        d[s] = (const T*)w->xcg[q];
-    }
-    const unsigned grid = cdiv(m->V, 256);
-    switch (S) {
-        case 1: hipLaunchKernelGGL((k_add_correction<T, 1>), dim3(grid), dim3(256), 0, st, m->V, x[0], x[1], x[2], d[0], d[1], d[2], mask); break;
-        case 2: hipLaunchKernelGGL((k_add_correction<T, 2>), dim3(grid), dim3(256), 0, st, m->V, x[0], x[1], x[2], d[0], d[1], d[2], mask); break;
-        default: hipLaunchKernelGGL((k_add_correction<T, 3>), dim3(grid), dim3(256), 0, st, m->V, x[0], x[1], x[2], d[0], d[1], d[2], mask); break;
-    }
-}
-
 // inner CG on A d = r64 (d starts at 0), T = CG precision
-template <class T>
-static void inner_cg(dsv_mesh* m, Workspace* w, int S, const double* tol2, const int* active,
-                     int max_it, hipStream_t st) {
+template <class T, int S>
+static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* active, int max_it,
+                     hipStream_t st) {
     CGState h = read_state(w, st);
     for (int s = 0; s < MAXS; ++s) {
         h.tol2[s] = s < S ? tol2[s] : 0.0;
@@ -1135,55 +1140,55 @@ static void inner_cg(dsv_mesh* m, Workspace* w, int S, const double* tol2, const
     h.max_it = max_it;
     h.any_active = 1;
     write_state(w, h, st);
-    launch_update<T, true>(S, m, w, 1, st);  // INIT = U_{-1}
+    launch_update<T, S, true>(m, w, 1, st);  // INIT = U_{-1}
     DSV_HIP(hipGetLastError());
-    run_cg<T>(m, w, S, max_it, st);
+    run_cg<T, S>(m, w, max_it, st);
 }
 
-static void solve_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt, int lumped,
-                       int method, double rtol, int max_it, hipStream_t user, dsv_stats* stats) {
-    if (method < 0 || method > 2) throw ArgError{"unknown method"};
-    if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
-    if (max_it < 1) throw ArgError{"max_it must be >= 1"};
-    check_solve_args(m, S, sp, true);
-    DSV_HIP(hipSetDevice(m->device));
-    DSV_HIP(hipStreamSynchronize(user));
+template <int S>
+static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped, int method,
+                       double rtol, int max_it, dsv_stats* stats) {
     Workspace* w = get_ws(m, method, S);
     hipStream_t st = w->stream;
     const auto t0 = std::chrono::steady_clock::now();
     const bool internal = method != DSV_METHOD_CSR64;
+    const int32_t* perm = internal ? m->perm : nullptr;
     const unsigned gv = cdiv(m->V, 256);
-    for (int s = 0; s < S; ++s) {
-        if (internal) {
-            hipLaunchKernelGGL((k_gather_perm<double, double>), dim3(gv), dim3(256), 0, st, m->V, m->perm, sp[s].b, w->b64[s]);
-            hipLaunchKernelGGL((k_gather_perm<double, double>), dim3(gv), dim3(256), 0, st, m->V, m->perm, (const double*)sp[s].x, w->x64[s]);
-        } else {
-            DSV_HIP(hipMemcpyAsync(w->b64[s], sp[s].b, m->V * sizeof(double), hipMemcpyDeviceToDevice, st));
-            DSV_HIP(hipMemcpyAsync(w->x64[s], sp[s].x, m->V * sizeof(double), hipMemcpyDeviceToDevice, st));
-        }
+    const double* bs[3] = {sp[0].b, S > 1 ? sp[1].b : nullptr, S > 2 ? sp[2].b : nullptr};
+    const double* xs[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
+    hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
+                       bs[2], (SV<double, S>*)w->b64);
+    hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0], xs[1],
+                       xs[2], (SV<double, S>*)w->x64);
+    for (int s = 0; s < S; ++s)
         if (sp[s].n_bc > 0)
-            hipLaunchKernelGGL(k_set_bc_values, dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st, sp[s].n_bc,
-                               sp[s].bc_dofs, internal ? m->iperm : (const int32_t*)nullptr, sp[s].bc_value,
-                               w->x64[s]);
-    }
+            hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
+                               sp[s].n_bc, sp[s].bc_dofs, internal ? m->iperm : (const int32_t*)nullptr,
+                               s, sp[s].bc_value, (SV<double, S>*)w->x64);
     // operator set-up: assembled values / coefficients and the Jacobi diagonal
     if (method == DSV_METHOD_CSR64) {
         for (int s = 0; s < S; ++s) {
             assemble_into(m, sp[s], inv_dt, lumped, w->csr[s], st);
-            hipLaunchKernelGGL(k_diag_inv_csr, dim3(gv), dim3(256), 0, st, m->V, m->diag_pos, w->csr[s],
-                               (double*)w->dinv[s]);
+            hipLaunchKernelGGL((k_diag_inv_csr<S>), dim3(gv), dim3(256), 0, st, m->V, m->diag_pos,
+                               w->csr[s], s, (SV<double, S>*)w->dinv);
         }
+        if (S == 3)  // padding lane of the records
+            hipLaunchKernelGGL((k_diag_inv_csr<S>), dim3(gv), dim3(256), 0, st, m->V, m->diag_pos,
+                               w->csr[0], 3, (SV<double, S>*)w->dinv);
     } else if (method == DSV_METHOD_MF64) {
-        prepare_op<double>(m, w, S, sp, inv_dt, lumped, true, st);
-        Vecs<double> v{};
-        for (int s = 0; s < MAXS; ++s) v.y[s] = (double*)w->dinv[s < S ? s : 0];
-        launch_mf_s<double, MF_DIAG>(S, m, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
+        const Partition& P = m->part;
+        prepare_bc(m, w, S, sp, st);
+        prepare_op<double, S>(m, w, P, sp, inv_dt, lumped, st, 0);
+        MFVec<double, S> v{};
+        v.y = (SV<double, S>*)w->dinv;
+        launch_mf<double, S, MF_DIAG>(m, P, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
     } else {
-        prepare_op<float>(m, w, S, sp, inv_dt, lumped, true, st);
-        prepare_op<double>(m, w, S, sp, inv_dt, lumped, true, st, true);
-        Vecs<float> v{};
-        for (int s = 0; s < MAXS; ++s) v.y[s] = (float*)w->dinv[s < S ? s : 0];
-        launch_mf_s<float, MF_DIAG>(S, m, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
+        prepare_bc(m, w, S, sp, st);
+        prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
+        prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1);
+        MFVec<float, S> v{};
+        v.y = (SV<float, S>*)w->dinv;
+        launch_mf<float, S, MF_DIAG>(m, m->part, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
     }
     DSV_HIP(hipGetLastError());
 
@@ -1193,7 +1198,7 @@ static void solve_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt,
     const int max_rounds = method == DSV_METHOD_MF32_IR64 ? m->max_outer : 4;
     for (int round = 0;; ++round) {
         double rr[MAXS] = {0, 0, 0}, bb[MAXS] = {0, 0, 0};
-        residual64(m, w, S, st, rr, bb);
+        residual64<S>(m, w, st, rr, bb);
         bool all = true;
         for (int s = 0; s < S; ++s) {
             const double bn = std::sqrt(bb[s]), rn = std::sqrt(rr[s]);
@@ -1213,8 +1218,9 @@ static void solve_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt,
         for (int s = 0; s < S; ++s) {
             if (conv[s] || dead[s]) continue;
             const double rn = std::sqrt(rr[s]);
-            double t = method == DSV_METHOD_MF32_IR64 ? std::max(m->inner_rtol * rn, 0.3 * tol_abs[s])
-                                                      : 0.9 * tol_abs[s];
+            const double t = method == DSV_METHOD_MF32_IR64
+                                 ? std::max(m->inner_rtol * rn, 0.3 * tol_abs[s])
+                                 : 0.9 * tol_abs[s];
             tol2[s] = t * t;
             act[s] = 1;
             mask |= 1 << s;
@@ -1225,11 +1231,13 @@ static void solve_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt,
         for (int s = 0; s < S; ++s)
             if (act[s]) budget = std::min(budget, max_it - iters_total[s]);
         if (method == DSV_METHOD_MF32_IR64) {
-            inner_cg<float>(m, w, S, tol2, act, std::max(1, budget), st);
-            add_correction<float>(S, m, w, mask, st);
+            inner_cg<float, S>(m, w, tol2, act, std::max(1, budget), st);
+            hipLaunchKernelGGL((k_add_correction<float, S>), dim3(gv), dim3(256), 0, st, m->V,
+                               (SV<double, S>*)w->x64, (const SV<float, S>*)w->xcg, mask);
         } else {
-            inner_cg<double>(m, w, S, tol2, act, std::max(1, budget), st);
-            add_correction<double>(S, m, w, mask, st);
+            inner_cg<double, S>(m, w, tol2, act, std::max(1, budget), st);
+            hipLaunchKernelGGL((k_add_correction<double, S>), dim3(gv), dim3(256), 0, st, m->V,
+                               (SV<double, S>*)w->x64, (const SV<double, S>*)w->xcg, mask);
         }
         DSV_HIP(hipGetLastError());
         // the state's iteration counters restart in every inner solve; accumulate them
@@ -1240,13 +1248,10 @@ static void solve_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt,
             if (h2.status[s] == ST_BREAKDOWN) dead[s] = 1;
         }
     }
-    for (int s = 0; s < S; ++s) {
-        if (internal)
-            hipLaunchKernelGGL((k_scatter_perm<double, double>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
-                               (const double*)w->x64[s], sp[s].x);
-        else
-            DSV_HIP(hipMemcpyAsync(sp[s].x, w->x64[s], m->V * sizeof(double), hipMemcpyDeviceToDevice, st));
-    }
+    double* xo[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
+    hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm,
+                       (const SV<double, S>*)w->x64, xo[0], xo[1], xo[2]);
+    DSV_HIP(hipGetLastError());
     DSV_HIP(hipStreamSynchronize(st));
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1263,54 +1268,52 @@ static void solve_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt,
     }
 }
 
-template <class T>
-static void apply_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt, int lumped,
-                       const double* const* x, double* const* y, hipStream_t user) {
+template <class T, int S>
+static void apply_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped,
+                       const double* const* x, double* const* y) {
     const int method = sizeof(T) == 4 ? DSV_METHOD_MF32_IR64 : DSV_METHOD_MF64;
     Workspace* w = get_ws(m, method, S);
     hipStream_t st = w->stream;
-    DSV_HIP(hipStreamSynchronize(user));
-    prepare_op<T>(m, w, S, sp, inv_dt, lumped, true, st);
+    const Partition& P = m->part;
+    prepare_bc(m, w, S, sp, st);
+    prepare_op<T, S>(m, w, P, sp, inv_dt, lumped, st, 0);
     const unsigned gv = cdiv(m->V, 256);
-    Vecs<T> v{};
-    for (int s = 0; s < MAXS; ++s) {
-        const int q = s < S ? s : 0;
-        v.x[s] = (T*)w->zcg[q];
-        v.y[s] = (T*)w->qcg[q];
-    }
-    for (int s = 0; s < S; ++s)
-        hipLaunchKernelGGL((k_gather_perm<double, T>), dim3(gv), dim3(256), 0, st, m->V, m->perm, x[s], v.x[s]);
-    launch_mf_s<T, MF_APPLY>(S, m, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
-    for (int s = 0; s < S; ++s)
-        hipLaunchKernelGGL((k_scatter_perm<T, double>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
-                           (const T*)v.y[s], y[s]);
+    MFVec<T, S> v{};
+    v.x = (const SV<T, S>*)w->zcg;
+    v.y = (SV<T, S>*)w->qcg;
+    hipLaunchKernelGGL((k_pack<T, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, x[0],
+                       S > 1 ? x[1] : nullptr, S > 2 ? x[2] : nullptr, (SV<T, S>*)w->zcg);
+    launch_mf<T, S, MF_APPLY>(m, P, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
+    hipLaunchKernelGGL((k_unpack<T, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
+                       (const SV<T, S>*)w->qcg, y[0], S > 1 ? y[1] : nullptr,
+                       S > 2 ? y[2] : nullptr);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipStreamSynchronize(st));
 }
 
 // Times the fused kernel on `stream`: mode 0 = apply, mode 1 = CG form (A_k, parity 0, with
 // every species active and beta = 0 so that repeated launches are identical).
-template <class T>
-static double time_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt, int lumped,
-                        int mode, int n_rep, hipStream_t stream) {
+template <class T, int S>
+static double time_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped, int mode,
+                        int n_rep, hipStream_t stream) {
     const int method = sizeof(T) == 4 ? DSV_METHOD_MF32_IR64 : DSV_METHOD_MF64;
     Workspace* w = get_ws(m, method, S);
-    DSV_HIP(hipStreamSynchronize(stream));
-    prepare_op<T>(m, w, S, sp, inv_dt, lumped, true, w->stream);
+    hipStream_t ws = w->stream;
+    const Partition& P = m->part;
+    prepare_bc(m, w, S, sp, ws);
+    prepare_op<T, S>(m, w, P, sp, inv_dt, lumped, ws, 0);
     const unsigned gv = cdiv(m->V, 256);
-    for (int s = 0; s < S; ++s) {
-        // z = x (internal order) when x is given, else the lumped mass; p_old = 0
-        if (sp[s].x)
-            hipLaunchKernelGGL((k_gather_perm<double, T>), dim3(gv), dim3(256), 0, w->stream, m->V, m->perm,
-                               (const double*)sp[s].x, (T*)w->zcg[s]);
-        else
-            hipLaunchKernelGGL((k_convert<double, T>), dim3(gv), dim3(256), 0, w->stream, (int64_t)m->V,
-                               m->lumped_m, (T*)w->zcg[s]);
-        DSV_HIP(hipMemsetAsync(w->pa[s], 0, m->V * sizeof(T), w->stream));
-    }
+    // z = x (internal order) when every x is given, else the lumped mass; p_old = 0
+    const bool have_x = sp[0].x && (S < 2 || sp[1].x) && (S < 3 || sp[2].x);
+    const double* x0 = have_x ? sp[0].x : m->lumped_m;
+    const double* x1 = S > 1 ? (have_x ? sp[1].x : m->lumped_m) : nullptr;
+    const double* x2 = S > 2 ? (have_x ? sp[2].x : m->lumped_m) : nullptr;
+    hipLaunchKernelGGL((k_pack<T, S>), dim3(gv), dim3(256), 0, ws, m->V, m->perm, x0, x1, x2,
+                       (SV<T, S>*)w->zcg);
+    DSV_HIP(hipMemsetAsync(w->pa, 0, m->V * rec_bytes(sizeof(T), S), ws));
     // state: every species active before A_0 (act[1]), k = 0, tolerance 0; the U partials
     // the kernel consumes hold r.r = 1 per species
-    CGState h = read_state(w, w->stream);
+    CGState h = read_state(w, ws);
     for (int s = 0; s < MAXS; ++s) {
         h.act[1][s] = s < S ? 1 : 0;
         h.tol2[s] = 0.0;
@@ -1318,17 +1321,17 @@ static double time_impl(dsv_mesh* m, int S, const dsv_species* sp, double inv_dt
     }
     h.k = 0;
     h.max_it = 1 << 30;
-    write_state(w, h, w->stream);
+    write_state(w, h, ws);
     std::vector<double> pu((size_t)2 * MAXS * MAX_PROD, 0.0);
     const int nU = upd_grid(m);
     for (int s = 0; s < S; ++s) pu[(size_t)(S + s) * nU] = 1.0;
     DSV_HIP(hipMemcpy(w->pU[1], pu.data(), pu.size() * sizeof(double), hipMemcpyHostToDevice));
-    const Vecs<T> v = cg_vecs<T>(w, S, 0);
+    const MFVec<T, S> v = cg_vecs<T, S>(w, 0);
     auto launch = [&] {
         if (mode == 1)
-            launch_mf_s<T, MF_CG>(S, m, w->prm, v, w->st, w->pU[1], nU, w->pA, 0, stream);
+            launch_mf<T, S, MF_CG>(m, P, w->prm, v, w->st, w->pU[1], nU, w->pA, 0, stream);
         else
-            launch_mf_s<T, MF_APPLY>(S, m, w->prm, v, w->st, nullptr, 0, nullptr, 0, stream);
+            launch_mf<T, S, MF_APPLY>(m, P, w->prm, v, w->st, nullptr, 0, nullptr, 0, stream);
     };
     for (int i = 0; i < 3; ++i) launch();
     hipEvent_t e0, e1;
@@ -1362,10 +1365,15 @@ int dsv_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
             if (!x[s] || !y[s]) throw ArgError{"dsv_apply: NULL x[s] or y[s]"};
         if (precision != 32 && precision != 64) throw ArgError{"precision must be 32 or 64"};
         DSV_HIP(hipSetDevice(m->device));
-        if (precision == 32)
-            apply_impl<float>(m, n_species, sp, inv_dt, lumped, x, y, (hipStream_t)stream);
-        else
-            apply_impl<double>(m, n_species, sp, inv_dt, lumped, x, y, (hipStream_t)stream);
+        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        switch (n_species * 100 + precision) {
+            case 132: apply_impl<float, 1>(m, sp, inv_dt, lumped, x, y); break;
+            case 232: apply_impl<float, 2>(m, sp, inv_dt, lumped, x, y); break;
+            case 332: apply_impl<float, 3>(m, sp, inv_dt, lumped, x, y); break;
+            case 164: apply_impl<double, 1>(m, sp, inv_dt, lumped, x, y); break;
+            case 264: apply_impl<double, 2>(m, sp, inv_dt, lumped, x, y); break;
+            default: apply_impl<double, 3>(m, sp, inv_dt, lumped, x, y); break;
+        }
     });
 }
 
@@ -1373,7 +1381,17 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
               int32_t lumped, int32_t method, double rtol, int32_t max_it, void* stream,
               dsv_stats* stats) {
     return guard([&] {
-        solve_impl(m, n_species, sp, inv_dt, lumped, method, rtol, max_it, (hipStream_t)stream, stats);
+        if (method < 0 || method > 2) throw ArgError{"unknown method"};
+        if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
+        if (max_it < 1) throw ArgError{"max_it must be >= 1"};
+        check_solve_args(m, n_species, sp, true);
+        DSV_HIP(hipSetDevice(m->device));
+        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        switch (n_species) {
+            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
+            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
+            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
+        }
     });
 }
 
@@ -1401,10 +1419,18 @@ int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         check_solve_args(m, n_species, sp, false);
         if (!ms || n_rep < 1) throw ArgError{"dsv_time_apply: bad arguments"};
         if (precision != 32 && precision != 64) throw ArgError{"precision must be 32 or 64"};
+        if (mode != 0 && mode != 1) throw ArgError{"mode must be 0 (apply) or 1 (CG form)"};
         DSV_HIP(hipSetDevice(m->device));
-        *ms = precision == 32
-                  ? time_impl<float>(m, n_species, sp, inv_dt, lumped, mode, n_rep, (hipStream_t)stream)
-                  : time_impl<double>(m, n_species, sp, inv_dt, lumped, mode, n_rep, (hipStream_t)stream);
+        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        hipStream_t s = (hipStream_t)stream;
+        switch (n_species * 100 + precision) {
+            case 132: *ms = time_impl<float, 1>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
+            case 232: *ms = time_impl<float, 2>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
+            case 332: *ms = time_impl<float, 3>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
+            case 164: *ms = time_impl<double, 1>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
+            case 264: *ms = time_impl<double, 2>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
+            default: *ms = time_impl<double, 3>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
+        }
     });
 }
 

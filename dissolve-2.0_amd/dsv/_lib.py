@@ -92,7 +92,7 @@ def load():
         "dsv_mesh_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int64)]),
         "dsv_mesh_view": (ctypes.c_int, [P, I32, ctypes.POINTER(P)]),
         "dsv_mesh_copy_view": (ctypes.c_int, [P, I32, P, P]),
-        "dsv_mesh_set_partition": (ctypes.c_int, [P, I32, I32]),
+        "dsv_mesh_set_partition": (ctypes.c_int, [P, I32, I32, I32]),
         "dsv_assemble": (ctypes.c_int, [P, ctypes.POINTER(Species), D, I32, P, P]),
         "dsv_rhs": (ctypes.c_int, [P, P, P, I32, D, I32, P, I32, D, P, P]),
         "dsv_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, ctypes.POINTER(P),

@@ -99,8 +99,9 @@ class Mesh:
         self._views[which] = t
         return t
 
-    def set_partition(self, vertices_per_block, block_threads):
-        _lib.call("dsv_mesh_set_partition", self._h, int(vertices_per_block), int(block_threads))
+    def set_partition(self, vertices_per_block, block_threads, lanes_per_vertex=4):
+        _lib.call("dsv_mesh_set_partition", self._h, int(vertices_per_block), int(block_threads),
+                  int(lanes_per_vertex))
         self.refresh_info()
 
     def set_solver_options(self, inner_rtol=1e-3, max_outer=12, graph_chunk=8, use_graphs=True):

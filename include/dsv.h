@@ -74,13 +74,15 @@ int dsv_device_count(int32_t* n);
 
 /* Copies conn (C x 4, caller numbering) and xyz (V x 3) to the current device, checks the
  * indices and that no cell is degenerate, and builds the CSR pattern, the scatter map, the
- * per-cell geometry and the matrix-free partition.  vertices_per_block <= 0 picks the default. */
+ * per-cell geometry and the matrix-free partition.  vertices_per_block: a multiple of 64 in
+ * [64, 1024] whose blocks have at most 1024 local (owned + halo) vertices, or <= 0 for the
+ * largest of 256, 128, 64 that satisfies this. */
 int dsv_mesh_create(dsv_mesh** out, int32_t n_cells, int32_t n_verts, const int32_t* conn,
                     const double* xyz, int32_t vertices_per_block);
 int dsv_mesh_destroy(dsv_mesh* m);
 /* sizes (8 values): n_cells, n_verts, nnz, number of matrix-free blocks, vertices per block,
- * number of (block, cell) entries of the partition, total halo vertices, max local vertices
- * of a block */
+ * number of incidence-list entries of the partition (sliced ELL, padding included), total
+ * halo vertices, max local vertices of a block */
 int dsv_mesh_info(const dsv_mesh* m, int64_t* info8);
 
 /* Device views owned by the mesh (valid until dsv_mesh_destroy): the attributes
@@ -92,9 +94,13 @@ int dsv_mesh_view(const dsv_mesh* m, int32_t which, void** ptr);
 /* Copies view `which` (see dsv_mesh_view) into the caller's device buffer dst. */
 int dsv_mesh_copy_view(const dsv_mesh* m, int32_t which, void* dst, void* stream);
 
-/* Repartitions the matrix-free blocks (vertices_per_block) and sets the workgroup size of
- * the matrix-free kernels; used by the autotuner. */
-int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t block_threads);
+/* Repartitions the matrix-free blocks (vertices_per_block, constraints as in
+ * dsv_mesh_create; DSV_E_ARG leaves the previous partition in place), the lanes that share
+ * one vertex's incident cells (lanes_per_vertex: 1, 2 or 4; default 4) and sets the
+ * workgroup size (64, 128, 256, 512 or 1024) of the matrix-free kernels; used by the
+ * autotuner.  No counterpart in the reference (launch-shape tuning of this library). */
+int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t block_threads,
+                           int32_t lanes_per_vertex);
 
 /* --- assembly: GPUSparseAssembler.assemble (`gpu/gpu_assembler.py:472-532`) ---
  * values (device, nnz, pattern order) = A = (inv_dt + alpha_e) M_e + De_e K_e summed over

@@ -142,6 +142,33 @@ def test_matrix_free_apply_three_species(torch_dev, name, lumped):
             assert _relerr(y.cpu().numpy(), ref) < tol, (prec, name)
 
 
+@pytest.mark.parametrize("nv,threads,lanes", [(64, 64, 1), (64, 256, 2), (128, 128, 4),
+                                              (128, 1024, 4), (256, 512, 1), (256, 256, 4)])
+def test_matrix_free_apply_launch_shapes(torch_dev, nv, threads, lanes):
+    """Every (block size, workgroup size, lanes per vertex) computes the same operator: slices
+    with padded lanes, workgroups that loop over several slices and the cross-lane sums."""
+    from dsv.ceed_operators import apply_batched
+    m, conn, xyz = _m("disc_small")
+    m.set_partition(nv, threads, lanes)
+    assert m.vertices_per_block == nv
+    rng = np.random.default_rng(9)
+    C, V = len(conn), len(xyz)
+    species = [dict(D=rng.lognormal(0, 0.5, C), alpha=rng.uniform(0, 3, V),
+                    bc_dofs=np.arange(0, V, 7, dtype=np.int32)),
+               dict(D_value=2.0, alpha_value=1.5),
+               dict(D=rng.uniform(1, 5, V))]
+    xs = [rng.uniform(-1, 1, V) for _ in species]
+    for prec, tol in ((64, 1e-12), (32, 2e-5)):
+        ys = apply_batched(m, species, 200.0, xs, lumped=1, precision=prec)
+        for s, x, y in zip(species, xs, ys):
+            ref = _species_csr(conn, xyz, s, 200.0, 1) @ x
+            assert _relerr(y.cpu().numpy(), ref) < tol, (prec, nv, threads, lanes)
+    with pytest.raises(ValueError):
+        m.set_partition(96, 256, 4)     # not a multiple of 64
+    with pytest.raises(ValueError):
+        m.set_partition(128, 256, 3)    # lanes per vertex must be 1, 2 or 4
+
+
 def _solve_case(name, lumped=1, seed=6):
     conn, xyz = CASES[name]
     rng = np.random.default_rng(seed)
