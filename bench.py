@@ -4,7 +4,7 @@ One step = RHS assembly + batched solve of O2, Cl-, OH- to ||b - A x|| <= 1e-8 |
 true residual).  Two GPU paths run on the same trajectory from the same initial state:
   plain  -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64)
   chosen -- FP32 matrix-free fused 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64)
-`value` is the chosen path's steps/s.  The roofline entry is the fused FP32 3-species CG apply
+`value` is the chosen path's steps/s.  The roofline entry is the fused FP32 3-species CG iteration
 kernel (the solve's dominant kernel), timed with HIP events on its own stream; the CPU baseline
 is the oracle (numpy assembly + C OpenMP Jacobi-PCG) on one 3-species step of the same mesh.
 
@@ -27,22 +27,22 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def apply_bytes(C, V, species):
-    """Algorithmic bytes of one fused FP32 apply: conn 16C + coords 12V + per species
-    x in / y out 8V + coefficient bytes (4C per-cell, 4V per-vertex, 0 scalar)."""
-    b = 16 * C + 12 * V
-    for s in species:
-        b += 8 * V
-        for key in ("D", "alpha"):
-            a = s.get(key)
-            if a is not None:
-                b += 4 * len(a)
-    return b
+def cg_iteration_bytes(C, V, n_species=3, word=4):
+    """Algorithmic bytes of one fused matrix-free CG iteration (FP32 words): the incidence
+    lists (one 4-byte word per (cell, corner), 16C), coordinates and the folded mass diagonal
+    (3 + S words per vertex), and the species vectors -- dinv, r, w, s, p, x read and r, w, s,
+    p, x written (11 S words per vertex)."""
+    return 16 * C + word * V * (3 + n_species + 11 * n_species)
+
+
+def apply_bytes(C, V, n_species=3, word=4):
+    """Algorithmic bytes of one matrix-free apply: incidence lists 16C, coordinates, mass
+    diagonal, x read and y written."""
+    return 16 * C + word * V * (3 + 3 * n_species)
 
 
 def load_workload(name):
     from dsv import workloads
-    cache = os.path.join("/tmp", f"dsv_{name}.npz")
     if name == "disc3":
         return workloads.disc3()
     if name == "cube_jitter":
@@ -95,6 +95,43 @@ def cpu_baseline(w, threads):
     return time.perf_counter() - t0
 
 
+def time_kernels(mesh, w, reps, stream=None):
+    """(ms per fused CG iteration, ms per apply) of the FP32 3-species kernels, HIP events
+    around `reps` back-to-back launches on one stream."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import species_array, species_struct
+    keep = []
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), bc_dofs=s.get("bc_dofs"),
+                              bc_value=s.get("bc_value", 0.0), keep=keep) for s in w["species"]]
+    arr = species_array(structs)
+    stream = stream or torch.cuda.Stream()
+    out = []
+    for mode in (1, 0):
+        ms = ctypes.c_double()
+        _lib.call("dsv_time_apply", mesh.handle, len(structs), arr, 1.0 / w["dt"], 1, 32, mode,
+                  reps, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ms))
+        out.append(ms.value)
+    return out[0], out[1]
+
+
+def profile_run(mesh, w, reps):
+    import torch
+    n = 64 * 1024 * 1024  # 256 MiB of float32
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy reference: 3 x {4 * n} bytes read and written")
+    ms_iter, ms_apply = time_kernels(mesh, w, reps)
+    torch.cuda.synchronize()
+    log(f"fused CG iteration {ms_iter * 1e3:.1f} us, apply {ms_apply * 1e3:.1f} us "
+        f"({reps} launches each plus 3 warm-up)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,15 +141,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="kernel-only run for rocprofv3: a 256 MiB device copy (the counter "
+                         "reference), then the fused CG iteration and the apply, --apply-reps "
+                         "launches each, on the default launch shape")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
-    import torch
-    from dsv import _lib
-    from dsv.core import Mesh, species_array, species_struct, stream_ptr
+    from dsv.core import Mesh
 
     t = time.perf_counter()
     w = load_workload(a.workload)
@@ -123,6 +161,9 @@ def main():
     log(f"device mesh: nnz={mesh.nnz} blocks={mesh.n_blocks} entries={mesh.n_entries} "
         f"({time.perf_counter() - t:.1f}s)")
     rtol = w["rtol"]
+    if a.profile_run:
+        profile_run(mesh, w, a.apply_reps)
+        return
     tuned = None
     if not a.no_tune:
         from dsv.autotune import tune
@@ -139,25 +180,13 @@ def main():
         log(f"{method}: {r['steps_per_s']:.2f} steps/s, iterations/step {r['iterations_per_step'][-1]}, "
             f"converged={r['converged']}")
 
-    # roofline of the fused FP32 3-species CG apply (dominant kernel of the chosen path)
-    keep = []
-    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
-                              alpha=s.get("alpha"), bc_dofs=s.get("bc_dofs"),
-                              bc_value=s.get("bc_value", 0.0), keep=keep) for s in w["species"]]
-    arr = species_array(structs)
-    stream = torch.cuda.Stream()
-    import ctypes
-    ms = ctypes.c_double()
-    _lib.call("dsv_time_apply", mesh.handle, len(structs), arr, 1.0 / w["dt"], 1, 32, 1,
-              a.apply_reps, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ms))
-    ms_apply = ms.value
-    _lib.call("dsv_time_apply", mesh.handle, len(structs), arr, 1.0 / w["dt"], 1, 32, 0,
-              a.apply_reps, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ms))
-    ms_plain_apply = ms.value
-    nbytes = apply_bytes(C, V, w["species"])
-    achieved = nbytes / (ms_apply * 1e-3) / 1e9
-    log(f"fused FP32 CG apply: {ms_apply * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
-        f"(plain apply {ms_plain_apply * 1e3:.1f} us)")
+    # roofline of the fused FP32 3-species CG iteration (the chosen path's dominant kernel)
+    ms_iter, ms_plain_apply = time_kernels(mesh, w, a.apply_reps)
+    nbytes = cg_iteration_bytes(C, V, len(w["species"]))
+    achieved = nbytes / (ms_iter * 1e-3) / 1e9
+    log(f"fused FP32 CG iteration: {ms_iter * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
+        f"(matrix-free apply alone {ms_plain_apply * 1e3:.1f} us, "
+        f"{apply_bytes(C, V, len(w['species'])) / (ms_plain_apply * 1e-3) / 1e9:.0f} GB/s)")
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -189,8 +218,9 @@ def main():
         "converged": chosen["converged"] and plain["converged"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_mf<float,3,MF_CG> (fused 3-species FP32 CG apply)",
-                     "bytes_per_launch": nbytes, "us_per_launch": round(ms_apply * 1e3, 2)},
+                     "kernel": "k_cg<float,3> (fused 3-species FP32 Jacobi-PCG iteration)",
+                     "bytes_per_launch": nbytes, "us_per_launch": round(ms_iter * 1e3, 2),
+                     "apply_us_per_launch": round(ms_plain_apply * 1e3, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

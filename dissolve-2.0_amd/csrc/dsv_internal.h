@@ -89,6 +89,8 @@ struct Partition {
     int64_t n_halo = 0;
     int32_t* hptr = nullptr;      // nb+1 halo ranges
     int32_t* hlist = nullptr;     // halo vertices (internal ids)
+    int32_t* lvl = nullptr;       // nb*max_local: local -> internal vertex (owned, halo, padding
+                                  // repeats the block's first vertex)
     uint16_t* lperm = nullptr;    // nb*NV: degree rank -> owned local index
     int32_t* sptr = nullptr;      // nb*(NV*L/64)+1 slice offsets (multiples of 64)
     uint32_t* ell = nullptr;      // packed corner triples
@@ -128,7 +130,7 @@ struct dsv_mesh {
     // solver options
     double inner_rtol = 1e-3;
     int max_outer = 12;
-    int graph_chunk = 8;
+    int graph_chunk = 16;
     int use_graphs = 1;
     dsv::Workspace* ws[3][4] = {};  // [method][n_species]
 };

@@ -158,6 +158,7 @@ static void morton_order(const double* xyz_aos, int32_t V, std::vector<int32_t>&
 void free_partition(Partition& p) {
     dfree(p.hptr);
     dfree(p.hlist);
+    dfree(p.lvl);
     dfree(p.lperm);
     dfree(p.sptr);
     dfree(p.ell);
@@ -183,6 +184,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
     if (NV < 64 || NV > 1024 || NV % 64 != 0)
         throw ArgError{"vertices_per_block must be a multiple of 64 in [64, 1024]"};
     if (L != 1 && L != 2 && L != 4) throw ArgError{"lanes_per_vertex must be 1, 2 or 4"};
+    if (NV * L > 1024) throw ArgError{"vertices_per_block * lanes_per_vertex must be <= 1024"};
     const int32_t C = m->C, V = m->V;
     const int nb = (V + NV - 1) / NV;
     const int vps = 64 / L;        // vertices per slice
@@ -268,6 +270,15 @@ void build_partition(dsv_mesh* m, int NV, int L) {
         if ((int64_t)hlist.size() >= (int64_t)INT32_MAX) throw ArgError{"halo lists too large"};
         hptr[b + 1] = (int32_t)hlist.size();
     }
+    // padded local vertex lists: one load per staged vertex, no dependence on the halo ranges
+    std::vector<int32_t> lvl((size_t)nb * max_local);
+    for (int b = 0; b < nb; ++b) {
+        const int v0 = b * NV, nv = std::min(NV, V - v0);
+        const int nh = hptr[b + 1] - hptr[b];
+        int32_t* row = lvl.data() + (size_t)b * max_local;
+        for (int i = 0; i < max_local; ++i)
+            row[i] = i < nv ? v0 + i : (i < nv + nh ? hlist[hptr[b] + i - nv] : v0);
+    }
     Partition p;
     p.NV = NV;
     p.L = L;
@@ -279,6 +290,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
     try {
         p.hptr = upload(hptr);
         p.hlist = upload(hlist);
+        p.lvl = upload(lvl);
         p.lperm = upload(lperm);
         p.sptr = upload(sptr);
         p.ell = upload(ell);
@@ -551,9 +563,8 @@ int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t bloc
                            int32_t lanes_per_vertex) {
     return guard([&] {
         if (!m) throw ArgError{"dsv_mesh_set_partition: NULL mesh"};
-        if (block_threads != 64 && block_threads != 128 && block_threads != 256 &&
-            block_threads != 512 && block_threads != 1024)
-            throw ArgError{"block_threads must be 64, 128, 256, 512 or 1024"};
+        if (block_threads != 256 && block_threads != 512)
+            throw ArgError{"block_threads must be 256 or 512"};
         free_workspaces(m);  // workspaces and captured graphs depend on the launch shape
         if (vertices_per_block != m->part.NV || lanes_per_vertex != m->part.L)
             build_partition(m, vertices_per_block, lanes_per_vertex);

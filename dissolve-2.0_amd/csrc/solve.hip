@@ -1,4 +1,4 @@
-// Matrix-free multi-species apply and the batched Jacobi-PCG solver.
+// Matrix-free multi-species apply and the batched Jacobi-PCG solvers.
 //
 // Replaces CeedDiffusionMassOp.apply (`gpu/ceed_operators.py:341-343`, whose QFunction is a
 // different operator -- the apply here reproduces the ASSEMBLED operator of
@@ -11,21 +11,32 @@
 // a vertex.  Coordinates are (x, y, z, 0) records.
 //
 // Matrix-free apply (internal Morton vertex numbering), one workgroup per partition block:
-//   (1) stage the block's local vertices (owned range, then halo) in LDS: coordinates, the
-//       species input (x, or p = z + beta p_old in CG form) and per-vertex coefficients;
-//   (2) one lane per owned vertex walks the vertex's incident cells (sliced-ELL list of the
-//       other three corners' local indices), recomputes each cell's geometry from LDS and
-//       accumulates the vertex's row of the cell operator for every species in registers --
-//       no atomics, no second barrier, a fixed summation order -- and writes the row
-//       (identity rows at Dirichlet vertices).  The 8 XCDs each take a contiguous run of
-//       partition blocks, which keeps halo reads in the XCD's own L2.
+//   (1) stage the block's local vertices (owned range, then halo) in LDS: coordinates and the
+//       species input;
+//   (2) L lanes per owned vertex walk the vertex's incident cells (sliced-ELL lists of the
+//       other three corners' local indices), recompute each cell's geometry from LDS and
+//       accumulate the vertex's row of the cell operator for every species in registers --
+//       no atomics, one barrier, a fixed summation order -- then write the row (identity
+//       rows at Dirichlet vertices).  The 8 XCDs each take a contiguous run of partition
+//       blocks, which keeps halo reads in the XCD's own L2.
 //
-// CG: per iteration two kernels -- (A) the apply in CG form, writing p and q = A p and block
-// partials of p.q; (U) the vertex update x += alpha p, r -= alpha q, z = r/diag with partials
-// of r.z and r.r.  No kernel uses a fence or a global atomic: every workgroup of a consumer
-// kernel reduces ALL partials of the producer kernel in the same fixed order, so all derive
-// identical alpha / beta / convergence decisions; workgroup 0 records them for the host.
-// Chunks of iterations are captured in a hipGraph and replayed until every species is done.
+// Matrix-free CG (methods MF64 and MF32_IR64): Jacobi-PCG in the Chronopoulos-Gear form, ONE
+// kernel per iteration.  Iteration i reduces gamma_i = (r_i, u_i), delta_i = (w_i, u_i) and
+// rho_i = (r_i, r_i) from the previous kernel's block partials, then computes
+//   beta = gamma_i / gamma_{i-1}, alpha = gamma_i / (delta_i - beta gamma_i / alpha_{i-1}),
+//   s_i = w_i + beta s_{i-1}, r_{i+1} = r_i - alpha s_i, u_{i+1} = r_{i+1} / diag(A)
+// for every local vertex while staging it (halo copies are recomputed from the same inputs,
+// bit-identical to the owner's), updates p_i = u_i + beta p_{i-1} and x_{i+1} = x_i + alpha p_i
+// for the owned ones, and applies w_{i+1} = A u_{i+1}, writing the partials of iteration i+1.
+// r, w and s alternate between two buffers so that halo reads never see this kernel's writes.
+// No kernel uses a fence or a global atomic: every workgroup reduces ALL partials of the
+// previous kernel in the same fixed order, so all derive identical alpha / beta / convergence
+// decisions; workgroup 0 records them for the host.  Chunks of iterations are captured in a
+// hipGraph and replayed until every species is done.
+//
+// CSR CG (method CSR64, the plain FP64 path): standard Jacobi-PCG with two kernels per
+// iteration, the CSR apply (A_k) and the vector update (U_k), with the same fence-free
+// reductions.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -44,24 +55,26 @@ void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped,
 constexpr int MAXS = 3;
 constexpr int MAX_PROD = 2048;  // max workgroups of a partial-producing kernel
 
-enum MFMode { MF_APPLY = 0, MF_DIAG = 1, MF_CG = 2, MF_RESID = 3, MF_MASS = 4 };
+enum MFMode { MF_APPLY = 0, MF_DIAG = 1, MF_RESID = 3, MF_MASS = 4 };
 
 // status codes in CGState::status
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2, ST_MAXIT = 3, ST_IDLE = 4 };
 
 // Written by workgroup 0 of the kernel named in the comment, read by later kernels / host.
+// Entries indexed [par] are written by the kernel with parity par and read by the next one.
 struct CGState {
     double tol2[MAXS];     // host
-    double rz[2][MAXS];    // A_k -> rz[k&1] = r_k.z_k
-    double rr[MAXS];       // A_k -> r_k.r_k (host)
-    int act[2][MAXS];      // A_k -> act[k&1]; host sets act[1] before INIT
-    int dead[MAXS];        // U_k (breakdown)
-    int iters[MAXS];       // U_k
-    int status[MAXS];      // A_k / U_k
-    int k;                 // U_k increments; host resets
+    double rz[2][MAXS];    // CSR A_k -> r_k.z_k ; MF K_i -> gamma_i
+    double alp[2][MAXS];   // MF K_i -> alpha_i
+    double rr[MAXS];       // A_k / K_i -> r.r (host)
+    int act[2][MAXS];      // A_k / K_i; host sets act[1] before the first iteration
+    int dead[MAXS];        // CSR U_k (breakdown)
+    int iters[MAXS];       // U_k / K_i
+    int status[MAXS];      // A_k / U_k / K_i
+    int k;                 // CSR: U_k increments; host resets
+    int kk[2];             // MF: K_i -> i + 1; host sets kk[1] = 0
     int max_it;            // host
-    int any_active;        // A_k (host polls)
-    int pad;
+    int any_active;        // A_k / K_i (host polls)
 };
 
 // species-interleaved vertex record (S = 3 padded to 4 for aligned wide accesses)
@@ -111,11 +124,51 @@ struct OpParams {
 // vectors of the apply-type kernels (vertex order of the kernel)
 template <class T, int S>
 struct MFVec {
-    const SV<T, S>* x;       // APPLY / RESID: x ; CG: z
-    SV<T, S>* y;             // APPLY: y ; DIAG: dinv ; CG: q ; RESID: r ; MASS: mass diagonal
-    const SV<T, S>* pold;    // CG
-    SV<T, S>* pnew;          // CG
+    const SV<T, S>* x;       // APPLY / RESID: x
+    SV<T, S>* y;             // APPLY: y ; DIAG: dinv ; RESID: r ; MASS: mass diagonal
     const SV<double, S>* b;  // RESID
+};
+
+// vectors of the matrix-free CG kernel (internal order); R, W, S alternate by parity
+template <class T, int S>
+struct CGVec {
+    const SV<T, S>* dinv;
+    SV<T, S>* R[2];
+    SV<T, S>* W[2];
+    SV<T, S>* Sd[2];
+    SV<T, S>* P;
+    SV<T, S>* X;               // correction
+    const SV<double, S>* rin;  // INIT: FP64 residual
+};
+
+// vectors of the CSR path (caller order)
+template <int S>
+struct CSRVec {
+    const SV<double, S>* x;     // A_k: z ; RESID: x
+    SV<double, S>* y;           // A_k: q ; RESID: r
+    const SV<double, S>* pold;  // A_k
+    SV<double, S>* pnew;        // A_k
+    const SV<double, S>* b;     // RESID
+};
+template <int S>
+struct UVec {
+    SV<double, S>* x;  // correction d
+    SV<double, S>* r;
+    SV<double, S>* z;
+    const SV<double, S>* dinv;
+    const SV<double, S>* p;    // p_k (written by A_k)
+    const SV<double, S>* q;
+    SV<double, S>* pzero;      // INIT: p_old of A_0, cleared
+    const SV<double, S>* rin;  // INIT: residual
+};
+
+struct Geo {
+    int V, NV, L, nb, per_xcd, max_local;
+    const int32_t* hptr;
+    const int32_t* lvl;
+    const uint16_t* lperm;
+    const int32_t* sptr;
+    const uint32_t* ell;
 };
 
 template <class T>
@@ -128,28 +181,6 @@ template <>
 __device__ __forceinline__ double recip<double>(double x) {
     return 1.0 / x;
 }
-
-// vectors of the update kernel
-template <class T, int S>
-struct UVec {
-    SV<T, S>* x;  // correction d
-    SV<T, S>* r;
-    SV<T, S>* z;
-    const SV<T, S>* dinv;
-    const SV<T, S>* p;         // p_k (written by A_k)
-    const SV<T, S>* q;
-    SV<T, S>* pzero;           // INIT: p_old of A_0, cleared
-    const SV<double, S>* rin;  // INIT: FP64 residual
-};
-
-struct Geo {
-    int V, NV, L, nb, per_xcd, max_local;
-    const int32_t* hptr;
-    const int32_t* hlist;
-    const uint16_t* lperm;
-    const int32_t* sptr;
-    const uint32_t* ell;
-};
 
 // Block sums of N values, result in EVERY thread (fixed order: lanes, then waves).
 template <int N>
@@ -173,21 +204,420 @@ __device__ __forceinline__ void block_sum_all(double (&v)[N], double* scratch) {
 }
 
 // Sum partials[q*nprod + b] over b < nprod for q < NQ; identical result in every thread of
-// every workgroup that calls it.
+// every workgroup that calls it.  All NQ loads of a step are issued together.
 template <int NQ>
 __device__ __forceinline__ void consume_partials(const double* __restrict__ partials, int nprod,
                                                  double (&out)[NQ], double* scratch) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nprod; b += blockDim.x) s += partials[q * nprod + b];
-        out[q] = s;
-    }
+    for (int q = 0; q < NQ; ++q) out[q] = 0.0;
+    for (int b = threadIdx.x; b < nprod; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) out[q] += partials[q * nprod + b];
     block_sum_all<NQ>(out, scratch);
 }
 
-// Preamble of A_k shared by the matrix-free and CSR kernels: reduce U_{k-1}'s partials and
-// decide which species continue and their beta.  Returns false when no species is active.
+// Writes the NQ block sums of `tot` as this workgroup's partials.
+template <int NQ>
+__device__ __forceinline__ void write_partials(double (&tot)[NQ], double* scratch,
+                                               double* __restrict__ pout) {
+    block_sum_all<NQ>(tot, scratch);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) pout[q * gridDim.x + blockIdx.x] = tot[q];
+}
+
+// ---------------------------------------------------------------------------------------
+// matrix-free rows
+//
+// Row of owned vertex 0 of a cell with edges e_k = x_k - x_0 (k = 1..3), c1 = e2 x e3,
+// c2 = e3 x e1, c3 = e1 x e2, |det| = |e1 . c1|, cs = c1 + c2 + c3: the stiffness row is
+// K_0k = D c0.c_k / (6 |det|) with c0 = -cs, so (K u)_0 = -(D/6) sum_k g_k (u_k - u_0) with
+// g_k = cs.c_k / |det|; the diagonal is (D/6) cs.cs / |det|.  Lumped mass |det|/24 on the
+// diagonal, consistent mass |det| (1 + delta_0k) / 120.
+
+enum RowKind { ROW_MATVEC = 0, ROW_DIAG = 1, ROW_MASS = 2 };
+
+// Row sums of the owned vertex served by lane slot `ls` of block b, summed over its L lanes
+// (all lanes of the wave must call this).  Generic form: the full rows (matvec, diagonal or
+// lumped mass diagonal).  Fast form: matvec acc[s] = sum_k g_k (u_k - u_0), diagonal
+// acc[0] = sum cs.cs / |det| (species-independent); finish_fast turns these into rows.
+// Sets `ov` (owned local index), `live`, `own` (the vertex's input).
+template <class T, int S, int KIND, bool FAST>
+__device__ __forceinline__ void lane_rows(const Geo& g, const OpParams<T>& prm,
+                                          const Vec4<T>* sgeo, const SV<T, S>* spv, int b, int nv,
+                                          int ls, int& ov, bool& live, SV<T, S>& own, T (&acc)[S]) {
+    const int L = g.L;
+    const int vps = 64 / L;
+    const int spb = g.NV / vps;
+    const int lane = ls & 63;
+    const int rank = ls / L;  // < NV: ls < NV * L
+    const int* sp0 = g.sptr + b * spb + (ls >> 6);
+    const int jbeg = sp0[0] + lane;
+    const int width = (sp0[1] - sp0[0]) >> 6;
+    live = rank < nv;
+    const int ovl = (int)g.lperm[b * g.NV + rank];
+    ov = live ? ovl : 0;
+    const Vec4<T> q0 = sgeo[ov];
+    own = SV<T, S>{};
+    if constexpr (KIND == ROW_MATVEC) own = spv[ov];
+    const T inv_dt = prm.inv_dt;
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = T(0);
+    for (int t0 = 0; t0 < width; t0 += 4) {
+        // the list words of 4 steps are loaded together (clamped in-bounds addresses, steps
+        // past the list read as padding), off the dependent chain
+        uint32_t pk4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t wv = g.ell[jbeg + 64 * min(t0 + u, width - 1)];
+            pk4[u] = t0 + u < width ? wv : ELL_PAD;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t pk = pk4[u];
+            if (pk == ELL_PAD) continue;  // padding, also every entry of a lane past nv
+            const int j = jbeg + 64 * (t0 + u);
+            const int la = pk & 1023, lb = (pk >> 10) & 1023, lc = pk >> 20;
+            SV<T, S> w1{}, w2{}, w3{};
+            if constexpr (KIND == ROW_MATVEC) {
+                w1 = spv[la];
+                w2 = spv[lb];
+                w3 = spv[lc];
+            }
+            const Vec4<T> q1 = sgeo[la], q2 = sgeo[lb], q3 = sgeo[lc];
+            const Vec3<T> e1{q1.x - q0.x, q1.y - q0.y, q1.z - q0.z};
+            const Vec3<T> e2{q2.x - q0.x, q2.y - q0.y, q2.z - q0.z};
+            const Vec3<T> e3{q3.x - q0.x, q3.y - q0.y, q3.z - q0.z};
+            const Vec3<T> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
+            const T adet = fabs(dot(e1, c1));
+            const T ri = recip(adet);
+            const Vec3<T> cs{c1.x + c2.x + c3.x, c1.y + c2.y + c3.y, c1.z + c2.z + c3.z};
+            if constexpr (KIND == ROW_DIAG && FAST) {
+                acc[0] += dot(cs, cs) * ri;
+            } else if constexpr (KIND != ROW_MATVEC) {
+                const T gd = dot(cs, cs) * ri;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const T mf = inv_dt + prm.sp[s].A[j];
+                    if constexpr (KIND == ROW_MASS)
+                        acc[s] += mf * adet * T(5.0 / 120.0);
+                    else
+                        acc[s] += prm.sp[s].D[j] * T(1.0 / 6.0) * gd +
+                                  (prm.lumped ? mf * adet * T(5.0 / 120.0) : mf * adet * T(2.0 / 120.0));
+                }
+            } else {
+                const T g1 = dot(cs, c1) * ri, g2 = dot(cs, c2) * ri, g3 = dot(cs, c3) * ri;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const T u0 = own.v[s], u1 = w1.v[s], u2 = w2.v[s], u3 = w3.v[s];
+                    const T gsum = g1 * (u1 - u0) + g2 * (u2 - u0) + g3 * (u3 - u0);
+                    if constexpr (FAST) {
+                        acc[s] += gsum;
+                    } else {
+                        const T mf = inv_dt + prm.sp[s].A[j];
+                        const T kr = -prm.sp[s].D[j] * T(1.0 / 6.0) * gsum;
+                        if (prm.lumped)
+                            acc[s] += kr + mf * adet * T(5.0 / 120.0) * u0;
+                        else
+                            acc[s] += kr + mf * adet * T(1.0 / 120.0) * ((((u0 + u1) + u2) + u3) + u0);
+                    }
+                }
+            }
+        }
+    }
+    // sum the partial rows of the L adjacent lanes of a vertex (pairwise, so every lane holds
+    // the same bits)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if (L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
+        if (L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+    }
+}
+
+// fast form: rows from the geometric sums, -(D/6) sum + mdiag u (diagonal: (D/6) sum + mdiag)
+template <class T, int S, int KIND>
+__device__ __forceinline__ void finish_fast(const OpParams<T>& prm, int vtx, const SV<T, S>& own,
+                                            T (&acc)[S]) {
+    const SV<T, S> md = reinterpret_cast<const SV<T, S>*>(prm.mdiag)[vtx];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if constexpr (KIND == ROW_DIAG)
+            acc[s] = prm.sp[s].D6 * acc[0] + md.v[s];
+        else
+            acc[s] = -prm.sp[s].D6 * acc[s] + md.v[s] * own.v[s];
+    }
+}
+
+// Workgroup -> partition blocks: XCD x (= workgroup id mod 8) owns blocks
+// [x*per_xcd, (x+1)*per_xcd), taken round-robin by its workgroups.
+struct BlockRange {
+    int beg, end, step;
+};
+__device__ __forceinline__ BlockRange my_blocks(const Geo& g) {
+    const int xcd = blockIdx.x & 7;
+    const int bbeg = xcd * g.per_xcd;
+    return BlockRange{bbeg + (int)(blockIdx.x >> 3), min(g.nb, bbeg + g.per_xcd),
+                      (int)(gridDim.x >> 3)};
+}
+
+constexpr int MF_THREADS_MAX = 512;  // workgroup sizes 256 and 512
+
+// ---------------------------------------------------------------------------------------
+// matrix-free operator kernel.  MODE: APPLY y = A x; DIAG y = 1 / diag(A); RESID r = b - A x
+// (FP64 partials of r.r and b.b); MASS (generic form only) y = lumped (inv_dt + alpha) mass
+// diagonal, the `mdiag` of the fast form.
+template <class T, int S, int MODE, bool FAST>
+__global__ void __launch_bounds__(MF_THREADS_MAX)
+k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
+     double* __restrict__ pout) {
+    static_assert(!(FAST && MODE == MF_MASS), "the mass diagonal is built by the generic form");
+    constexpr int KIND = MODE == MF_DIAG ? ROW_DIAG : (MODE == MF_MASS ? ROW_MASS : ROW_MATVEC);
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
+    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
+    __shared__ double red[16 * 2 * MAXS];
+    const int ml = g.max_local;
+
+    double part[S], part2[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) part[s] = part2[s] = 0.0;
+    const BlockRange br = my_blocks(g);
+    for (int b = br.beg; b < br.end; b += br.step) {
+        const int v0 = b * g.NV;
+        const int nv = min(g.NV, g.V - v0);
+        const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
+        for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
+            const int gv = g.lvl[(size_t)b * ml + i];
+            sgeo[i] = xyz[gv];
+            if constexpr (KIND == ROW_MATVEC) spv[i] = v.x[gv];
+        }
+        __syncthreads();
+        const int nls = ((nv + 64 / g.L - 1) / (64 / g.L)) * 64;
+        for (int ls = threadIdx.x; ls < nls; ls += blockDim.x) {
+            int ov;
+            bool live;
+            SV<T, S> own;
+            T acc[S];
+            lane_rows<T, S, KIND, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
+            if (!live || (ls & (g.L - 1)) != 0) continue;
+            const int vtx = v0 + ov;
+            if constexpr (FAST) finish_fast<T, S, KIND>(prm, vtx, own, acc);
+            const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
+            SV<T, S> out{};
+            if constexpr (MODE == MF_RESID) {
+                const SV<double, S> bv = v.b[vtx];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const T val = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
+                    const double r = bv.v[s] - (double)val;
+                    out.v[s] = (T)r;
+                    part[s] += r * r;
+                    part2[s] += bv.v[s] * bv.v[s];
+                }
+            } else if constexpr (MODE == MF_APPLY) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) out.v[s] = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
+            } else if constexpr (MODE == MF_DIAG) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) out.v[s] = ((bcb >> s) & 1u) ? T(1) : T(1) / acc[s];
+            } else {  // MF_MASS
+#pragma unroll
+                for (int s = 0; s < S; ++s) out.v[s] = acc[s];
+            }
+            v.y[vtx] = out;
+        }
+        __syncthreads();  // LDS is reused by the next partition block
+    }
+    if constexpr (MODE == MF_RESID) {
+        double tot[2 * S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            tot[s] = part[s];
+            tot[S + s] = part2[s];
+        }
+        write_partials<2 * S>(tot, red, pout);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// matrix-free Chronopoulos-Gear PCG iteration K_i (parity par = i & 1), or the INIT kernel
+// (r_0 = (T) rin for the solved species, 0 otherwise; x = p = s_{-1} = 0; u_0 = r_0 / diag;
+// w_0 = A u_0).  Partials per workgroup: gamma, delta, rho of the next iteration.
+template <class T, int S, bool FAST, bool INIT>
+__global__ void __launch_bounds__(MF_THREADS_MAX)
+k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
+     CGState* __restrict__ st, const double* __restrict__ pin, int nin,
+     double* __restrict__ pout, int par) {
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
+    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
+    __shared__ double red[16 * 3 * MAXS];
+    const int ml = g.max_local;
+
+    bool act[S];
+    T alpha[S], beta[S];
+    bool any = false;
+    const int rd = INIT ? 0 : par, wr = INIT ? 0 : par ^ 1;  // buffers read / written
+    if constexpr (INIT) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            act[s] = st->act[1][s] != 0;
+            any |= act[s];
+            alpha[s] = beta[s] = T(0);
+        }
+        if (!any) return;
+    } else {
+        bool prev[S];
+        bool anyprev = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            prev[s] = st->act[par ^ 1][s] != 0;
+            anyprev |= prev[s];
+        }
+        const int k = st->kk[par ^ 1];
+        if (!anyprev) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                for (int s = 0; s < S; ++s) st->act[par][s] = 0;
+                st->kk[par] = k;
+                st->any_active = 0;
+            }
+            return;
+        }
+        double sums[3 * S];
+        consume_partials<3 * S>(pin, nin, sums, red);  // gamma_i, delta_i, rho_i
+        double gam[S], alp[S];
+        bool bad[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double gm = sums[s], dl = sums[S + s], rho = sums[2 * S + s];
+            act[s] = prev[s] && (rho > st->tol2[s]) && (k < st->max_it) && isfinite(rho);
+            double bb = 0.0, den = dl;
+            if (k > 0) {
+                bb = gm / st->rz[par ^ 1][s];
+                den = dl - bb * gm / st->alp[par ^ 1][s];
+            }
+            bad[s] = act[s] && (!(gm > 0.0) || !(den > 0.0) || !isfinite(den) || !isfinite(bb));
+            act[s] = act[s] && !bad[s];
+            gam[s] = gm;
+            alp[s] = act[s] ? gm / den : 0.0;
+            alpha[s] = (T)alp[s];
+            beta[s] = act[s] ? (T)bb : T(0);
+            any |= act[s];
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            for (int s = 0; s < S; ++s) {
+                st->act[par][s] = act[s] ? 1 : 0;
+                if (!prev[s]) continue;
+                st->rz[par][s] = gam[s];
+                st->alp[par][s] = alp[s];
+                st->rr[s] = sums[2 * S + s];
+                if (act[s])
+                    st->iters[s] += 1;
+                else
+                    st->status[s] = bad[s] ? ST_BREAKDOWN
+                                           : (sums[2 * S + s] <= st->tol2[s] ? ST_CONVERGED : ST_MAXIT);
+            }
+            st->kk[par] = k + 1;
+            st->any_active = any ? 1 : 0;
+        }
+        if (!any) return;
+    }
+
+    double pg[S], pd[S], pr[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) pg[s] = pd[s] = pr[s] = 0.0;
+    const BlockRange br = my_blocks(g);
+    for (int b = br.beg; b < br.end; b += br.step) {
+        const int v0 = b * g.NV;
+        const int nv = min(g.NV, g.V - v0);
+        const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
+        // (1) stage u_{i+1} of every local vertex; owned vertices also advance p, x, s, r
+        for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
+            const int gv = g.lvl[(size_t)b * ml + i];
+            sgeo[i] = xyz[gv];
+            const SV<T, S> d = v.dinv[gv];
+            SV<T, S> rn{}, un{}, sn{};
+            if constexpr (INIT) {
+                const SV<double, S> ri = v.rin[gv];
+#pragma unroll
+                for (int s = 0; s < S; ++s) rn.v[s] = act[s] ? (T)ri.v[s] : T(0);
+            } else {
+                const SV<T, S> r = v.R[rd][gv], w = v.W[rd][gv], so = v.Sd[rd][gv];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    sn.v[s] = w.v[s] + beta[s] * so.v[s];
+                    rn.v[s] = r.v[s] - alpha[s] * sn.v[s];
+                }
+                if (i < nv) {
+                    SV<T, S> p = v.P[gv], x = v.X[gv];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        p.v[s] = r.v[s] * d.v[s] + beta[s] * p.v[s];
+                        if (act[s]) x.v[s] += alpha[s] * p.v[s];
+                    }
+                    v.P[gv] = p;
+                    v.X[gv] = x;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) un.v[s] = rn.v[s] * d.v[s];
+            spv[i] = un;
+            if (i < nv) {
+                v.R[wr][gv] = rn;
+                v.Sd[wr][gv] = sn;  // INIT: s_{-1} = 0
+                if constexpr (INIT) {
+                    v.P[gv] = SV<T, S>{};
+                    v.X[gv] = SV<T, S>{};
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (act[s]) {
+                        pg[s] += (double)rn.v[s] * (double)un.v[s];
+                        pr[s] += (double)rn.v[s] * (double)rn.v[s];
+                    }
+            }
+        }
+        __syncthreads();
+        // (2) w_{i+1} = A u_{i+1} for the owned vertices
+        const int nls = ((nv + 64 / g.L - 1) / (64 / g.L)) * 64;
+        for (int ls = threadIdx.x; ls < nls; ls += blockDim.x) {
+            int ov;
+            bool live;
+            SV<T, S> own;
+            T acc[S];
+            lane_rows<T, S, ROW_MATVEC, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
+            if (!live || (ls & (g.L - 1)) != 0) continue;
+            const int vtx = v0 + ov;
+            if constexpr (FAST) finish_fast<T, S, ROW_MATVEC>(prm, vtx, own, acc);
+            const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
+            SV<T, S> out{};
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                out.v[s] = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
+                if (act[s]) pd[s] += (double)out.v[s] * (double)own.v[s];
+            }
+            v.W[wr][vtx] = out;
+        }
+        __syncthreads();  // LDS is reused by the next partition block
+    }
+    double tot[3 * S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        tot[s] = pg[s];
+        tot[S + s] = pd[s];
+        tot[2 * S + s] = pr[s];
+    }
+    write_partials<3 * S>(tot, red, pout);
+}
+
+// ---------------------------------------------------------------------------------------
+// CSR (caller order) kernels of the plain FP64 path: CG apply (A_k) and residual.
+constexpr int CSR_LANES = 8;
+constexpr int CSR_THREADS = 512;
+constexpr int CSR_ROWS = CSR_THREADS / CSR_LANES;
+
+// A_k preamble of the CSR CG: reduce U_{k-1}'s partials, decide which species continue and
+// their beta.  Returns false when no species is active.
 template <int S>
 __device__ __forceinline__ bool cg_begin(CGState* __restrict__ st, const double* __restrict__ pin,
                                          int nin, int par, bool (&act)[S], double (&beta)[S],
@@ -232,241 +662,13 @@ __device__ __forceinline__ bool cg_begin(CGState* __restrict__ st, const double*
     return any;
 }
 
-// ---------------------------------------------------------------------------------------
-// matrix-free kernel.  MODE: APPLY y = A x; DIAG y = 1 / diag(A); CG = A_k of the CG loop
-// with parity `par` = k & 1; RESID r = b - A x (FP64 partials of r.r and b.b); MASS (generic
-// form only) y = lumped (inv_dt + alpha) mass diagonal, the `mdiag` of the fast form.
-//
-// Row of owned vertex 0 of a cell with edges e_k = x_k - x_0 (k = 1..3), c1 = e2 x e3,
-// c2 = e3 x e1, c3 = e1 x e2, |det| = |e1 . c1|, cs = c1 + c2 + c3: the stiffness row is
-// K_0k = D c0.c_k / (6 |det|) with c0 = -cs, so (K u)_0 = -(D/6) sum_k g_k (u_k - u_0) with
-// g_k = cs.c_k / |det|; the diagonal is (D/6) cs.cs / |det|.  Lumped mass |det|/24 on the
-// diagonal, consistent mass |det| (1 + delta_0k) / 120.
-template <class T, int S, int MODE, bool FAST>
-__global__ void __launch_bounds__(1024)
-k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
-     CGState* __restrict__ st, const double* __restrict__ pin, int nin,
-     double* __restrict__ pout, int par) {
-    static_assert(!(FAST && MODE == MF_MASS), "the mass diagonal is built by the generic form");
-    extern __shared__ __align__(32) unsigned char smem_raw[];
-    const int ml = g.max_local;
-    Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
-    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + ml);  // species input per local vertex
-    __shared__ double red[16 * 2 * MAXS];
-    const int nwg = gridDim.x;
-    constexpr bool HAS_X = MODE != MF_DIAG && MODE != MF_MASS;
-
-    bool act[S];
-    double betad[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        act[s] = true;
-        betad[s] = 0.0;
-    }
-    if constexpr (MODE == MF_CG) {
-        if (!cg_begin<S>(st, pin, nin, par, act, betad, red)) return;
-    }
-    T beta[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) beta[s] = (T)betad[s];
-
-    double part[S];
-    double part2[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) part[s] = part2[s] = 0.0;
-
-    const T inv_dt = prm.inv_dt;
-    const int lumped = prm.lumped;
-    const SV<T, S>* __restrict__ mdiag = reinterpret_cast<const SV<T, S>*>(prm.mdiag);
-    // XCD x (= workgroup id mod 8) owns partition blocks [x*per_xcd, (x+1)*per_xcd)
-    const int xcd = blockIdx.x & 7;
-    const int wl = blockIdx.x >> 3;
-    const int wpx = nwg >> 3;  // workgroups per XCD
-    const int bbeg = xcd * g.per_xcd;
-    const int bend = min(g.nb, bbeg + g.per_xcd);
-    for (int b = bbeg + wl; b < bend; b += wpx) {
-        const int v0 = b * g.NV;
-        const int nv = min(g.NV, g.V - v0);
-        const int h0 = g.hptr[b];
-        const int nloc = nv + (g.hptr[b + 1] - h0);
-        // (1) stage local vertices
-        for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
-            const int gv = i < nv ? v0 + i : g.hlist[h0 + i - nv];
-            sgeo[i] = xyz[gv];
-            if constexpr (MODE == MF_CG) {
-                const SV<T, S> z = v.x[gv];
-                const SV<T, S> po = v.pold[gv];
-                SV<T, S> p{};
-#pragma unroll
-                for (int s = 0; s < S; ++s) p.v[s] = z.v[s] + beta[s] * po.v[s];
-                spv[i] = p;
-            } else if constexpr (HAS_X) {
-                spv[i] = v.x[gv];
-            }
-        }
-        __syncthreads();
-        // (2) L lanes per owned vertex (vertices in decreasing-degree order); whole waves step
-        // through the lanes of the block's slices
-        const int L = g.L;
-        const int vps = 64 / L;
-        const int spb = g.NV / vps;
-        const int nls = ((nv + vps - 1) / vps) * 64;
-        for (int ls = threadIdx.x; ls < nls; ls += blockDim.x) {
-            const int lane = ls & 63;
-            const int rank = ls / L;
-            const int* sp0 = g.sptr + b * spb + (ls >> 6);
-            const int jbeg = sp0[0] + lane;
-            const int width = (sp0[1] - sp0[0]) >> 6;
-            const bool live = rank < nv;
-            const int ov = live ? (int)g.lperm[b * g.NV + rank] : 0;
-            const Vec4<T> q0 = sgeo[ov];
-            SV<T, S> own{};
-            if constexpr (HAS_X) own = spv[ov];
-            T acc[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] = T(0);
-            for (int t0 = 0; t0 < width; t0 += 4) {
-              // the list words of 4 steps are loaded together (clamped in-bounds addresses,
-              // steps past the list read as padding), off the dependent chain
-              uint32_t pk4[4];
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                  const uint32_t wv = g.ell[jbeg + 64 * min(t0 + u, width - 1)];
-                  pk4[u] = t0 + u < width ? wv : ELL_PAD;
-              }
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const uint32_t pk = pk4[u];
-                if (pk == ELL_PAD) continue;  // padding, also every entry of a lane past nv
-                const int j = jbeg + 64 * (t0 + u);
-                const int la = pk & 1023, lb = (pk >> 10) & 1023, lc = pk >> 20;
-                const Vec4<T> q1 = sgeo[la], q2 = sgeo[lb], q3 = sgeo[lc];
-                const Vec3<T> e1{q1.x - q0.x, q1.y - q0.y, q1.z - q0.z};
-                const Vec3<T> e2{q2.x - q0.x, q2.y - q0.y, q2.z - q0.z};
-                const Vec3<T> e3{q3.x - q0.x, q3.y - q0.y, q3.z - q0.z};
-                const Vec3<T> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
-                const T adet = fabs(dot(e1, c1));
-                const T ri = recip(adet);
-                const Vec3<T> cs{c1.x + c2.x + c3.x, c1.y + c2.y + c3.y, c1.z + c2.z + c3.z};
-                if constexpr (MODE == MF_DIAG && FAST) {
-                    acc[0] += dot(cs, cs) * ri;  // species-independent; D/6 and mdiag at the end
-                } else if constexpr (MODE == MF_DIAG || MODE == MF_MASS) {
-                    const T gd = dot(cs, cs) * ri;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const T mf = inv_dt + prm.sp[s].A[j];
-                        if constexpr (MODE == MF_MASS)
-                            acc[s] += mf * adet * T(5.0 / 120.0);
-                        else
-                            acc[s] += prm.sp[s].D[j] * T(1.0 / 6.0) * gd +
-                                      (lumped ? mf * adet * T(5.0 / 120.0) : mf * adet * T(2.0 / 120.0));
-                    }
-                } else {
-                    const T g1 = dot(cs, c1) * ri, g2 = dot(cs, c2) * ri, g3 = dot(cs, c3) * ri;
-                    const SV<T, S> w1 = spv[la], w2 = spv[lb], w3 = spv[lc];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const T u0 = own.v[s], u1 = w1.v[s], u2 = w2.v[s], u3 = w3.v[s];
-                        const T gsum = g1 * (u1 - u0) + g2 * (u2 - u0) + g3 * (u3 - u0);
-                        if constexpr (FAST) {
-                            acc[s] += gsum;
-                        } else {
-                            const T mf = inv_dt + prm.sp[s].A[j];
-                            const T kr = -prm.sp[s].D[j] * T(1.0 / 6.0) * gsum;
-                            if (lumped)
-                                acc[s] += kr + mf * adet * T(5.0 / 120.0) * u0;
-                            else
-                                acc[s] += kr + mf * adet * T(1.0 / 120.0) *
-                                                   ((((u0 + u1) + u2) + u3) + u0);
-                        }
-                    }
-                }
-              }
-            }
-            // sum the partial rows of the L adjacent lanes of a vertex (pairwise, so every
-            // lane holds the same bits)
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
-                if (L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
-            }
-            if (!live || (lane & (L - 1)) != 0) continue;
-            const int vtx = v0 + ov;
-            if constexpr (FAST) {
-                // rows from the geometric sums: -(D/6) sum + mdiag u (diag: (D/6) sum + mdiag)
-                const SV<T, S> md = mdiag[vtx];
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    if constexpr (MODE == MF_DIAG)
-                        acc[s] = prm.sp[s].D6 * acc[0] + md.v[s];
-                    else
-                        acc[s] = -prm.sp[s].D6 * acc[s] + md.v[s] * own.v[s];
-                }
-            }
-            const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
-            SV<T, S> out{};
-            if constexpr (MODE == MF_RESID) {
-                const SV<double, S> bv = v.b[vtx];
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const T val = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
-                    const double r = bv.v[s] - (double)val;
-                    out.v[s] = (T)r;
-                    part[s] += r * r;
-                    part2[s] += bv.v[s] * bv.v[s];
-                }
-                v.y[vtx] = out;
-            } else if constexpr (MODE == MF_APPLY) {
-#pragma unroll
-                for (int s = 0; s < S; ++s) out.v[s] = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
-                v.y[vtx] = out;
-            } else if constexpr (MODE == MF_DIAG) {
-#pragma unroll
-                for (int s = 0; s < S; ++s) out.v[s] = ((bcb >> s) & 1u) ? T(1) : T(1) / acc[s];
-                v.y[vtx] = out;
-            } else if constexpr (MODE == MF_MASS) {
-#pragma unroll
-                for (int s = 0; s < S; ++s) out.v[s] = acc[s];
-                v.y[vtx] = out;
-            } else {  // MF_CG
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const T val = ((bcb >> s) & 1u) ? own.v[s] : acc[s];
-                    out.v[s] = val;
-                    if (act[s]) part[s] += (double)own.v[s] * (double)val;
-                }
-                v.pnew[vtx] = own;
-                v.y[vtx] = out;
-            }
-        }
-        __syncthreads();  // LDS is reused by the next partition block
-    }
-    if constexpr (MODE == MF_CG || MODE == MF_RESID) {
-        constexpr int NQ = MODE == MF_CG ? S : 2 * S;
-        double tot[NQ];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            tot[s] = part[s];
-            if constexpr (MODE == MF_RESID) tot[S + s] = part2[s];
-        }
-        block_sum_all<NQ>(tot, red);
-        if (threadIdx.x == 0)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) pout[q * nwg + blockIdx.x] = tot[q];
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// CSR (caller order) kernels of the plain FP64 path: CG apply (A_k) and residual.
-constexpr int CSR_LANES = 8;
-constexpr int CSR_THREADS = 512;
-constexpr int CSR_ROWS = CSR_THREADS / CSR_LANES;
+enum CSRMode { CSR_CG = 0, CSR_RESID = 1 };
 
 template <int S, int MODE>
 __global__ void __launch_bounds__(CSR_THREADS)
 k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
       const double* __restrict__ A0, const double* __restrict__ A1, const double* __restrict__ A2,
-      MFVec<double, S> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
+      CSRVec<S> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
       double* __restrict__ pout, int par) {
     __shared__ double red[16 * 2 * MAXS];
     const int nwg = gridDim.x;
@@ -477,7 +679,7 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
         act[s] = true;
         beta[s] = 0.0;
     }
-    if constexpr (MODE == MF_CG) {
+    if constexpr (MODE == CSR_CG) {
         if (!cg_begin<S>(st, pin, nin, par, act, beta, red)) return;
     }
     const double* __restrict__ Av[3] = {A0, A1, A2};
@@ -495,11 +697,11 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
             const int c = indices[j];
             const SV<double, S> z = v.x[c];
             SV<double, S> po{};
-            if constexpr (MODE == MF_CG) po = v.pold[c];
+            if constexpr (MODE == CSR_CG) po = v.pold[c];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 if (!act[s]) continue;
-                const double u = MODE == MF_CG ? z.v[s] + beta[s] * po.v[s] : z.v[s];
+                const double u = MODE == CSR_CG ? z.v[s] + beta[s] * po.v[s] : z.v[s];
                 acc[s] += Av[s][j] * u;
             }
         }
@@ -508,7 +710,7 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
 #pragma unroll
             for (int off = CSR_LANES / 2; off > 0; off >>= 1) acc[s] += __shfl_xor(acc[s], off, 64);
         if (lane == 0) {
-            if constexpr (MODE == MF_CG) {
+            if constexpr (MODE == CSR_CG) {
                 const SV<double, S> z = v.x[row];
                 const SV<double, S> po = v.pold[row];
                 SV<double, S> p{}, q{};
@@ -533,35 +735,32 @@ k_csr(int V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ ind
             }
         }
     }
-    constexpr int NQ = MODE == MF_CG ? S : 2 * S;
+    constexpr int NQ = MODE == CSR_CG ? S : 2 * S;
     double tot[NQ];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         tot[s] = part[s];
-        if constexpr (MODE != MF_CG) tot[S + s] = part2[s];
+        if constexpr (MODE != CSR_CG) tot[S + s] = part2[s];
     }
-    block_sum_all<NQ>(tot, red);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) pout[q * nwg + blockIdx.x] = tot[q];
+    write_partials<NQ>(tot, red, pout);
 }
 
 // U_k: x += alpha p; r -= alpha q; z = r * dinv; partials r.z, r.r (parity par = k & 1).
-// INIT (U_{-1}): r = (T) rin (0 for species not solved); x = 0; p_old = 0; z = r * dinv.
-template <class T, int S, bool INIT>
+// INIT (U_{-1}): r = rin (0 for species not solved); x = 0; p_old = 0; z = r * dinv.
+template <int S, bool INIT>
 __global__ void __launch_bounds__(256)
-k_update(int V, UVec<T, S> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
+k_update(int V, UVec<S> v, CGState* __restrict__ st, const double* __restrict__ pin, int nin,
          double* __restrict__ pout, int par) {
     __shared__ double red[16 * 2 * MAXS];
     const int nwg = gridDim.x;
     bool act[S];
-    T alpha[S];
+    double alpha[S];
     bool any = false;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         act[s] = st->act[par][s] != 0;
         any |= act[s];
-        alpha[s] = T(0);
+        alpha[s] = 0.0;
     }
     if (!any) return;
     if constexpr (!INIT) {
@@ -571,7 +770,7 @@ k_update(int V, UVec<T, S> v, CGState* __restrict__ st, const double* __restrict
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             bad[s] = act[s] && (!(pq[s] > 0.0) || !isfinite(pq[s]));
-            alpha[s] = (act[s] && !bad[s]) ? (T)(st->rz[par][s] / pq[s]) : T(0);
+            alpha[s] = (act[s] && !bad[s]) ? st->rz[par][s] / pq[s] : 0.0;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             for (int s = 0; s < S; ++s) {
@@ -591,15 +790,15 @@ k_update(int V, UVec<T, S> v, CGState* __restrict__ st, const double* __restrict
     for (int s = 0; s < S; ++s) prz[s] = prr[s] = 0.0;
     const int stride = nwg * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        const SV<T, S> d = v.dinv[i];
-        SV<T, S> r{}, z{}, x{};
+        const SV<double, S> d = v.dinv[i];
+        SV<double, S> r{}, z{}, x{};
         if constexpr (INIT) {
             const SV<double, S> ri = v.rin[i];
 #pragma unroll
-            for (int s = 0; s < S; ++s) r.v[s] = act[s] ? (T)ri.v[s] : T(0);
-            v.pzero[i] = SV<T, S>{};
+            for (int s = 0; s < S; ++s) r.v[s] = act[s] ? ri.v[s] : 0.0;
+            v.pzero[i] = SV<double, S>{};
         } else {
-            const SV<T, S> p = v.p[i], q = v.q[i];
+            const SV<double, S> p = v.p[i], q = v.q[i];
             x = v.x[i];
             r = v.r[i];
 #pragma unroll
@@ -613,8 +812,8 @@ k_update(int V, UVec<T, S> v, CGState* __restrict__ st, const double* __restrict
         for (int s = 0; s < S; ++s) {
             z.v[s] = r.v[s] * d.v[s];
             if (act[s]) {
-                prz[s] += (double)r.v[s] * (double)z.v[s];
-                prr[s] += (double)r.v[s] * (double)r.v[s];
+                prz[s] += r.v[s] * z.v[s];
+                prr[s] += r.v[s] * r.v[s];
             }
         }
         v.x[i] = x;
@@ -627,10 +826,7 @@ k_update(int V, UVec<T, S> v, CGState* __restrict__ st, const double* __restrict
         tot[s] = prz[s];
         tot[S + s] = prr[s];
     }
-    block_sum_all<2 * S>(tot, red);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < 2 * S; ++q) pout[q * nwg + blockIdx.x] = tot[q];
+    write_partials<2 * S>(tot, red, pout);
 }
 
 // x64 += (double) d over the species in `mask` (refinement correction)
@@ -683,16 +879,6 @@ __global__ void k_inc_coef(int64_t n, const int32_t* __restrict__ ell_cell,
     const int32_t e = ell_cell[i];
     out[i] = e >= 0 ? (T)cell_coef(p, mode, val, e, conn) : T(0);
 }
-
-// out[i] = in[perm[i]], 0 where perm[i] < 0
-template <class Ti, class To>
-__global__ void k_gather_perm(int n, const int32_t* __restrict__ perm, const Ti* __restrict__ in,
-                              To* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t p = perm[i];
-    out[i] = p >= 0 ? (To)in[p] : To(0);
-}
 __global__ void k_set_bc_bits(int n_bc, const int32_t* __restrict__ bc,
                               const int32_t* __restrict__ iperm, int s,
                               uint8_t* __restrict__ bits) {
@@ -722,13 +908,23 @@ __global__ void k_diag_inv_csr(int V, const int32_t* __restrict__ diag_pos,
 // ---------------------------------------------------------------------------------------
 // workspace
 
+// species-record vectors of a workspace (CG precision unless noted)
+enum VecId {
+    VB_B64 = 0, VB_X64, VB_R64,  // FP64
+    VB_DINV,
+    VB_X,                        // correction (both CG forms)
+    VB_V1, VB_V2, VB_V3, VB_V4, VB_V5, VB_V6, VB_V7,
+    VB_COUNT
+};
+// CSR CG: r, z, q, p_a, p_b = V1..V5.  MF CG: R[2] = V1,V2; W[2] = V3,V4; S[2] = V5,V6; P = V7.
+
 struct Workspace {
     int method = 0, S = 0, V = 0;
     hipStream_t stream = nullptr;
     CGState* st = nullptr;
     CGState* h_st = nullptr;  // pinned
-    double* pA = nullptr;     // partials of A kernels (S per workgroup)
-    double* pU[2] = {};       // partials of U kernels by parity (2S per workgroup)
+    double* pA = nullptr;     // CSR: partials of A_k (S per workgroup)
+    double* pU[2] = {};       // CSR: partials of U_k; MF: partials of K_i (3S per workgroup)
     double* pR = nullptr;     // partials of the residual kernel (2S per workgroup)
     double* h_pR = nullptr;   // pinned copy
     // operator coefficients: [0] kernels of the CG precision, [1] FP64 residual of the
@@ -740,18 +936,7 @@ struct Workspace {
     uint8_t* bcbits = nullptr;
     unsigned char prm[sizeof(OpParams<double>)] = {};   // OpParams of the CG precision
     unsigned char prm2[sizeof(OpParams<double>)] = {};  // OpParams<double> of the residual
-    // FP64 records
-    void* b64 = nullptr;
-    void* x64 = nullptr;
-    void* r64 = nullptr;
-    // CG records in the CG precision
-    void* xcg = nullptr;
-    void* rcg = nullptr;
-    void* zcg = nullptr;
-    void* qcg = nullptr;
-    void* pa = nullptr;
-    void* pb = nullptr;
-    void* dinv = nullptr;
+    void* vb[VB_COUNT] = {};
     double* csr[MAXS] = {};  // CSR64 values per species
     // captured CG chunk and the parameters it was captured with
     hipGraphExec_t graph = nullptr;
@@ -779,16 +964,7 @@ static void free_ws(Workspace* w) {
     dfree(w->mdiag[1]);
     for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
     dfree(w->bcbits);
-    dfree(w->b64);
-    dfree(w->x64);
-    dfree(w->r64);
-    dfree(w->xcg);
-    dfree(w->rcg);
-    dfree(w->zcg);
-    dfree(w->qcg);
-    dfree(w->pa);
-    dfree(w->pb);
-    dfree(w->dinv);
+    for (int i = 0; i < VB_COUNT; ++i) dfree(w->vb[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
 }
@@ -811,7 +987,7 @@ static int max_lds_per_block(int device) {
     return v;
 }
 
-// LDS bytes of the MF kernel: coordinates and species input of every local vertex
+// LDS bytes of the MF kernels: coordinates and species input of every local vertex
 template <class T>
 static size_t mf_lds_bytes(const Partition& P, int S) {
     return (size_t)P.max_local * (sizeof(Vec4<T>) + rec_bytes(sizeof(T), S));
@@ -822,9 +998,9 @@ static int mf_grid(const Partition& P) {
     const int per = (P.nb + 7) / 8;
     return 8 * std::min(per, MAX_PROD / 8);
 }
-static int csr_grid(const dsv_mesh* m) { return std::min((int)cdiv(m->V, CSR_ROWS), MAX_PROD); }
+static int csr_grid(const dsv_mesh* m) { return std::min((int)cdiv(m->V, CSR_ROWS), 1024); }
 static int upd_grid(const dsv_mesh* m) {
-    return std::max(1, std::min((int)cdiv(m->V, 256), MAX_PROD));
+    return std::max(1, std::min((int)cdiv(m->V, 1024), 1024));
 }
 
 static bool is_fp32(int method) { return method == DSV_METHOD_MF32_IR64; }
@@ -844,23 +1020,17 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S) {
         DSV_HIP(hipHostMalloc((void**)&w->h_pR, sizeof(double) * 2 * MAXS * MAX_PROD,
                               hipHostMallocDefault));
         w->pA = dalloc<double>((size_t)MAXS * MAX_PROD);
-        w->pU[0] = dalloc<double>((size_t)2 * MAXS * MAX_PROD);
-        w->pU[1] = dalloc<double>((size_t)2 * MAXS * MAX_PROD);
+        w->pU[0] = dalloc<double>((size_t)3 * MAXS * MAX_PROD);
+        w->pU[1] = dalloc<double>((size_t)3 * MAXS * MAX_PROD);
         w->pR = dalloc<double>((size_t)2 * MAXS * MAX_PROD);
         const size_t V = m->V;
         const size_t rt = rec_bytes(is_fp32(method) ? sizeof(float) : sizeof(double), S);
         const size_t r64 = rec_bytes(sizeof(double), S);
         w->bcbits = dalloc<uint8_t>(V);
-        DSV_HIP(hipMalloc(&w->b64, V * r64));
-        DSV_HIP(hipMalloc(&w->x64, V * r64));
-        DSV_HIP(hipMalloc(&w->r64, V * r64));
-        DSV_HIP(hipMalloc(&w->xcg, V * rt));
-        DSV_HIP(hipMalloc(&w->rcg, V * rt));
-        DSV_HIP(hipMalloc(&w->zcg, V * rt));
-        DSV_HIP(hipMalloc(&w->qcg, V * rt));
-        DSV_HIP(hipMalloc(&w->pa, V * rt));
-        DSV_HIP(hipMalloc(&w->pb, V * rt));
-        DSV_HIP(hipMalloc(&w->dinv, V * rt));
+        for (int i = 0; i < VB_COUNT; ++i) {
+            if (method == DSV_METHOD_CSR64 && (i == VB_V6 || i == VB_V7)) continue;
+            DSV_HIP(hipMalloc(&w->vb[i], V * (i <= VB_R64 ? r64 : rt)));
+        }
         if (method == DSV_METHOD_CSR64)
             for (int s = 0; s < S; ++s) w->csr[s] = dalloc<double>(m->nnz);
     } catch (...) {
@@ -890,35 +1060,44 @@ static Geo make_geo(const dsv_mesh* m, const Partition& P) {
     g.per_xcd = (P.nb + 7) / 8;
     g.max_local = P.max_local;
     g.hptr = P.hptr;
-    g.hlist = P.hlist;
+    g.lvl = P.lvl;
     g.lperm = P.lperm;
     g.sptr = P.sptr;
     g.ell = P.ell;
     return g;
 }
 
-template <class T, int S, int MODE>
-static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char* prm,
-                      const MFVec<T, S>& v, CGState* st, const double* pin, int nin, double* pout,
-                      int par, hipStream_t stream) {
-    const Geo g = make_geo(m, P);
-    const Vec4<T>* X = sizeof(T) == 4 ? (const Vec4<T>*)(const void*)m->xyz32
-                                      : (const Vec4<T>*)(const void*)m->xyz64;
-    OpParams<T> p;
-    std::memcpy(&p, prm, sizeof(p));
+template <class T>
+static const Vec4<T>* coords(const dsv_mesh* m) {
+    return sizeof(T) == 4 ? (const Vec4<T>*)(const void*)m->xyz32
+                          : (const Vec4<T>*)(const void*)m->xyz64;
+}
+
+template <class T, int S>
+static size_t checked_lds(const dsv_mesh* m, const Partition& P) {
     const size_t lds = mf_lds_bytes<T>(P, S);
     if (lds + 1024 > (size_t)max_lds_per_block(m->device))
         throw ArgError{"matrix-free partition needs " + std::to_string(lds) +
                        " B of LDS per workgroup; use fewer vertices_per_block"};
+    return lds;
+}
+
+template <class T, int S, int MODE>
+static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char* prm,
+                      const MFVec<T, S>& v, double* pout, hipStream_t stream) {
+    const Geo g = make_geo(m, P);
+    OpParams<T> p;
+    std::memcpy(&p, prm, sizeof(p));
+    const size_t lds = checked_lds<T, S>(m, P);
     if constexpr (MODE != MF_MASS) {
         if (p.fast) {
             hipLaunchKernelGGL((k_mf<T, S, MODE, true>), dim3(mf_grid(P)), dim3(P.threads), lds,
-                               stream, g, X, p, v, st, pin, nin, pout, par);
+                               stream, g, coords<T>(m), p, v, pout);
             return;
         }
     }
     hipLaunchKernelGGL((k_mf<T, S, MODE, false>), dim3(mf_grid(P)), dim3(P.threads), lds, stream, g,
-                       X, p, v, st, pin, nin, pout, par);
+                       coords<T>(m), p, v, pout);
 }
 
 // Builds the operator parameters of precision T in coefficient set `set`: per-ELL-entry cell
@@ -936,6 +1115,7 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
     const size_t n = (size_t)P.n_entries;
     for (int s = 0; s < S; ++s) {
         const double* src[2] = {sp[s].D, sp[s].alpha};
+        const int32_t len[2] = {sp[s].D_len, sp[s].alpha_len};
         const double val[2] = {sp[s].D_value, sp[s].alpha ? 0.0 : sp[s].alpha_value};
         const T** out[2] = {&h.sp[s].D, &h.sp[s].A};
         for (int c = 0; c < 2; ++c) {
@@ -950,8 +1130,8 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
                 cap = n * sizeof(T);
             }
             hipLaunchKernelGGL((k_inc_coef<T>), dim3(cdiv(n, 256)), dim3(256), 0, st, (int64_t)n,
-                               P.ell_cell, m->conn, src[c], coef_mode(src[c], c ? sp[s].alpha_len : sp[s].D_len, m),
-                               val[c], (T*)buf);
+                               P.ell_cell, m->conn, src[c], coef_mode(src[c], len[c], m), val[c],
+                               (T*)buf);
             *out[c] = (const T*)buf;
         }
         h.sp[s].D6 = (T)(sp[s].D_value / 6.0);
@@ -964,7 +1144,7 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
         v.y = (SV<T, S>*)w->mdiag[set];
         unsigned char hb[sizeof(OpParams<double>)] = {};
         std::memcpy(hb, &h, sizeof(h));
-        launch_mf<T, S, MF_MASS>(m, P, hb, v, w->st, nullptr, 0, nullptr, 0, st);
+        launch_mf<T, S, MF_MASS>(m, P, hb, v, nullptr, st);
         h.fast = 1;
         h.mdiag = w->mdiag[set];
     }
@@ -972,9 +1152,45 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
     DSV_HIP(hipGetLastError());
 }
 
+template <class T, int S>
+static CGVec<T, S> cg_vecs(Workspace* w) {
+    CGVec<T, S> v{};
+    v.dinv = (const SV<T, S>*)w->vb[VB_DINV];
+    v.R[0] = (SV<T, S>*)w->vb[VB_V1];
+    v.R[1] = (SV<T, S>*)w->vb[VB_V2];
+    v.W[0] = (SV<T, S>*)w->vb[VB_V3];
+    v.W[1] = (SV<T, S>*)w->vb[VB_V4];
+    v.Sd[0] = (SV<T, S>*)w->vb[VB_V5];
+    v.Sd[1] = (SV<T, S>*)w->vb[VB_V6];
+    v.P = (SV<T, S>*)w->vb[VB_V7];
+    v.X = (SV<T, S>*)w->vb[VB_X];
+    v.rin = (const SV<double, S>*)w->vb[VB_R64];
+    return v;
+}
+
+// MF CG kernel: INIT (writes the partials K_0 reads, pU[0]) or K_i with parity par
+template <class T, int S, bool INIT>
+static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
+    const Partition& P = m->part;
+    const Geo g = make_geo(m, P);
+    OpParams<T> p;
+    std::memcpy(&p, w->prm, sizeof(p));
+    const size_t lds = checked_lds<T, S>(m, P);
+    const CGVec<T, S> v = cg_vecs<T, S>(w);
+    const int n = mf_grid(P);
+    const double* pin = INIT ? nullptr : w->pU[par];
+    double* pout = INIT ? w->pU[0] : w->pU[par ^ 1];
+    if (p.fast)
+        hipLaunchKernelGGL((k_cg<T, S, true, INIT>), dim3(n), dim3(P.threads), lds, stream, g,
+                           coords<T>(m), p, v, w->st, pin, n, pout, par);
+    else
+        hipLaunchKernelGGL((k_cg<T, S, false, INIT>), dim3(n), dim3(P.threads), lds, stream, g,
+                           coords<T>(m), p, v, w->st, pin, n, pout, par);
+}
+
 template <int S, int MODE>
-static void launch_csr(const dsv_mesh* m, Workspace* w, const MFVec<double, S>& v,
-                       const double* pin, int nin, double* pout, int par, hipStream_t stream) {
+static void launch_csr(const dsv_mesh* m, Workspace* w, const CSRVec<S>& v, const double* pin,
+                       int nin, double* pout, int par, hipStream_t stream) {
     const double* A0 = w->csr[0];
     const double* A1 = S > 1 ? w->csr[1] : A0;
     const double* A2 = S > 2 ? w->csr[2] : A0;
@@ -982,54 +1198,43 @@ static void launch_csr(const dsv_mesh* m, Workspace* w, const MFVec<double, S>& 
                        m->indptr, m->indices, A0, A1, A2, v, w->st, pin, nin, pout, par);
 }
 
-// producer grid of the A kernels of this workspace
-static int a_grid(dsv_mesh* m, const Workspace* w) {
-    if (w->method == DSV_METHOD_CSR64) return csr_grid(m);
-    return mf_grid(m->part);
-}
-
-template <class T, int S, bool INIT>
+template <int S, bool INIT>
 static void launch_update(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
-    UVec<T, S> u{};
-    u.x = (SV<T, S>*)w->xcg;
-    u.r = (SV<T, S>*)w->rcg;
-    u.z = (SV<T, S>*)w->zcg;
-    u.dinv = (const SV<T, S>*)w->dinv;
-    u.p = (const SV<T, S>*)(parity ? w->pa : w->pb);  // p_new of A_k with this parity
-    u.q = (const SV<T, S>*)w->qcg;
-    u.pzero = (SV<T, S>*)w->pa;  // INIT clears the p_old of A_0 (parity 0 reads pa)
-    u.rin = (const SV<double, S>*)w->r64;
-    hipLaunchKernelGGL((k_update<T, S, INIT>), dim3(upd_grid(m)), dim3(256), 0, stream, m->V, u,
-                       w->st, w->pA, a_grid(m, w), w->pU[parity], parity);
+    UVec<S> u{};
+    u.x = (SV<double, S>*)w->vb[VB_X];
+    u.r = (SV<double, S>*)w->vb[VB_V1];
+    u.z = (SV<double, S>*)w->vb[VB_V2];
+    u.dinv = (const SV<double, S>*)w->vb[VB_DINV];
+    u.p = (const SV<double, S>*)(parity ? w->vb[VB_V4] : w->vb[VB_V5]);  // p_new of A_k
+    u.q = (const SV<double, S>*)w->vb[VB_V3];
+    u.pzero = (SV<double, S>*)w->vb[VB_V4];  // INIT clears the p_old of A_0 (parity 0)
+    u.rin = (const SV<double, S>*)w->vb[VB_R64];
+    hipLaunchKernelGGL((k_update<S, INIT>), dim3(upd_grid(m)), dim3(256), 0, stream, m->V, u, w->st,
+                       w->pA, csr_grid(m), w->pU[parity], parity);
 }
 
-// vectors of A_k: z in x, q in y, p_old / p_new alternate between pa / pb by parity
-template <class T, int S>
-static MFVec<T, S> cg_vecs(Workspace* w, int parity) {
-    MFVec<T, S> v{};
-    v.x = (const SV<T, S>*)w->zcg;
-    v.y = (SV<T, S>*)w->qcg;
-    v.pold = (const SV<T, S>*)(parity ? w->pb : w->pa);
-    v.pnew = (SV<T, S>*)(parity ? w->pa : w->pb);
-    v.b = nullptr;
-    return v;
+// one CSR CG iteration (A_k + U_k) with parity k & 1: z in x, q in y, p_old / p_new alternate
+template <int S>
+static void csr_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
+    CSRVec<S> v{};
+    v.x = (const SV<double, S>*)w->vb[VB_V2];
+    v.y = (SV<double, S>*)w->vb[VB_V3];
+    v.pold = (const SV<double, S>*)(parity ? w->vb[VB_V5] : w->vb[VB_V4]);
+    v.pnew = (SV<double, S>*)(parity ? w->vb[VB_V4] : w->vb[VB_V5]);
+    launch_csr<S, CSR_CG>(m, w, v, w->pU[parity ^ 1], upd_grid(m), w->pA, parity, stream);
+    launch_update<S, false>(m, w, parity, stream);
 }
 
-// one CG iteration (A_k + U_k) with parity k & 1
+// one CG iteration of the workspace's method (iteration parity `parity`)
 template <class T, int S>
 static void cg_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
-    const MFVec<T, S> v = cg_vecs<T, S>(w, parity);
-    const int nU = upd_grid(m);
     if constexpr (std::is_same<T, double>::value) {
         if (w->method == DSV_METHOD_CSR64) {
-            launch_csr<S, MF_CG>(m, w, v, w->pU[parity ^ 1], nU, w->pA, parity, stream);
-            launch_update<T, S, false>(m, w, parity, stream);
+            csr_iteration<S>(m, w, parity, stream);
             return;
         }
     }
-    launch_mf<T, S, MF_CG>(m, m->part, w->prm, v, w->st, w->pU[parity ^ 1], nU,
-                           w->pA, parity, stream);
-    launch_update<T, S, false>(m, w, parity, stream);
+    launch_cg<T, S, false>(m, w, parity, stream);
 }
 
 // Runs CG iterations until no species is active.  Chunks of an even number of iterations
@@ -1085,19 +1290,22 @@ static void write_state(Workspace* w, const CGState& h, hipStream_t stream) {
 // residual r64 = b - A x64 (FP64, vertex order of the method); returns rr and bb per species
 template <int S>
 static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb) {
-    MFVec<double, S> v{};
-    v.x = (const SV<double, S>*)w->x64;
-    v.y = (SV<double, S>*)w->r64;
-    v.b = (const SV<double, S>*)w->b64;
     int nprod;
     if (w->method == DSV_METHOD_CSR64) {
-        launch_csr<S, MF_RESID>(m, w, v, nullptr, 0, w->pR, 0, stream);
+        CSRVec<S> v{};
+        v.x = (const SV<double, S>*)w->vb[VB_X64];
+        v.y = (SV<double, S>*)w->vb[VB_R64];
+        v.b = (const SV<double, S>*)w->vb[VB_B64];
+        launch_csr<S, CSR_RESID>(m, w, v, nullptr, 0, w->pR, 0, stream);
         nprod = csr_grid(m);
     } else {
-        const Partition& P = m->part;
-        launch_mf<double, S, MF_RESID>(m, P, w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm,
-                                       v, w->st, nullptr, 0, w->pR, 0, stream);
-        nprod = mf_grid(P);
+        MFVec<double, S> v{};
+        v.x = (const SV<double, S>*)w->vb[VB_X64];
+        v.y = (SV<double, S>*)w->vb[VB_R64];
+        v.b = (const SV<double, S>*)w->vb[VB_B64];
+        launch_mf<double, S, MF_RESID>(m, m->part, w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm,
+                                       v, w->pR, stream);
+        nprod = mf_grid(m->part);
     }
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * 2 * S * nprod, hipMemcpyDeviceToHost,
@@ -1137,10 +1345,14 @@ static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* a
         h.status[s] = (s < S && active[s]) ? ST_RUNNING : ST_IDLE;
     }
     h.k = 0;
+    h.kk[0] = h.kk[1] = 0;
     h.max_it = max_it;
     h.any_active = 1;
     write_state(w, h, st);
-    launch_update<T, S, true>(m, w, 1, st);  // INIT = U_{-1}
+    if (w->method == DSV_METHOD_CSR64)
+        launch_update<S, true>(m, w, 1, st);  // U_{-1}
+    else
+        launch_cg<T, S, true>(m, w, 0, st);
     DSV_HIP(hipGetLastError());
     run_cg<T, S>(m, w, max_it, st);
 }
@@ -1156,39 +1368,40 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     const unsigned gv = cdiv(m->V, 256);
     const double* bs[3] = {sp[0].b, S > 1 ? sp[1].b : nullptr, S > 2 ? sp[2].b : nullptr};
     const double* xs[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
+    SV<double, S>* x64 = (SV<double, S>*)w->vb[VB_X64];
     hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
-                       bs[2], (SV<double, S>*)w->b64);
+                       bs[2], (SV<double, S>*)w->vb[VB_B64]);
     hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0], xs[1],
-                       xs[2], (SV<double, S>*)w->x64);
+                       xs[2], x64);
     for (int s = 0; s < S; ++s)
         if (sp[s].n_bc > 0)
             hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
                                sp[s].n_bc, sp[s].bc_dofs, internal ? m->iperm : (const int32_t*)nullptr,
-                               s, sp[s].bc_value, (SV<double, S>*)w->x64);
+                               s, sp[s].bc_value, x64);
     // operator set-up: assembled values / coefficients and the Jacobi diagonal
     if (method == DSV_METHOD_CSR64) {
+        SV<double, S>* dinv = (SV<double, S>*)w->vb[VB_DINV];
         for (int s = 0; s < S; ++s) {
             assemble_into(m, sp[s], inv_dt, lumped, w->csr[s], st);
             hipLaunchKernelGGL((k_diag_inv_csr<S>), dim3(gv), dim3(256), 0, st, m->V, m->diag_pos,
-                               w->csr[s], s, (SV<double, S>*)w->dinv);
+                               w->csr[s], s, dinv);
         }
         if (S == 3)  // padding lane of the records
             hipLaunchKernelGGL((k_diag_inv_csr<S>), dim3(gv), dim3(256), 0, st, m->V, m->diag_pos,
-                               w->csr[0], 3, (SV<double, S>*)w->dinv);
+                               w->csr[0], 3, dinv);
     } else if (method == DSV_METHOD_MF64) {
-        const Partition& P = m->part;
         prepare_bc(m, w, S, sp, st);
-        prepare_op<double, S>(m, w, P, sp, inv_dt, lumped, st, 0);
+        prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
         MFVec<double, S> v{};
-        v.y = (SV<double, S>*)w->dinv;
-        launch_mf<double, S, MF_DIAG>(m, P, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
+        v.y = (SV<double, S>*)w->vb[VB_DINV];
+        launch_mf<double, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
     } else {
         prepare_bc(m, w, S, sp, st);
         prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
         prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1);
         MFVec<float, S> v{};
-        v.y = (SV<float, S>*)w->dinv;
-        launch_mf<float, S, MF_DIAG>(m, m->part, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
+        v.y = (SV<float, S>*)w->vb[VB_DINV];
+        launch_mf<float, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
     }
     DSV_HIP(hipGetLastError());
 
@@ -1232,12 +1445,12 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
             if (act[s]) budget = std::min(budget, max_it - iters_total[s]);
         if (method == DSV_METHOD_MF32_IR64) {
             inner_cg<float, S>(m, w, tol2, act, std::max(1, budget), st);
-            hipLaunchKernelGGL((k_add_correction<float, S>), dim3(gv), dim3(256), 0, st, m->V,
-                               (SV<double, S>*)w->x64, (const SV<float, S>*)w->xcg, mask);
+            hipLaunchKernelGGL((k_add_correction<float, S>), dim3(gv), dim3(256), 0, st, m->V, x64,
+                               (const SV<float, S>*)w->vb[VB_X], mask);
         } else {
             inner_cg<double, S>(m, w, tol2, act, std::max(1, budget), st);
-            hipLaunchKernelGGL((k_add_correction<double, S>), dim3(gv), dim3(256), 0, st, m->V,
-                               (SV<double, S>*)w->x64, (const SV<double, S>*)w->xcg, mask);
+            hipLaunchKernelGGL((k_add_correction<double, S>), dim3(gv), dim3(256), 0, st, m->V, x64,
+                               (const SV<double, S>*)w->vb[VB_X], mask);
         }
         DSV_HIP(hipGetLastError());
         // the state's iteration counters restart in every inner solve; accumulate them
@@ -1250,7 +1463,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     }
     double* xo[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
     hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm,
-                       (const SV<double, S>*)w->x64, xo[0], xo[1], xo[2]);
+                       (const SV<double, S>*)x64, xo[0], xo[1], xo[2]);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipStreamSynchronize(st));
     const double ms =
@@ -1274,65 +1487,67 @@ static void apply_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     const int method = sizeof(T) == 4 ? DSV_METHOD_MF32_IR64 : DSV_METHOD_MF64;
     Workspace* w = get_ws(m, method, S);
     hipStream_t st = w->stream;
-    const Partition& P = m->part;
     prepare_bc(m, w, S, sp, st);
-    prepare_op<T, S>(m, w, P, sp, inv_dt, lumped, st, 0);
+    prepare_op<T, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
     const unsigned gv = cdiv(m->V, 256);
     MFVec<T, S> v{};
-    v.x = (const SV<T, S>*)w->zcg;
-    v.y = (SV<T, S>*)w->qcg;
+    v.x = (const SV<T, S>*)w->vb[VB_V1];
+    v.y = (SV<T, S>*)w->vb[VB_V2];
     hipLaunchKernelGGL((k_pack<T, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, x[0],
-                       S > 1 ? x[1] : nullptr, S > 2 ? x[2] : nullptr, (SV<T, S>*)w->zcg);
-    launch_mf<T, S, MF_APPLY>(m, P, w->prm, v, w->st, nullptr, 0, nullptr, 0, st);
+                       S > 1 ? x[1] : nullptr, S > 2 ? x[2] : nullptr, (SV<T, S>*)w->vb[VB_V1]);
+    launch_mf<T, S, MF_APPLY>(m, m->part, w->prm, v, nullptr, st);
     hipLaunchKernelGGL((k_unpack<T, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
-                       (const SV<T, S>*)w->qcg, y[0], S > 1 ? y[1] : nullptr,
+                       (const SV<T, S>*)w->vb[VB_V2], y[0], S > 1 ? y[1] : nullptr,
                        S > 2 ? y[2] : nullptr);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipStreamSynchronize(st));
 }
 
-// Times the fused kernel on `stream`: mode 0 = apply, mode 1 = CG form (A_k, parity 0, with
-// every species active and beta = 0 so that repeated launches are identical).
+// Times one kernel on `stream`: mode 0 = matrix-free apply; mode 1 = one matrix-free CG
+// iteration K_0 (every species active, partials gamma = delta = rho = 1, so that repeated
+// launches take identical decisions).
 template <class T, int S>
 static double time_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped, int mode,
                         int n_rep, hipStream_t stream) {
     const int method = sizeof(T) == 4 ? DSV_METHOD_MF32_IR64 : DSV_METHOD_MF64;
     Workspace* w = get_ws(m, method, S);
     hipStream_t ws = w->stream;
-    const Partition& P = m->part;
     prepare_bc(m, w, S, sp, ws);
-    prepare_op<T, S>(m, w, P, sp, inv_dt, lumped, ws, 0);
+    prepare_op<T, S>(m, w, m->part, sp, inv_dt, lumped, ws, 0);
     const unsigned gv = cdiv(m->V, 256);
-    // z = x (internal order) when every x is given, else the lumped mass; p_old = 0
+    // inputs: x (internal order) when every x is given, else the lumped mass; the diagonal
     const bool have_x = sp[0].x && (S < 2 || sp[1].x) && (S < 3 || sp[2].x);
     const double* x0 = have_x ? sp[0].x : m->lumped_m;
     const double* x1 = S > 1 ? (have_x ? sp[1].x : m->lumped_m) : nullptr;
     const double* x2 = S > 2 ? (have_x ? sp[2].x : m->lumped_m) : nullptr;
-    hipLaunchKernelGGL((k_pack<T, S>), dim3(gv), dim3(256), 0, ws, m->V, m->perm, x0, x1, x2,
-                       (SV<T, S>*)w->zcg);
-    DSV_HIP(hipMemsetAsync(w->pa, 0, m->V * rec_bytes(sizeof(T), S), ws));
-    // state: every species active before A_0 (act[1]), k = 0, tolerance 0; the U partials
-    // the kernel consumes hold r.r = 1 per species
+    MFVec<T, S> dv{};
+    dv.y = (SV<T, S>*)w->vb[VB_DINV];
+    launch_mf<T, S, MF_DIAG>(m, m->part, w->prm, dv, nullptr, ws);
+    for (int id : {VB_V1, VB_V3, VB_V5, VB_V7, VB_X})
+        hipLaunchKernelGGL((k_pack<T, S>), dim3(gv), dim3(256), 0, ws, m->V, m->perm, x0, x1, x2,
+                           (SV<T, S>*)w->vb[id]);
     CGState h = read_state(w, ws);
     for (int s = 0; s < MAXS; ++s) {
         h.act[1][s] = s < S ? 1 : 0;
         h.tol2[s] = 0.0;
-        h.dead[s] = 0;
     }
-    h.k = 0;
+    h.kk[1] = 0;
     h.max_it = 1 << 30;
     write_state(w, h, ws);
-    std::vector<double> pu((size_t)2 * MAXS * MAX_PROD, 0.0);
-    const int nU = upd_grid(m);
-    for (int s = 0; s < S; ++s) pu[(size_t)(S + s) * nU] = 1.0;
-    DSV_HIP(hipMemcpy(w->pU[1], pu.data(), pu.size() * sizeof(double), hipMemcpyHostToDevice));
-    const MFVec<T, S> v = cg_vecs<T, S>(w, 0);
+    std::vector<double> pu((size_t)3 * MAXS * MAX_PROD, 0.0);
+    const int n = mf_grid(m->part);
+    for (int q = 0; q < 3 * S; ++q) pu[(size_t)q * n] = 1.0;
+    DSV_HIP(hipMemcpy(w->pU[0], pu.data(), pu.size() * sizeof(double), hipMemcpyHostToDevice));
+    MFVec<T, S> v{};
+    v.x = (const SV<T, S>*)w->vb[VB_V1];
+    v.y = (SV<T, S>*)w->vb[VB_V2];
     auto launch = [&] {
         if (mode == 1)
-            launch_mf<T, S, MF_CG>(m, P, w->prm, v, w->st, w->pU[1], nU, w->pA, 0, stream);
+            launch_cg<T, S, false>(m, w, 0, stream);
         else
-            launch_mf<T, S, MF_APPLY>(m, P, w->prm, v, w->st, nullptr, 0, nullptr, 0, stream);
+            launch_mf<T, S, MF_APPLY>(m, m->part, w->prm, v, nullptr, stream);
     };
+    DSV_HIP(hipStreamSynchronize(ws));
     for (int i = 0; i < 3; ++i) launch();
     hipEvent_t e0, e1;
     DSV_HIP(hipEventCreate(&e0));
@@ -1419,7 +1634,7 @@ int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         check_solve_args(m, n_species, sp, false);
         if (!ms || n_rep < 1) throw ArgError{"dsv_time_apply: bad arguments"};
         if (precision != 32 && precision != 64) throw ArgError{"precision must be 32 or 64"};
-        if (mode != 0 && mode != 1) throw ArgError{"mode must be 0 (apply) or 1 (CG form)"};
+        if (mode != 0 && mode != 1) throw ArgError{"mode must be 0 (apply) or 1 (CG iteration)"};
         DSV_HIP(hipSetDevice(m->device));
         DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
         hipStream_t s = (hipStream_t)stream;

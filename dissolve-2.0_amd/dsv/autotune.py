@@ -17,7 +17,7 @@ from . import _lib
 from .core import species_array, species_struct
 
 DEFAULT_NV = (64, 128, 256, 512)
-DEFAULT_THREADS = (128, 256, 512, 1024)
+DEFAULT_THREADS = (256, 512)
 DEFAULT_LANES = (1, 2, 4)
 
 

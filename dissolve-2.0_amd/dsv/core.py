@@ -104,7 +104,7 @@ class Mesh:
                   int(lanes_per_vertex))
         self.refresh_info()
 
-    def set_solver_options(self, inner_rtol=1e-3, max_outer=12, graph_chunk=8, use_graphs=True):
+    def set_solver_options(self, inner_rtol=1e-3, max_outer=12, graph_chunk=16, use_graphs=True):
         _lib.call("dsv_set_solver_options", self._h, float(inner_rtol), int(max_outer),
                   int(graph_chunk), int(bool(use_graphs)))
 

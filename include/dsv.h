@@ -96,9 +96,10 @@ int dsv_mesh_copy_view(const dsv_mesh* m, int32_t which, void* dst, void* stream
 
 /* Repartitions the matrix-free blocks (vertices_per_block, constraints as in
  * dsv_mesh_create; DSV_E_ARG leaves the previous partition in place), the lanes that share
- * one vertex's incident cells (lanes_per_vertex: 1, 2 or 4; default 4) and sets the
- * workgroup size (64, 128, 256, 512 or 1024) of the matrix-free kernels; used by the
- * autotuner.  No counterpart in the reference (launch-shape tuning of this library). */
+ * one vertex's incident cells (lanes_per_vertex: 1, 2 or 4, with vertices_per_block *
+ * lanes_per_vertex <= 1024; default 4) and sets the workgroup size (256 or 512) of the
+ * matrix-free kernels; used by the autotuner.  No counterpart in the reference (launch-shape
+ * tuning of this library). */
 int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t block_threads,
                            int32_t lanes_per_vertex);
 

@@ -142,8 +142,9 @@ def test_matrix_free_apply_three_species(torch_dev, name, lumped):
             assert _relerr(y.cpu().numpy(), ref) < tol, (prec, name)
 
 
-@pytest.mark.parametrize("nv,threads,lanes", [(64, 64, 1), (64, 256, 2), (128, 128, 4),
-                                              (128, 1024, 4), (256, 512, 1), (256, 256, 4)])
+@pytest.mark.parametrize("nv,threads,lanes", [(64, 256, 1), (64, 512, 2), (128, 256, 4),
+                                              (128, 512, 4), (256, 512, 1), (256, 256, 4),
+                                              (512, 512, 2)])
 def test_matrix_free_apply_launch_shapes(torch_dev, nv, threads, lanes):
     """Every (block size, workgroup size, lanes per vertex) computes the same operator: slices
     with padded lanes, workgroups that loop over several slices and the cross-lane sums."""
@@ -167,6 +168,10 @@ def test_matrix_free_apply_launch_shapes(torch_dev, nv, threads, lanes):
         m.set_partition(96, 256, 4)     # not a multiple of 64
     with pytest.raises(ValueError):
         m.set_partition(128, 256, 3)    # lanes per vertex must be 1, 2 or 4
+    with pytest.raises(ValueError):
+        m.set_partition(512, 256, 4)    # at most 1024 lane slots per block
+    with pytest.raises(ValueError):
+        m.set_partition(128, 128, 4)    # workgroups of 256 or 512 threads
 
 
 def _solve_case(name, lumped=1, seed=6):
@@ -211,6 +216,30 @@ def test_batched_solve_matches_direct(torch_dev, method, name, lumped):
         assert st["rel_residual"][k] <= 1e-11
         err = _relerr(sp[k]["x"].cpu().numpy(), refs[k])
         assert err < 1e-8, (method, k, err, st)
+
+
+@pytest.mark.parametrize("nv,threads,lanes", [(64, 512, 2), (256, 256, 4), (128, 256, 1)])
+@pytest.mark.parametrize("method", ["mf64", "mf32_ir64"])
+def test_matrix_free_solve_launch_shapes(torch_dev, nv, threads, lanes, method):
+    """The fused CG iteration (several staging and lane rounds per thread, workgroups with
+    several partition blocks) converges to the direct solution on every launch shape."""
+    import torch
+    from dsv.core import Mesh
+    from dsv.solvers import solve_batched
+    conn, xyz, sps, u0, bs, refs, inv_dt = _solve_case("disc_small", 1, seed=8)
+    m = Mesh(conn, xyz)
+    m.set_partition(nv, threads, lanes)
+    sp = []
+    for s, u, b in zip(sps, u0, bs):
+        d = dict(s)
+        d.pop("f", None)
+        d["b"] = torch.tensor(b, dtype=torch.float64, device=torch_dev)
+        d["x"] = torch.tensor(u, dtype=torch.float64, device=torch_dev)
+        sp.append(d)
+    st = solve_batched(m, sp, inv_dt, rtol=1e-11, method=method, lumped=1)
+    for k in range(3):
+        assert st["converged"][k], st
+        assert _relerr(sp[k]["x"].cpu().numpy(), refs[k]) < 1e-8, (k, st)
 
 
 def test_single_species_and_warm_start(torch_dev):
