@@ -95,6 +95,15 @@ struct Partition {
     int32_t* sptr = nullptr;      // nb*(NV*L/64)+1 slice offsets (multiples of 64)
     uint32_t* ell = nullptr;      // packed corner triples
     int32_t* ell_cell = nullptr;  // caller cell id per entry, -1 for padding
+    // Stored-geometry operator: the same sliced-ELL layout over the graph neighbours w != v
+    // of every owned vertex (lanes ranked by neighbour count, `elperm`), holding the local
+    // index of w and the FP32 weight K_vw of the unit-diffusivity stiffness (padding: the
+    // lane's own vertex with weight 0).
+    int64_t n_edge_entries = 0;
+    int32_t* eptr = nullptr;      // nb*(NV*L/64)+1 slice offsets
+    uint16_t* elperm = nullptr;   // nb*NV: neighbour-count rank -> owned local index
+    uint16_t* eidx = nullptr;     // local neighbour index per entry
+    float* ewt = nullptr;         // K_vw(D = 1) per entry
 };
 
 struct Workspace;  // solver buffers, defined in solve.hip
@@ -125,6 +134,10 @@ struct dsv_mesh {
     float* xyz32 = nullptr;       // 4V AoS (x, y, z, 0) internal
     double* xyz64 = nullptr;      // 4V AoS (x, y, z, 0) internal
     dsv::Partition part;          // matrix-free kernels (autotuned)
+    // unit-diffusivity stiffness K(D = 1) in CSR order (FP64, host), built once for the
+    // stored-geometry operator
+    std::vector<double> h_kgeo;
+    std::vector<int32_t> h_indptr, h_indices;
     std::vector<int32_t> h_conn;  // caller numbering
     std::vector<int32_t> h_iperm;
     // solver options
@@ -132,11 +145,14 @@ struct dsv_mesh {
     int max_outer = 12;
     int graph_chunk = 16;
     int use_graphs = 1;
+    int operator_form = DSV_FORM_STORED;  // inner FP32 CG operator
     dsv::Workspace* ws[3][4] = {};  // [method][n_species]
 };
 
 namespace dsv {
 void build_partition(dsv_mesh* m, int NV, int L);
+void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped, double* values,
+                   hipStream_t st);
 void free_partition(Partition& p);
 void free_workspaces(dsv_mesh* m);
 void ensure_geometry_views(dsv_mesh* m);

@@ -163,10 +163,40 @@ void free_partition(Partition& p) {
     dfree(p.sptr);
     dfree(p.ell);
     dfree(p.ell_cell);
+    dfree(p.eptr);
+    dfree(p.elperm);
+    dfree(p.eidx);
+    dfree(p.ewt);
+    p.n_edge_entries = 0;
     p.nb = 0;
     p.n_entries = 0;
     p.n_halo = 0;
     p.max_local = 0;
+}
+
+// K(D = 1) (stiffness of unit diffusivity, no mass, no Dirichlet rows) and the CSR pattern on
+// the host, assembled once by the deterministic FP64 assembly
+static void ensure_unit_stiffness(dsv_mesh* m) {
+    if (!m->h_kgeo.empty()) return;
+    dsv_species unit{};
+    unit.D_value = 1.0;
+    double* kd = dalloc<double>(m->nnz);
+    try {
+        assemble_into(m, unit, 0.0, 1, kd, 0);
+        DSV_HIP(hipDeviceSynchronize());
+        m->h_kgeo.resize(m->nnz);
+        m->h_indptr.resize((size_t)m->V + 1);
+        m->h_indices.resize(m->nnz);
+        DSV_HIP(hipMemcpy(m->h_kgeo.data(), kd, m->nnz * sizeof(double), hipMemcpyDeviceToHost));
+        DSV_HIP(hipMemcpy(m->h_indptr.data(), m->indptr, ((size_t)m->V + 1) * sizeof(int32_t),
+                          hipMemcpyDeviceToHost));
+        DSV_HIP(hipMemcpy(m->h_indices.data(), m->indices, m->nnz * sizeof(int32_t),
+                          hipMemcpyDeviceToHost));
+    } catch (...) {
+        dfree(kd);
+        throw;
+    }
+    dfree(kd);
 }
 
 template <class T>
@@ -208,6 +238,17 @@ void build_partition(dsv_mesh* m, int NV, int L) {
     std::vector<int32_t> ell_cell;
     std::vector<int> order;
     int max_local = 0;
+    // stored-geometry operator: K(D = 1) per graph edge, assembled once (FP64, deterministic)
+    ensure_unit_stiffness(m);
+    const int32_t* kp = m->h_indptr.data();
+    const int32_t* ki = m->h_indices.data();
+    const double* kv = m->h_kgeo.data();
+    std::vector<int32_t> perm_h(V);
+    for (int32_t c = 0; c < V; ++c) perm_h[ip[c]] = c;
+    std::vector<int32_t> eptr((size_t)nb * spb + 1, 0);
+    std::vector<uint16_t> elperm((size_t)nb * NV, 0), eidx;
+    std::vector<float> ewt;
+    std::vector<int> eorder;
     for (int b = 0; b < nb; ++b) {
         const int v0 = b * NV, nv = std::min(NV, V - v0);
         halo.clear();
@@ -266,6 +307,39 @@ void build_partition(dsv_mesh* m, int NV, int L) {
                 }
             }
         }
+        // edge lists of the block's owned vertices (graph neighbours w != v, CSR order)
+        auto ndeg = [&](int i) { return kp[perm_h[v0 + i] + 1] - kp[perm_h[v0 + i]] - 1; };
+        eorder.resize(nv);
+        std::iota(eorder.begin(), eorder.end(), 0);
+        std::stable_sort(eorder.begin(), eorder.end(), [&](int a, int c) { return ndeg(a) > ndeg(c); });
+        for (int r = 0; r < nv; ++r) elperm[(size_t)b * NV + r] = (uint16_t)eorder[r];
+        for (int s = 0; s < spb; ++s) {
+            const size_t sidx = (size_t)b * spb + s;
+            const int r0 = vps * s, r1 = std::min(nv, vps * s + vps);
+            int width = 0;
+            for (int r = r0; r < r1; ++r) width = std::max(width, (ndeg(eorder[r]) + L - 1) / L);
+            const size_t base = eidx.size();
+            if (base + 64 * (size_t)width >= (size_t)INT32_MAX)
+                throw ArgError{"stored-geometry lists too large for int32 offsets"};
+            eptr[sidx + 1] = (int32_t)(base + 64 * (size_t)width);
+            eidx.resize(base + 64 * (size_t)width, 0);
+            ewt.resize(base + 64 * (size_t)width, 0.0f);
+            for (int r = r0; r < r1; ++r) {
+                const int ov = eorder[r];
+                const int c = perm_h[v0 + ov];
+                for (int k = 0; k < L; ++k)  // padding: the lane's own vertex, weight 0
+                    for (int t = 0; t < width; ++t)
+                        eidx[base + 64 * (size_t)t + (r - r0) * L + k] = (uint16_t)ov;
+                int i = 0;
+                for (int32_t j = kp[c]; j < kp[c + 1]; ++j) {
+                    if (ki[j] == c) continue;
+                    const size_t at = base + 64 * (size_t)(i / L) + (r - r0) * L + i % L;
+                    eidx[at] = (uint16_t)local(ip[ki[j]]);
+                    ewt[at] = (float)kv[j];
+                    ++i;
+                }
+            }
+        }
         hlist.insert(hlist.end(), halo.begin(), halo.end());
         if ((int64_t)hlist.size() >= (int64_t)INT32_MAX) throw ArgError{"halo lists too large"};
         hptr[b + 1] = (int32_t)hlist.size();
@@ -285,12 +359,17 @@ void build_partition(dsv_mesh* m, int NV, int L) {
     p.nb = nb;
     p.threads = m->part.threads ? m->part.threads : 256;
     p.n_entries = (int64_t)ell.size();
+    p.n_edge_entries = (int64_t)eidx.size();
     p.n_halo = (int64_t)hlist.size();
     p.max_local = max_local;
     try {
         p.hptr = upload(hptr);
         p.hlist = upload(hlist);
         p.lvl = upload(lvl);
+        p.eptr = upload(eptr);
+        p.elperm = upload(elperm);
+        p.eidx = upload(eidx);
+        p.ewt = upload(ewt);
         p.lperm = upload(lperm);
         p.sptr = upload(sptr);
         p.ell = upload(ell);

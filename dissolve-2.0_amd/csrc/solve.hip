@@ -108,7 +108,8 @@ template <class T>
 struct SpeciesP {
     const T* D;
     const T* A;
-    T D6;
+    T D6;  // fast forms: D / 6
+    T D1;  // stored-geometry form: D
 };
 
 template <class T>
@@ -119,6 +120,7 @@ struct OpParams {
     T inv_dt;
     int lumped;
     int fast;
+    int stored;  // stored-geometry form of the CG kernel (fast form, FP32)
 };
 
 // vectors of the apply-type kernels (vertex order of the kernel)
@@ -169,6 +171,10 @@ struct Geo {
     const uint16_t* lperm;
     const int32_t* sptr;
     const uint32_t* ell;
+    const int32_t* eptr;
+    const uint16_t* elperm;
+    const uint16_t* eidx;
+    const float* ewt;
 };
 
 template <class T>
@@ -348,6 +354,52 @@ __device__ __forceinline__ void finish_fast(const OpParams<T>& prm, int vtx, con
     }
 }
 
+// Stored-geometry rows: acc[s] = sum over graph neighbours w of K_vw(D=1) (u_w - u_v) for the
+// owned vertex of lane slot `ls`, summed over its L lanes (all lanes of the wave must call
+// this).  Padding entries point at the vertex itself with weight 0, so there is no branch.
+template <int S>
+__device__ __forceinline__ void edge_rows(const Geo& g, const SV<float, S>* spv, int b, int nv,
+                                          int ls, int& ov, bool& live, SV<float, S>& own,
+                                          float (&acc)[S]) {
+    const int L = g.L;
+    const int vps = 64 / L;
+    const int spb = g.NV / vps;
+    const int lane = ls & 63;
+    const int rank = ls / L;  // < NV: ls < NV * L
+    const int* sp0 = g.eptr + b * spb + (ls >> 6);
+    const int jbeg = sp0[0] + lane;
+    const int width = (sp0[1] - sp0[0]) >> 6;
+    live = rank < nv;
+    const int ovl = (int)g.elperm[b * g.NV + rank];
+    ov = live ? ovl : 0;
+    own = spv[ov];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+    for (int t0 = 0; t0 < width; t0 += 4) {
+        int ix[4];
+        float wt[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = jbeg + 64 * min(t0 + u, width - 1);
+            const int iv = g.eidx[j];
+            const float wv = g.ewt[j];
+            ix[u] = t0 + u < width ? iv : ov;
+            wt[u] = t0 + u < width ? wv : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const SV<float, S> nb = spv[ix[u]];
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[s] += wt[u] * (nb.v[s] - own.v[s]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if (L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
+        if (L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+    }
+}
+
 // Workgroup -> partition blocks: XCD x (= workgroup id mod 8) owns blocks
 // [x*per_xcd, (x+1)*per_xcd), taken round-robin by its workgroups.
 struct BlockRange {
@@ -443,7 +495,7 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
 // matrix-free Chronopoulos-Gear PCG iteration K_i (parity par = i & 1), or the INIT kernel
 // (r_0 = (T) rin for the solved species, 0 otherwise; x = p = s_{-1} = 0; u_0 = r_0 / diag;
 // w_0 = A u_0).  Partials per workgroup: gamma, delta, rho of the next iteration.
-template <class T, int S, bool FAST, bool INIT>
+template <class T, int S, bool FAST, bool INIT, bool STORED>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
      CGState* __restrict__ st, const double* __restrict__ pin, int nin,
@@ -534,7 +586,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
         // (1) stage u_{i+1} of every local vertex; owned vertices also advance p, x, s, r
         for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
             const int gv = g.lvl[(size_t)b * ml + i];
-            sgeo[i] = xyz[gv];
+            if constexpr (!STORED) sgeo[i] = xyz[gv];
             const SV<T, S> d = v.dinv[gv];
             SV<T, S> rn{}, un{}, sn{};
             if constexpr (INIT) {
@@ -585,10 +637,22 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
             bool live;
             SV<T, S> own;
             T acc[S];
-            lane_rows<T, S, ROW_MATVEC, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
+            if constexpr (STORED) {
+                static_assert(std::is_same<T, float>::value && FAST, "stored geometry: FP32 fast form");
+                edge_rows<S>(g, spv, b, nv, ls, ov, live, own, acc);
+            } else {
+                lane_rows<T, S, ROW_MATVEC, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
+            }
             if (!live || (ls & (g.L - 1)) != 0) continue;
             const int vtx = v0 + ov;
-            if constexpr (FAST) finish_fast<T, S, ROW_MATVEC>(prm, vtx, own, acc);
+            if constexpr (STORED) {
+                // D K(D=1) u + mdiag u
+                const SV<T, S> md = reinterpret_cast<const SV<T, S>*>(prm.mdiag)[vtx];
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[s] = prm.sp[s].D1 * acc[s] + md.v[s] * own.v[s];
+            } else if constexpr (FAST) {
+                finish_fast<T, S, ROW_MATVEC>(prm, vtx, own, acc);
+            }
             const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
             SV<T, S> out{};
 #pragma unroll
@@ -1064,6 +1128,10 @@ static Geo make_geo(const dsv_mesh* m, const Partition& P) {
     g.lperm = P.lperm;
     g.sptr = P.sptr;
     g.ell = P.ell;
+    g.eptr = P.eptr;
+    g.elperm = P.elperm;
+    g.eidx = P.eidx;
+    g.ewt = P.ewt;
     return g;
 }
 
@@ -1135,8 +1203,10 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
             *out[c] = (const T*)buf;
         }
         h.sp[s].D6 = (T)(sp[s].D_value / 6.0);
+        h.sp[s].D1 = (T)sp[s].D_value;
     }
     h.fast = 0;
+    h.stored = 0;
     h.mdiag = nullptr;
     if (fast) {
         if (!w->mdiag[set]) DSV_HIP(hipMalloc(&w->mdiag[set], (size_t)m->V * rec_bytes(sizeof(T), S)));
@@ -1147,6 +1217,7 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
         launch_mf<T, S, MF_MASS>(m, P, hb, v, nullptr, st);
         h.fast = 1;
         h.mdiag = w->mdiag[set];
+        h.stored = (sizeof(T) == 4 && m->operator_form == DSV_FORM_STORED && P.eptr) ? 1 : 0;
     }
     std::memcpy(set ? w->prm2 : w->prm, &h, sizeof(h));
     DSV_HIP(hipGetLastError());
@@ -1180,11 +1251,18 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     const int n = mf_grid(P);
     const double* pin = INIT ? nullptr : w->pU[par];
     double* pout = INIT ? w->pU[0] : w->pU[par ^ 1];
+    if constexpr (std::is_same<T, float>::value) {
+        if (p.stored) {
+            hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds, stream,
+                               g, coords<T>(m), p, v, w->st, pin, n, pout, par);
+            return;
+        }
+    }
     if (p.fast)
-        hipLaunchKernelGGL((k_cg<T, S, true, INIT>), dim3(n), dim3(P.threads), lds, stream, g,
+        hipLaunchKernelGGL((k_cg<T, S, true, INIT, false>), dim3(n), dim3(P.threads), lds, stream, g,
                            coords<T>(m), p, v, w->st, pin, n, pout, par);
     else
-        hipLaunchKernelGGL((k_cg<T, S, false, INIT>), dim3(n), dim3(P.threads), lds, stream, g,
+        hipLaunchKernelGGL((k_cg<T, S, false, INIT, false>), dim3(n), dim3(P.threads), lds, stream, g,
                            coords<T>(m), p, v, w->st, pin, n, pout, par);
 }
 
@@ -1624,6 +1702,15 @@ int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 4; ++b)
                 if (m->ws[a][b]) m->ws[a][b]->graph_valid = 0;
+    });
+}
+
+int dsv_set_operator_form(dsv_mesh* m, int32_t form) {
+    return guard([&] {
+        if (!m) throw ArgError{"NULL mesh"};
+        if (form != DSV_FORM_RECOMPUTE && form != DSV_FORM_STORED)
+            throw ArgError{"form must be DSV_FORM_RECOMPUTE or DSV_FORM_STORED"};
+        m->operator_form = form;
     });
 }
 

@@ -27,7 +27,7 @@ METHODS = {"csr64": METHOD_CSR64, "mf32_ir64": METHOD_MF32_IR64, "mf64": METHOD_
 EXPORTS = (
     "dsv_last_error", "dsv_device_count", "dsv_mesh_create", "dsv_mesh_destroy", "dsv_mesh_info",
     "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs", "dsv_apply",
-    "dsv_solve", "dsv_set_solver_options", "dsv_time_apply",
+    "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form", "dsv_time_apply",
 )
 
 
@@ -100,6 +100,7 @@ def load():
         "dsv_solve": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, D, I32, P,
                                      ctypes.POINTER(Stats)]),
         "dsv_set_solver_options": (ctypes.c_int, [P, D, I32, I32, I32]),
+        "dsv_set_operator_form": (ctypes.c_int, [P, I32]),
         "dsv_time_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, I32, I32,
                                           P, ctypes.POINTER(D)]),
     }

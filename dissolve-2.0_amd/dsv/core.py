@@ -108,6 +108,13 @@ class Mesh:
         _lib.call("dsv_set_solver_options", self._h, float(inner_rtol), int(max_outer),
                   int(graph_chunk), int(bool(use_graphs)))
 
+    FORM_RECOMPUTE, FORM_STORED = 0, 1
+
+    def set_operator_form(self, form):
+        """Operator of the FP32 inner CG (scalar D, lumped mass): FORM_STORED (default) or
+        FORM_RECOMPUTE (element Jacobians recomputed on the fly)."""
+        _lib.call("dsv_set_operator_form", self._h, int(form))
+
     def close(self):
         if getattr(self, "_h", None):
             try:

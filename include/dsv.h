@@ -34,6 +34,10 @@ extern "C" {
 #define DSV_E_NOMEM (-4)    /* device allocation failed */
 #define DSV_E_BREAKDOWN (-5) /* CG breakdown (p.Ap <= 0 or non-finite) */
 
+/* operator forms of the inner FP32 matrix-free CG (dsv_set_operator_form) */
+#define DSV_FORM_RECOMPUTE 0  /* element Jacobians recomputed on the fly from coordinates */
+#define DSV_FORM_STORED 1     /* FP32 stored geometry: unit-diffusivity stiffness weights */
+
 /* solve methods */
 #define DSV_METHOD_CSR64 0    /* assembled FP64 CSR + FP64 Jacobi-PCG ("plain FP64 path") */
 #define DSV_METHOD_MF32_IR64 1 /* FP32 matrix-free Jacobi-PCG inside FP64 iterative refinement */
@@ -137,6 +141,13 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
  * sweeps) and the number of CG iterations per captured hipGraph chunk. */
 int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
                            int32_t graph_chunk, int32_t use_graphs);
+
+/* Operator form of the FP32 inner CG of DSV_METHOD_MF32_IR64 when every species has a scalar
+ * D and the mass is lumped (otherwise the generic recomputing form is used):
+ * DSV_FORM_STORED (default) applies D K(D=1) + diag((inv_dt + alpha) M) with K(D=1) stored
+ * per graph edge in FP32; DSV_FORM_RECOMPUTE recomputes every element Jacobian.  No
+ * counterpart in the reference (its operator choice is libCEED's). */
+int dsv_set_operator_form(dsv_mesh* m, int32_t form);
 
 /* Average device duration (ms) of one launch of the fused n_species matrix-free kernel,
  * measured with HIP events around n_rep back-to-back launches on `stream`.  mode 0 times the
