@@ -1,10 +1,13 @@
 """Benchmark: three-species backward-Euler steps/s on the ~800k-tet graded disc, rtol 1e-8.
 
 One step = RHS assembly + batched solve of O2, Cl-, OH- to ||b - A x|| <= 1e-8 ||b|| (FP64
-true residual).  Two GPU paths run on the same trajectory from the same initial state:
-  plain  -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64)
-  chosen -- FP32 matrix-free fused 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64)
-`value` is the chosen path's steps/s.  The roofline entry is the fused FP32 3-species CG iteration
+true residual).  Three GPU paths advance the same trajectory from the same initial state (W
+untimed warm-up steps, then K timed steps):
+  plain   -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64), previous-solution start
+  fp32    -- fused FP32 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64), previous start
+  chosen  -- the fp32 path started from the learned warm start, trained on the W warm-up steps
+`value` is the chosen path's steps/s; the timed steps include the warm start's features,
+inference and residual judge.  The roofline entry is the fused FP32 3-species CG iteration
 kernel (the solve's dominant kernel), timed with HIP events on its own stream; the CPU baseline
 is the oracle (numpy assembly + C OpenMP Jacobi-PCG) on one 3-species step of the same mesh.
 
@@ -27,12 +30,14 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cg_iteration_bytes(C, V, n_species=3, word=4):
-    """Algorithmic bytes of one fused matrix-free CG iteration (FP32 words): the incidence
-    lists (one 4-byte word per (cell, corner), 16C), coordinates and the folded mass diagonal
-    (3 + S words per vertex), and the species vectors -- dinv, r, w, s, p, x read and r, w, s,
-    p, x written (11 S words per vertex)."""
-    return 16 * C + word * V * (3 + n_species + 11 * n_species)
+def cg_iteration_bytes(C, V, nnz, n_species=3, word=4, stored=True):
+    """Algorithmic bytes of one fused FP32 CG iteration.  Operator: stored geometry -- one
+    (2-byte local index, 4-byte weight) entry per directed graph edge, 6 (nnz - V); recomputed
+    geometry -- one 4-byte incidence word per (cell, corner), 16C, plus the coordinates (3 words
+    per vertex).  Per vertex: the folded mass diagonal (S words) and the species vectors --
+    dinv, r, w, s, p, x read and r, w, s, p, x written (11 S words)."""
+    op = 6 * (nnz - V) if stored else 16 * C + word * 3 * V
+    return op + word * V * (n_species + 11 * n_species)
 
 
 def apply_bytes(C, V, n_species=3, word=4):
@@ -52,12 +57,24 @@ def load_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_path(mesh, w, method, steps, warmup, rtol):
+def run_path(mesh, w, method, steps, warmup, rtol, learned=False, epochs=20):
     import torch
     from dsv.transport import Stepper
-    st = Stepper(mesh, w["species"], w["dt"], method=method, rtol=rtol)
-    for i in range(warmup):
-        st.step()
+    mlp = None
+    if learned:
+        from dsv.mlp import WarmStartMLP
+        mlp = WarmStartMLP(mesh.device)
+    st = Stepper(mesh, w["species"], w["dt"], method=method, rtol=rtol, phi=w.get("phi"),
+                 grad_phi=w.get("grad_norm"), eps=w.get("eps"), mlp=mlp)
+    train_loss = None
+    if learned:
+        t = time.perf_counter()
+        train_loss = st.train_warm_start(warmup, epochs=epochs)
+        log(f"warm start trained on {warmup} steps in {time.perf_counter() - t:.1f}s, "
+            f"loss {train_loss[0]:.3g} -> {train_loss[-1]:.3g}")
+    else:
+        for i in range(warmup):
+            st.step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     its = []
@@ -72,6 +89,8 @@ def run_path(mesh, w, method, steps, warmup, rtol):
         "iterations_per_step": [h["iterations"] for h in its],
         "outer_per_step": [h["outer"] for h in its],
         "rel_residual_last": its[-1]["rel_residual"],
+        "train_loss": train_loss,
+        "warm_choice": [e["mlp"] for e in st.warm_log[-steps:]] if learned else None,
     }, st
 
 
@@ -136,7 +155,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=8,
+                    help="untimed steps of every path; the learned warm start trains on them")
     ap.add_argument("--workload", default="disc3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-reps", type=int, default=50)
@@ -174,15 +194,23 @@ def main():
         tuned = {"vertices_per_block": best[0], "threads": best[1], "lanes_per_vertex": best[2]}
         log(f"tuned: {tuned}")
     res = {}
-    for method in ("csr64", "mf32_ir64"):
-        r, _ = run_path(mesh, w, method, a.steps, a.warmup, rtol)
-        res[method] = r
-        log(f"{method}: {r['steps_per_s']:.2f} steps/s, iterations/step {r['iterations_per_step'][-1]}, "
+    for name, method, learned in (("plain", "csr64", False), ("fp32", "mf32_ir64", False),
+                                  ("chosen", "mf32_ir64", True)):
+        r, st = run_path(mesh, w, method, a.steps, a.warmup, rtol, learned=learned)
+        res[name] = r
+        log(f"{name} ({method}{', learned warm start' if learned else ''}): "
+            f"{r['steps_per_s']:.2f} steps/s, iterations/step {r['iterations_per_step'][-1]}, "
             f"converged={r['converged']}")
+        if name == "fp32":
+            ref_u = [u.clone() for u in st.u]
+        if name == "chosen":
+            dev = max(float((u - v).abs().max() / v.abs().max().clamp_min(1e-300))
+                      for u, v in zip(st.u, ref_u))
+            log(f"chosen vs fp32-previous-start solutions after the timed steps: max rel diff {dev:.2e}")
 
     # roofline of the fused FP32 3-species CG iteration (the chosen path's dominant kernel)
     ms_iter, ms_plain_apply = time_kernels(mesh, w, a.apply_reps)
-    nbytes = cg_iteration_bytes(C, V, len(w["species"]))
+    nbytes = cg_iteration_bytes(C, V, mesh.nnz, len(w["species"]))
     achieved = nbytes / (ms_iter * 1e-3) / 1e9
     log(f"fused FP32 CG iteration: {ms_iter * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
         f"(matrix-free apply alone {ms_plain_apply * 1e3:.1f} us, "
@@ -197,8 +225,11 @@ def main():
                          f"Jacobi-PCG to rtol {rtol:g}), {t_cpu:.1f}s"}
         log(f"cpu baseline: {t_cpu:.1f}s per step on {threads} threads")
 
-    chosen = res["mf32_ir64"]
-    plain = res["csr64"]
+    chosen = res["chosen"]
+    plain = res["plain"]
+    fp32 = res["fp32"]
+    it_prev = sum(sum(i) for i in fp32["iterations_per_step"]) / a.steps
+    it_learned = sum(sum(i) for i in chosen["iterations_per_step"]) / a.steps
     out = {
         "metric": "3-species BE steps/s at 800k tets, rtol 1e-8: chosen vs plain FP64; HBM %",
         "value": round(chosen["steps_per_s"], 3), "unit": "steps/s", "n_gpus": 1,
@@ -207,11 +238,20 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{w['name']}: graded Delaunay disc r=1 h=0.2, {C} tets, {V} "
                                f"vertices, O2/Cl/OH D=2.0/2.0/5.3, dt={w['dt']}, rtol={rtol:g}",
-                   "method": "FP32 matrix-free fused 3-species Jacobi-PCG in FP64 refinement, "
-                             "previous-solution warm start, hipGraph chunks",
+                   "method": "FP32 fused 3-species Jacobi-PCG (Chronopoulos-Gear, one kernel per "
+                             "iteration, stored FP32 geometry) in FP64 refinement, "
+                             "learned MLP warm start (previous solution as fallback), "
+                             "hipGraph chunks",
                    "parallelism": "1 GPU", "launch_shape": tuned},
         "plain_fp64": {"steps_per_s": round(plain["steps_per_s"], 3),
                        "iterations_last_step": plain["iterations_per_step"][-1]},
+        "fp32_previous_start": {"steps_per_s": round(fp32["steps_per_s"], 3),
+                                "iterations_last_step": fp32["iterations_per_step"][-1]},
+        "learned_warm_start": {"cg_iterations_per_step": round(it_learned, 1),
+                               "cg_iterations_per_step_previous_start": round(it_prev, 1),
+                               "cg_iterations_saved_pct": round(100 * (1 - it_learned / it_prev), 1),
+                               "train_loss_first_last": [chosen["train_loss"][0], chosen["train_loss"][-1]],
+                               "mlp_chosen_per_species_last_step": chosen["warm_choice"][-1]},
         "chosen_vs_plain": round(chosen["steps_per_s"] / plain["steps_per_s"], 3),
         "iterations_last_step": chosen["iterations_per_step"][-1],
         "outer_last_step": chosen["outer_per_step"][-1],

@@ -1,0 +1,304 @@
+// Learned CG warm start: a per-vertex MLP 7 -> 64 -> 64 -> 1 (tanh) that predicts the next
+// backward-Euler increment of a species from local features; inference, MSE gradient and
+// Adam on the GPU.  No counterpart in the reference (its solves start from the previous
+// solution, `numerics/solvers.py:322-372`).
+//
+// Parameters (FP32, flat): W1[64][7], b1[64], W2[64][64], b2[64], w3[64], b3.
+// Features of vertex v of species s at step n (row of X, 7 FP32 values):
+//   u_n, du/sigma, clamp(phi/eps, -4, 4)/4, min(|grad phi|, 4)/4, log(D)/2, alpha*dt, dirichlet
+// with du = u_n - u_{n-1} and sigma = max_v |du| (a per-species, per-step scale); the network
+// predicts (u_{n+1} - u_n)/sigma.
+//
+// The gradient is computed without atomics: workgroups of 64 lanes take 64-sample chunks, lane
+// j owns row j of every weight matrix and accumulates over the chunks it sees in a fixed
+// order; a second kernel sums the per-workgroup gradients in workgroup order.
+#include <algorithm>
+#include <cmath>
+
+#include "dsv_internal.h"
+#include "kernels_common.h"
+
+namespace dsv {
+
+constexpr int MLP_IN = 7;
+constexpr int MLP_H = 64;
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + MLP_H * MLP_IN;
+constexpr int OFF_W2 = OFF_B1 + MLP_H;
+constexpr int OFF_B2 = OFF_W2 + MLP_H * MLP_H;
+constexpr int OFF_W3 = OFF_B2 + MLP_H;
+constexpr int OFF_B3 = OFF_W3 + MLP_H;
+constexpr int MLP_NPARAM = OFF_B3 + 1;  // 4737
+constexpr int GRAD_WG = 64;             // lanes (and samples per chunk) of the gradient kernel
+constexpr int GRAD_MAX_WG = 1024;
+
+static_assert(MLP_NPARAM == DSV_MLP_NPARAM, "parameter count of the C-ABI");
+
+__device__ __forceinline__ void load_params(const float* __restrict__ p, float* sp) {
+    for (int i = threadIdx.x; i < MLP_NPARAM; i += blockDim.x) sp[i] = p[i];
+    __syncthreads();
+}
+
+// forward of one sample with parameters in LDS; h1 (and h2 when given) kept for backprop
+__device__ __forceinline__ float mlp_forward(const float* sp, const float (&x)[MLP_IN],
+                                             float (&h1)[MLP_H], float* h2_out) {
+#pragma unroll
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = sp[OFF_B1 + j];
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) a += sp[OFF_W1 + j * MLP_IN + i] * x[i];
+        h1[j] = tanhf(a);
+    }
+    float out = sp[OFF_B3];
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = sp[OFF_B2 + j];
+#pragma unroll
+        for (int k = 0; k < MLP_H; ++k) a += sp[OFF_W2 + j * MLP_H + k] * h1[k];
+        const float h = tanhf(a);
+        if (h2_out) h2_out[j] = h;
+        out += sp[OFF_W3 + j] * h;
+    }
+    return out;
+}
+
+__global__ void __launch_bounds__(256)
+k_mlp_forward(const float* __restrict__ params, const float* __restrict__ X, int n,
+              float* __restrict__ out) {
+    __shared__ float sp[MLP_NPARAM];
+    load_params(params, sp);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[MLP_IN];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) x[k] = X[(size_t)i * MLP_IN + k];
+    float h1[MLP_H];
+    out[i] = mlp_forward(sp, x, h1, nullptr);
+}
+
+// Gradient of L = mean_i (f(x_i) - y_i)^2 (partials per workgroup: NPARAM values + loss sum)
+__global__ void __launch_bounds__(GRAD_WG)
+k_mlp_grad(const float* __restrict__ params, const float* __restrict__ X,
+           const float* __restrict__ y, int n, float* __restrict__ gpart) {
+    __shared__ float sp[MLP_NPARAM];
+    __shared__ float sx[GRAD_WG][MLP_IN];
+    __shared__ float sh1[GRAD_WG][MLP_H + 1];  // +1: lane-strided rows without bank conflicts
+    __shared__ float sh2[GRAD_WG][MLP_H + 1];  // h2, then dL/da2, then dL/da1
+    __shared__ float sg[GRAD_WG];
+    load_params(params, sp);
+    const int t = threadIdx.x;  // sample slot in a chunk, and the row j = t owned for gradients
+    const float inv_n = 1.0f / (float)n;
+    float gW1[MLP_IN], gW2[MLP_H];
+    float gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, loss = 0.f;
+#pragma unroll
+    for (int i = 0; i < MLP_IN; ++i) gW1[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < MLP_H; ++k) gW2[k] = 0.f;
+    const int nchunks = (n + GRAD_WG - 1) / GRAD_WG;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const int i = c * GRAD_WG + t;
+        const bool live = i < n;
+        float x[MLP_IN];
+#pragma unroll
+        for (int k = 0; k < MLP_IN; ++k) {
+            x[k] = live ? X[(size_t)i * MLP_IN + k] : 0.f;
+            sx[t][k] = x[k];
+        }
+        float h1[MLP_H];
+        const float out = mlp_forward(sp, x, h1, sh2[t]);
+#pragma unroll
+        for (int k = 0; k < MLP_H; ++k) sh1[t][k] = h1[k];
+        const float r = live ? out - y[i] : 0.f;
+        loss += r * r;
+        const float g = 2.f * r * inv_n;  // dL/dout of this sample
+        sg[t] = g;
+        __syncthreads();
+        // row t of the output layer, and dL/da2 of sample t (in place of h2 after use)
+        {
+            float s = 0.f;
+            for (int q = 0; q < GRAD_WG; ++q) s += sg[q] * sh2[q][t];
+            gw3 += s;
+            if (t == 0)
+                for (int q = 0; q < GRAD_WG; ++q) gb3 += sg[q];
+        }
+        __syncthreads();
+        for (int j = 0; j < MLP_H; ++j) {
+            const float h = sh2[t][j];
+            sh2[t][j] = g * sp[OFF_W3 + j] * (1.f - h * h);
+        }
+        __syncthreads();
+        // row t of W2 / b2
+        for (int q = 0; q < GRAD_WG; ++q) {
+            const float d = sh2[q][t];
+            gb2 += d;
+#pragma unroll
+            for (int k = 0; k < MLP_H; ++k) gW2[k] += d * sh1[q][k];
+        }
+        // dL/da1 of sample t (from its own dL/da2 row), then stored over that row
+        float d1[MLP_H];
+#pragma unroll
+        for (int k = 0; k < MLP_H; ++k) {
+            float s = 0.f;
+            for (int j = 0; j < MLP_H; ++j) s += sp[OFF_W2 + j * MLP_H + k] * sh2[t][j];
+            d1[k] = s * (1.f - h1[k] * h1[k]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MLP_H; ++k) sh2[t][k] = d1[k];
+        __syncthreads();
+        // row t of W1 / b1
+        for (int q = 0; q < GRAD_WG; ++q) {
+            const float d = sh2[q][t];
+            gb1 += d;
+#pragma unroll
+            for (int k = 0; k < MLP_IN; ++k) gW1[k] += d * sx[q][k];
+        }
+        __syncthreads();  // LDS rows are refilled by the next chunk
+    }
+    float* gp = gpart + (size_t)blockIdx.x * (MLP_NPARAM + 1);
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) gp[OFF_W1 + t * MLP_IN + k] = gW1[k];
+    gp[OFF_B1 + t] = gb1;
+#pragma unroll
+    for (int k = 0; k < MLP_H; ++k) gp[OFF_W2 + t * MLP_H + k] = gW2[k];
+    gp[OFF_B2 + t] = gb2;
+    gp[OFF_W3 + t] = gw3;
+    // loss: lane sums in lane order by lane 0
+    sg[t] = loss;
+    __syncthreads();
+    if (t == 0) {
+        float l = 0.f;
+        for (int q = 0; q < GRAD_WG; ++q) l += sg[q];
+        gp[OFF_B3] = gb3;
+        gp[MLP_NPARAM] = l;
+    }
+}
+
+// grad[p] = sum over workgroups of the partials, in workgroup order; loss = mean square error
+__global__ void k_mlp_grad_reduce(const float* __restrict__ gpart, int nwg, int n,
+                                  float* __restrict__ grad, float* __restrict__ loss) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > MLP_NPARAM) return;
+    float s = 0.f;
+    for (int w = 0; w < nwg; ++w) s += gpart[(size_t)w * (MLP_NPARAM + 1) + p];
+    if (p < MLP_NPARAM)
+        grad[p] = s;
+    else if (loss)
+        *loss = s / (float)n;
+}
+
+__global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                           float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
+                           float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// |x| maximum over n values as float bits (non-negative floats order like their bits)
+__global__ void k_absmax_bits(int n, const double* __restrict__ a, const double* __restrict__ b,
+                              unsigned* __restrict__ out) {
+    unsigned best = 0u;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float d = (float)fabs(a[i] - b[i]);
+        best = max(best, __float_as_uint(d));
+    }
+    for (int off = 32; off > 0; off >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+}
+
+__global__ void k_mlp_features(int n, const double* __restrict__ u, const double* __restrict__ u_prev,
+                               const double* __restrict__ phi, const double* __restrict__ gphi,
+                               const double* __restrict__ alpha, double alpha_value, double D,
+                               double dt, double eps, const uint8_t* __restrict__ bcmask,
+                               const unsigned* __restrict__ sigma_bits, float* __restrict__ X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float sig = fmaxf(__uint_as_float(*sigma_bits), 1e-30f);
+    float* x = X + (size_t)i * MLP_IN;
+    x[0] = (float)u[i];
+    x[1] = (float)(u[i] - u_prev[i]) / sig;
+    x[2] = fminf(fmaxf((float)(phi[i] / eps), -4.f), 4.f) * 0.25f;
+    x[3] = fminf((float)gphi[i], 4.f) * 0.25f;
+    x[4] = (float)(0.5 * log(D));
+    x[5] = (float)((alpha ? alpha[i] : alpha_value) * dt);
+    x[6] = bcmask && bcmask[i] ? 1.f : 0.f;
+}
+
+}  // namespace dsv
+
+using namespace dsv;
+
+extern "C" {
+
+int dsv_mlp_forward(const float* params, const float* X, int32_t n, float* out, void* stream) {
+    return guard([&] {
+        if (!params || !X || !out || n < 0) throw ArgError{"dsv_mlp_forward: bad arguments"};
+        if (n == 0) return;
+        hipLaunchKernelGGL(k_mlp_forward, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                           params, X, n, out);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+size_t dsv_mlp_grad_workspace(int32_t n) {
+    const int chunks = (int)cdiv(n > 0 ? n : 1, GRAD_WG);
+    const int nwg = chunks < GRAD_MAX_WG ? chunks : GRAD_MAX_WG;
+    return (size_t)nwg * (MLP_NPARAM + 1) * sizeof(float);
+}
+
+int dsv_mlp_grad(const float* params, const float* X, const float* y, int32_t n, float* grad,
+                 float* loss, void* workspace, size_t workspace_bytes, void* stream) {
+    return guard([&] {
+        if (!params || !X || !y || !grad || !workspace || n < 1)
+            throw ArgError{"dsv_mlp_grad: bad arguments"};
+        if (workspace_bytes < dsv_mlp_grad_workspace(n))
+            throw ArgError{"dsv_mlp_grad: workspace smaller than dsv_mlp_grad_workspace(n)"};
+        const int chunks = (int)cdiv(n, GRAD_WG);
+        const int nwg = std::min(chunks, GRAD_MAX_WG);
+        hipStream_t st = (hipStream_t)stream;
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(GRAD_WG), 0, st, params, X, y, n,
+                           (float*)workspace);
+        hipLaunchKernelGGL(k_mlp_grad_reduce, dim3(cdiv(MLP_NPARAM + 1, 256)), dim3(256), 0, st,
+                           (const float*)workspace, nwg, n, grad, loss);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr,
+                 float beta1, float beta2, float eps, void* stream) {
+    return guard([&] {
+        if (!params || !grad || !m || !v || step < 1) throw ArgError{"dsv_mlp_adam: bad arguments"};
+        const float c1 = 1.f - std::pow(beta1, (float)step);
+        const float c2 = 1.f - std::pow(beta2, (float)step);
+        hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(MLP_NPARAM, 256)), dim3(256), 0, (hipStream_t)stream,
+                           params, grad, m, v, MLP_NPARAM, lr, beta1, beta2, eps, c1, c2);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_mlp_features(int32_t n, const double* u, const double* u_prev, const double* phi,
+                     const double* grad_phi, const double* alpha, double alpha_value, double D,
+                     double dt, double eps, const uint8_t* bc_mask, float* X, float* sigma,
+                     void* stream) {
+    return guard([&] {
+        if (n < 1 || !u || !u_prev || !phi || !grad_phi || !X || !sigma)
+            throw ArgError{"dsv_mlp_features: bad arguments"};
+        if (!(D > 0.0) || !(dt > 0.0) || !(eps > 0.0))
+            throw ArgError{"dsv_mlp_features: D, dt and eps must be > 0"};
+        hipStream_t st = (hipStream_t)stream;
+        unsigned* bits = reinterpret_cast<unsigned*>(sigma);  // max |du| as float bits
+        DSV_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned), st));
+        hipLaunchKernelGGL(k_absmax_bits, dim3(std::min<unsigned>(cdiv(n, 256), 1024)), dim3(256), 0,
+                           st, n, u, u_prev, bits);
+        hipLaunchKernelGGL(k_mlp_features, dim3(cdiv(n, 256)), dim3(256), 0, st, n, u, u_prev, phi,
+                           grad_phi, alpha, alpha_value, D, dt, eps, bc_mask, bits, X);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -210,26 +210,28 @@ __device__ __forceinline__ void block_sum_all(double (&v)[N], double* scratch) {
 }
 
 // Sum partials[q*nprod + b] over b < nprod for q < NQ; identical result in every thread of
-// every workgroup that calls it.  All NQ loads of a step are issued together.
-template <int NQ>
-__device__ __forceinline__ void consume_partials(const double* __restrict__ partials, int nprod,
+// every workgroup that calls it.  All NQ loads of a step are issued together.  Partials are
+// FP64, or FP32 (P = float) where the consumer is the FP32 inner CG: every workgroup of the
+// consuming kernel reads all of them, so their size sets the cost of the reduction.
+template <int NQ, class P = double>
+__device__ __forceinline__ void consume_partials(const P* __restrict__ partials, int nprod,
                                                  double (&out)[NQ], double* scratch) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) out[q] = 0.0;
     for (int b = threadIdx.x; b < nprod; b += blockDim.x)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) out[q] += partials[q * nprod + b];
+        for (int q = 0; q < NQ; ++q) out[q] += (double)partials[q * nprod + b];
     block_sum_all<NQ>(out, scratch);
 }
 
 // Writes the NQ block sums of `tot` as this workgroup's partials.
-template <int NQ>
+template <int NQ, class P = double>
 __device__ __forceinline__ void write_partials(double (&tot)[NQ], double* scratch,
-                                               double* __restrict__ pout) {
+                                               P* __restrict__ pout) {
     block_sum_all<NQ>(tot, scratch);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) pout[q * gridDim.x + blockIdx.x] = tot[q];
+        for (int q = 0; q < NQ; ++q) pout[q * gridDim.x + blockIdx.x] = (P)tot[q];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -498,8 +500,8 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
 template <class T, int S, bool FAST, bool INIT, bool STORED>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
-     CGState* __restrict__ st, const double* __restrict__ pin, int nin,
-     double* __restrict__ pout, int par) {
+     CGState* __restrict__ st, const T* __restrict__ pin, int nin, T* __restrict__ pout,
+     int par) {
     extern __shared__ __align__(32) unsigned char smem_raw[];
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
     SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
@@ -536,7 +538,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
             return;
         }
         double sums[3 * S];
-        consume_partials<3 * S>(pin, nin, sums, red);  // gamma_i, delta_i, rho_i
+        consume_partials<3 * S, T>(pin, nin, sums, red);  // gamma_i, delta_i, rho_i
         double gam[S], alp[S];
         bool bad[S];
 #pragma unroll
@@ -671,7 +673,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
         tot[S + s] = pd[s];
         tot[2 * S + s] = pr[s];
     }
-    write_partials<3 * S>(tot, red, pout);
+    write_partials<3 * S, T>(tot, red, pout);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -988,9 +990,12 @@ struct Workspace {
     CGState* st = nullptr;
     CGState* h_st = nullptr;  // pinned
     double* pA = nullptr;     // CSR: partials of A_k (S per workgroup)
-    double* pU[2] = {};       // CSR: partials of U_k; MF: partials of K_i (3S per workgroup)
+    double* pU[2] = {};       // CSR: partials of U_k; MF: partials of K_i (3S per workgroup,
+                              // in the CG precision)
     double* pR = nullptr;     // partials of the residual kernel (2S per workgroup)
     double* h_pR = nullptr;   // pinned copy
+    int* h_flags = nullptr;   // pinned activity flags of the two chunks in flight
+    hipEvent_t ev[2] = {};
     // operator coefficients: [0] kernels of the CG precision, [1] FP64 residual of the
     // refinement path; per-ELL-entry D then alpha per species, and the fast form's mass
     // diagonal (species records)
@@ -1017,6 +1022,9 @@ static void free_ws(Workspace* w) {
     dfree(w->st);
     if (w->h_st) (void)hipHostFree(w->h_st);
     if (w->h_pR) (void)hipHostFree(w->h_pR);
+    if (w->h_flags) (void)hipHostFree(w->h_flags);
+    for (int i = 0; i < 2; ++i)
+        if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
     dfree(w->pA);
     dfree(w->pU[0]);
     dfree(w->pU[1]);
@@ -1083,6 +1091,9 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S) {
         DSV_HIP(hipHostMalloc((void**)&w->h_st, sizeof(CGState), hipHostMallocDefault));
         DSV_HIP(hipHostMalloc((void**)&w->h_pR, sizeof(double) * 2 * MAXS * MAX_PROD,
                               hipHostMallocDefault));
+        DSV_HIP(hipHostMalloc((void**)&w->h_flags, 2 * sizeof(int), hipHostMallocDefault));
+        for (int i = 0; i < 2; ++i)
+            DSV_HIP(hipEventCreateWithFlags(&w->ev[i], hipEventDisableTiming));
         w->pA = dalloc<double>((size_t)MAXS * MAX_PROD);
         w->pU[0] = dalloc<double>((size_t)3 * MAXS * MAX_PROD);
         w->pU[1] = dalloc<double>((size_t)3 * MAXS * MAX_PROD);
@@ -1249,8 +1260,8 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     const size_t lds = checked_lds<T, S>(m, P);
     const CGVec<T, S> v = cg_vecs<T, S>(w);
     const int n = mf_grid(P);
-    const double* pin = INIT ? nullptr : w->pU[par];
-    double* pout = INIT ? w->pU[0] : w->pU[par ^ 1];
+    const T* pin = INIT ? nullptr : (const T*)w->pU[par];
+    T* pout = (T*)(INIT ? w->pU[0] : w->pU[par ^ 1]);
     if constexpr (std::is_same<T, float>::value) {
         if (p.stored) {
             hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds, stream,
@@ -1337,17 +1348,25 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream) {
         w->graph_chunk = chunk;
         std::memcpy(w->graph_prm, w->prm, sizeof(w->prm));
     }
-    while (true) {
+    // Chunk k+1 is queued before the host waits for chunk k's activity flag, so the device
+    // never idles on the poll; once every species is done the extra chunk's kernels return at
+    // their first check.
+    auto enqueue = [&](int slot) {
         if (m->use_graphs) {
             DSV_HIP(hipGraphLaunch(w->graph, stream));
         } else {
             for (int i = 0; i < chunk; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
         }
         launched += chunk;
-        DSV_HIP(hipMemcpyAsync(&w->h_st->any_active, &w->st->any_active, sizeof(int),
+        DSV_HIP(hipMemcpyAsync(&w->h_flags[slot], &w->st->any_active, sizeof(int),
                                hipMemcpyDeviceToHost, stream));
-        DSV_HIP(hipStreamSynchronize(stream));
-        if (!w->h_st->any_active || launched > max_it + 2 * chunk) break;
+        DSV_HIP(hipEventRecord(w->ev[slot], stream));
+    };
+    enqueue(0);
+    for (int k = 0;; ++k) {
+        if (launched <= max_it + 2 * chunk) enqueue((k + 1) & 1);
+        DSV_HIP(hipEventSynchronize(w->ev[k & 1]));
+        if (!w->h_flags[k & 1] || launched > max_it + 2 * chunk) break;
     }
     return launched;
 }
@@ -1559,6 +1578,30 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     }
 }
 
+// ||b - A x||^2 and ||b||^2 per species for x = sp[s].x with x[bc] = bc_value (FP64,
+// matrix-free), the check dsv_solve makes before its first iteration
+template <int S>
+static void residual_norms_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped,
+                               double* rr, double* bb) {
+    Workspace* w = get_ws(m, DSV_METHOD_MF64, S);
+    hipStream_t st = w->stream;
+    const unsigned gv = cdiv(m->V, 256);
+    const double* bs[3] = {sp[0].b, S > 1 ? sp[1].b : nullptr, S > 2 ? sp[2].b : nullptr};
+    const double* xs[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
+    SV<double, S>* x64 = (SV<double, S>*)w->vb[VB_X64];
+    hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, bs[0], bs[1],
+                       bs[2], (SV<double, S>*)w->vb[VB_B64]);
+    hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, xs[0], xs[1],
+                       xs[2], x64);
+    for (int s = 0; s < S; ++s)
+        if (sp[s].n_bc > 0)
+            hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
+                               sp[s].n_bc, sp[s].bc_dofs, m->iperm, s, sp[s].bc_value, x64);
+    prepare_bc(m, w, S, sp, st);
+    prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
+    residual64<S>(m, w, st, rr, bb);
+}
+
 template <class T, int S>
 static void apply_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped,
                        const double* const* x, double* const* y) {
@@ -1612,10 +1655,10 @@ static double time_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int l
     h.kk[1] = 0;
     h.max_it = 1 << 30;
     write_state(w, h, ws);
-    std::vector<double> pu((size_t)3 * MAXS * MAX_PROD, 0.0);
+    std::vector<T> pu((size_t)3 * MAXS * MAX_PROD, T(0));  // partials in the CG precision
     const int n = mf_grid(m->part);
-    for (int q = 0; q < 3 * S; ++q) pu[(size_t)q * n] = 1.0;
-    DSV_HIP(hipMemcpy(w->pU[0], pu.data(), pu.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int q = 0; q < 3 * S; ++q) pu[(size_t)q * n] = T(1);
+    DSV_HIP(hipMemcpy(w->pU[0], pu.data(), pu.size() * sizeof(T), hipMemcpyHostToDevice));
     MFVec<T, S> v{};
     v.x = (const SV<T, S>*)w->vb[VB_V1];
     v.y = (SV<T, S>*)w->vb[VB_V2];
@@ -1702,6 +1745,26 @@ int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 4; ++b)
                 if (m->ws[a][b]) m->ws[a][b]->graph_valid = 0;
+    });
+}
+
+int dsv_residual_norms(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                       int32_t lumped, double* rnorm, double* bnorm, void* stream) {
+    return guard([&] {
+        check_solve_args(m, n_species, sp, true);
+        if (!rnorm) throw ArgError{"dsv_residual_norms: NULL rnorm"};
+        DSV_HIP(hipSetDevice(m->device));
+        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        double rr[MAXS] = {0, 0, 0}, bb[MAXS] = {0, 0, 0};
+        switch (n_species) {
+            case 1: residual_norms_impl<1>(m, sp, inv_dt, lumped, rr, bb); break;
+            case 2: residual_norms_impl<2>(m, sp, inv_dt, lumped, rr, bb); break;
+            default: residual_norms_impl<3>(m, sp, inv_dt, lumped, rr, bb); break;
+        }
+        for (int s = 0; s < n_species; ++s) {
+            rnorm[s] = std::sqrt(rr[s]);
+            if (bnorm) bnorm[s] = std::sqrt(bb[s]);
+        }
     });
 }
 

@@ -21,6 +21,7 @@
 #ifndef DSV_H
 #define DSV_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -137,6 +138,12 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
               int32_t lumped, int32_t method, double rtol, int32_t max_it, void* stream,
               dsv_stats* stats);
 
+/* ||b_s - A_s x_s||_2 (rnorm, host array of n_species) and ||b_s||_2 (bnorm, may be NULL)
+ * for x_s = sp[s].x with its Dirichlet entries replaced by bc_value -- the residual dsv_solve
+ * starts from; FP64, matrix-free.  Used to judge candidate initial guesses. */
+int dsv_residual_norms(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                       int32_t lumped, double* rnorm, double* bnorm, void* stream);
+
 /* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
  * sweeps) and the number of CG iterations per captured hipGraph chunk. */
 int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
@@ -156,6 +163,32 @@ int dsv_set_operator_form(dsv_mesh* m, int32_t form);
 int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
                    int32_t lumped, int32_t precision, int32_t mode, int32_t n_rep, void* stream,
                    double* ms);
+
+/* --- learned warm start (no counterpart in the reference, whose solves start from the
+ * previous solution): a per-vertex MLP 7 -> 64 -> 64 -> 1 with tanh activations, FP32
+ * parameters packed as W1[64][7], b1[64], W2[64][64], b2[64], w3[64], b3.  Device pointers. */
+#define DSV_MLP_NPARAM 4737
+#define DSV_MLP_NFEAT 7
+
+/* Features (X: n x 7 FP32) of one species at step n, per vertex (caller numbering): u_n,
+ * (u_n - u_prev)/sigma, clamp(phi/eps, -4, 4)/4, min(|grad phi|, 4)/4, log(D)/2,
+ * alpha dt (alpha NULL -> alpha_value), Dirichlet flag (bc_mask NULL -> 0).  sigma (one device
+ * FP32) = max |u_n - u_prev|; the network's output times sigma is the predicted increment. */
+int dsv_mlp_features(int32_t n, const double* u, const double* u_prev, const double* phi,
+                     const double* grad_phi, const double* alpha, double alpha_value, double D,
+                     double dt, double eps, const uint8_t* bc_mask, float* X, float* sigma,
+                     void* stream);
+/* out[i] = f(X[i]) for n samples */
+int dsv_mlp_forward(const float* params, const float* X, int32_t n, float* out, void* stream);
+/* Bytes of device workspace dsv_mlp_grad needs for n samples. */
+size_t dsv_mlp_grad_workspace(int32_t n);
+/* grad = dL/dparams and *loss = L of L = mean_i (f(X[i]) - y[i])^2 (loss may be NULL);
+ * deterministic (fixed summation order, no atomics). */
+int dsv_mlp_grad(const float* params, const float* X, const float* y, int32_t n, float* grad,
+                 float* loss, void* workspace, size_t workspace_bytes, void* stream);
+/* One Adam step (step >= 1 counts from 1) on all DSV_MLP_NPARAM parameters. */
+int dsv_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr,
+                 float beta1, float beta2, float eps, void* stream);
 
 #ifdef __cplusplus
 }

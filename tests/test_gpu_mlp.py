@@ -1,0 +1,129 @@
+"""GPU kernels of the learned warm start against the CPU oracle (oracle/mlp.py), and the gain
+of the trained network on a solver-generated time series.
+
+Bars: forward within 1e-5 of the FP64 oracle (absolute, outputs O(1)); gradient within 1e-4 of
+the largest gradient component; Adam within FP32 rounding; features within FP32 rounding.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as OM
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(seed=0, n=1000):
+    rng = np.random.default_rng(seed)
+    p = OM.init(seed)
+    p[OM.OFF_B1:OM.OFF_W2] = rng.normal(0, 0.3, OM.NH)
+    p[OM.OFF_B2:OM.OFF_W3] = rng.normal(0, 0.3, OM.NH)
+    X = rng.normal(0, 1, (n, OM.NIN)).astype(np.float32)
+    y = rng.normal(0, 1, n).astype(np.float32)
+    return p.astype(np.float32), X, y
+
+
+@pytest.fixture(scope="module")
+def dev(gpu):
+    return gpu
+
+
+def _mlp(dev, p):
+    import torch
+    from dsv.mlp import WarmStartMLP
+    m = WarmStartMLP(dev)
+    m.params.copy_(torch.as_tensor(p))
+    return m
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 70001])
+def test_forward_and_gradient_match_oracle(dev, n):
+    import torch
+    p, X, y = _net(1, n)
+    m = _mlp(dev, p)
+    Xt, yt = torch.as_tensor(X, device=dev), torch.as_tensor(y, device=dev)
+    out = m.forward(Xt).cpu().numpy()
+    p64, X64, y64 = p.astype(np.float64), X.astype(np.float64), y.astype(np.float64)
+    assert np.abs(out - OM.forward(p64, X64)).max() < 1e-5
+    g, loss = m.grad(Xt, yt)
+    gref = OM.grad(p64, X64, y64)
+    assert np.abs(g.cpu().numpy() - gref).max() <= 1e-4 * np.abs(gref).max()
+    assert abs(loss.item() - OM.loss(p64, X64, y64)) <= 1e-5 * OM.loss(p64, X64, y64)
+    # deterministic: a second evaluation is bit-identical
+    g2, _ = m.grad(Xt, yt)
+    assert torch.equal(g, g2)
+
+
+def test_adam_matches_oracle(dev):
+    import torch
+    p, X, y = _net(2, 500)
+    m = _mlp(dev, p)
+    Xt, yt = torch.as_tensor(X, device=dev), torch.as_tensor(y, device=dev)
+    pr, mr, vr = p.astype(np.float64), np.zeros(OM.NPARAM), np.zeros(OM.NPARAM)
+    for step in range(1, 4):
+        g, _ = m.grad(Xt, yt)
+        gn = g.cpu().numpy().astype(np.float64)
+        m.adam(g)
+        pr, mr, vr = OM.adam_step(pr, gn, mr, vr, step, m.lr)
+        assert np.abs(m.params.cpu().numpy() - pr).max() < 1e-6
+
+
+def test_features_match_oracle(dev):
+    import torch
+    from dsv.mlp import WarmStartMLP
+    rng = np.random.default_rng(3)
+    n = 5000
+    u, up = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    phi, gp, al = rng.normal(0, 0.05, n), rng.uniform(0, 6, n), rng.uniform(0, 40, n)
+    bc = (rng.uniform(0, 1, n) < 0.1).astype(np.uint8)
+    t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)
+    m = WarmStartMLP(dev)
+    X, sigma = m.features(t(u), t(up), t(phi), t(gp), 2.0, 0.005, 0.011, alpha=t(al),
+                          bc_mask=t(bc, torch.uint8))
+    Xr, sr = OM.features(u, up, phi, gp, al, 2.0, 0.005, 0.011, bc)
+    assert abs(sigma.item() - sr) <= 1e-7 * sr
+    assert np.abs(X.cpu().numpy() - Xr).max() < 2e-6
+    X2, _ = m.features(t(u), t(up), t(phi), t(gp), 2.0, 0.005, 0.011, alpha_value=1.5)
+    assert np.allclose(X2.cpu().numpy()[:, 5], 1.5 * 0.005) and X2.cpu().numpy()[:, 6].max() == 0
+
+
+def test_training_reduces_loss(dev):
+    import torch
+    from dsv.mlp import WarmStartMLP
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (20000, OM.NIN)).astype(np.float32)
+    y = (0.8 * X[:, 1] + 0.2 * np.tanh(X[:, 2]) * X[:, 0]).astype(np.float32)
+    m = WarmStartMLP(dev, lr=3e-3)
+    hist = m.fit(torch.as_tensor(X, device=dev), torch.as_tensor(y, device=dev), epochs=15,
+                 batch=2048)
+    assert hist[-1] < 0.02 * float(np.mean(y * y)), hist
+
+
+def test_learned_warm_start_saves_iterations(dev):
+    """Train on the first steps of a 3-species graded-disc run, then compare the following steps
+    against the previous-solution start: fewer CG iterations, same converged solutions."""
+    import torch
+    from dsv import workloads
+    from dsv.core import Mesh
+    from dsv.mlp import WarmStartMLP
+    from dsv.transport import Stepper
+    w = workloads.disc3(seed=1, scale=6.0)
+    mesh = Mesh(w["conn"], w["xyz"])
+    kw = dict(method="mf32_ir64", rtol=w["rtol"], phi=w["phi"], grad_phi=w["grad_norm"],
+              eps=w["eps"])
+    plain = Stepper(mesh, w["species"], w["dt"], **kw)
+    learned = Stepper(mesh, w["species"], w["dt"], mlp=WarmStartMLP(mesh.device), **kw)
+    n_train, n_eval = 8, 6
+    for _ in range(n_train):
+        plain.step()
+    hist = learned.train_warm_start(n_train, epochs=30)
+    assert hist[-1] < hist[0]
+    it_plain = it_learned = 0
+    for _ in range(n_eval):
+        it_plain += sum(plain.step()["iterations"])
+        it_learned += sum(learned.step()["iterations"])
+        for k in range(3):
+            a, b = plain.u[k].cpu().numpy(), learned.u[k].cpu().numpy()
+            assert np.abs(a - b).max() <= 1e-5 * max(np.abs(a).max(), 1e-30)
+    assert all(all(h["converged"]) for h in learned.history)
+    assert any(any(e["mlp"]) for e in learned.warm_log)
+    assert it_learned < it_plain, (it_learned, it_plain, learned.warm_log[-1])
