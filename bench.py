@@ -1,15 +1,20 @@
 """Benchmark: three-species backward-Euler steps/s on the ~800k-tet graded disc, rtol 1e-8.
 
 One step = RHS assembly + batched solve of O2, Cl-, OH- to ||b - A x|| <= 1e-8 ||b|| (FP64
-true residual).  Three GPU paths advance the same trajectory from the same initial state (W
+true residual).  Four GPU paths advance the same trajectory from the same initial state (W
 untimed warm-up steps, then K timed steps):
-  plain   -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64), previous-solution start
-  fp32    -- fused FP32 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64), previous start
-  chosen  -- the fp32 path started from the learned warm start, trained on the W warm-up steps
-`value` is the chosen path's steps/s; the timed steps include the warm start's features,
-inference and residual judge.  The roofline entry is the fused FP32 3-species CG iteration
-kernel (the solve's dominant kernel), timed with HIP events on its own stream; the CPU baseline
-is the oracle (numpy assembly + C OpenMP Jacobi-PCG) on one 3-species step of the same mesh.
+  plain    -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64), previous-solution
+              start (the reference's operator path and warm start)
+  fp32     -- fused FP32 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64), previous start
+  chosen   -- the fp32 path started from the Galerkin warm start over the last three solution
+              increments (previous solution as fallback)
+  learned  -- the fp32 path started from the Galerkin warm start over two increments and the
+              learned MLP direction, trained on the GPU on the W warm-up steps
+`value` is the chosen path's steps/s; the timed steps include every warm-start computation
+(features, inference, projection).  The roofline entry is the fused FP32 3-species CG
+iteration kernel (the solve's dominant kernel), timed with HIP events on its own stream; the
+CPU baseline is the oracle (numpy assembly + C OpenMP Jacobi-PCG) on one 3-species step of
+the same mesh.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload disc3]
 """
@@ -57,22 +62,30 @@ def load_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_path(mesh, w, method, steps, warmup, rtol, learned=False, epochs=20):
+def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30):
     import torch
     from dsv.transport import Stepper
     mlp = None
-    if learned:
+    if warm == "learned":
         from dsv.mlp import WarmStartMLP
         mlp = WarmStartMLP(mesh.device)
     st = Stepper(mesh, w["species"], w["dt"], method=method, rtol=rtol, phi=w.get("phi"),
                  grad_phi=w.get("grad_norm"), eps=w.get("eps"), mlp=mlp)
-    train_loss = None
-    if learned:
+    train = None
+    if warm == "learned":
+        st.set_warm_start("extrapolate", history_dirs=2)
+        torch.cuda.synchronize()
         t = time.perf_counter()
-        train_loss = st.train_warm_start(warmup, epochs=epochs)
-        log(f"warm start trained on {warmup} steps in {time.perf_counter() - t:.1f}s, "
-            f"loss {train_loss[0]:.3g} -> {train_loss[-1]:.3g}")
+        loss = st.train_warm_start(warmup, epochs=epochs)
+        torch.cuda.synchronize()
+        train = {"pairs_from_steps": warmup, "epochs": epochs,
+                 "seconds_incl_steps": round(time.perf_counter() - t, 2),
+                 "loss_first_last": [round(loss[0], 5), round(loss[-1], 5)]}
+        log(f"warm start trained on {warmup} steps in {train['seconds_incl_steps']}s, "
+            f"loss {loss[0]:.3g} -> {loss[-1]:.3g}")
     else:
+        if warm != "previous":
+            st.set_warm_start(warm, history_dirs=3)
         for i in range(warmup):
             st.step()
     torch.cuda.synchronize()
@@ -84,13 +97,15 @@ def run_path(mesh, w, method, steps, warmup, rtol, learned=False, epochs=20):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     conv = all(all(h["converged"]) for h in its)
+    log_tail = st.warm_log[-steps:] if warm != "previous" else []
     return {
         "steps_per_s": steps / t, "ms_per_step": 1e3 * t / steps, "converged": conv,
         "iterations_per_step": [h["iterations"] for h in its],
         "outer_per_step": [h["outer"] for h in its],
         "rel_residual_last": its[-1]["rel_residual"],
-        "train_loss": train_loss,
-        "warm_choice": [e["mlp"] for e in st.warm_log[-steps:]] if learned else None,
+        "train": train,
+        "taken_per_step": [e["taken"] for e in log_tail],
+        "coef_last_step": log_tail[-1]["coef"] if log_tail else None,
     }, st
 
 
@@ -194,19 +209,23 @@ def main():
         tuned = {"vertices_per_block": best[0], "threads": best[1], "lanes_per_vertex": best[2]}
         log(f"tuned: {tuned}")
     res = {}
-    for name, method, learned in (("plain", "csr64", False), ("fp32", "mf32_ir64", False),
-                                  ("chosen", "mf32_ir64", True)):
-        r, st = run_path(mesh, w, method, a.steps, a.warmup, rtol, learned=learned)
+    ref_u = None
+    for name, method, warm in (("plain", "csr64", "previous"), ("fp32", "mf32_ir64", "previous"),
+                               ("chosen", "mf32_ir64", "extrapolate"),
+                               ("learned", "mf32_ir64", "learned")):
+        r, st = run_path(mesh, w, method, a.steps, a.warmup, rtol, warm=warm)
         res[name] = r
-        log(f"{name} ({method}{', learned warm start' if learned else ''}): "
-            f"{r['steps_per_s']:.2f} steps/s, iterations/step {r['iterations_per_step'][-1]}, "
-            f"converged={r['converged']}")
+        log(f"{name} ({method}, {warm} start): {r['steps_per_s']:.2f} steps/s, "
+            f"iterations/step {r['iterations_per_step']}, converged={r['converged']}")
         if name == "fp32":
             ref_u = [u.clone() for u in st.u]
-        if name == "chosen":
+        elif ref_u is not None:
             dev = max(float((u - v).abs().max() / v.abs().max().clamp_min(1e-300))
                       for u, v in zip(st.u, ref_u))
-            log(f"chosen vs fp32-previous-start solutions after the timed steps: max rel diff {dev:.2e}")
+            r["max_rel_diff_vs_fp32_previous_start"] = dev
+            log(f"{name} vs fp32 previous-start solutions after the timed steps: "
+                f"max rel diff {dev:.2e}")
+        del st
 
     # roofline of the fused FP32 3-species CG iteration (the chosen path's dominant kernel)
     ms_iter, ms_plain_apply = time_kernels(mesh, w, a.apply_reps)
@@ -225,11 +244,14 @@ def main():
                          f"Jacobi-PCG to rtol {rtol:g}), {t_cpu:.1f}s"}
         log(f"cpu baseline: {t_cpu:.1f}s per step on {threads} threads")
 
-    chosen = res["chosen"]
-    plain = res["plain"]
-    fp32 = res["fp32"]
-    it_prev = sum(sum(i) for i in fp32["iterations_per_step"]) / a.steps
-    it_learned = sum(sum(i) for i in chosen["iterations_per_step"]) / a.steps
+    chosen, plain, fp32, learned = res["chosen"], res["plain"], res["fp32"], res["learned"]
+
+    def its(r):
+        return round(sum(sum(i) for i in r["iterations_per_step"]) / a.steps, 1)
+
+    def saved(r):
+        return round(100 * (1 - its(r) / its(fp32)), 1)
+
     out = {
         "metric": "3-species BE steps/s at 800k tets, rtol 1e-8: chosen vs plain FP64; HBM %",
         "value": round(chosen["steps_per_s"], 3), "unit": "steps/s", "n_gpus": 1,
@@ -239,23 +261,29 @@ def main():
         "config": {"workload": f"{w['name']}: graded Delaunay disc r=1 h=0.2, {C} tets, {V} "
                                f"vertices, O2/Cl/OH D=2.0/2.0/5.3, dt={w['dt']}, rtol={rtol:g}",
                    "method": "FP32 fused 3-species Jacobi-PCG (Chronopoulos-Gear, one kernel per "
-                             "iteration, stored FP32 geometry) in FP64 refinement, "
-                             "learned MLP warm start (previous solution as fallback), "
-                             "hipGraph chunks",
+                             "iteration, stored FP32 geometry) in FP64 refinement, FP64 "
+                             "Galerkin warm start over the last 3 solution increments "
+                             "(previous solution as fallback), hipGraph chunks",
                    "parallelism": "1 GPU", "launch_shape": tuned},
         "plain_fp64": {"steps_per_s": round(plain["steps_per_s"], 3),
-                       "iterations_last_step": plain["iterations_per_step"][-1]},
+                       "cg_iterations_per_step": its(plain)},
         "fp32_previous_start": {"steps_per_s": round(fp32["steps_per_s"], 3),
-                                "iterations_last_step": fp32["iterations_per_step"][-1]},
-        "learned_warm_start": {"cg_iterations_per_step": round(it_learned, 1),
-                               "cg_iterations_per_step_previous_start": round(it_prev, 1),
-                               "cg_iterations_saved_pct": round(100 * (1 - it_learned / it_prev), 1),
-                               "train_loss_first_last": [chosen["train_loss"][0], chosen["train_loss"][-1]],
-                               "mlp_chosen_per_species_last_step": chosen["warm_choice"][-1]},
+                                "cg_iterations_per_step": its(fp32)},
+        "chosen_galerkin_extrapolation": {"steps_per_s": round(chosen["steps_per_s"], 3),
+                                          "cg_iterations_per_step": its(chosen),
+                                          "cg_iterations_saved_pct_vs_previous_start": saved(chosen)},
+        "learned_warm_start": {"steps_per_s": round(learned["steps_per_s"], 3),
+                               "cg_iterations_per_step": its(learned),
+                               "cg_iterations_saved_pct_vs_previous_start": saved(learned),
+                               "training": learned["train"],
+                               "mlp_coef_last_step": learned["coef_last_step"][-1]
+                               if learned["coef_last_step"] else None,
+                               "taken_last_step": learned["taken_per_step"][-1]
+                               if learned["taken_per_step"] else None},
         "chosen_vs_plain": round(chosen["steps_per_s"] / plain["steps_per_s"], 3),
         "iterations_last_step": chosen["iterations_per_step"][-1],
         "outer_last_step": chosen["outer_per_step"][-1],
-        "converged": chosen["converged"] and plain["converged"],
+        "converged": all(r["converged"] for r in res.values()),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_cg<float,3> (fused 3-species FP32 Jacobi-PCG iteration)",

@@ -1602,6 +1602,198 @@ static void residual_norms_impl(dsv_mesh* m, const dsv_species* sp, double inv_d
     residual64<S>(m, w, st, rr, bb);
 }
 
+// ---------------------------------------------------------------------------------------
+// Galerkin warm start: per species, x = x0 + sum_k c_k d_k with c minimising the A-energy
+// norm of the error over x0 + span{d_k}.  The directions are zero at Dirichlet vertices and
+// x0 holds the Dirichlet values there, so only the free block of A -- symmetric positive
+// definite -- enters, and c solves (d_j . A d_k) c_k = d_j . r0 with r0 = b - A x0.
+
+constexpr int PROJ_THREADS = 256;
+constexpr int PROJ_GRID = 256;  // S * 9 * PROJ_GRID partials fit the residual partial buffer
+static_assert(MAXS * 9 * PROJ_GRID <= 2 * MAXS * MAX_PROD, "projection partials");
+
+template <int S>
+struct ProjVec {
+    const SV<double, S>* d[3];
+    const SV<double, S>* Ad[3];
+    const SV<double, S>* r0;
+    const SV<double, S>* x0;
+    SV<double, S>* x;
+    double c[3][MAXS];
+};
+
+// partials per species: d_j . A d_k (j <= k), then d_j . r0
+template <int S, int ND>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv,
+                                                           double* __restrict__ pout) {
+    constexpr int NP = ND * (ND + 1) / 2 + ND;
+    __shared__ double red[S * NP * 16];
+    double tot[S * NP];
+#pragma unroll
+    for (int q = 0; q < S * NP; ++q) tot[q] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        SV<double, S> d[ND], Ad[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            d[k] = pv.d[k][i];
+            Ad[k] = pv.Ad[k][i];
+        }
+        const SV<double, S> r = pv.r0[i];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            int q = s * NP;
+#pragma unroll
+            for (int j = 0; j < ND; ++j)
+#pragma unroll
+                for (int k = j; k < ND; ++k) tot[q++] += d[j].v[s] * Ad[k].v[s];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) tot[q++] += d[j].v[s] * r.v[s];
+        }
+    }
+    write_partials<S * NP>(tot, red, pout);
+}
+
+// x = x0 + sum_k c_k d_k, and partials of ||r0 - sum_k c_k A d_k||^2 per species
+template <int S, int ND>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_combine(int V, ProjVec<S> pv,
+                                                              double* __restrict__ pout) {
+    __shared__ double red[S * 16];
+    double tot[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) tot[s] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        SV<double, S> x = pv.x0[i], r = pv.r0[i];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                x.v[s] += pv.c[k][s] * d.v[s];
+                r.v[s] -= pv.c[k][s] * Ad.v[s];
+            }
+        }
+        pv.x[i] = x;
+#pragma unroll
+        for (int s = 0; s < S; ++s) tot[s] += r.v[s] * r.v[s];
+    }
+    write_partials<S>(tot, red, pout);
+}
+
+// Solves G c = g for the n x n SPD Gram matrix G (row-major) by LDL^T, dropping a direction
+// whose pivot falls below rel_tol of its own energy d.Ad (numerically dependent on the
+// previous ones, or zero); a dropped direction gets c = 0.
+static void solve_gram(int n, const double* G, const double* g, double* c, double rel_tol) {
+    double L[3][3] = {}, Dg[3] = {}, y[3] = {};
+    bool keep[3] = {};
+    for (int j = 0; j < n; ++j) {
+        double dj = G[j * n + j];
+        for (int k = 0; k < j; ++k)
+            if (keep[k]) dj -= L[j][k] * L[j][k] * Dg[k];
+        keep[j] = G[j * n + j] > 0.0 && std::isfinite(dj) && dj > rel_tol * G[j * n + j];
+        if (!keep[j]) continue;
+        Dg[j] = dj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = G[i * n + j];
+            for (int k = 0; k < j; ++k)
+                if (keep[k]) v -= L[i][k] * L[j][k] * Dg[k];
+            L[i][j] = v / dj;
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        if (!keep[j]) continue;
+        double v = g[j];
+        for (int k = 0; k < j; ++k)
+            if (keep[k]) v -= L[j][k] * y[k];
+        y[j] = v;
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        c[j] = 0.0;
+        if (!keep[j]) continue;
+        double v = y[j] / Dg[j];
+        for (int i = j + 1; i < n; ++i)
+            if (keep[i]) v -= L[i][j] * c[i];
+        c[j] = v;
+    }
+}
+
+template <int S, int ND>
+static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped,
+                            const double* const* dirs, double* coef, double* r_before,
+                            double* r_after) {
+    constexpr int NP = ND * (ND + 1) / 2 + ND;
+    double rr[MAXS] = {}, bb[MAXS] = {};
+    residual_norms_impl<S>(m, sp, inv_dt, lumped, rr, bb);  // x0 -> X64, r0 -> R64
+    Workspace* w = get_ws(m, DSV_METHOD_MF64, S);
+    hipStream_t st = w->stream;
+    const unsigned gv = cdiv(m->V, 256);
+    ProjVec<S> pv{};
+    for (int k = 0; k < ND; ++k) {
+        auto* d = (SV<double, S>*)w->vb[VB_V1 + 2 * k];
+        auto* Ad = (SV<double, S>*)w->vb[VB_V2 + 2 * k];
+        const double* const* dk = dirs + (size_t)k * S;
+        hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, dk[0],
+                           S > 1 ? dk[1] : nullptr, S > 2 ? dk[2] : nullptr, d);
+        for (int s = 0; s < S; ++s)
+            if (sp[s].n_bc > 0)
+                hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0,
+                                   st, sp[s].n_bc, sp[s].bc_dofs, m->iperm, s, 0.0, d);
+        MFVec<double, S> v{};
+        v.x = d;
+        v.y = Ad;
+        launch_mf<double, S, MF_APPLY>(m, m->part, w->prm, v, nullptr, st);
+        pv.d[k] = d;
+        pv.Ad[k] = Ad;
+    }
+    pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
+    pv.x0 = (const SV<double, S>*)w->vb[VB_X64];
+    pv.x = (SV<double, S>*)w->vb[VB_X];
+    hipLaunchKernelGGL((k_proj_dots<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
+                       w->pR);
+    DSV_HIP(hipGetLastError());
+    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * NP * PROJ_GRID,
+                           hipMemcpyDeviceToHost, st));
+    DSV_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < S; ++s) {
+        double q[NP];
+        for (int j = 0; j < NP; ++j) {
+            double a = 0.0;
+            for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)(s * NP + j) * PROJ_GRID + b];
+            q[j] = a;
+        }
+        double G[ND * ND], g[ND], c[ND];
+        int t = 0;
+        for (int j = 0; j < ND; ++j)
+            for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
+        for (int j = 0; j < ND; ++j) g[j] = q[t++];
+        solve_gram(ND, G, g, c, 1e-10);
+        for (int k = 0; k < ND; ++k) pv.c[k][s] = c[k];
+    }
+    hipLaunchKernelGGL((k_proj_combine<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V,
+                       pv, w->pR);
+    DSV_HIP(hipGetLastError());
+    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * PROJ_GRID, hipMemcpyDeviceToHost,
+                           st));
+    DSV_HIP(hipStreamSynchronize(st));
+    double* out[3] = {nullptr, nullptr, nullptr};
+    for (int s = 0; s < S; ++s) {
+        double a = 0.0;
+        for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)s * PROJ_GRID + b];
+        const double r0 = std::sqrt(rr[s]), r1 = std::sqrt(a);
+        const bool take = std::isfinite(r1) && r1 < r0;  // the start point is the fallback
+        if (take) out[s] = sp[s].x;
+        if (r_before) r_before[s] = r0;
+        if (r_after) r_after[s] = take ? r1 : r0;
+        if (coef)
+            for (int k = 0; k < ND; ++k) coef[k * S + s] = take ? pv.c[k][s] : 0.0;
+    }
+    if (out[0] || out[1] || out[2]) {
+        hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
+                           (const SV<double, S>*)pv.x, out[0], out[1], out[2]);
+        DSV_HIP(hipGetLastError());
+        DSV_HIP(hipStreamSynchronize(st));
+    }
+}
+
 template <class T, int S>
 static void apply_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped,
                        const double* const* x, double* const* y) {
@@ -1764,6 +1956,29 @@ int dsv_residual_norms(dsv_mesh* m, int32_t n_species, const dsv_species* sp, do
         for (int s = 0; s < n_species; ++s) {
             rnorm[s] = std::sqrt(rr[s]);
             if (bnorm) bnorm[s] = std::sqrt(bb[s]);
+        }
+    });
+}
+
+int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                   int32_t lumped, int32_t n_dir, const double* const* dirs, double* coef,
+                   double* r_before, double* r_after, void* stream) {
+    return guard([&] {
+        check_solve_args(m, n_species, sp, true);
+        if (n_dir < 1 || n_dir > 3) throw ArgError{"n_dir must be 1, 2 or 3"};
+        if (!dirs) throw ArgError{"dsv_warm_start: NULL dirs"};
+        DSV_HIP(hipSetDevice(m->device));
+        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        switch (n_species * 10 + n_dir) {
+#define DSV_WS_CASE(S_, N_)                                                                  \
+    case S_ * 10 + N_:                                                                     \
+        warm_start_impl<S_, N_>(m, sp, inv_dt, lumped, dirs, coef, r_before, r_after); \
+        break;
+            DSV_WS_CASE(1, 1) DSV_WS_CASE(1, 2) DSV_WS_CASE(1, 3)
+            DSV_WS_CASE(2, 1) DSV_WS_CASE(2, 2) DSV_WS_CASE(2, 3)
+            DSV_WS_CASE(3, 1) DSV_WS_CASE(3, 2) DSV_WS_CASE(3, 3)
+#undef DSV_WS_CASE
+            default: throw ArgError{"n_species must be 1, 2 or 3"};
         }
     });
 }

@@ -28,8 +28,8 @@ EXPORTS = (
     "dsv_last_error", "dsv_device_count", "dsv_mesh_create", "dsv_mesh_destroy", "dsv_mesh_info",
     "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs", "dsv_apply",
     "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form", "dsv_time_apply",
-    "dsv_residual_norms", "dsv_mlp_features", "dsv_mlp_forward", "dsv_mlp_grad_workspace",
-    "dsv_mlp_grad", "dsv_mlp_adam",
+    "dsv_residual_norms", "dsv_warm_start", "dsv_mlp_features", "dsv_mlp_forward",
+    "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam",
 )
 
 
@@ -107,6 +107,9 @@ def load():
                                           P, ctypes.POINTER(D)]),
         "dsv_residual_norms": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32,
                                               ctypes.POINTER(D), ctypes.POINTER(D), P]),
+        "dsv_warm_start": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32,
+                                          ctypes.POINTER(P), ctypes.POINTER(D), ctypes.POINTER(D),
+                                          ctypes.POINTER(D), P]),
         "dsv_mlp_features": (ctypes.c_int, [I32, P, P, P, P, P, D, D, D, D, P, P, P, P]),
         "dsv_mlp_forward": (ctypes.c_int, [P, P, I32, P, P]),
         "dsv_mlp_grad_workspace": (ctypes.c_size_t, [I32]),

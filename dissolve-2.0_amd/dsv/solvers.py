@@ -141,6 +141,40 @@ def solve_batched(mesh, species, inv_dt, rtol=1e-8, max_it=5000, method="mf32_ir
     return st.as_dict(len(structs))
 
 
+def galerkin_warm_start(mesh, species, inv_dt, dirs, lumped=1, stream=None):
+    """Galerkin warm start of up to three species (libdsv dsv_warm_start): replaces each
+    species' x by the point of x + span{dirs[k][s]} minimising the A-energy norm of the error
+    when that lowers the FP64 residual (x stays the fallback).
+
+    species: dicts as for solve_batched (b and x required); dirs: list (1..3) of per-species
+    lists of device FP64 vectors (None = zero).  Returns (coefficients [k][s], residual norm
+    of the given x, residual norm of the start left in x) per species."""
+    import ctypes
+    keep = []
+    structs = []
+    for sp in species:
+        structs.append(species_struct(mesh, D=sp.get("D"), D_value=sp.get("D_value", 0.0),
+                                      alpha=sp.get("alpha"), alpha_value=sp.get("alpha_value", 0.0),
+                                      bc_dofs=sp.get("bc_dofs"), bc_value=sp.get("bc_value", 0.0),
+                                      b=sp["b"], x=sp["x"], keep=keep))
+    S, nd = len(structs), len(dirs)
+    if any(len(dk) != S for dk in dirs):
+        raise ValueError("every direction needs one vector per species")
+    x0 = species[0]["x"]
+    for dk in dirs:
+        for d in dk:
+            if d is not None and (d.numel() != mesh.n_dofs or d.dtype != x0.dtype
+                                  or d.device != x0.device or not d.is_contiguous()):
+                raise ValueError("directions must be contiguous FP64 device vectors of length V")
+    arr = (ctypes.c_void_p * (nd * S))(*[ptr(d).value for dk in dirs for d in dk])
+    coef = (ctypes.c_double * (nd * S))()
+    r0 = (ctypes.c_double * S)()
+    r1 = (ctypes.c_double * S)()
+    _lib.call("dsv_warm_start", mesh.handle, S, species_array(structs), float(inv_dt), int(lumped),
+              nd, arr, coef, r0, r1, stream if stream is not None else stream_ptr(mesh.device))
+    return ([[coef[k * S + s] for s in range(S)] for k in range(nd)], list(r0), list(r1))
+
+
 class MatrixFreeOperator:
     """Matrix-free operator handle with the reference's `mult(A, x, y)` hook
     (`numerics/solvers.py:172-203`); x and y are host or device arrays of length n_dofs."""

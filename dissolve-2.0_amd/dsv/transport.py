@@ -3,15 +3,22 @@
 One step, for every species s: b_s = M (u_s / dt + f_s) with b_s[bc] = bc value
 (`gpu/gpu_assembler.py:534-603`), then A_s x_s = b_s solved for all species in the same
 kernels by libdsv (`numerics/solvers.py:287-372`), warm-started from the previous solution
-(`numerics/solvers.py:329,366-367`) or from the learned predictor (dsv.mlp).  Method "csr64"
+(`numerics/solvers.py:329,366-367`) or from a Galerkin warm start.  Method "csr64"
 re-assembles the FP64 CSR of every species each step (the reference's operator path);
 "mf32_ir64" applies the operator in FP32 inside FP64 refinement.
 
-Learned warm start: `train_warm_start(n)` runs n steps from the previous-solution start while
-collecting (features at step n, increment to step n+1) pairs, fits the MLP on them, and from
-then on every step starts from u_n + sigma f(X) for every species whose initial FP64 residual
-is smaller than the previous solution's (the previous solution stays the fallback and the
-judge); `warm_log` records the choice and both residuals per step.
+Warm starts (no counterpart in the reference, whose solves start from the previous
+solution): "previous" starts every solve from u_n; "extrapolate" and "learned" start from the
+Galerkin warm start of libdsv (dsv_warm_start) over u_n + span{directions}, per species --
+the combination minimising the A-energy norm of the error, taken only where its FP64
+residual is below that of u_n (the previous solution stays the fallback and the judge).
+"extrapolate" uses the last history_dirs (default 3) increments u_n - u_{n-1},
+u_{n-1} - u_{n-2}, ...; "learned" uses at most two of them plus the per-vertex MLP's
+prediction of the part of the next increment that extrapolation misses,
+(u_{n+1} - 2 u_n + u_{n-1}) / (sigma tau), sigma = max |u_n - u_{n-1}| and tau the training
+targets' RMS.  `train_warm_start(n)` runs n steps collecting (features, target) pairs, fits
+the MLP on the GPU and switches to "learned"; `warm_log` records per step and species the
+coefficients and both residual norms.
 """
 from __future__ import annotations
 
@@ -40,7 +47,7 @@ class Stepper:
         self._keep = []
         self.defs = species
         self.u = [to_device(s["u0"], dev, torch.float64).clone() for s in species]
-        self.u_prev = [u.clone() for u in self.u]
+        self.u_hist = [[u.clone() for u in self.u] for _ in range(3)]  # u_{n-1}, u_{n-2}, u_{n-3}
         self.b = [torch.zeros_like(u) for u in self.u]
         self.x = [torch.zeros_like(u) for u in self.u]
         self.f = [to_device(s["f"], dev, torch.float64) if s.get("f") is not None else None
@@ -62,11 +69,32 @@ class Stepper:
                 mask = torch.zeros(self.u[k].numel(), dtype=torch.uint8, device=dev)
                 mask[self.bc[k].long()] = 1
             self.bc_mask.append(mask)
-        self.use_mlp = False
+        self.warm = "previous"
+        self.history_dirs = 3
         self.collect = False
         self.pairs = []
+        self.tau = 1.0
         self.warm_log = []
         self.n_steps = 0
+
+    @property
+    def u_prev(self):
+        """The previous step's solutions u_{n-1}."""
+        return self.u_hist[0]
+
+    def set_warm_start(self, mode, history_dirs=None):
+        """"previous", "extrapolate" or "learned" (needs a trained MLP); history_dirs (1..3) is
+        the number of past increments u_n - u_{n-1}, u_{n-1} - u_{n-2}, ... among the
+        directions (at most 2 are used beside the learned direction)."""
+        if mode not in ("previous", "extrapolate", "learned"):
+            raise ValueError(f"unknown warm start {mode!r}")
+        if mode == "learned" and (self.mlp is None or not self.mlp.trained):
+            raise ValueError("the learned warm start needs a trained MLP (train_warm_start)")
+        if history_dirs is not None:
+            if history_dirs not in (1, 2, 3):
+                raise ValueError("history_dirs must be 1, 2 or 3")
+            self.history_dirs = int(history_dirs)
+        self.warm = mode
 
     def set_coefficients(self, species):
         """(Re)bind D / alpha / f / bc of every species (device copies)."""
@@ -100,7 +128,7 @@ class Stepper:
         s = self.defs[k]
         al = s.get("alpha")
         alpha = to_device(al, self.mesh.device, self.u[k].dtype) if al is not None else None
-        return self.mlp.features(self.u[k], self.u_prev[k], self.phi, self.grad_phi,
+        return self.mlp.features(self.u[k], self.u_hist[0][k], self.phi, self.grad_phi,
                                  s.get("D_value", 1.0) if s.get("D") is None else float(np.mean(s["D"])),
                                  self.dt, self.eps, alpha=alpha,
                                  alpha_value=s.get("alpha_value", 0.0), bc_mask=self.bc_mask[k])
@@ -112,46 +140,57 @@ class Stepper:
                   self.lumped, rn, None, stream_ptr(self.mesh.device))
         return [rn[k] for k in range(self.S)]
 
+    def galerkin_start(self, dirs):
+        """Replaces self.x (the start points) by the Galerkin warm start over
+        self.x + span{dirs[k][s]} where that lowers the FP64 residual; returns
+        (coefficients [k][s], residual norms before, after)."""
+        nd = len(dirs)
+        keep = [d for dk in dirs for d in dk]
+        arr = (ctypes.c_void_p * (nd * self.S))(*[ptr(d).value for d in keep])
+        coef = (ctypes.c_double * (nd * self.S))()
+        r0 = (ctypes.c_double * self.S)()
+        r1 = (ctypes.c_double * self.S)()
+        _lib.call("dsv_warm_start", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
+                  self.lumped, nd, arr, coef, r0, r1, stream_ptr(self.mesh.device))
+        return ([[coef[k * self.S + s] for s in range(self.S)] for k in range(nd)],
+                list(r0), list(r1))
+
+    def _mlp_direction(self, feats, k):
+        X, sigma = feats[k]
+        return (sigma.double() * self.tau) * self.mlp.forward(X).double()
+
     def step(self, warm=None):
         """One backward-Euler step of every species; `warm` (list of tensors) replaces the
         initial guess.  Returns the solver statistics."""
         t0 = time.perf_counter()
         self.rhs()
         feats = None
-        want = self.mlp is not None and (self.collect or self.use_mlp) and self.n_steps >= 1
-        if want:
+        learned = self.warm == "learned" and self.n_steps >= 1
+        if self.mlp is not None and (self.collect or learned) and self.n_steps >= 1:
             feats = [self._features(k) for k in range(self.S)]
-        if warm is not None:
-            for k in range(self.S):
-                self.x[k].copy_(warm[k])
-        elif self.use_mlp and feats is not None:
-            cand = [self.u[k] + feats[k][1].double() * self.mlp.forward(feats[k][0]).double()
-                    for k in range(self.S)]
-            for k in range(self.S):
-                self.x[k].copy_(cand[k])
-            r_mlp = self.residual_norms()
-            for k in range(self.S):
-                self.x[k].copy_(self.u[k])
-            r_prev = self.residual_norms()
-            use = [r_mlp[k] < r_prev[k] for k in range(self.S)]
-            for k in range(self.S):
-                if use[k]:
-                    self.x[k].copy_(cand[k])
-            self.warm_log.append({"step": self.n_steps, "mlp": use, "r_mlp": r_mlp,
-                                  "r_prev": r_prev})
-        else:
-            for k in range(self.S):
-                self.x[k].copy_(self.u[k])
+        for k in range(self.S):
+            self.x[k].copy_(self.u[k] if warm is None else warm[k])
+        if warm is None and self.warm != "previous" and self.n_steps >= 1:
+            seq = [self.u] + self.u_hist
+            nh = min(self.history_dirs, 2 if learned else 3, self.n_steps)
+            dirs = [[seq[j][k] - seq[j + 1][k] for k in range(self.S)] for j in range(nh)]
+            if learned:
+                dirs.append([self._mlp_direction(feats, k) for k in range(self.S)])
+            coef, r0, r1 = self.galerkin_start(dirs)
+            self.warm_log.append({"step": self.n_steps, "mode": self.warm, "coef": coef,
+                                  "r_previous": r0, "r_start": r1,
+                                  "taken": [r1[k] < r0[k] for k in range(self.S)]})
         st = _lib.Stats()
         _lib.call("dsv_solve", self.mesh.handle, self.S, self._arr, 1.0 / self.dt, self.lumped,
                   _lib.METHODS[self.method], self.rtol, self.max_it, stream_ptr(self.mesh.device), st)
         if self.collect and feats is not None:
             for k in range(self.S):
                 X, sigma = feats[k]
-                y = ((self.x[k] - self.u[k]) / sigma.double()).float()
+                y = ((self.x[k] - 2.0 * self.u[k] + self.u_hist[0][k]) / sigma.double()).float()
                 self.pairs.append((X, y))
+        self.u_hist = [self.u_hist[-1]] + self.u_hist[:-1]
         for k in range(self.S):
-            self.u_prev[k].copy_(self.u[k])
+            self.u_hist[0][k].copy_(self.u[k])
             self.u[k].copy_(self.x[k])
         self.n_steps += 1
         d = st.as_dict(self.S)
@@ -160,8 +199,9 @@ class Stepper:
         return d
 
     def train_warm_start(self, n_steps, epochs=20, batch=16384):
-        """Run n_steps steps from the previous-solution start collecting training pairs, fit
-        the MLP on them, and switch to the learned warm start.  Returns the loss per epoch."""
+        """Run n_steps steps (with the current warm start) collecting training pairs, fit the
+        MLP on them, and switch to the learned warm start.  Returns the loss per epoch
+        (relative to the targets' mean square, which is 1 after scaling by tau)."""
         import torch
         if self.mlp is None or self.phi is None or self.grad_phi is None or self.eps is None:
             raise ValueError("the learned warm start needs mlp, phi, grad_phi and eps")
@@ -173,7 +213,8 @@ class Stepper:
             raise ValueError("no training pairs (needs at least 2 steps)")
         X = torch.cat([p[0] for p in self.pairs])
         y = torch.cat([p[1] for p in self.pairs])
-        hist = self.mlp.fit(X, y, epochs=epochs, batch=batch)
+        self.tau = max(float(torch.sqrt(torch.mean(y.double() ** 2))), 1e-30)
+        hist = self.mlp.fit(X, (y / self.tau).float(), epochs=epochs, batch=batch)
         self.pairs = []
-        self.use_mlp = True
+        self.warm = "learned"
         return hist

@@ -11,7 +11,8 @@
  *     array argument is a DEVICE pointer (memory of the device the mesh was created on),
  *     in the caller's vertex / cell numbering.
  *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are stream-ordered;
- *     dsv_solve and dsv_pattern synchronise the stream before returning.
+ *     dsv_solve, dsv_residual_norms and dsv_warm_start synchronise the stream before
+ *     returning.
  *   - Cell / vertex coefficient arrays follow the reference's _to_cell_avg rule: length C is
  *     taken per cell, length V is averaged over the 4 vertices of each cell.
  *
@@ -143,6 +144,20 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
  * starts from; FP64, matrix-free.  Used to judge candidate initial guesses. */
 int dsv_residual_norms(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
                        int32_t lumped, double* rnorm, double* bnorm, void* stream);
+
+/* Galerkin warm start (no counterpart in the reference, whose solves start from the previous
+ * solution, `numerics/solvers.py:329,366-367`).  For each species s, with x0 = sp[s].x
+ * (Dirichlet entries set to bc_value) and the n_dir (1..3) directions d_k = dirs[k*n_species
+ * + s] (device arrays, caller numbering, taken as 0 at Dirichlet vertices; NULL = zero),
+ * computes x = x0 + sum_k c_k d_k with c minimising the A_s-energy norm of the error over
+ * x0 + span{d_k}: (d_j . A d_k) c = d_j . (b - A x0), FP64, directions numerically dependent
+ * on earlier ones dropped (c_k = 0).  sp[s].x is replaced by x only when its FP64 residual
+ * 2-norm is below that of x0, so x0 stays the fallback.  coef (host, n_dir*n_species, may be
+ * NULL) receives c (all 0 where x0 was kept, layout as dirs); r_before / r_after (host,
+ * n_species, may be NULL) the residual norms of x0 and of the start left in sp[s].x. */
+int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                   int32_t lumped, int32_t n_dir, const double* const* dirs, double* coef,
+                   double* r_before, double* r_after, void* stream);
 
 /* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
  * sweeps) and the number of CG iterations per captured hipGraph chunk. */

@@ -100,7 +100,8 @@ def test_training_reduces_loss(dev):
 
 def test_learned_warm_start_saves_iterations(dev):
     """Train on the first steps of a 3-species graded-disc run, then compare the following steps
-    against the previous-solution start: fewer CG iterations, same converged solutions."""
+    against the previous-solution start: fewer CG iterations, same converged solutions, the
+    learned direction used by the Galerkin combination."""
     import torch
     from dsv import workloads
     from dsv.core import Mesh
@@ -115,8 +116,9 @@ def test_learned_warm_start_saves_iterations(dev):
     n_train, n_eval = 8, 6
     for _ in range(n_train):
         plain.step()
+    learned.set_warm_start("extrapolate", history_dirs=2)
     hist = learned.train_warm_start(n_train, epochs=30)
-    assert hist[-1] < hist[0]
+    assert hist[-1] < hist[0] and learned.warm == "learned"
     it_plain = it_learned = 0
     for _ in range(n_eval):
         it_plain += sum(plain.step()["iterations"])
@@ -125,5 +127,8 @@ def test_learned_warm_start_saves_iterations(dev):
             a, b = plain.u[k].cpu().numpy(), learned.u[k].cpu().numpy()
             assert np.abs(a - b).max() <= 1e-5 * max(np.abs(a).max(), 1e-30)
     assert all(all(h["converged"]) for h in learned.history)
-    assert any(any(e["mlp"]) for e in learned.warm_log)
-    assert it_learned < it_plain, (it_learned, it_plain, learned.warm_log[-1])
+    log = learned.warm_log[-n_eval:]
+    assert all(e["mode"] == "learned" and len(e["coef"]) == 3 for e in log)
+    assert any(any(e["taken"]) for e in log)
+    assert any(c != 0.0 for e in log for c in e["coef"][-1])  # the MLP direction is used
+    assert it_learned < it_plain, (it_learned, it_plain, log[-1])

@@ -1,0 +1,114 @@
+"""GPU Galerkin warm start (dsv_warm_start) against the CPU oracle (oracle/warmstart.py),
+through the C-ABI.  Bars: coefficients within 1e-8 relative, start points within 1e-10
+relative to max|x|, residual norms within 1e-9 relative; the exact increment reproduces the
+direct solution to 1e-9."""
+import numpy as np
+import pytest
+
+from dsv import meshes
+from oracle import assembly as OA
+from oracle import solve as OS
+from oracle import warmstart as OW
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, seed=3):
+    rng = np.random.default_rng(seed)
+    if name == "disc_small":
+        conn, xyz = meshes.graded_disc(seed=1, scale=6.0)
+    else:
+        conn, xyz = meshes.kuhn_cube(7)
+        xyz = meshes.jitter(xyz, 1 / 7, 0.2, rng, ~meshes.boundary_vertex_mask(xyz))
+        conn, xyz = meshes.locality_order(conn, xyz)
+    C, V = len(conn), len(xyz)
+    bc = np.nonzero(xyz[:, 2] <= xyz[:, 2].min() + 1e-9)[0].astype(np.int32)
+    dt = 0.005
+    sps = [dict(D=rng.lognormal(0, 0.5, C), alpha=rng.uniform(0, 50, V), bc_dofs=bc, bc_value=1.0),
+           dict(D_value=2.0),
+           dict(D_value=5.3, alpha_value=0.3)]
+    u0 = [rng.uniform(0, 1, V) for _ in sps]
+    mats, bs = [], []
+    for s, u in zip(sps, u0):
+        D = s["D"] if "D" in s else s["D_value"]
+        al = s["alpha"] if "alpha" in s else s.get("alpha_value", 0.0)
+        ip, ix, data = OA.assemble_values(conn, xyz, D, 1 / dt, al, 1, s.get("bc_dofs"))
+        mats.append(OA.to_scipy(ip, ix, data, V))
+        bs.append(OA.assemble_rhs(conn, xyz, u, 1 / dt, rng.uniform(0, 1, V), 1, s.get("bc_dofs"),
+                                  s.get("bc_value", 0.0)))
+    return conn, xyz, sps, u0, mats, bs, 1 / dt, rng
+
+
+def _run(dev, mesh, sps, u0, bs, inv_dt, dirs):
+    import torch
+    from dsv.solvers import galerkin_warm_start
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    sp = []
+    for s, u, b in zip(sps, u0, bs):
+        d = dict(s)
+        d["b"], d["x"] = t(b), t(u)
+        sp.append(d)
+    tdirs = [[t(d) if d is not None else None for d in dk] for dk in dirs]
+    coef, r0, r1 = galerkin_warm_start(mesh, sp, inv_dt, tdirs)
+    return [d["x"].cpu().numpy() for d in sp], coef, r0, r1
+
+
+@pytest.mark.parametrize("name", ["kuhn7_jitter_morton", "disc_small"])
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_matches_oracle(gpu, name, nd):
+    from dsv.core import Mesh
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case(name)
+    mesh = Mesh(conn, xyz)
+    V = len(xyz)
+    # smooth-ish directions (increments of the exact solution plus noise) and pure noise
+    exact = [OS.direct(A, b) for A, b in zip(mats, bs)]
+    dirs = []
+    for k in range(nd):
+        dirs.append([(exact[s] - u0[s]) * rng.uniform(0.5, 1.5) + rng.normal(0, 0.1, V)
+                     if k == 0 else rng.normal(0, 1, V) for s in range(3)])
+    x, coef, r0, r1 = _run(gpu, mesh, sps, u0, bs, inv_dt, dirs)
+    for s in range(3):
+        xr, cr, n0, n1 = OW.galerkin_start(mats[s], bs[s], u0[s], [d[s] for d in dirs],
+                                           sps[s].get("bc_dofs"), sps[s].get("bc_value", 0.0))
+        assert abs(r0[s] - n0) <= 1e-9 * n0
+        assert abs(r1[s] - n1) <= 1e-9 * n0
+        for k in range(nd):
+            assert abs(coef[k][s] - cr[k]) <= 1e-8 * max(np.abs(cr).max(), 1e-30), (s, k)
+        assert np.abs(x[s] - xr).max() <= 1e-10 * np.abs(xr).max()
+        assert n1 < n0
+
+
+def test_exact_increment_and_dropped_directions(gpu):
+    from dsv.core import Mesh
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("kuhn7_jitter_morton", seed=5)
+    mesh = Mesh(conn, xyz)
+    exact = [OS.direct(A, b) for A, b in zip(mats, bs)]
+    inc = []
+    for s in range(3):
+        x0 = u0[s].copy()
+        if sps[s].get("bc_dofs") is not None:
+            x0[sps[s]["bc_dofs"]] = sps[s]["bc_value"]
+        inc.append(exact[s] - x0)
+    # the exact increment, a duplicate of it and an absent (NULL) direction
+    x, coef, r0, r1 = _run(gpu, mesh, sps, u0, bs, inv_dt,
+                           [inc, [2.0 * d for d in inc], [None, None, None]])
+    for s in range(3):
+        assert abs(coef[0][s] - 1.0) < 1e-9 and coef[1][s] == 0.0 and coef[2][s] == 0.0
+        assert np.abs(x[s] - exact[s]).max() <= 1e-9 * np.abs(exact[s]).max()
+        assert r1[s] < 1e-8 * r0[s]
+
+
+def test_argument_errors(gpu):
+    import torch
+    from dsv.core import Mesh
+    from dsv.solvers import galerkin_warm_start
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("kuhn7_jitter_morton")
+    mesh = Mesh(conn, xyz)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
+    sp = [dict(sps[0], b=t(bs[0]), x=t(u0[0]))]
+    d = t(np.ones(len(xyz)))
+    for bad in ([], [[d]] * 4):
+        with pytest.raises(ValueError):
+            galerkin_warm_start(mesh, sp, inv_dt, bad)
+    with pytest.raises(ValueError):
+        galerkin_warm_start(mesh, sp, inv_dt, [[d[:-1]]])
