@@ -35,20 +35,41 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cg_iteration_bytes(C, V, nnz, n_species=3, word=4, stored=True):
-    """Algorithmic bytes of one fused FP32 CG iteration.  Operator: stored geometry -- one
-    (2-byte local index, 4-byte weight) entry per directed graph edge, 6 (nnz - V); recomputed
-    geometry -- one 4-byte incidence word per (cell, corner), 16C, plus the coordinates (3 words
-    per vertex).  Per vertex: the folded mass diagonal (S words) and the species vectors --
-    dinv, r, w, s, p, x read and r, w, s, p, x written (11 S words)."""
-    op = 6 * (nnz - V) if stored else 16 * C + word * 3 * V
-    return op + word * V * (n_species + 11 * n_species)
+def _coef_bytes(a, C, V):
+    """4 bytes per value of a per-element (length C) or per-vertex (length V) field, 0 for a
+    scalar."""
+    if a is None:
+        return 0
+    n = len(a)
+    return 4 * C if n == C else 4 * V
 
 
-def apply_bytes(C, V, n_species=3, word=4):
-    """Algorithmic bytes of one matrix-free apply: incidence lists 16C, coordinates, mass
-    diagonal, x read and y written."""
-    return 16 * C + word * V * (3 + 3 * n_species)
+def apply_bytes(C, V, species):
+    """Algorithmic bytes of one FP32 multi-species matrix-free apply (the minimal-input
+    formulation, geometry recomputed): connectivity 16 C, coordinates 12 V, and per species x
+    read and y written (8 V) plus its coefficient fields."""
+    return 16 * C + 12 * V + sum(8 * V + _coef_bytes(s.get("D"), C, V) +
+                                 _coef_bytes(s.get("alpha"), C, V) for s in species)
+
+
+def cg_iteration_bytes(C, V, species):
+    """Algorithmic bytes of one fused FP32 Jacobi-PCG iteration: the apply plus the vector
+    passes of the iteration, 9 FP32 words per vertex and species."""
+    return apply_bytes(C, V, species) + 36 * V * len(species)
+
+
+def measured_traffic(path=os.path.join(ROOT, "profiles", "k_cg_disc3_traffic.md")):
+    """HBM bytes per launch of the fused CG iteration from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel factors), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return {"bytes_per_launch": int(line.split(":")[1]),
+                            "source": os.path.relpath(path, ROOT)}
+    except OSError:
+        return None
+    return None
 
 
 def load_workload(name):
@@ -228,12 +249,14 @@ def main():
         del st
 
     # roofline of the fused FP32 3-species CG iteration (the chosen path's dominant kernel)
-    ms_iter, ms_plain_apply = time_kernels(mesh, w, a.apply_reps)
-    nbytes = cg_iteration_bytes(C, V, mesh.nnz, len(w["species"]))
+    ms_iter, ms_apply = time_kernels(mesh, w, a.apply_reps)
+    nbytes = cg_iteration_bytes(C, V, w["species"])
+    abytes = apply_bytes(C, V, w["species"])
     achieved = nbytes / (ms_iter * 1e-3) / 1e9
-    log(f"fused FP32 CG iteration: {ms_iter * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
-        f"(matrix-free apply alone {ms_plain_apply * 1e3:.1f} us, "
-        f"{apply_bytes(C, V, len(w['species'])) / (ms_plain_apply * 1e-3) / 1e9:.0f} GB/s)")
+    apply_gbs = abytes / (ms_apply * 1e-3) / 1e9
+    log(f"fused FP32 CG iteration: {ms_iter * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
+        f"matrix-free apply alone {ms_apply * 1e3:.1f} us, {apply_gbs:.0f} GB/s")
+    traffic = measured_traffic()
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -285,10 +308,16 @@ def main():
         "outer_last_step": chosen["outer_per_step"][-1],
         "converged": all(r["converged"] for r in res.values()),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": "k_cg<float,3> (fused 3-species FP32 Jacobi-PCG iteration)",
-                     "bytes_per_launch": nbytes, "us_per_launch": round(ms_iter * 1e3, 2),
-                     "apply_us_per_launch": round(ms_plain_apply * 1e3, 2)},
+                     "algorithmic_bytes_per_launch": nbytes,
+                     "us_per_launch": round(ms_iter * 1e3, 2),
+                     "traffic_source": traffic["source"] if traffic else None},
+        "apply_hbm": {"kernel": "k_mf<float,3,APPLY> (fused 3-species FP32 apply)",
+                      "algorithmic_bytes_per_launch": abytes,
+                      "us_per_launch": round(ms_apply * 1e3, 2), "achieved_gbs": round(apply_gbs, 1),
+                      "frac": round(apply_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -7,7 +7,8 @@
 // Features of vertex v of species s at step n (row of X, 7 FP32 values):
 //   u_n, du/sigma, clamp(phi/eps, -4, 4)/4, min(|grad phi|, 4)/4, log(D)/2, alpha*dt, dirichlet
 // with du = u_n - u_{n-1} and sigma = max_v |du| (a per-species, per-step scale); the network
-// predicts (u_{n+1} - u_n)/sigma.
+// predicts a per-vertex quantity in units of sigma (the stepper trains it on the part of the
+// next increment that linear extrapolation misses).
 //
 // The gradient is computed without atomics: workgroups of 64 lanes take 64-sample chunks, lane
 // j owns row j of every weight matrix and accumulates over the chunks it sees in a fixed

@@ -11,7 +11,9 @@ Modules mirroring the reference's call surface:
 New:
   dsv.core            Mesh (device mesh handle)
   dsv.transport       batched three-species backward-Euler stepper
-  dsv.warmstart       learned CG warm start (per-vertex MLP, HIP kernels)
+  dsv.mlp             learned CG warm start direction (per-vertex MLP, HIP kernels)
+  dsv.autotune        launch-shape autotuner of the matrix-free kernels
+  dsv.workloads       the transport workloads (meshes, coefficients, initial states)
   dsv.meshes          mesh generators of the transport workloads
 """
 from ._lib import DsvError, DsvLibraryError, load as load_library  # noqa: F401

@@ -1,13 +1,14 @@
-"""Learned CG warm start: a per-vertex MLP (7 -> 64 -> 64 -> 1, tanh) predicting the next
-backward-Euler increment of each species, trained on the GPU by libdsv's forward, gradient
-and Adam kernels on time-step pairs the solver itself produces.
+"""Learned CG warm-start direction: a per-vertex MLP (7 -> 64 -> 64 -> 1, tanh) trained on the
+GPU by libdsv's forward, gradient and Adam kernels on time-step pairs the solver itself
+produces.
 
 Features per vertex (libdsv dsv_mlp_features): u_n, (u_n - u_{n-1})/sigma, clamp(phi/eps)/4,
 min(|grad phi|, 4)/4, log(D)/2, alpha dt, Dirichlet flag, with sigma = max |u_n - u_{n-1}|.
-Target: (u_{n+1} - u_n)/sigma.  The warm start u_n + sigma f(X) competes with the plain
-previous-solution start (the reference's, `numerics/solvers.py:329,366-367`) on the initial FP64
-residual, so a poor prediction can never slow a solve down; every solve still converges to the
-same tolerance.
+dsv.transport.Stepper trains it on (u_{n+1} - 2 u_n + u_{n-1}) / (sigma tau) -- the part of the
+next increment that linear extrapolation misses, tau the targets' RMS -- and uses
+sigma tau f(X) as one direction of the Galerkin warm start, whose result is taken only when it
+lowers the FP64 residual of the previous-solution start (the reference's,
+`numerics/solvers.py:329,366-367`); every solve still converges to the same tolerance.
 """
 from __future__ import annotations
 
