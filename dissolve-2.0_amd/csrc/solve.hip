@@ -420,15 +420,19 @@ constexpr int MF_THREADS_MAX = 512;  // workgroup sizes 256 and 512
 // matrix-free operator kernel.  MODE: APPLY y = A x; DIAG y = 1 / diag(A); RESID r = b - A x
 // (FP64 partials of r.r and b.b); MASS (generic form only) y = lumped (inv_dt + alpha) mass
 // diagonal, the `mdiag` of the fast form.
-template <class T, int S, int MODE, bool FAST>
+template <class T, int S, int MODE, bool FAST, bool STORED = false>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
      double* __restrict__ pout) {
     static_assert(!(FAST && MODE == MF_MASS), "the mass diagonal is built by the generic form");
+    static_assert(!STORED || (MODE == MF_APPLY && FAST && std::is_same<T, float>::value),
+                  "stored geometry: FP32 fast-form apply");
     constexpr int KIND = MODE == MF_DIAG ? ROW_DIAG : (MODE == MF_MASS ? ROW_MASS : ROW_MATVEC);
     extern __shared__ __align__(32) unsigned char smem_raw[];
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
-    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
+    // the stored form stages no coordinates
+    SV<T, S>* spv = STORED ? reinterpret_cast<SV<T, S>*>(smem_raw)
+                           : reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
     __shared__ double red[16 * 2 * MAXS];
     const int ml = g.max_local;
 
@@ -442,7 +446,7 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
         const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
         for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
             const int gv = g.lvl[(size_t)b * ml + i];
-            sgeo[i] = xyz[gv];
+            if constexpr (!STORED) sgeo[i] = xyz[gv];
             if constexpr (KIND == ROW_MATVEC) spv[i] = v.x[gv];
         }
         __syncthreads();
@@ -452,10 +456,20 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
             bool live;
             SV<T, S> own;
             T acc[S];
-            lane_rows<T, S, KIND, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
+            if constexpr (STORED)
+                edge_rows<S>(g, spv, b, nv, ls, ov, live, own, acc);
+            else
+                lane_rows<T, S, KIND, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
             if (!live || (ls & (g.L - 1)) != 0) continue;
             const int vtx = v0 + ov;
-            if constexpr (FAST) finish_fast<T, S, KIND>(prm, vtx, own, acc);
+            if constexpr (STORED) {
+                // D K(D=1) x + mdiag x
+                const SV<T, S> md = reinterpret_cast<const SV<T, S>*>(prm.mdiag)[vtx];
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[s] = prm.sp[s].D1 * acc[s] + md.v[s] * own.v[s];
+            } else if constexpr (FAST) {
+                finish_fast<T, S, KIND>(prm, vtx, own, acc);
+            }
             const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
             SV<T, S> out{};
             if constexpr (MODE == MF_RESID) {
@@ -504,7 +518,9 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
      int par) {
     extern __shared__ __align__(32) unsigned char smem_raw[];
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
-    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
+    // the stored form stages no coordinates
+    SV<T, S>* spv = STORED ? reinterpret_cast<SV<T, S>*>(smem_raw)
+                           : reinterpret_cast<SV<T, S>*>(sgeo + g.max_local);
     __shared__ double red[16 * 3 * MAXS];
     const int ml = g.max_local;
 
@@ -1153,8 +1169,9 @@ static const Vec4<T>* coords(const dsv_mesh* m) {
 }
 
 template <class T, int S>
-static size_t checked_lds(const dsv_mesh* m, const Partition& P) {
-    const size_t lds = mf_lds_bytes<T>(P, S);
+static size_t checked_lds(const dsv_mesh* m, const Partition& P, bool stored = false) {
+    const size_t lds = stored ? (size_t)P.max_local * rec_bytes(sizeof(T), S)
+                              : mf_lds_bytes<T>(P, S);
     if (lds + 1024 > (size_t)max_lds_per_block(m->device))
         throw ArgError{"matrix-free partition needs " + std::to_string(lds) +
                        " B of LDS per workgroup; use fewer vertices_per_block"};
@@ -1167,6 +1184,14 @@ static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char
     const Geo g = make_geo(m, P);
     OpParams<T> p;
     std::memcpy(&p, prm, sizeof(p));
+    if constexpr (MODE == MF_APPLY && std::is_same<T, float>::value) {
+        if (p.stored) {
+            const size_t lds_st = checked_lds<T, S>(m, P, true);
+            hipLaunchKernelGGL((k_mf<T, S, MODE, true, true>), dim3(mf_grid(P)), dim3(P.threads),
+                               lds_st, stream, g, coords<T>(m), p, v, pout);
+            return;
+        }
+    }
     const size_t lds = checked_lds<T, S>(m, P);
     if constexpr (MODE != MF_MASS) {
         if (p.fast) {
@@ -1257,18 +1282,19 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     const Geo g = make_geo(m, P);
     OpParams<T> p;
     std::memcpy(&p, w->prm, sizeof(p));
-    const size_t lds = checked_lds<T, S>(m, P);
     const CGVec<T, S> v = cg_vecs<T, S>(w);
     const int n = mf_grid(P);
     const T* pin = INIT ? nullptr : (const T*)w->pU[par];
     T* pout = (T*)(INIT ? w->pU[0] : w->pU[par ^ 1]);
     if constexpr (std::is_same<T, float>::value) {
         if (p.stored) {
-            hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds, stream,
-                               g, coords<T>(m), p, v, w->st, pin, n, pout, par);
+            const size_t lds_st = checked_lds<T, S>(m, P, true);
+            hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds_st,
+                               stream, g, coords<T>(m), p, v, w->st, pin, n, pout, par);
             return;
         }
     }
+    const size_t lds = checked_lds<T, S>(m, P);
     if (p.fast)
         hipLaunchKernelGGL((k_cg<T, S, true, INIT, false>), dim3(n), dim3(P.threads), lds, stream, g,
                            coords<T>(m), p, v, w->st, pin, n, pout, par);
