@@ -197,6 +197,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--paths", default="plain,fp32,chosen,learned",
+                    help="comma list of the paths to run (the JSON line needs all four)")
+    ap.add_argument("--partition", default=None,
+                    help="NV,THREADS,LANES launch shape of the matrix-free kernels (skips tuning)")
     ap.add_argument("--profile-run", action="store_true",
                     help="kernel-only run for rocprofv3: a 256 MiB device copy (the counter "
                          "reference), then the fused CG iteration and the apply, --apply-reps "
@@ -217,10 +221,17 @@ def main():
     log(f"device mesh: nnz={mesh.nnz} blocks={mesh.n_blocks} entries={mesh.n_entries} "
         f"({time.perf_counter() - t:.1f}s)")
     rtol = w["rtol"]
+    if a.partition:
+        nv, th, lanes = (int(t) for t in a.partition.split(","))
+        mesh.set_partition(nv, th, lanes)
+        a.no_tune = True
+        log(f"launch shape: vertices_per_block={nv} threads={th} lanes_per_vertex={lanes}")
     if a.profile_run:
         profile_run(mesh, w, a.apply_reps)
         return
     tuned = None
+    if a.partition:
+        tuned = {"vertices_per_block": nv, "threads": th, "lanes_per_vertex": lanes}
     if not a.no_tune:
         from dsv.autotune import tune
         best, table = tune(mesh, w["species"], 1.0 / w["dt"])
@@ -231,9 +242,12 @@ def main():
         log(f"tuned: {tuned}")
     res = {}
     ref_u = None
+    paths = a.paths.split(",")
     for name, method, warm in (("plain", "csr64", "previous"), ("fp32", "mf32_ir64", "previous"),
                                ("chosen", "mf32_ir64", "extrapolate"),
                                ("learned", "mf32_ir64", "learned")):
+        if name not in paths:
+            continue
         r, st = run_path(mesh, w, method, a.steps, a.warmup, rtol, warm=warm)
         res[name] = r
         log(f"{name} ({method}, {warm} start): {r['steps_per_s']:.2f} steps/s, "
@@ -248,6 +262,9 @@ def main():
                 f"max rel diff {dev:.2e}")
         del st
 
+    if any(p not in res for p in ("plain", "fp32", "chosen", "learned")):
+        log("not every path ran: no JSON line")
+        return
     # roofline of the fused FP32 3-species CG iteration (the chosen path's dominant kernel)
     ms_iter, ms_apply = time_kernels(mesh, w, a.apply_reps)
     nbytes = cg_iteration_bytes(C, V, w["species"])

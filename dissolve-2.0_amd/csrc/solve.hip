@@ -6,9 +6,8 @@
 // (`numerics/solvers.py:322-372`, GMRES(50)+bjacobi on a CSR that is copied through the host
 // every step).
 //
-// Data layout: every solver vector is species-interleaved -- one record SV<T,S> per vertex
-// holding the S species (padded to 4 when S = 3) -- so one wide load fetches every species of
-// a vertex.  Coordinates are (x, y, z, 0) records.
+// Data layout: every solver vector is species-interleaved -- one unpadded record SV<T,S> per
+// vertex holding the S species -- so one load fetches every species of a vertex.  Coordinates are (x, y, z, 0) records.
 //
 // Matrix-free apply (internal Morton vertex numbering), one workgroup per partition block:
 //   (1) stage the block's local vertices (owned range, then halo) in LDS: coordinates and the
@@ -77,7 +76,8 @@ struct CGState {
     int any_active;        // A_k / K_i (host polls)
 };
 
-// species-interleaved vertex record (S = 3 padded to 4 for aligned wide accesses)
+// species-interleaved vertex record: the S values of one vertex, unpadded (12 B for three
+// FP32 species), so that every byte a vector pass moves is a species value
 template <class T, int S>
 struct SVs;
 template <class T>
@@ -89,9 +89,10 @@ struct alignas(2 * sizeof(T)) SVs<T, 2> {
     T v[2];
 };
 template <class T>
-struct alignas(4 * sizeof(T)) SVs<T, 3> {
-    T v[4];
+struct alignas(sizeof(T)) SVs<T, 3> {
+    T v[3];
 };
+static_assert(sizeof(SVs<float, 3>) == 12 && sizeof(SVs<double, 3>) == 24, "unpadded records");
 template <class T, int S>
 using SV = SVs<T, S>;
 
@@ -693,6 +694,286 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
 }
 
 // ---------------------------------------------------------------------------------------
+// Stored-geometry FP32 CG iteration K_i (i > 0), the same recurrences as k_cg, scheduled for
+// latency: the loads that do not depend on this iteration's alpha / beta -- the vectors of the
+// first PRE local vertices of each thread, the halo vertices' local-list entries, the
+// edge-slice headers of the first PRE lane slots -- are issued before the previous
+// iteration's partials are reduced, and those of the next partition block before the current
+// one's rows are computed, so that the reduction, the gathers and the headers overlap.  Owned
+// vertices are the contiguous range [v0, v0 + nv) and are addressed directly.  Local vertices
+// or lane slots past PRE per thread (large blocks) are loaded in place.
+constexpr int PRE = 2;
+
+template <int S>
+struct StagePre {
+    int gv[PRE];
+    SV<float, S> d[PRE], r[PRE], w[PRE], so[PRE], p[PRE], x[PRE];
+};
+struct SlotPre {
+    int jbeg[PRE], width[PRE], ov[PRE];
+};
+
+template <int S>
+__device__ __forceinline__ void stage_loads(const Geo& g, const CGVec<float, S>& v, int rd, int b,
+                                            int i, int& gv, SV<float, S>& d, SV<float, S>& r,
+                                            SV<float, S>& w, SV<float, S>& so, SV<float, S>& p,
+                                            SV<float, S>& x) {
+    const int v0 = b * g.NV;
+    const int nv = min(g.NV, g.V - v0);
+    gv = i < nv ? v0 + i : g.lvl[(size_t)b * g.max_local + i];
+    d = v.dinv[gv];
+    r = v.R[rd][gv];
+    w = v.W[rd][gv];
+    so = v.Sd[rd][gv];
+    if (i < nv) {
+        p = v.P[gv];
+        x = v.X[gv];
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void prefetch_stage(const Geo& g, const CGVec<float, S>& v, int rd,
+                                               int b, StagePre<S>& pf) {
+    const int v0 = b * g.NV;
+    const int nv = min(g.NV, g.V - v0);
+    const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
+#pragma unroll
+    for (int j = 0; j < PRE; ++j) {
+        const int i = threadIdx.x + j * blockDim.x;
+        if (i < nloc)
+            stage_loads<S>(g, v, rd, b, i, pf.gv[j], pf.d[j], pf.r[j], pf.w[j], pf.so[j], pf.p[j],
+                           pf.x[j]);
+    }
+}
+
+__device__ __forceinline__ void slot_header(const Geo& g, int b, int nv, int ls, int& jbeg,
+                                            int& width, int& ov) {
+    const int vps = 64 / g.L;
+    const int spb = g.NV / vps;
+    const int* sp0 = g.eptr + b * spb + (ls >> 6);
+    jbeg = sp0[0] + (ls & 63);
+    width = (sp0[1] - sp0[0]) >> 6;
+    const int rank = ls / g.L;
+    ov = rank < nv ? (int)g.elperm[b * g.NV + rank] : -1;
+}
+
+__device__ __forceinline__ void prefetch_slots(const Geo& g, int b, SlotPre& sp) {
+    const int nv = min(g.NV, g.V - b * g.NV);
+    const int nls = ((nv + 64 / g.L - 1) / (64 / g.L)) * 64;
+#pragma unroll
+    for (int j = 0; j < PRE; ++j) {
+        const int ls = threadIdx.x + j * blockDim.x;
+        if (ls < nls) slot_header(g, b, nv, ls, sp.jbeg[j], sp.width[j], sp.ov[j]);
+    }
+}
+
+// sum over the slot's edges of K_vw(D=1) (u_w - u_v), reduced over the L lanes of the vertex;
+// ov < 0 marks a lane past the block's vertices (it still takes part in the lane reduction)
+template <int S>
+__device__ __forceinline__ void slot_rows(const Geo& g, const SV<float, S>* spv, int jbeg,
+                                          int width, int ov, SV<float, S>& own, float (&acc)[S]) {
+    const int o = ov < 0 ? 0 : ov;
+    own = spv[o];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+    for (int t0 = 0; t0 < width; t0 += 4) {
+        int ix[4];
+        float wt[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = jbeg + 64 * min(t0 + u, width - 1);
+            const int iv = g.eidx[j];
+            const float wv = g.ewt[j];
+            ix[u] = t0 + u < width ? iv : o;
+            wt[u] = t0 + u < width ? wv : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const SV<float, S> nb = spv[ix[u]];
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[s] += wt[u] * (nb.v[s] - own.v[s]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if (g.L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
+        if (g.L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+    }
+}
+
+template <int S>
+__global__ void __launch_bounds__(MF_THREADS_MAX)
+k_cg_st(Geo g, OpParams<float> prm, CGVec<float, S> v, CGState* __restrict__ st,
+        const float* __restrict__ pin, int nin, float* __restrict__ pout, int par) {
+    using T = float;
+    extern __shared__ __align__(32) unsigned char smem_raw[];
+    SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(smem_raw);
+    __shared__ double red[16 * 3 * MAXS];
+    const int rd = par, wr = par ^ 1;
+
+    bool prev[S];
+    bool anyprev = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        prev[s] = st->act[par ^ 1][s] != 0;
+        anyprev |= prev[s];
+    }
+    const int k = st->kk[par ^ 1];
+    if (!anyprev) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            for (int s = 0; s < S; ++s) st->act[par][s] = 0;
+            st->kk[par] = k;
+            st->any_active = 0;
+        }
+        return;
+    }
+    const BlockRange br = my_blocks(g);
+    StagePre<S> pf;
+    SlotPre sl;
+    if (br.beg < br.end) {
+        prefetch_stage<S>(g, v, rd, br.beg, pf);
+        prefetch_slots(g, br.beg, sl);
+    }
+
+    // scalars of this iteration from the previous kernel's partials (as in k_cg)
+    bool act[S];
+    T alpha[S], beta[S];
+    bool any = false;
+    {
+        double sums[3 * S];
+        consume_partials<3 * S, T>(pin, nin, sums, red);  // gamma_i, delta_i, rho_i
+        double gam[S], alp[S];
+        bool bad[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double gm = sums[s], dl = sums[S + s], rho = sums[2 * S + s];
+            act[s] = prev[s] && (rho > st->tol2[s]) && (k < st->max_it) && isfinite(rho);
+            double bb = 0.0, den = dl;
+            if (k > 0) {
+                bb = gm / st->rz[par ^ 1][s];
+                den = dl - bb * gm / st->alp[par ^ 1][s];
+            }
+            bad[s] = act[s] && (!(gm > 0.0) || !(den > 0.0) || !isfinite(den) || !isfinite(bb));
+            act[s] = act[s] && !bad[s];
+            gam[s] = gm;
+            alp[s] = act[s] ? gm / den : 0.0;
+            alpha[s] = (T)alp[s];
+            beta[s] = act[s] ? (T)bb : T(0);
+            any |= act[s];
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            for (int s = 0; s < S; ++s) {
+                st->act[par][s] = act[s] ? 1 : 0;
+                if (!prev[s]) continue;
+                st->rz[par][s] = gam[s];
+                st->alp[par][s] = alp[s];
+                st->rr[s] = sums[2 * S + s];
+                if (act[s])
+                    st->iters[s] += 1;
+                else
+                    st->status[s] = bad[s] ? ST_BREAKDOWN
+                                           : (sums[2 * S + s] <= st->tol2[s] ? ST_CONVERGED : ST_MAXIT);
+            }
+            st->kk[par] = k + 1;
+            st->any_active = any ? 1 : 0;
+        }
+    }
+    if (!any) return;
+
+    double pg[S], pd[S], pr[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) pg[s] = pd[s] = pr[s] = 0.0;
+    for (int b = br.beg; b < br.end; b += br.step) {
+        const int v0 = b * g.NV;
+        const int nv = min(g.NV, g.V - v0);
+        const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
+        // (1) stage u_{i+1}; owned vertices also advance p, x, s, r
+        auto stage = [&](int i, int gv, const SV<T, S>& d, const SV<T, S>& r, const SV<T, S>& w,
+                         const SV<T, S>& so, SV<T, S> p, SV<T, S> x) {
+            SV<T, S> rn, un, sn;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                sn.v[s] = w.v[s] + beta[s] * so.v[s];
+                rn.v[s] = r.v[s] - alpha[s] * sn.v[s];
+                un.v[s] = rn.v[s] * d.v[s];
+            }
+            spv[i] = un;
+            if (i < nv) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    p.v[s] = r.v[s] * d.v[s] + beta[s] * p.v[s];
+                    if (act[s]) x.v[s] += alpha[s] * p.v[s];
+                }
+                v.P[gv] = p;
+                v.X[gv] = x;
+                v.R[wr][gv] = rn;
+                v.Sd[wr][gv] = sn;
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (act[s]) {
+                        pg[s] += (double)rn.v[s] * (double)un.v[s];
+                        pr[s] += (double)rn.v[s] * (double)rn.v[s];
+                    }
+            }
+        };
+#pragma unroll
+        for (int j = 0; j < PRE; ++j) {
+            const int i = threadIdx.x + j * blockDim.x;
+            if (i < nloc) stage(i, pf.gv[j], pf.d[j], pf.r[j], pf.w[j], pf.so[j], pf.p[j], pf.x[j]);
+        }
+        for (int i = threadIdx.x + PRE * blockDim.x; i < nloc; i += blockDim.x) {
+            int gv;
+            SV<T, S> d, r, w, so, p{}, x{};
+            stage_loads<S>(g, v, rd, b, i, gv, d, r, w, so, p, x);
+            stage(i, gv, d, r, w, so, p, x);
+        }
+        // the next block's staging inputs are read while this block's rows are computed
+        const int bn = b + br.step;
+        if (bn < br.end) prefetch_stage<S>(g, v, rd, bn, pf);
+        __syncthreads();
+        // (2) w_{i+1} = A u_{i+1} for the owned vertices
+        const int nls = ((nv + 64 / g.L - 1) / (64 / g.L)) * 64;
+        auto rows = [&](int ls, int jbeg, int width, int ov) {
+            SV<T, S> own;
+            T acc[S];
+            slot_rows<S>(g, spv, jbeg, width, ov, own, acc);
+            if (ov < 0 || (ls & (g.L - 1)) != 0) return;
+            const int vtx = v0 + ov;
+            const SV<T, S> md = reinterpret_cast<const SV<T, S>*>(prm.mdiag)[vtx];
+            const unsigned bcb = prm.bcbits ? (unsigned)prm.bcbits[vtx] : 0u;
+            SV<T, S> out;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const T a = prm.sp[s].D1 * acc[s] + md.v[s] * own.v[s];
+                out.v[s] = ((bcb >> s) & 1u) ? own.v[s] : a;
+                if (act[s]) pd[s] += (double)out.v[s] * (double)own.v[s];
+            }
+            v.W[wr][vtx] = out;
+        };
+#pragma unroll
+        for (int j = 0; j < PRE; ++j) {
+            const int ls = threadIdx.x + j * blockDim.x;
+            if (ls < nls) rows(ls, sl.jbeg[j], sl.width[j], sl.ov[j]);
+        }
+        for (int ls = threadIdx.x + PRE * blockDim.x; ls < nls; ls += blockDim.x) {
+            int jbeg, width, ov;
+            slot_header(g, b, nv, ls, jbeg, width, ov);
+            rows(ls, jbeg, width, ov);
+        }
+        if (bn < br.end) prefetch_slots(g, bn, sl);
+        __syncthreads();  // LDS is reused by the next partition block
+    }
+    double tot[3 * S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        tot[s] = pg[s];
+        tot[S + s] = pd[s];
+        tot[2 * S + s] = pr[s];
+    }
+    write_partials<3 * S, T>(tot, red, pout);
+}
+
+// ---------------------------------------------------------------------------------------
 // CSR (caller order) kernels of the plain FP64 path: CG apply (A_k) and residual.
 constexpr int CSR_LANES = 8;
 constexpr int CSR_THREADS = 512;
@@ -1030,7 +1311,7 @@ struct Workspace {
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
 };
 
-static size_t rec_bytes(size_t t, int S) { return t * (S == 3 ? 4 : S); }
+static size_t rec_bytes(size_t t, int S) { return t * S; }
 
 static void free_ws(Workspace* w) {
     if (!w) return;
@@ -1289,8 +1570,12 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     if constexpr (std::is_same<T, float>::value) {
         if (p.stored) {
             const size_t lds_st = checked_lds<T, S>(m, P, true);
-            hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds_st,
-                               stream, g, coords<T>(m), p, v, w->st, pin, n, pout, par);
+            if constexpr (!INIT)
+                hipLaunchKernelGGL((k_cg_st<S>), dim3(n), dim3(P.threads), lds_st, stream, g, p, v,
+                                   w->st, pin, n, pout, par);
+            else
+                hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds_st,
+                                   stream, g, coords<T>(m), p, v, w->st, pin, n, pout, par);
             return;
         }
     }
@@ -1509,9 +1794,6 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
             hipLaunchKernelGGL((k_diag_inv_csr<S>), dim3(gv), dim3(256), 0, st, m->V, m->diag_pos,
                                w->csr[s], s, dinv);
         }
-        if (S == 3)  // padding lane of the records
-            hipLaunchKernelGGL((k_diag_inv_csr<S>), dim3(gv), dim3(256), 0, st, m->V, m->diag_pos,
-                               w->csr[0], 3, dinv);
     } else if (method == DSV_METHOD_MF64) {
         prepare_bc(m, w, S, sp, st);
         prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
@@ -1880,25 +2162,45 @@ static double time_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int l
     MFVec<T, S> v{};
     v.x = (const SV<T, S>*)w->vb[VB_V1];
     v.y = (SV<T, S>*)w->vb[VB_V2];
-    auto launch = [&] {
+    auto launch = [&](hipStream_t s) {
         if (mode == 1)
-            launch_cg<T, S, false>(m, w, 0, stream);
+            launch_cg<T, S, false>(m, w, 0, s);
         else
-            launch_mf<T, S, MF_APPLY>(m, m->part, w->prm, v, nullptr, stream);
+            launch_mf<T, S, MF_APPLY>(m, m->part, w->prm, v, nullptr, s);
     };
     DSV_HIP(hipStreamSynchronize(ws));
-    for (int i = 0; i < 3; ++i) launch();
-    hipEvent_t e0, e1;
-    DSV_HIP(hipEventCreate(&e0));
-    DSV_HIP(hipEventCreate(&e1));
-    DSV_HIP(hipEventRecord(e0, stream));
-    for (int i = 0; i < n_rep; ++i) launch();
-    DSV_HIP(hipEventRecord(e1, stream));
-    DSV_HIP(hipEventSynchronize(e1));
+    for (int i = 0; i < 3; ++i) launch(stream);
+    DSV_HIP(hipStreamSynchronize(stream));
+    // the n_rep launches are captured in one graph and replayed, so that host launch overhead
+    // does not show in a short kernel's time (the solver replays its iterations the same way)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DSV_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n_rep; ++i) launch(ws);
+    DSV_HIP(hipStreamEndCapture(ws, &graph));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms = 0.f;
-    DSV_HIP(hipEventElapsedTime(&ms, e0, e1));
+    try {
+        DSV_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        DSV_HIP(hipGraphLaunch(exec, ws));  // warm-up replay
+        DSV_HIP(hipEventCreate(&e0));
+        DSV_HIP(hipEventCreate(&e1));
+        DSV_HIP(hipEventRecord(e0, ws));
+        DSV_HIP(hipGraphLaunch(exec, ws));
+        DSV_HIP(hipEventRecord(e1, ws));
+        DSV_HIP(hipEventSynchronize(e1));
+        DSV_HIP(hipEventElapsedTime(&ms, e0, e1));
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        throw;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
     DSV_HIP(hipGetLastError());
     return (double)ms / n_rep;
 }

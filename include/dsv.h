@@ -171,10 +171,12 @@ int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
  * counterpart in the reference (its operator choice is libCEED's). */
 int dsv_set_operator_form(dsv_mesh* m, int32_t form);
 
-/* Average device duration (ms) of one launch of the fused n_species matrix-free kernel,
- * measured with HIP events around n_rep back-to-back launches on `stream`.  mode 0 times the
- * plain apply y = A x (x = sp[s].x, or the lumped mass when NULL); mode 1 times the CG
- * iteration form (p = z + beta p_old on the fly, q = A p, p.q partials) used inside dsv_solve. */
+/* Average device duration (ms) of one launch of the fused n_species matrix-free kernel: three
+ * warm-up launches on `stream`, then n_rep launches captured in one hipGraph and replayed on
+ * the library's stream of the mesh between two HIP events.  mode 0 times the plain apply
+ * y = A x (x = sp[s].x, or the lumped mass when NULL); mode 1 times one fused CG iteration as
+ * dsv_solve launches it (vector updates, w = A u and the partials of the next iteration).  The
+ * FP32 kernels use the operator form set by dsv_set_operator_form. */
 int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
                    int32_t lumped, int32_t precision, int32_t mode, int32_t n_rep, void* stream,
                    double* ms);
