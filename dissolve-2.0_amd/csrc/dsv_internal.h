@@ -104,6 +104,7 @@ struct Partition {
     uint16_t* elperm = nullptr;   // nb*NV: neighbour-count rank -> owned local index
     uint16_t* eidx = nullptr;     // local neighbour index per entry
     float* ewt = nullptr;         // K_vw(D = 1) per entry
+    double* ewt64 = nullptr;      // the same weights in FP64 (FP64 residual and apply)
 };
 
 struct Workspace;  // solver buffers, defined in solve.hip
@@ -146,6 +147,8 @@ struct dsv_mesh {
     int graph_chunk = 16;
     int use_graphs = 1;
     int operator_form = DSV_FORM_STORED;  // inner FP32 CG operator
+    int op_reuse = 0;        // dsv_set_operator_reuse
+    unsigned op_epoch = 0;   // bumped by every dsv_set_operator_reuse call
     dsv::Workspace* ws[3][4] = {};  // [method][n_species]
 };
 

@@ -167,6 +167,7 @@ void free_partition(Partition& p) {
     dfree(p.elperm);
     dfree(p.eidx);
     dfree(p.ewt);
+    dfree(p.ewt64);
     p.n_edge_entries = 0;
     p.nb = 0;
     p.n_entries = 0;
@@ -248,6 +249,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
     std::vector<int32_t> eptr((size_t)nb * spb + 1, 0);
     std::vector<uint16_t> elperm((size_t)nb * NV, 0), eidx;
     std::vector<float> ewt;
+    std::vector<double> ewt64;
     std::vector<int> eorder;
     for (int b = 0; b < nb; ++b) {
         const int v0 = b * NV, nv = std::min(NV, V - v0);
@@ -324,6 +326,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
             eptr[sidx + 1] = (int32_t)(base + 64 * (size_t)width);
             eidx.resize(base + 64 * (size_t)width, 0);
             ewt.resize(base + 64 * (size_t)width, 0.0f);
+            ewt64.resize(base + 64 * (size_t)width, 0.0);
             for (int r = r0; r < r1; ++r) {
                 const int ov = eorder[r];
                 const int c = perm_h[v0 + ov];
@@ -336,6 +339,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
                     const size_t at = base + 64 * (size_t)(i / L) + (r - r0) * L + i % L;
                     eidx[at] = (uint16_t)local(ip[ki[j]]);
                     ewt[at] = (float)kv[j];
+                    ewt64[at] = kv[j];
                     ++i;
                 }
             }
@@ -370,6 +374,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
         p.elperm = upload(elperm);
         p.eidx = upload(eidx);
         p.ewt = upload(ewt);
+        p.ewt64 = upload(ewt64);
         p.lperm = upload(lperm);
         p.sptr = upload(sptr);
         p.ell = upload(ell);

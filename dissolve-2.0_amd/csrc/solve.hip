@@ -176,6 +176,7 @@ struct Geo {
     const uint16_t* elperm;
     const uint16_t* eidx;
     const float* ewt;
+    const double* ewt64;
 };
 
 template <class T>
@@ -360,10 +361,22 @@ __device__ __forceinline__ void finish_fast(const OpParams<T>& prm, int vtx, con
 // Stored-geometry rows: acc[s] = sum over graph neighbours w of K_vw(D=1) (u_w - u_v) for the
 // owned vertex of lane slot `ls`, summed over its L lanes (all lanes of the wave must call
 // this).  Padding entries point at the vertex itself with weight 0, so there is no branch.
-template <int S>
-__device__ __forceinline__ void edge_rows(const Geo& g, const SV<float, S>* spv, int b, int nv,
-                                          int ls, int& ov, bool& live, SV<float, S>& own,
-                                          float (&acc)[S]) {
+template <class T>
+__device__ __forceinline__ const T* edge_weights(const Geo& g);
+template <>
+__device__ __forceinline__ const float* edge_weights<float>(const Geo& g) {
+    return g.ewt;
+}
+template <>
+__device__ __forceinline__ const double* edge_weights<double>(const Geo& g) {
+    return g.ewt64;
+}
+
+template <class T, int S>
+__device__ __forceinline__ void edge_rows(const Geo& g, const SV<T, S>* spv, int b, int nv,
+                                          int ls, int& ov, bool& live, SV<T, S>& own,
+                                          T (&acc)[S]) {
+    const T* __restrict__ ewt = edge_weights<T>(g);
     const int L = g.L;
     const int vps = 64 / L;
     const int spb = g.NV / vps;
@@ -377,21 +390,21 @@ __device__ __forceinline__ void edge_rows(const Geo& g, const SV<float, S>* spv,
     ov = live ? ovl : 0;
     own = spv[ov];
 #pragma unroll
-    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+    for (int s = 0; s < S; ++s) acc[s] = T(0);
     for (int t0 = 0; t0 < width; t0 += 4) {
         int ix[4];
-        float wt[4];
+        T wt[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = jbeg + 64 * min(t0 + u, width - 1);
             const int iv = g.eidx[j];
-            const float wv = g.ewt[j];
+            const T wv = ewt[j];
             ix[u] = t0 + u < width ? iv : ov;
-            wt[u] = t0 + u < width ? wv : 0.0f;
+            wt[u] = t0 + u < width ? wv : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const SV<float, S> nb = spv[ix[u]];
+            const SV<T, S> nb = spv[ix[u]];
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[s] += wt[u] * (nb.v[s] - own.v[s]);
         }
@@ -426,8 +439,8 @@ __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
      double* __restrict__ pout) {
     static_assert(!(FAST && MODE == MF_MASS), "the mass diagonal is built by the generic form");
-    static_assert(!STORED || (MODE == MF_APPLY && FAST && std::is_same<T, float>::value),
-                  "stored geometry: FP32 fast-form apply");
+    static_assert(!STORED || ((MODE == MF_APPLY || MODE == MF_RESID) && FAST),
+                  "stored geometry: fast-form apply and residual");
     constexpr int KIND = MODE == MF_DIAG ? ROW_DIAG : (MODE == MF_MASS ? ROW_MASS : ROW_MATVEC);
     extern __shared__ __align__(32) unsigned char smem_raw[];
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
@@ -458,7 +471,7 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
             SV<T, S> own;
             T acc[S];
             if constexpr (STORED)
-                edge_rows<S>(g, spv, b, nv, ls, ov, live, own, acc);
+                edge_rows<T, S>(g, spv, b, nv, ls, ov, live, own, acc);
             else
                 lane_rows<T, S, KIND, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
             if (!live || (ls & (g.L - 1)) != 0) continue;
@@ -658,7 +671,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
             T acc[S];
             if constexpr (STORED) {
                 static_assert(std::is_same<T, float>::value && FAST, "stored geometry: FP32 fast form");
-                edge_rows<S>(g, spv, b, nv, ls, ov, live, own, acc);
+                edge_rows<T, S>(g, spv, b, nv, ls, ov, live, own, acc);
             } else {
                 lane_rows<T, S, ROW_MATVEC, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
             }
@@ -1281,6 +1294,49 @@ enum VecId {
 };
 // CSR CG: r, z, q, p_a, p_b = V1..V5.  MF CG: R[2] = V1,V2; W[2] = V3,V4; S[2] = V5,V6; P = V7.
 
+// Inputs an operator coefficient set / the Dirichlet bits were built from: with operator reuse
+// on, a call whose inputs match skips the rebuild.
+struct OpKey {
+    int valid = 0;
+    unsigned epoch = 0;
+    int S = 0, lumped = 0, form = 0;
+    double inv_dt = 0.0;
+    const void* ptr[MAXS][3] = {};
+    int32_t len[MAXS][3] = {};
+    double val[MAXS][2] = {};
+};
+static bool same_key(const OpKey& a, const OpKey& b) {
+    if (!a.valid || !b.valid || a.epoch != b.epoch || a.S != b.S || a.lumped != b.lumped ||
+        a.form != b.form || a.inv_dt != b.inv_dt)
+        return false;
+    for (int s = 0; s < a.S; ++s)
+        for (int c = 0; c < 3; ++c)
+            if (a.ptr[s][c] != b.ptr[s][c] || a.len[s][c] != b.len[s][c] ||
+                (c < 2 && a.val[s][c] != b.val[s][c]))
+                return false;
+    return true;
+}
+static OpKey make_key(const dsv_mesh* m, int S, const dsv_species* sp, double inv_dt, int lumped) {
+    OpKey k;
+    k.valid = 1;
+    k.epoch = m->op_epoch;
+    k.S = S;
+    k.lumped = lumped ? 1 : 0;
+    k.form = m->operator_form;
+    k.inv_dt = inv_dt;
+    for (int s = 0; s < S; ++s) {
+        k.ptr[s][0] = sp[s].D;
+        k.ptr[s][1] = sp[s].alpha;
+        k.ptr[s][2] = sp[s].bc_dofs;
+        k.len[s][0] = sp[s].D_len;
+        k.len[s][1] = sp[s].alpha_len;
+        k.len[s][2] = sp[s].n_bc;
+        k.val[s][0] = sp[s].D_value;
+        k.val[s][1] = sp[s].alpha_value;
+    }
+    return k;
+}
+
 struct Workspace {
     int method = 0, S = 0, V = 0;
     hipStream_t stream = nullptr;
@@ -1291,7 +1347,7 @@ struct Workspace {
                               // in the CG precision)
     double* pR = nullptr;     // partials of the residual kernel (2S per workgroup)
     double* h_pR = nullptr;   // pinned copy
-    int* h_flags = nullptr;   // pinned activity flags of the two chunks in flight
+    CGState* h_poll = nullptr;  // pinned state snapshots after the two chunks in flight
     hipEvent_t ev[2] = {};
     // operator coefficients: [0] kernels of the CG precision, [1] FP64 residual of the
     // refinement path; per-ELL-entry D then alpha per species, and the fast form's mass
@@ -1304,8 +1360,14 @@ struct Workspace {
     unsigned char prm2[sizeof(OpParams<double>)] = {};  // OpParams<double> of the residual
     void* vb[VB_COUNT] = {};
     double* csr[MAXS] = {};  // CSR64 values per species
+    // operator reuse: the inputs of coefficient sets 0 / 1, of the Dirichlet bits, and whether
+    // the Jacobi diagonal (VB_DINV) belongs to set 0
+    OpKey key[2];
+    OpKey bckey;
+    int dinv_valid = 0;
     // captured CG chunk and the parameters it was captured with
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph_tail = nullptr;  // the short chunk used near convergence
     int graph_chunk = 0;
     int graph_valid = 0;
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
@@ -1316,10 +1378,11 @@ static size_t rec_bytes(size_t t, int S) { return t * S; }
 static void free_ws(Workspace* w) {
     if (!w) return;
     if (w->graph) (void)hipGraphExecDestroy(w->graph);
+    if (w->graph_tail) (void)hipGraphExecDestroy(w->graph_tail);
     dfree(w->st);
     if (w->h_st) (void)hipHostFree(w->h_st);
     if (w->h_pR) (void)hipHostFree(w->h_pR);
-    if (w->h_flags) (void)hipHostFree(w->h_flags);
+    if (w->h_poll) (void)hipHostFree(w->h_poll);
     for (int i = 0; i < 2; ++i)
         if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
     dfree(w->pA);
@@ -1388,7 +1451,7 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S) {
         DSV_HIP(hipHostMalloc((void**)&w->h_st, sizeof(CGState), hipHostMallocDefault));
         DSV_HIP(hipHostMalloc((void**)&w->h_pR, sizeof(double) * 2 * MAXS * MAX_PROD,
                               hipHostMallocDefault));
-        DSV_HIP(hipHostMalloc((void**)&w->h_flags, 2 * sizeof(int), hipHostMallocDefault));
+        DSV_HIP(hipHostMalloc((void**)&w->h_poll, 2 * sizeof(CGState), hipHostMallocDefault));
         for (int i = 0; i < 2; ++i)
             DSV_HIP(hipEventCreateWithFlags(&w->ev[i], hipEventDisableTiming));
         w->pA = dalloc<double>((size_t)MAXS * MAX_PROD);
@@ -1415,6 +1478,10 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S) {
 
 // Dirichlet bits (internal vertex order): bit s at the Dirichlet vertices of species s
 static void prepare_bc(dsv_mesh* m, Workspace* w, int S, const dsv_species* sp, hipStream_t st) {
+    OpKey k = make_key(m, S, sp, 0.0, 0);
+    if (m->op_reuse && same_key(k, w->bckey)) return;
+    w->bckey = k;
+    w->bckey.valid = m->op_reuse;
     DSV_HIP(hipMemsetAsync(w->bcbits, 0, m->V, st));
     for (int s = 0; s < S; ++s)
         if (sp[s].n_bc > 0)
@@ -1440,6 +1507,7 @@ static Geo make_geo(const dsv_mesh* m, const Partition& P) {
     g.elperm = P.elperm;
     g.eidx = P.eidx;
     g.ewt = P.ewt;
+    g.ewt64 = P.ewt64;
     return g;
 }
 
@@ -1465,7 +1533,7 @@ static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char
     const Geo g = make_geo(m, P);
     OpParams<T> p;
     std::memcpy(&p, prm, sizeof(p));
-    if constexpr (MODE == MF_APPLY && std::is_same<T, float>::value) {
+    if constexpr (MODE == MF_APPLY || MODE == MF_RESID) {
         if (p.stored) {
             const size_t lds_st = checked_lds<T, S>(m, P, true);
             hipLaunchKernelGGL((k_mf<T, S, MODE, true, true>), dim3(mf_grid(P)), dim3(P.threads),
@@ -1491,6 +1559,11 @@ static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char
 template <class T, int S>
 static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_species* sp,
                        double inv_dt, int lumped, hipStream_t st, int set) {
+    const OpKey key = make_key(m, S, sp, inv_dt, lumped);
+    if (m->op_reuse && same_key(key, w->key[set])) return;  // prm / prm2 still hold the set
+    w->key[set] = key;
+    w->key[set].valid = m->op_reuse;
+    if (set == 0) w->dinv_valid = 0;
     OpParams<T> h{};
     h.inv_dt = (T)inv_dt;
     h.lumped = lumped ? 1 : 0;
@@ -1534,7 +1607,7 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
         launch_mf<T, S, MF_MASS>(m, P, hb, v, nullptr, st);
         h.fast = 1;
         h.mdiag = w->mdiag[set];
-        h.stored = (sizeof(T) == 4 && m->operator_form == DSV_FORM_STORED && P.eptr) ? 1 : 0;
+        h.stored = (m->operator_form == DSV_FORM_STORED && P.eptr) ? 1 : 0;
     }
     std::memcpy(set ? w->prm2 : w->prm, &h, sizeof(h));
     DSV_HIP(hipGetLastError());
@@ -1643,41 +1716,83 @@ static void cg_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stre
 template <class T, int S>
 static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream) {
     const int chunk = std::max(2, m->graph_chunk & ~1);
+    const int tail = std::max(2, (chunk / 4) & ~1);
     int launched = 0;
     if (m->use_graphs && (!w->graph_valid || w->graph_chunk != chunk ||
                           std::memcmp(w->graph_prm, w->prm, sizeof(w->prm)) != 0)) {
-        if (w->graph) (void)hipGraphExecDestroy(w->graph);
-        w->graph = nullptr;
+        for (hipGraphExec_t* ge : {&w->graph, &w->graph_tail}) {
+            if (*ge) (void)hipGraphExecDestroy(*ge);
+            *ge = nullptr;
+        }
         w->graph_valid = 0;
-        hipGraph_t gr;
-        DSV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < chunk; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
-        DSV_HIP(hipStreamEndCapture(stream, &gr));
-        DSV_HIP(hipGraphInstantiate(&w->graph, gr, nullptr, nullptr, 0));
-        DSV_HIP(hipGraphDestroy(gr));
+        for (int which = 0; which < 2; ++which) {
+            hipGraph_t gr;
+            DSV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < (which ? tail : chunk); ++i) cg_iteration<T, S>(m, w, i & 1, stream);
+            DSV_HIP(hipStreamEndCapture(stream, &gr));
+            DSV_HIP(hipGraphInstantiate(which ? &w->graph_tail : &w->graph, gr, nullptr, nullptr, 0));
+            DSV_HIP(hipGraphDestroy(gr));
+        }
         w->graph_valid = 1;
         w->graph_chunk = chunk;
         std::memcpy(w->graph_prm, w->prm, sizeof(w->prm));
     }
-    // Chunk k+1 is queued before the host waits for chunk k's activity flag, so the device
-    // never idles on the poll; once every species is done the extra chunk's kernels return at
-    // their first check.
-    auto enqueue = [&](int slot) {
+    // Chunk k+1 is queued before the host waits for chunk k's state, so the device never idles
+    // on the poll; once every species is done the queued chunk's kernels return at their first
+    // check.  From the r.r of the species at two chunk ends the host predicts how many
+    // iterations remain, and queues short chunks once less than a full chunk is predicted, so
+    // that few no-op iterations follow convergence.
+    auto enqueue = [&](int slot, int len) {
         if (m->use_graphs) {
-            DSV_HIP(hipGraphLaunch(w->graph, stream));
+            DSV_HIP(hipGraphLaunch(len == chunk ? w->graph : w->graph_tail, stream));
         } else {
-            for (int i = 0; i < chunk; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
+            for (int i = 0; i < len; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
         }
-        launched += chunk;
-        DSV_HIP(hipMemcpyAsync(&w->h_flags[slot], &w->st->any_active, sizeof(int),
-                               hipMemcpyDeviceToHost, stream));
+        launched += len;
+        DSV_HIP(hipMemcpyAsync(&w->h_poll[slot], w->st, sizeof(CGState), hipMemcpyDeviceToHost,
+                               stream));
         DSV_HIP(hipEventRecord(w->ev[slot], stream));
+        return len;
     };
-    enqueue(0);
+    const int Sn = w->S;
+    CGState last{};  // state after the most recent completed chunk (host copy)
+    bool have_last = false;
+    double rr0[MAXS] = {0, 0, 0};  // r.r and iteration counts at the chunk end before it
+    int it0[MAXS] = {0, 0, 0};
+    bool have0 = false;
+    int len_q[2] = {chunk, 0};
+    enqueue(0, chunk);
     for (int k = 0;; ++k) {
-        if (launched <= max_it + 2 * chunk) enqueue((k + 1) & 1);
-        DSV_HIP(hipEventSynchronize(w->ev[k & 1]));
-        if (!w->h_flags[k & 1] || launched > max_it + 2 * chunk) break;
+        const int cur = k & 1, nxt = cur ^ 1;
+        int next_len = chunk;
+        if (have_last && have0) {
+            double remaining = 0.0;
+            bool known = true;
+            for (int s = 0; s < Sn; ++s) {
+                if (last.status[s] != ST_RUNNING) continue;
+                const int dn = last.iters[s] - it0[s];
+                if (dn <= 0 || !(last.rr[s] > 0.0) || !(rr0[s] > last.rr[s])) {
+                    known = false;
+                    continue;
+                }
+                const double rate = std::log(last.rr[s] / rr0[s]) / dn;  // per iteration, < 0
+                remaining = std::max(remaining, std::log(last.tol2[s] / last.rr[s]) / rate);
+            }
+            // chunk k (in flight) runs len_q[cur] more iterations before chunk k + 1
+            if (known && remaining - len_q[cur] < chunk) next_len = tail;
+        }
+        if (launched <= max_it + 2 * chunk) len_q[nxt] = enqueue(nxt, next_len);
+        DSV_HIP(hipEventSynchronize(w->ev[cur]));
+        if (have_last) {
+            for (int s = 0; s < Sn; ++s) {
+                rr0[s] = last.rr[s];
+                it0[s] = last.iters[s];
+            }
+            have0 = true;
+        }
+        last = w->h_poll[cur];  // copied before the next enqueue reuses this slot
+        have_last = true;
+        if (!last.any_active || launched > max_it + 2 * chunk) break;
     }
     return launched;
 }
@@ -1797,16 +1912,22 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     } else if (method == DSV_METHOD_MF64) {
         prepare_bc(m, w, S, sp, st);
         prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
-        MFVec<double, S> v{};
-        v.y = (SV<double, S>*)w->vb[VB_DINV];
-        launch_mf<double, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
+        if (!w->dinv_valid) {
+            MFVec<double, S> v{};
+            v.y = (SV<double, S>*)w->vb[VB_DINV];
+            launch_mf<double, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
+            w->dinv_valid = m->op_reuse;
+        }
     } else {
         prepare_bc(m, w, S, sp, st);
         prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
         prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1);
-        MFVec<float, S> v{};
-        v.y = (SV<float, S>*)w->vb[VB_DINV];
-        launch_mf<float, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
+        if (!w->dinv_valid) {
+            MFVec<float, S> v{};
+            v.y = (SV<float, S>*)w->vb[VB_DINV];
+            launch_mf<float, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
+            w->dinv_valid = m->op_reuse;
+        }
     }
     DSV_HIP(hipGetLastError());
 
@@ -2308,6 +2429,14 @@ int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
 #undef DSV_WS_CASE
             default: throw ArgError{"n_species must be 1, 2 or 3"};
         }
+    });
+}
+
+int dsv_set_operator_reuse(dsv_mesh* m, int32_t reuse) {
+    return guard([&] {
+        if (!m) throw ArgError{"NULL mesh"};
+        m->op_reuse = reuse ? 1 : 0;
+        m->op_epoch += 1;  // discards every operator built so far
     });
 }
 

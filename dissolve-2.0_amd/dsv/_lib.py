@@ -26,8 +26,9 @@ METHODS = {"csr64": METHOD_CSR64, "mf32_ir64": METHOD_MF32_IR64, "mf64": METHOD_
 # every symbol include/dsv.h declares
 EXPORTS = (
     "dsv_last_error", "dsv_device_count", "dsv_mesh_create", "dsv_mesh_destroy", "dsv_mesh_info",
-    "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs", "dsv_apply",
-    "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form", "dsv_time_apply",
+    "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs",
+    "dsv_apply", "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form",
+    "dsv_set_operator_reuse", "dsv_time_apply",
     "dsv_residual_norms", "dsv_warm_start", "dsv_mlp_features", "dsv_mlp_forward",
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam",
 )
@@ -103,6 +104,7 @@ def load():
                                      ctypes.POINTER(Stats)]),
         "dsv_set_solver_options": (ctypes.c_int, [P, D, I32, I32, I32]),
         "dsv_set_operator_form": (ctypes.c_int, [P, I32]),
+        "dsv_set_operator_reuse": (ctypes.c_int, [P, I32]),
         "dsv_time_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, I32, I32,
                                           P, ctypes.POINTER(D)]),
         "dsv_residual_norms": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32,

@@ -115,6 +115,11 @@ class Mesh:
         FORM_RECOMPUTE (element Jacobians recomputed on the fly)."""
         _lib.call("dsv_set_operator_form", self._h, int(form))
 
+    def set_operator_reuse(self, reuse=True):
+        """Reuse the operator data built from unchanged coefficient arrays across calls
+        (libdsv dsv_set_operator_reuse); every call discards what was built."""
+        _lib.call("dsv_set_operator_reuse", self._h, 1 if reuse else 0)
+
     def close(self):
         if getattr(self, "_h", None):
             try:

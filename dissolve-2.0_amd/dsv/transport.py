@@ -33,9 +33,10 @@ from .core import Mesh, ptr, species_array, species_struct, stream_ptr, to_devic
 
 class Stepper:
     def __init__(self, mesh: Mesh, species, dt, method="mf32_ir64", rtol=1e-8, max_it=20000,
-                 lumped=1, phi=None, grad_phi=None, eps=None, mlp=None):
+                 lumped=1, phi=None, grad_phi=None, eps=None, mlp=None, reuse_operator=True):
         import torch
         self.mesh = mesh
+        self.reuse_operator = bool(reuse_operator)
         self.dt = float(dt)
         self.method = method
         self.rtol = float(rtol)
@@ -113,6 +114,9 @@ class Stepper:
         self._keep = keep
         self._arr = species_array(structs)
         self.defs = species
+        # the coefficient arrays above stay unchanged until the next call: the library may
+        # keep the operator it builds from them (this call discards any earlier one)
+        self.mesh.set_operator_reuse(self.reuse_operator)
 
     def rhs(self):
         m = self.mesh

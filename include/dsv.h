@@ -164,6 +164,16 @@ int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
 int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
                            int32_t graph_chunk, int32_t use_graphs);
 
+/* Operator reuse (no counterpart in the reference, which re-assembles every call).  With
+ * reuse != 0, dsv_solve, dsv_apply, dsv_residual_norms and dsv_warm_start keep the operator
+ * data they build (per-entry coefficients, mass diagonal, Jacobi diagonal, Dirichlet bits) and
+ * skip rebuilding it when a later call names the same coefficient and Dirichlet arrays (same
+ * pointers and lengths), the same scalar D / alpha values, inv_dt, mass lumping and operator
+ * form.  The caller guarantees that the contents of those arrays do not change while reuse is
+ * on; every call of this function discards what was built (call it again after changing a
+ * coefficient array in place).  Default: off. */
+int dsv_set_operator_reuse(dsv_mesh* m, int32_t reuse);
+
 /* Operator form of the FP32 inner CG of DSV_METHOD_MF32_IR64 when every species has a scalar
  * D and the mass is lumped (otherwise the generic recomputing form is used):
  * DSV_FORM_STORED (default) applies D K(D=1) + diag((inv_dt + alpha) M) with K(D=1) stored
