@@ -1,0 +1,133 @@
+// Upstream coefficient fields of the transport step, on the GPU:
+//   - the nodal gradient of a P1 field, the volume-weighted average of the cells' constant
+//     gradients (`gpu/gpu_assembler.py:251-307,646-671`), and |grad u| with the reference's
+//     1e-12 regularisation (`physics/interface_velocity.py:65-87`);
+//   - the regularised-delta ORR sink of O2 and the OH source (`dissolve.py:866-875,923-936`);
+//   - the film factor of the effective diffusivities and its medium-masked form
+//     (`physics/interface_velocity.py:90-155`).
+//
+// The reference scatters every cell's weighted gradient to its 4 vertices with FP64 atomics.
+// Here each vertex gathers over its incident cells in ascending cell order -- the cells of the
+// vertex's diagonal CSR entry, already sorted by the set-up -- so the result is bit-reproducible.
+// A cell's gradient is J^-T (u1 - u0, u2 - u0, u3 - u0) = (du1 c1 + du2 c2 + du3 c3) / det J with
+// c1 = e2 x e3, c2 = e3 x e1, c3 = e1 x e2 (e_k = x_k - x_0), recomputed from the FP64
+// coordinates; the weight is |det J| / 6.
+#include <cmath>
+
+#include "dsv_internal.h"
+#include "kernels_common.h"
+
+namespace dsv {
+
+__global__ void k_nodal_gradient(int32_t V, const int32_t* __restrict__ conn,
+                                 const double* __restrict__ xyz, const double* __restrict__ detJ,
+                                 const int32_t* __restrict__ diag_pos,
+                                 const int32_t* __restrict__ cptr, const int32_t* __restrict__ cslot,
+                                 const double* __restrict__ u, double* __restrict__ grad,
+                                 double* __restrict__ gnorm) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= V) return;
+    const double* X = xyz;
+    const double* Y = xyz + V;
+    const double* Z = xyz + 2 * (size_t)V;
+    double gx = 0.0, gy = 0.0, gz = 0.0, ws = 0.0;
+    const int32_t p = diag_pos[r];
+    for (int32_t k = cptr[p]; k < cptr[p + 1]; ++k) {
+        const int32_t c = cslot[k] >> 4;
+        const int32_t n0 = conn[4 * (size_t)c], n1 = conn[4 * (size_t)c + 1];
+        const int32_t n2 = conn[4 * (size_t)c + 2], n3 = conn[4 * (size_t)c + 3];
+        const Vec3<double> x0{X[n0], Y[n0], Z[n0]};
+        const Vec3<double> e1{X[n1] - x0.x, Y[n1] - x0.y, Z[n1] - x0.z};
+        const Vec3<double> e2{X[n2] - x0.x, Y[n2] - x0.y, Z[n2] - x0.z};
+        const Vec3<double> e3{X[n3] - x0.x, Y[n3] - x0.y, Z[n3] - x0.z};
+        const Vec3<double> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
+        const double det = dot(e1, c1);
+        const double u0 = u[n0];
+        const double d1 = u[n1] - u0, d2 = u[n2] - u0, d3 = u[n3] - u0;
+        const double w = detJ[c] / 6.0;
+        const double s = w / det;
+        gx += s * (d1 * c1.x + d2 * c2.x + d3 * c3.x);
+        gy += s * (d1 * c1.y + d2 * c2.y + d3 * c3.y);
+        gz += s * (d1 * c1.z + d2 * c2.z + d3 * c3.z);
+        ws += w;
+    }
+    gx /= ws;
+    gy /= ws;
+    gz /= ws;
+    if (grad) {
+        grad[3 * (size_t)r] = gx;
+        grad[3 * (size_t)r + 1] = gy;
+        grad[3 * (size_t)r + 2] = gz;
+    }
+    if (gnorm) gnorm[r] = sqrt(gx * gx + gy * gy + gz * gz + 1e-12);
+}
+
+// delta_eps(phi) = (1 + cos(pi phi / eps)) / (2 eps) for |phi| < eps, else 0
+__device__ __forceinline__ double delta_eps(double phi, double eps) {
+    return fabs(phi) < eps ? (1.0 / (2.0 * eps)) * (1.0 + cos(M_PI * phi / eps)) : 0.0;
+}
+
+__global__ void k_transport_coefficients(int32_t V, const double* __restrict__ phi,
+                                         const double* __restrict__ gnorm,
+                                         const double* __restrict__ F, const double* __restrict__ Cl,
+                                         const double* __restrict__ O2, dsv_coef_params p,
+                                         double* __restrict__ alpha_o2, double* __restrict__ f_oh,
+                                         double* __restrict__ film, double* __restrict__ medium) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double ph = phi[v];
+    const double dl = delta_eps(ph, p.eps) * gnorm[v];
+    const double Fv = F ? F[v] : 0.0;
+    const double in_medium = ph <= 0.0 ? 1.0 : 0.0;
+    if (alpha_o2) alpha_o2[v] = dl * p.k_orr;
+    if (f_oh) {
+        const double cl = Cl ? Cl[v] : 0.0;
+        const double o2 = O2 ? O2[v] : 0.0;
+        f_oh[v] = in_medium * p.kd * Fv * cl * cl + 4.0 * dl * p.k_orr * o2;
+    }
+    if (film || medium) {
+        const double sat = fmin(fmax(Fv / fmax(p.Fmax, 1e-20), 0.0), 1.0);
+        const double ff = (1.0 - sat) + sat * p.film_floor;
+        if (film) film[v] = ff;
+        if (medium) medium[v] = ff * in_medium;
+    }
+}
+
+}  // namespace dsv
+
+using namespace dsv;
+
+extern "C" {
+
+int dsv_nodal_gradient(dsv_mesh* m, const double* u, double* grad, double* grad_norm,
+                       void* stream) {
+    return guard([&] {
+        if (!m || !u) throw ArgError{"dsv_nodal_gradient: NULL mesh or u"};
+        if (!grad && !grad_norm) throw ArgError{"dsv_nodal_gradient: no output"};
+        DSV_HIP(hipSetDevice(m->device));
+        hipStream_t st = (hipStream_t)stream;
+        hipLaunchKernelGGL(k_nodal_gradient, dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
+                           m->conn, m->xyz, m->detJ, m->diag_pos, m->cptr, m->cslot, u, grad,
+                           grad_norm);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_transport_coefficients(dsv_mesh* m, const double* phi, const double* grad_norm,
+                               const double* F_film, const double* Cl, const double* O2,
+                               const dsv_coef_params* params, double* alpha_o2, double* f_oh,
+                               double* film_factor, double* medium_factor, void* stream) {
+    return guard([&] {
+        if (!m || !phi || !grad_norm || !params)
+            throw ArgError{"dsv_transport_coefficients: NULL mesh, phi, grad_norm or params"};
+        if (!(params->eps > 0.0)) throw ArgError{"dsv_transport_coefficients: eps must be > 0"};
+        DSV_HIP(hipSetDevice(m->device));
+        hipStream_t st = (hipStream_t)stream;
+        hipLaunchKernelGGL(k_transport_coefficients, dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
+                           phi, grad_norm, F_film, Cl, O2, *params, alpha_o2, f_oh, film_factor,
+                           medium_factor);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -30,7 +30,8 @@ EXPORTS = (
     "dsv_apply", "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form",
     "dsv_set_operator_reuse", "dsv_time_apply",
     "dsv_residual_norms", "dsv_warm_start", "dsv_mlp_features", "dsv_mlp_forward",
-    "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam",
+    "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam", "dsv_nodal_gradient",
+    "dsv_transport_coefficients",
 )
 
 
@@ -51,6 +52,12 @@ class Species(ctypes.Structure):
         ("bc_dofs", ctypes.c_void_p), ("n_bc", ctypes.c_int32), ("bc_value", ctypes.c_double),
         ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
     ]
+
+
+class CoefParams(ctypes.Structure):
+    """dsv_coef_params (include/dsv.h)."""
+    _fields_ = [("eps", ctypes.c_double), ("k_orr", ctypes.c_double), ("kd", ctypes.c_double),
+                ("Fmax", ctypes.c_double), ("film_floor", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -112,6 +119,9 @@ def load():
         "dsv_warm_start": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32,
                                           ctypes.POINTER(P), ctypes.POINTER(D), ctypes.POINTER(D),
                                           ctypes.POINTER(D), P]),
+        "dsv_nodal_gradient": (ctypes.c_int, [P, P, P, P, P]),
+        "dsv_transport_coefficients": (ctypes.c_int, [P, P, P, P, P, P, ctypes.POINTER(CoefParams),
+                                                      P, P, P, P, P]),
         "dsv_mlp_features": (ctypes.c_int, [I32, P, P, P, P, P, D, D, D, D, P, P, P, P]),
         "dsv_mlp_forward": (ctypes.c_int, [P, P, I32, P, P]),
         "dsv_mlp_grad_workspace": (ctypes.c_size_t, [I32]),

@@ -165,7 +165,13 @@ class GPUSparseAssembler:
         return self.d_b
 
     def nodal_gradient(self, u_vert):
-        raise NotImplementedError("nodal_gradient serves the level-set path, outside this library")
+        """Per-vertex grad u (torch, n_dofs x 3) of a P1 field: the cells' constant gradients
+        averaged with weights |det J| / 6 (`gpu/gpu_assembler.py:646-671`), on the GPU."""
+        from .coefficients import nodal_gradient
+        if hasattr(u_vert, "x"):
+            u_vert = u_vert.x.array
+        self._check_len(u_vert)
+        return nodal_gradient(self.mesh, u_vert)
 
     def scaffold_volume(self, phi_arr, epsilon):
         raise NotImplementedError("scaffold_volume serves the level-set path, outside this library")

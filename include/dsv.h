@@ -191,6 +191,38 @@ int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
                    int32_t lumped, int32_t precision, int32_t mode, int32_t n_rep, void* stream,
                    double* ms);
 
+/* --- upstream coefficient fields of the transport step --- */
+
+/* Nodal gradient of a P1 field u (device, V): per vertex, the average of the constant gradients
+ * of its incident cells weighted by cell volume |det J| / 6 -- GPUSparseAssembler.nodal_gradient
+ * (`gpu/gpu_assembler.py:251-307,646-671`), gathered per vertex in ascending cell order instead
+ * of scattered with atomics.  grad (V x 3, interleaved gx, gy, gz) and grad_norm
+ * (sqrt(gx^2 + gy^2 + gz^2 + 1e-12), compute_norm_grad_phi, `physics/interface_velocity.py:65-87`)
+ * may each be NULL, not both. */
+int dsv_nodal_gradient(dsv_mesh* m, const double* u, double* grad, double* grad_norm,
+                       void* stream);
+
+typedef struct dsv_coef_params {
+    double eps;         /* half-width of the regularised delta band */
+    double k_orr;       /* ORR rate */
+    double kd;          /* film-dissolution coefficient of the OH source */
+    double Fmax;        /* film saturation */
+    double film_floor;  /* porosity * constrictivity / tortuosity^2 */
+} dsv_coef_params;
+
+/* Per vertex (device arrays, length V; every output may be NULL):
+ *   alpha_o2 = delta_eps(phi) |grad phi| k_orr                          (`dissolve.py:866-875`)
+ *   f_oh = [phi <= 0] kd F Cl^2 + 4 delta_eps(phi) |grad phi| k_orr O2  (`dissolve.py:923-936`)
+ *   film_factor = (1 - s) + s film_floor, s = clip(F / Fmax, 0, 1), and
+ *   medium_factor = film_factor [phi <= 0]       (`physics/interface_velocity.py:90-155`: the
+ *   effective diffusivities are D_Zn0 film_factor and D_i0 medium_factor for Cl, OH, O2)
+ * with delta_eps(phi) = (1 + cos(pi phi / eps)) / (2 eps) for |phi| < eps, else 0.  F_film, Cl and
+ * O2 may be NULL (taken as 0). */
+int dsv_transport_coefficients(dsv_mesh* m, const double* phi, const double* grad_norm,
+                               const double* F_film, const double* Cl, const double* O2,
+                               const dsv_coef_params* params, double* alpha_o2, double* f_oh,
+                               double* film_factor, double* medium_factor, void* stream);
+
 /* --- learned warm start (no counterpart in the reference, whose solves start from the
  * previous solution): a per-vertex MLP 7 -> 64 -> 64 -> 1 with tanh activations, FP32
  * parameters packed as W1[64][7], b1[64], W2[64][64], b2[64], w3[64], b3.  Device pointers. */
