@@ -187,6 +187,55 @@ def profile_run(mesh, w, reps):
         f"({reps} launches each plus 3 warm-up)")
 
 
+def run_series(mesh, w, n=200, epochs=5):
+    """Receding-interface series: the previous-solution start, the Galerkin extrapolation and
+    the learned start (trained on steps 1..n/2) over steps n/2+1..n; prints one JSON line."""
+    import torch
+    from dsv import workloads
+    from dsv.mlp import WarmStartMLP
+    from dsv.transport import Stepper
+    out = {"metric": "receding-interface series, 3-species BE steps/s and CG iterations over "
+                     f"steps {n // 2 + 1}-{n}, rtol {w['rtol']:g}",
+           "workload": f"{w['name']}: {len(w['conn'])} tets, interface speed 0.008 O2, "
+                       "coefficients recomputed on the GPU every step", "paths": {}}
+    for warm in ("previous", "extrapolate", "learned"):
+        st = Stepper(mesh, w["species"], w["dt"], method="mf32_ir64", rtol=w["rtol"],
+                     phi=w["phi"], grad_phi=w["grad_norm"], eps=w["eps"],
+                     mlp=WarmStartMLP(mesh.device) if warm == "learned" else None)
+        drv = workloads.RecedingInterface(st, w)
+        half = n // 2
+        train = None
+        if warm == "learned":
+            t = time.perf_counter()
+            loss = st.train_warm_start(half, epochs=epochs, before_step=drv)
+            train = {"epochs": epochs, "seconds_incl_steps": round(time.perf_counter() - t, 2),
+                     "loss_first_last": [round(loss[0], 4), round(loss[-1], 4)]}
+            st.set_warm_start("learned", history_dirs=2)
+        else:
+            if warm == "extrapolate":
+                st.set_warm_start("extrapolate", history_dirs=3)
+            for _ in range(half):
+                drv()
+                st.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(n - half):
+            drv()
+            its.append(st.step()["iterations"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out["paths"][warm] = {"steps_per_s": round((n - half) / dt, 2),
+                              "cg_iterations": sum(sum(i) for i in its),
+                              "converged": all(all(h["converged"]) for h in st.history),
+                              "training": train}
+        log(f"series {warm}: {out['paths'][warm]}")
+    base = out["paths"]["previous"]["cg_iterations"]
+    for p in out["paths"].values():
+        p["cg_iterations_saved_pct"] = round(100 * (1 - p["cg_iterations"] / base), 1)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +246,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--series", action="store_true",
+                    help="instead of the headline: the receding-interface time series on the "
+                         "disc (200 steps; the MLP trains on steps 1-100), one JSON line")
     ap.add_argument("--paths", default="plain,fp32,chosen,learned",
                     help="comma list of the paths to run (the JSON line needs all four)")
     ap.add_argument("--partition", default=None,
@@ -228,6 +280,9 @@ def main():
         log(f"launch shape: vertices_per_block={nv} threads={th} lanes_per_vertex={lanes}")
     if a.profile_run:
         profile_run(mesh, w, a.apply_reps)
+        return
+    if a.series:
+        run_series(mesh, w)
         return
     tuned = None
     if a.partition:

@@ -51,7 +51,7 @@ class Stepper:
         self.u_hist = [[u.clone() for u in self.u] for _ in range(3)]  # u_{n-1}, u_{n-2}, u_{n-3}
         self.b = [torch.zeros_like(u) for u in self.u]
         self.x = [torch.zeros_like(u) for u in self.u]
-        self.f = [to_device(s["f"], dev, torch.float64) if s.get("f") is not None else None
+        self.f = [to_device(s["f"], dev, torch.float64).clone() if s.get("f") is not None else None
                   for s in species]
         self.bc = [to_device(s["bc_dofs"], dev, torch.int32) if s.get("bc_dofs") is not None
                    and len(s["bc_dofs"]) > 0 else None for s in species]
@@ -60,8 +60,9 @@ class Stepper:
         self.history = []
         # learned warm start
         self.mlp = mlp
-        self.phi = to_device(phi, dev, torch.float64) if phi is not None else None
-        self.grad_phi = to_device(grad_phi, dev, torch.float64) if grad_phi is not None else None
+        self.phi = to_device(phi, dev, torch.float64).clone() if phi is not None else None
+        self.grad_phi = (to_device(grad_phi, dev, torch.float64).clone() if grad_phi is not None
+                         else None)
         self.eps = float(eps) if eps is not None else None
         self.bc_mask = []
         for k, s in enumerate(species):
@@ -99,23 +100,48 @@ class Stepper:
 
     def set_coefficients(self, species):
         """(Re)bind D / alpha / f / bc of every species (device copies)."""
+        import torch
         keep = []
         structs = []
+        self.alpha_dev = []
         for k, s in enumerate(species):
+            al = s.get("alpha")
+            al = to_device(al, self.mesh.device, torch.float64).clone() if al is not None else None
+            self.alpha_dev.append(al)
             st = species_struct(self.mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
-                                alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                                alpha=al, alpha_value=s.get("alpha_value", 0.0),
                                 bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
                                 b=self.b[k], x=self.x[k], keep=keep)
             structs.append(st)
             if "f" in s:
                 import torch
-                self.f[k] = (to_device(s["f"], self.mesh.device, torch.float64)
+                self.f[k] = (to_device(s["f"], self.mesh.device, torch.float64).clone()
                              if s["f"] is not None else None)
         self._keep = keep
         self._arr = species_array(structs)
         self.defs = species
         # the coefficient arrays above stay unchanged until the next call: the library may
         # keep the operator it builds from them (this call discards any earlier one)
+        self.mesh.set_operator_reuse(self.reuse_operator)
+
+    def refresh_coefficients(self, alpha=None, f=None):
+        """Overwrite, in place, the per-vertex reaction coefficients (alpha[k]) and sources
+        (f[k]) of the species (lists with None for the ones left unchanged; a species' alpha must
+        already be a per-vertex array), and discard the operator the library built from the old
+        values."""
+        import torch
+        dev = self.mesh.device
+        for k in range(self.S):
+            if alpha is not None and alpha[k] is not None:
+                if self.alpha_dev[k] is None:
+                    raise ValueError(f"species {self.names[k]} has no per-vertex alpha to refresh")
+                self.alpha_dev[k].copy_(to_device(alpha[k], dev, torch.float64))
+            if f is not None and f[k] is not None:
+                fk = to_device(f[k], dev, torch.float64)
+                if self.f[k] is None or self.f[k].numel() != fk.numel():
+                    self.f[k] = fk.clone()
+                else:
+                    self.f[k].copy_(fk)
         self.mesh.set_operator_reuse(self.reuse_operator)
 
     def rhs(self):
@@ -130,8 +156,7 @@ class Stepper:
 
     def _features(self, k):
         s = self.defs[k]
-        al = s.get("alpha")
-        alpha = to_device(al, self.mesh.device, self.u[k].dtype) if al is not None else None
+        alpha = self.alpha_dev[k]
         return self.mlp.features(self.u[k], self.u_hist[0][k], self.phi, self.grad_phi,
                                  s.get("D_value", 1.0) if s.get("D") is None else float(np.mean(s["D"])),
                                  self.dt, self.eps, alpha=alpha,
@@ -202,15 +227,18 @@ class Stepper:
         self.history.append(d)
         return d
 
-    def train_warm_start(self, n_steps, epochs=20, batch=16384):
-        """Run n_steps steps (with the current warm start) collecting training pairs, fit the
-        MLP on them, and switch to the learned warm start.  Returns the loss per epoch
-        (relative to the targets' mean square, which is 1 after scaling by tau)."""
+    def train_warm_start(self, n_steps, epochs=20, batch=16384, before_step=None):
+        """Run n_steps steps (with the current warm start; `before_step()` is called before each
+        one, e.g. to move the interface) collecting training pairs, fit the MLP on them, and
+        switch to the learned warm start.  Returns the loss per epoch (relative to the targets'
+        mean square, which is 1 after scaling by tau)."""
         import torch
         if self.mlp is None or self.phi is None or self.grad_phi is None or self.eps is None:
             raise ValueError("the learned warm start needs mlp, phi, grad_phi and eps")
         self.collect = True
         for _ in range(n_steps):
+            if before_step is not None:
+                before_step()
             self.step()
         self.collect = False
         if not self.pairs:

@@ -15,6 +15,8 @@ cosine delta of `physics/weak_forms.py:18-24`.
                     D = 2.0 / 2.0 / 5.3, dt = 0.005, ORR sink on O2, OH source on phi < 0,
                     O2 Dirichlet (bulk value) on the bottom face.
 * gyroid3        -- gyroid lattice 160^3 Kuhn (14.7M tets), same three species.
+* RecedingInterface -- the disc3 time series with the interface receding at a speed
+                    proportional to local O2, coefficients recomputed on the GPU every step.
 """
 from __future__ import annotations
 
@@ -138,3 +140,33 @@ def gyroid3(n=160, level=0.3):
     phi = g                                     # metal shell 0 < g < level, medium g <= 0
     bc = np.nonzero(x <= 1e-12)[0].astype(np.int32)
     return three_species(conn, xyz, phi, bc, 0.005, 1.5 * h, name="gyroid3")
+
+
+class RecedingInterface:
+    """The receding-interface time series on a three-species workload: before each step the
+    level set moves with a normal speed proportional to the local O2 concentration,
+    phi <- phi - dt kappa O2 (the metal, phi > 0, dissolves where oxygen reaches it), and the
+    step's coefficients follow on the GPU: |grad phi| (libdsv dsv_nodal_gradient), the ORR sink
+    alpha_O2 = delta_eps(phi) |grad phi| k_orr and the OH source 4 delta_eps |grad phi| k_orr O2
+    (dsv_transport_coefficients, the species blocks' `dissolve.py:866-875,923-936` with no film).
+
+    Call the instance before each stepper.step(); it updates stepper.phi and stepper.grad_phi
+    in place (the learned warm start's features) and refreshes the stepper's alpha_O2 and f_OH.
+    """
+
+    def __init__(self, stepper, w, kappa=0.008):
+        self.st = stepper
+        self.w = w
+        self.kappa = float(kappa)
+        if stepper.phi is None or stepper.grad_phi is None:
+            raise ValueError("the stepper needs phi and grad_phi")
+
+    def __call__(self):
+        from .coefficients import grad_norm, transport_coefficients
+        st, w = self.st, self.w
+        mesh = st.mesh
+        o2 = st.u[0]
+        st.phi.sub_(st.dt * self.kappa * o2)
+        grad_norm(mesh, st.phi, out=st.grad_phi)
+        c = transport_coefficients(mesh, st.phi, st.grad_phi, w["eps"], w["k_orr"], O2=o2)
+        st.refresh_coefficients(alpha=[c["alpha_o2"], None, None], f=[None, None, c["f_oh"]])
