@@ -382,11 +382,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "k_cg<float,3> (fused 3-species FP32 Jacobi-PCG iteration)",
+                     "kernel": "k_cg_st<3> (fused 3-species FP32 Jacobi-PCG iteration, "
+                               "stored geometry)",
                      "algorithmic_bytes_per_launch": nbytes,
                      "us_per_launch": round(ms_iter * 1e3, 2),
                      "traffic_source": traffic["source"] if traffic else None},
-        "apply_hbm": {"kernel": "k_mf<float,3,APPLY> (fused 3-species FP32 apply)",
+        "apply_hbm": {"kernel": "k_mf<float,3,APPLY,stored> (fused 3-species FP32 apply)",
                       "algorithmic_bytes_per_launch": abytes,
                       "us_per_launch": round(ms_apply * 1e3, 2), "achieved_gbs": round(apply_gbs, 1),
                       "frac": round(apply_gbs / HBM_PEAK_GBS, 4)},
