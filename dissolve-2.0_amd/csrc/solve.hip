@@ -459,7 +459,7 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
         const int nv = min(g.NV, g.V - v0);
         const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
         for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
-            const int gv = g.lvl[(size_t)b * ml + i];
+            const int gv = i < nv ? v0 + i : g.lvl[(size_t)b * ml + i];  // owned: contiguous
             if constexpr (!STORED) sgeo[i] = xyz[gv];
             if constexpr (KIND == ROW_MATVEC) spv[i] = v.x[gv];
         }
@@ -617,7 +617,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
         const int nloc = nv + (g.hptr[b + 1] - g.hptr[b]);
         // (1) stage u_{i+1} of every local vertex; owned vertices also advance p, x, s, r
         for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
-            const int gv = g.lvl[(size_t)b * ml + i];
+            const int gv = i < nv ? v0 + i : g.lvl[(size_t)b * ml + i];  // owned: contiguous
             if constexpr (!STORED) sgeo[i] = xyz[gv];
             const SV<T, S> d = v.dinv[gv];
             SV<T, S> rn{}, un{}, sn{};
