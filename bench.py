@@ -157,6 +157,7 @@ def time_kernels(mesh, w, reps, stream=None):
     import torch
     from dsv import _lib
     from dsv.core import species_array, species_struct
+    mesh.fresh_operator()
     keep = []
     structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
                               alpha=s.get("alpha"), bc_dofs=s.get("bc_dofs"),

@@ -24,6 +24,7 @@ DEFAULT_LANES = (1, 2, 4)
 def time_apply(mesh, species, inv_dt, precision=32, mode=1, reps=30, lumped=1, stream=None):
     """Average ms per launch of the fused kernel for `species` (list of dicts)."""
     import torch
+    mesh.fresh_operator()
     keep = []
     structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
                               alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),

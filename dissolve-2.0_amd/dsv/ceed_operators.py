@@ -96,6 +96,7 @@ class CeedDiffusionMassOp:
         m = self.mesh
         x = to_device(u_vec, m.device, torch.float64)
         y = torch.empty_like(x)
+        m.fresh_operator()
         keep = []
         s = species_struct(m, D=self.De_vals, alpha=self.alpha, bc_dofs=self.bc_dofs, keep=keep)
         import ctypes
@@ -116,6 +117,7 @@ def apply_batched(mesh, species, inv_dt, xs, lumped=1, precision=32):
     dsv.solvers.solve_batched's.  Returns a list of device tensors."""
     import ctypes
     import torch
+    mesh.fresh_operator()
     keep = []
     structs = [species_struct(mesh, D=sp.get("D"), D_value=sp.get("D_value", 0.0),
                               alpha=sp.get("alpha"), alpha_value=sp.get("alpha_value", 0.0),

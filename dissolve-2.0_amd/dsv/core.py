@@ -119,6 +119,13 @@ class Mesh:
         """Reuse the operator data built from unchanged coefficient arrays across calls
         (libdsv dsv_set_operator_reuse); every call discards what was built."""
         _lib.call("dsv_set_operator_reuse", self._h, 1 if reuse else 0)
+        self._reuse = bool(reuse)
+
+    def fresh_operator(self):
+        """Discard any reusable operator before a call whose coefficient arrays are new
+        (allocated per call, so their addresses may repeat those of freed arrays)."""
+        if getattr(self, "_reuse", False):
+            _lib.call("dsv_set_operator_reuse", self._h, 1)
 
     def close(self):
         if getattr(self, "_h", None):

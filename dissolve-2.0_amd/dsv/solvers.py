@@ -106,6 +106,7 @@ def _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type, pc_type, rtol, max_
     x = cache["x"]
     x.copy_(torch.as_tensor(np.asarray(u.x.array), dtype=torch.float64))
     keep = []
+    mesh.fresh_operator()
     s = asm.species(_arr(de_field), alpha_arr, bc_dofs, bc_value, keep=keep)
     s.b = ptr(d_b).value
     s.x = ptr(x).value
@@ -126,6 +127,7 @@ def solve_batched(mesh, species, inv_dt, rtol=1e-8, max_it=5000, method="mf32_ir
     species: list of dicts with keys D / D_value, alpha / alpha_value, bc_dofs, bc_value,
     b (torch f64 device), x (torch f64 device, warm start in, solution out).
     Returns the libdsv statistics as a dict."""
+    mesh.fresh_operator()
     keep = []
     structs = []
     for sp in species:
@@ -150,6 +152,7 @@ def galerkin_warm_start(mesh, species, inv_dt, dirs, lumped=1, stream=None):
     lists of device FP64 vectors (None = zero).  Returns (coefficients [k][s], residual norm
     of the given x, residual norm of the start left in x) per species."""
     import ctypes
+    mesh.fresh_operator()
     keep = []
     structs = []
     for sp in species:

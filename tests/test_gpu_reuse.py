@@ -1,6 +1,6 @@
-"""Operator reuse (dsv_set_operator_reuse) through the C-ABI: with reuse on, solves and
-residuals are bit-identical to rebuilding the operator every call; a coefficient array at a new
-address, and an array changed in place followed by a new reuse call, are picked up."""
+"""Operator reuse (dsv_set_operator_reuse): a stepper that keeps its operator across steps is
+bit-identical to one that rebuilds it, also after refreshed coefficients; the per-call APIs
+never serve a stale operator."""
 import numpy as np
 import pytest
 
@@ -35,29 +35,41 @@ def _solve(mesh, sps, u0, bs, t, method):
     return [s["x"].cpu().numpy() for s in sp], st
 
 
-@pytest.mark.parametrize("method", ["mf32_ir64", "mf64"])
-def test_reuse_is_bit_identical_and_tracks_changes(gpu, method):
-    mesh, sps, u0, bs, t = _setup(gpu)
-    ref, st_ref = _solve(mesh, sps, u0, bs, t, method)
+def test_stepper_reuse_is_bit_identical_and_follows_refresh(gpu):
+    """A stepper that keeps its operator across steps (reuse on) against one that rebuilds it
+    every call (reuse off, its own mesh): the same iterations and bit-identical solutions, also
+    after the coefficients are refreshed in place."""
+    from dsv import workloads
+    from dsv.core import Mesh
+    from dsv.transport import Stepper
+    w = workloads.disc3(seed=1, scale=6.0)
+    steppers = []
+    for reuse in (True, False):
+        st = Stepper(Mesh(w["conn"], w["xyz"]), w["species"], w["dt"], method="mf32_ir64",
+                     rtol=w["rtol"], reuse_operator=reuse)
+        st.set_warm_start("extrapolate")
+        steppers.append(st)
+    a, b = steppers
+    for n in range(6):
+        if n == 3:
+            for st in steppers:
+                st.refresh_coefficients(alpha=[st.alpha_dev[0] * 2.5, None, None])
+        ha, hb = a.step(), b.step()
+        assert ha["iterations"] == hb["iterations"], (n, ha, hb)
+        for ua, ub in zip(a.u, b.u):
+            assert np.array_equal(ua.cpu().numpy(), ub.cpu().numpy()), n
+
+
+def test_per_call_apis_never_reuse_a_stale_operator(gpu):
+    """With reuse on (e.g. left on by a Stepper), solve_batched builds its operator afresh, so an
+    array changed in place at the same address -- what a freed and reallocated array looks
+    like -- is never served from the cache."""
+    mesh, sps, u0, bs, t = _setup(gpu, seed=6)
     mesh.set_operator_reuse(True)
-    for _ in range(2):  # the first call builds, the second reuses
-        got, st = _solve(mesh, sps, u0, bs, t, method)
-        assert st["iterations"] == st_ref["iterations"]
-        for a, b in zip(got, ref):
-            assert np.array_equal(a, b)
-    # a new alpha array (new address): picked up without a reuse call
-    sps2 = [dict(sps[0], alpha=sps[0]["alpha"] * 3.0), sps[1], sps[2]]
+    _solve(mesh, sps, u0, bs, t, "mf32_ir64")
+    sps[0]["alpha"].mul_(3.0)  # same address, new values
+    got, _ = _solve(mesh, sps, u0, bs, t, "mf32_ir64")
     mesh.set_operator_reuse(False)
-    ref2, _ = _solve(mesh, sps2, u0, bs, t, method)
-    mesh.set_operator_reuse(True)
-    _solve(mesh, sps, u0, bs, t, method)           # cache the first operator
-    got2, _ = _solve(mesh, sps2, u0, bs, t, method)
-    for a, b in zip(got2, ref2):
+    ref, _ = _solve(mesh, sps, u0, bs, t, "mf32_ir64")
+    for a, b in zip(got, ref):
         assert np.array_equal(a, b)
-    # the same array changed in place, then a new reuse call
-    sps[0]["alpha"].mul_(3.0)
-    mesh.set_operator_reuse(True)
-    got3, _ = _solve(mesh, sps, u0, bs, t, method)
-    for a, b in zip(got3, ref2):
-        assert np.array_equal(a, b)
-    mesh.set_operator_reuse(False)
