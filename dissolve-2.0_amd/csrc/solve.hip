@@ -1365,6 +1365,9 @@ struct Workspace {
     OpKey key[2];
     OpKey bckey;
     int dinv_valid = 0;
+    // persistent grid of the MF CG kernels and the kernel variant it was sized for
+    int cg_grid = 0;
+    int cg_grid_key = 0;
     // captured CG chunk and the parameters it was captured with
     hipGraphExec_t graph = nullptr;
     hipGraphExec_t graph_tail = nullptr;  // the short chunk used near convergence
@@ -1629,6 +1632,42 @@ static CGVec<T, S> cg_vecs(Workspace* w) {
     return v;
 }
 
+// Persistent grid of the MF CG kernels: the partition's workgroup count capped at the number of
+// workgroups of the iteration kernel K_i the device keeps resident (its occupancy times the
+// CUs), so that no second round of workgroups queues behind the first.  INIT uses the same grid
+// (K_0 reads its partials).  Sized once per workspace and kernel variant.
+template <class T, int S>
+static int cg_grid(dsv_mesh* m, Workspace* w) {
+    const Partition& P = m->part;
+    OpParams<T> p;
+    std::memcpy(&p, w->prm, sizeof(p));
+    const bool stored = std::is_same<T, float>::value && p.stored;
+    const int key = stored ? 1 : (p.fast ? 2 : 3);
+    if (w->cg_grid_key == key) return w->cg_grid;
+    const void* f = nullptr;
+    size_t lds = 0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (stored) {
+            f = reinterpret_cast<const void*>(&k_cg_st<S>);
+            lds = checked_lds<T, S>(m, P, true);
+        }
+    }
+    if (!f) {
+        f = p.fast ? reinterpret_cast<const void*>(&k_cg<T, S, true, false, false>)
+                   : reinterpret_cast<const void*>(&k_cg<T, S, false, false, false>);
+        lds = checked_lds<T, S>(m, P);
+    }
+    int per_cu = 0, ncu = 0;
+    DSV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, P.threads, lds));
+    DSV_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
+    int n = mf_grid(P);
+    const int cap = (per_cu * ncu) & ~7;  // a multiple of 8: the XCD mapping
+    if (cap >= 8) n = std::min(n, cap);
+    w->cg_grid = n;
+    w->cg_grid_key = key;
+    return n;
+}
+
 // MF CG kernel: INIT (writes the partials K_0 reads, pU[0]) or K_i with parity par
 template <class T, int S, bool INIT>
 static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
@@ -1637,7 +1676,7 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     OpParams<T> p;
     std::memcpy(&p, w->prm, sizeof(p));
     const CGVec<T, S> v = cg_vecs<T, S>(w);
-    const int n = mf_grid(P);
+    const int n = cg_grid<T, S>(m, w);
     const T* pin = INIT ? nullptr : (const T*)w->pU[par];
     T* pout = (T*)(INIT ? w->pU[0] : w->pU[par ^ 1]);
     if constexpr (std::is_same<T, float>::value) {
@@ -2277,7 +2316,7 @@ static double time_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int l
     h.max_it = 1 << 30;
     write_state(w, h, ws);
     std::vector<T> pu((size_t)3 * MAXS * MAX_PROD, T(0));  // partials in the CG precision
-    const int n = mf_grid(m->part);
+    const int n = cg_grid<T, S>(m, w);
     for (int q = 0; q < 3 * S; ++q) pu[(size_t)q * n] = T(1);
     DSV_HIP(hipMemcpy(w->pU[0], pu.data(), pu.size() * sizeof(T), hipMemcpyHostToDevice));
     MFVec<T, S> v{};
