@@ -188,6 +188,38 @@ def profile_run(mesh, w, reps):
         f"({reps} launches each plus 3 warm-up)")
 
 
+def kernels_at_scale(shapes=((256, 256, 1), (256, 256, 2))):
+    """The fused CG iteration and the apply on the 14.7M-tet gyroid lattice, where the working
+    set (about 1 GB per CG iteration) is far beyond the 256 MB MALL: HIP-event times and
+    fractions of HBM peak of the algorithmic bytes, best of the given launch shapes."""
+    from dsv import workloads
+    from dsv.core import Mesh
+    t = time.perf_counter()
+    w = workloads.gyroid3()
+    mesh = Mesh(w["conn"], w["xyz"])
+    C, V = len(w["conn"]), len(w["xyz"])
+    nb, ab = cg_iteration_bytes(C, V, w["species"]), apply_bytes(C, V, w["species"])
+    best = None
+    for nv, th, lanes in shapes:
+        mesh.set_partition(nv, th, lanes)
+        it, ap = time_kernels(mesh, w, 30)
+        if best is None or it < best[0]:
+            best = (it, ap, (nv, th, lanes))
+    it, ap, shape = best
+    out = {"workload": f"gyroid3: {C} tets, {V} vertices",
+           "launch_shape": {"vertices_per_block": shape[0], "threads": shape[1],
+                            "lanes_per_vertex": shape[2]},
+           "cg_iteration": {"us": round(it * 1e3, 2), "algorithmic_bytes": nb,
+                            "achieved_gbs": round(nb / (it * 1e-3) / 1e9, 1),
+                            "frac": round(nb / (it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "apply": {"us": round(ap * 1e3, 2), "algorithmic_bytes": ab,
+                     "achieved_gbs": round(ab / (ap * 1e-3) / 1e9, 1),
+                     "frac": round(ab / (ap * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    log(f"at scale ({time.perf_counter() - t:.0f}s incl. mesh): {out}")
+    mesh.close()
+    return out
+
+
 def run_series(mesh, w, n=200, epochs=5):
     """Receding-interface series: the previous-solution start, the Galerkin extrapolation and
     the learned start (trained on steps 1..n/2) over steps n/2+1..n; prints one JSON line."""
@@ -245,6 +277,8 @@ def main():
                     help="untimed steps of every path; the learned warm start trains on them")
     ap.add_argument("--workload", default="disc3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scale", action="store_true",
+                    help="skip the kernel timings on the 14.7M-tet gyroid lattice")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--series", action="store_true",
@@ -331,6 +365,10 @@ def main():
         f"matrix-free apply alone {ms_apply * 1e3:.1f} us, {apply_gbs:.0f} GB/s")
     traffic = measured_traffic()
 
+    scale = None
+    if not a.no_scale and a.workload == "disc3":
+        scale = kernels_at_scale()
+
     cpu = None
     if not a.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -392,6 +430,7 @@ def main():
                       "algorithmic_bytes_per_launch": abytes,
                       "us_per_launch": round(ms_apply * 1e3, 2), "achieved_gbs": round(apply_gbs, 1),
                       "frac": round(apply_gbs / HBM_PEAK_GBS, 4)},
+        "kernels_at_scale": scale,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1360,6 +1360,7 @@ struct Workspace {
     unsigned char prm2[sizeof(OpParams<double>)] = {};  // OpParams<double> of the residual
     void* vb[VB_COUNT] = {};
     double* csr[MAXS] = {};  // CSR64 values per species
+    void* wd[6] = {};        // FP64 warm-start directions d_k and A d_k (dsv_solve_warm)
     // operator reuse: the inputs of coefficient sets 0 / 1, of the Dirichlet bits, and whether
     // the Jacobi diagonal (VB_DINV) belongs to set 0
     OpKey key[2];
@@ -1398,6 +1399,7 @@ static void free_ws(Workspace* w) {
     dfree(w->mdiag[0]);
     dfree(w->mdiag[1]);
     for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
+    for (int i = 0; i < 6; ++i) dfree(w->wd[i]);
     dfree(w->bcbits);
     for (int i = 0; i < VB_COUNT; ++i) dfree(w->vb[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -1919,9 +1921,24 @@ static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* a
     run_cg<T, S>(m, w, max_it, st);
 }
 
+// Galerkin warm start folded into a solve (dsv_solve_warm): directions from past solutions
+// (hist) and explicit ones (extra); outputs as dsv_warm_start's.
+struct WarmArgs {
+    int n_hist = 0;
+    const double* const* hist = nullptr;
+    int n_extra = 0;
+    const double* const* extra = nullptr;
+    double* coef = nullptr;
+    double* r_before = nullptr;
+    double* r_after = nullptr;
+};
+template <int S>
+static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
+                       double* rr, hipStream_t st);
+
 template <int S>
 static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped, int method,
-                       double rtol, int max_it, dsv_stats* stats) {
+                       double rtol, int max_it, dsv_stats* stats, const WarmArgs* warm = nullptr) {
     Workspace* w = get_ws(m, method, S);
     hipStream_t st = w->stream;
     const auto t0 = std::chrono::steady_clock::now();
@@ -1977,6 +1994,14 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     for (int round = 0;; ++round) {
         double rr[MAXS] = {0, 0, 0}, bb[MAXS] = {0, 0, 0};
         residual64<S>(m, w, st, rr, bb);
+        if (round == 0 && warm && warm->n_hist + warm->n_extra > 0) {
+            // x64 and r64 move to the Galerkin start (rr: its updated residual); should that
+            // already meet the tolerance, the true residual confirms it
+            fused_warm<S>(m, w, sp, *warm, rr, st);
+            bool met = true;
+            for (int s = 0; s < S; ++s) met &= std::sqrt(rr[s]) <= rtol * std::sqrt(bb[s]);
+            if (met) residual64<S>(m, w, st, rr, bb);
+        }
         bool all = true;
         for (int s = 0; s < S; ++s) {
             const double bn = std::sqrt(bb[s]), rn = std::sqrt(rr[s]);
@@ -2147,6 +2172,64 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_combine(int V, ProjVec<S>
     write_partials<S>(tot, red, pout);
 }
 
+// ||r0 - sum_k c_k A d_k||^2 per species, as partials (nothing written)
+template <int S, int ND>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_norms(int V, ProjVec<S> pv,
+                                                            double* __restrict__ pout) {
+    __shared__ double red[S * 16];
+    double tot[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) tot[s] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        SV<double, S> r = pv.r0[i];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            const SV<double, S> Ad = pv.Ad[k][i];
+#pragma unroll
+            for (int s = 0; s < S; ++s) r.v[s] -= pv.c[k][s] * Ad.v[s];
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) tot[s] += r.v[s] * r.v[s];
+    }
+    write_partials<S>(tot, red, pout);
+}
+
+// in place: x += sum_k c_k d_k, r -= sum_k c_k A d_k (c = 0 leaves a species unchanged)
+template <int S, int ND>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_update(int V, ProjVec<S> pv,
+                                                             SV<double, S>* __restrict__ r) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        SV<double, S> x = pv.x[i], rv = r[i];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                x.v[s] += pv.c[k][s] * d.v[s];
+                rv.v[s] -= pv.c[k][s] * Ad.v[s];
+            }
+        }
+        pv.x[i] = x;
+        r[i] = rv;
+    }
+}
+
+// d = a - b per species (caller order -> internal order; NULL b: d = a)
+template <int S>
+__global__ void k_pack_diff(int V, const int32_t* __restrict__ perm, const double* a0,
+                            const double* a1, const double* a2, const double* b0, const double* b1,
+                            const double* b2, SV<double, S>* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int c = perm ? perm[i] : i;
+    const double* A[3] = {a0, a1, a2};
+    const double* B[3] = {b0, b1, b2};
+    SV<double, S> r{};
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.v[s] = (A[s] ? A[s][c] : 0.0) - (B[s] ? B[s][c] : 0.0);
+    out[i] = r;
+}
+
 // Solves G c = g for the n x n SPD Gram matrix G (row-major) by LDL^T, dropping a direction
 // whose pivot falls below rel_tol of its own energy d.Ad (numerically dependent on the
 // previous ones, or zero); a dropped direction gets c = 0.
@@ -2247,7 +2330,9 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
         double a = 0.0;
         for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)s * PROJ_GRID + b];
         const double r0 = std::sqrt(rr[s]), r1 = std::sqrt(a);
-        const bool take = std::isfinite(r1) && r1 < r0;  // the start point is the fallback
+        bool moved = false;  // every direction dropped: nothing to take
+        for (int k = 0; k < ND; ++k) moved |= pv.c[k][s] != 0.0;
+        const bool take = moved && std::isfinite(r1) && r1 < r0;  // the start is the fallback
         if (take) out[s] = sp[s].x;
         if (r_before) r_before[s] = r0;
         if (r_after) r_after[s] = take ? r1 : r0;
@@ -2259,6 +2344,105 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
                            (const SV<double, S>*)pv.x, out[0], out[1], out[2]);
         DSV_HIP(hipGetLastError());
         DSV_HIP(hipStreamSynchronize(st));
+    }
+}
+
+template <int S, int ND>
+static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
+                          double* rr, hipStream_t st) {
+    constexpr int NP = ND * (ND + 1) / 2 + ND;
+    const unsigned gv = cdiv(m->V, 256);
+    const unsigned char* prm64 = w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm;
+    ProjVec<S> pv{};
+    for (int k = 0; k < ND; ++k) {
+        for (int j = 0; j < 2; ++j)
+            if (!w->wd[2 * k + j])
+                DSV_HIP(hipMalloc(&w->wd[2 * k + j], (size_t)m->V * rec_bytes(sizeof(double), S)));
+        auto* d = (SV<double, S>*)w->wd[2 * k];
+        auto* Ad = (SV<double, S>*)w->wd[2 * k + 1];
+        const double* a[3] = {nullptr, nullptr, nullptr};
+        const double* b[3] = {nullptr, nullptr, nullptr};
+        for (int s = 0; s < S; ++s) {
+            if (k < wa.n_hist) {  // d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
+                a[s] = k == 0 ? sp[s].x : wa.hist[(size_t)(k - 1) * S + s];
+                b[s] = wa.hist[(size_t)k * S + s];
+            } else {
+                a[s] = wa.extra[(size_t)(k - wa.n_hist) * S + s];
+            }
+        }
+        hipLaunchKernelGGL((k_pack_diff<S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, a[0],
+                           a[1], a[2], b[0], b[1], b[2], d);
+        for (int s = 0; s < S; ++s)
+            if (sp[s].n_bc > 0)
+                hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0,
+                                   st, sp[s].n_bc, sp[s].bc_dofs, m->iperm, s, 0.0, d);
+        MFVec<double, S> v{};
+        v.x = d;
+        v.y = Ad;
+        launch_mf<double, S, MF_APPLY>(m, m->part, prm64, v, nullptr, st);
+        pv.d[k] = d;
+        pv.Ad[k] = Ad;
+    }
+    pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
+    pv.x = (SV<double, S>*)w->vb[VB_X64];
+    hipLaunchKernelGGL((k_proj_dots<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
+                       w->pR);
+    DSV_HIP(hipGetLastError());
+    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * NP * PROJ_GRID,
+                           hipMemcpyDeviceToHost, st));
+    DSV_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < S; ++s) {
+        double q[NP];
+        for (int j = 0; j < NP; ++j) {
+            double a = 0.0;
+            for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)(s * NP + j) * PROJ_GRID + b];
+            q[j] = a;
+        }
+        double G[ND * ND], g[ND], c[ND];
+        int t = 0;
+        for (int j = 0; j < ND; ++j)
+            for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
+        for (int j = 0; j < ND; ++j) g[j] = q[t++];
+        solve_gram(ND, G, g, c, 1e-10);
+        for (int k = 0; k < ND; ++k) pv.c[k][s] = c[k];
+    }
+    hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
+                       w->pR);
+    DSV_HIP(hipGetLastError());
+    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * PROJ_GRID, hipMemcpyDeviceToHost,
+                           st));
+    DSV_HIP(hipStreamSynchronize(st));
+    bool any = false;
+    for (int s = 0; s < S; ++s) {
+        double a = 0.0;
+        for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)s * PROJ_GRID + b];
+        const double r0 = std::sqrt(rr[s]), r1 = std::sqrt(a);
+        bool moved = false;  // every direction dropped: nothing to take
+        for (int k = 0; k < ND; ++k) moved |= pv.c[k][s] != 0.0;
+        const bool take = moved && std::isfinite(r1) && r1 < r0;  // the start is the fallback
+        if (!take)
+            for (int k = 0; k < ND; ++k) pv.c[k][s] = 0.0;
+        any |= take;
+        if (wa.r_before) wa.r_before[s] = r0;
+        if (wa.r_after) wa.r_after[s] = take ? r1 : r0;
+        if (wa.coef)
+            for (int k = 0; k < ND; ++k) wa.coef[k * S + s] = pv.c[k][s];
+        if (take) rr[s] = a;
+    }
+    if (any) {
+        hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+                           m->V, pv, (SV<double, S>*)w->vb[VB_R64]);
+        DSV_HIP(hipGetLastError());
+    }
+}
+
+template <int S>
+static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
+                       double* rr, hipStream_t st) {
+    switch (wa.n_hist + wa.n_extra) {
+        case 1: fused_warm_nd<S, 1>(m, w, sp, wa, rr, st); break;
+        case 2: fused_warm_nd<S, 2>(m, w, sp, wa, rr, st); break;
+        default: fused_warm_nd<S, 3>(m, w, sp, wa, rr, st); break;
     }
 }
 
@@ -2407,6 +2591,41 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
             case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
             case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
             default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
+        }
+    });
+}
+
+int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                   int32_t lumped, int32_t method, double rtol, int32_t max_it, int32_t n_hist,
+                   const double* const* hist, int32_t n_extra, const double* const* extra,
+                   void* stream, dsv_stats* stats, double* coef, double* r_before,
+                   double* r_after) {
+    return guard([&] {
+        if (method != DSV_METHOD_MF32_IR64 && method != DSV_METHOD_MF64)
+            throw ArgError{"dsv_solve_warm: method must be matrix-free (MF32_IR64 or MF64)"};
+        if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
+        if (max_it < 1) throw ArgError{"max_it must be >= 1"};
+        if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > 3)
+            throw ArgError{"dsv_solve_warm: n_hist + n_extra must be in [0, 3]"};
+        if ((n_hist > 0 && !hist) || (n_extra > 0 && !extra))
+            throw ArgError{"dsv_solve_warm: NULL hist or extra"};
+        check_solve_args(m, n_species, sp, true);
+        for (int i = 0; i < n_hist * n_species; ++i)
+            if (!hist[i]) throw ArgError{"dsv_solve_warm: NULL history vector"};
+        DSV_HIP(hipSetDevice(m->device));
+        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        WarmArgs wa;
+        wa.n_hist = n_hist;
+        wa.hist = hist;
+        wa.n_extra = n_extra;
+        wa.extra = extra;
+        wa.coef = coef;
+        wa.r_before = r_before;
+        wa.r_after = r_after;
+        switch (n_species) {
+            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, &wa); break;
+            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, &wa); break;
+            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, &wa); break;
         }
     });
 }

@@ -90,6 +90,8 @@ class Stepper:
         directions (at most 2 are used beside the learned direction)."""
         if mode not in ("previous", "extrapolate", "learned"):
             raise ValueError(f"unknown warm start {mode!r}")
+        if mode != "previous" and self.method == "csr64":
+            raise ValueError("the Galerkin warm starts need a matrix-free method (mf32_ir64, mf64)")
         if mode == "learned" and (self.mlp is None or not self.mlp.trained):
             raise ValueError("the learned warm start needs a trained MLP (train_warm_start)")
         if history_dirs is not None:
@@ -199,19 +201,31 @@ class Stepper:
             feats = [self._features(k) for k in range(self.S)]
         for k in range(self.S):
             self.x[k].copy_(self.u[k] if warm is None else warm[k])
-        if warm is None and self.warm != "previous" and self.n_steps >= 1:
-            seq = [self.u] + self.u_hist
-            nh = min(self.history_dirs, 2 if learned else 3, self.n_steps)
-            dirs = [[seq[j][k] - seq[j + 1][k] for k in range(self.S)] for j in range(nh)]
-            if learned:
-                dirs.append([self._mlp_direction(feats, k) for k in range(self.S)])
-            coef, r0, r1 = self.galerkin_start(dirs)
-            self.warm_log.append({"step": self.n_steps, "mode": self.warm, "coef": coef,
-                                  "r_previous": r0, "r_start": r1,
-                                  "taken": [r1[k] < r0[k] for k in range(self.S)]})
         st = _lib.Stats()
-        _lib.call("dsv_solve", self.mesh.handle, self.S, self._arr, 1.0 / self.dt, self.lumped,
-                  _lib.METHODS[self.method], self.rtol, self.max_it, stream_ptr(self.mesh.device), st)
+        if warm is None and self.warm != "previous" and self.n_steps >= 1:
+            # Galerkin warm start over past increments (and the MLP direction), folded into
+            # the solve: the directions are formed from the stored past solutions in libdsv
+            nh = min(self.history_dirs, 2 if learned else 3, self.n_steps)
+            extra = [self._mlp_direction(feats, k) for k in range(self.S)] if learned else []
+            nd = nh + (1 if learned else 0)
+            hist = (ctypes.c_void_p * max(1, nh * self.S))(
+                *[ptr(self.u_hist[j][k]).value for j in range(nh) for k in range(self.S)])
+            ext = (ctypes.c_void_p * max(1, len(extra)))(*[ptr(e).value for e in extra])
+            coef = (ctypes.c_double * (nd * self.S))()
+            r0 = (ctypes.c_double * self.S)()
+            r1 = (ctypes.c_double * self.S)()
+            _lib.call("dsv_solve_warm", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
+                      self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it, nh, hist,
+                      1 if learned else 0, ext, stream_ptr(self.mesh.device), st, coef, r0, r1)
+            self.warm_log.append({
+                "step": self.n_steps, "mode": self.warm,
+                "coef": [[coef[k * self.S + s] for s in range(self.S)] for k in range(nd)],
+                "r_previous": list(r0), "r_start": list(r1),
+                "taken": [r1[k] < r0[k] for k in range(self.S)]})
+        else:
+            _lib.call("dsv_solve", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
+                      self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it,
+                      stream_ptr(self.mesh.device), st)
         if self.collect and feats is not None:
             for k in range(self.S):
                 X, sigma = feats[k]

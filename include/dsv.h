@@ -159,6 +159,21 @@ int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
                    int32_t lumped, int32_t n_dir, const double* const* dirs, double* coef,
                    double* r_before, double* r_after, void* stream);
 
+/* dsv_solve preceded by the Galerkin warm start of dsv_warm_start, folded into the solve (the
+ * start's FP64 residual is the solve's first; no further synchronisation).  The directions are
+ * the differences of past solutions -- hist[j*n_species + s] is species s at step n-1-j, the
+ * start sp[s].x is step n, so d_0 = x - hist_0, d_j = hist_(j-1) - hist_j (n_hist of them) --
+ * followed by n_extra explicit directions extra[k*n_species + s] (NULL = zero);
+ * n_hist + n_extra <= 3.  The combination is taken per species only where it lowers the FP64
+ * residual (the start stays the fallback); coef (n_dir*n_species), r_before and r_after
+ * (n_species) are as in dsv_warm_start and may be NULL.  Matrix-free methods only
+ * (DSV_METHOD_MF32_IR64, DSV_METHOD_MF64).  No counterpart in the reference. */
+int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                   int32_t lumped, int32_t method, double rtol, int32_t max_it, int32_t n_hist,
+                   const double* const* hist, int32_t n_extra, const double* const* extra,
+                   void* stream, dsv_stats* stats, double* coef, double* r_before,
+                   double* r_after);
+
 /* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
  * sweeps) and the number of CG iterations per captured hipGraph chunk. */
 int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,

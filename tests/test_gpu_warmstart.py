@@ -112,3 +112,67 @@ def test_argument_errors(gpu):
             galerkin_warm_start(mesh, sp, inv_dt, bad)
     with pytest.raises(ValueError):
         galerkin_warm_start(mesh, sp, inv_dt, [[d[:-1]]])
+
+
+@pytest.mark.parametrize("method", ["mf32_ir64", "mf64"])
+def test_solve_warm_matches_separate_warm_start_and_solve(gpu, method):
+    """dsv_solve_warm (directions from past solutions, folded into the solve) against
+    dsv_warm_start with the same directions followed by dsv_solve: the same coefficients, and
+    solutions within the solve tolerance's reach."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import Mesh, species_array, species_struct, stream_ptr, ptr
+    from dsv.solvers import galerkin_warm_start, solve_batched
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("disc_small", seed=9)
+    mesh = Mesh(conn, xyz)
+    V = len(xyz)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
+    exact = [OS.direct(A, b) for A, b in zip(mats, bs)]
+    # a synthetic history: past solutions drifting toward the exact one
+    hist = [[t(u0[s] - (j + 1) * 0.3 * (exact[s] - u0[s]) + rng.normal(0, 1e-3, V))
+             for s in range(3)] for j in range(2)]
+    dirs = [[t(u0[s]) - hist[0][s] for s in range(3)], [hist[0][s] - hist[1][s] for s in range(3)]]
+    # separate: warm start, then solve
+    sp_a = [dict(s, b=t(b), x=t(u)) for s, u, b in zip(sps, u0, bs)]
+    coef_a, r0_a, r1_a = galerkin_warm_start(mesh, sp_a, inv_dt, dirs)
+    st_a = solve_batched(mesh, sp_a, inv_dt, rtol=1e-10, method=method)
+    # fused
+    keep = []
+    sp_b = [dict(s, b=t(b), x=t(u)) for s, u, b in zip(sps, u0, bs)]
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                              bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                              b=s["b"], x=s["x"], keep=keep) for s in sp_b]
+    harr = (ctypes.c_void_p * 6)(*[ptr(hist[j][s]).value for j in range(2) for s in range(3)])
+    coef = (ctypes.c_double * 6)()
+    r0, r1 = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+    st = _lib.Stats()
+    _lib.call("dsv_solve_warm", mesh.handle, 3, species_array(structs), inv_dt, 1,
+              _lib.METHODS[method], 1e-10, 5000, 2, harr, 0, None, stream_ptr(gpu), st, coef, r0, r1)
+    d = st.as_dict(3)
+    assert all(d["converged"]) and all(st_a["converged"])
+    for s in range(3):
+        assert abs(r0[s] - r0_a[s]) <= 1e-10 * r0_a[s]
+        for k in range(2):
+            assert abs(coef[k * 3 + s] - coef_a[k][s]) <= 1e-10 * max(abs(coef_a[k][s]), 1e-300)
+        a, b = sp_a[s]["x"].cpu().numpy(), sp_b[s]["x"].cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
+        assert np.abs(b - exact[s]).max() <= 1e-7 * np.abs(exact[s]).max()
+    # a direction that does not help is dropped: the start stays and the solve still converges
+    sp_c = [dict(s, b=t(b), x=t(u)) for s, u, b in zip(sps, u0, bs)]
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                              bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                              b=s["b"], x=s["x"], keep=keep) for s in sp_c]
+    same = [t(u0[s]) for s in range(3)]  # the start itself as u_{n-1}: d_0 = x - x = 0
+    keep.extend(same)
+    zero = (ctypes.c_void_p * 3)(*[ptr(v).value for v in same])
+    _lib.call("dsv_solve_warm", mesh.handle, 3, species_array(structs), inv_dt, 1,
+              _lib.METHODS[method], 1e-10, 5000, 1, zero, 0, None, stream_ptr(gpu), st, coef, r0, r1)
+    d = st.as_dict(3)
+    assert all(d["converged"]) and all(r1[s] == r0[s] for s in range(3))
+    with pytest.raises(ValueError):
+        _lib.call("dsv_solve_warm", mesh.handle, 3, species_array(structs), inv_dt, 1,
+                  _lib.METHODS["csr64"], 1e-10, 5000, 1, zero, 0, None, stream_ptr(gpu), st,
+                  None, None, None)
