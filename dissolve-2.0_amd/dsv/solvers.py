@@ -71,11 +71,18 @@ def solve_transport(name: str, a_form, L_form, u, bcs: list = None, ksp_type: st
                                   "pc_type in {None, 'bjacobi', 'jacobi'}) is provided")
     if ksp_type not in ("gmres", "fgmres", "cg", "bcgs"):
         raise ValueError(f"unsupported ksp_type {ksp_type!r}")
-    converged, iterations, rel = _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type,
-                                                pc_type or "bjacobi", rtol, max_it, comm,
-                                                alpha_field=alpha_field, u_old=u_old,
-                                                source_field=source_field, bc_value=bc_value,
-                                                method=method)
+    try:
+        converged, iterations, rel = _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type,
+                                                    pc_type or "bjacobi", rtol, max_it, comm,
+                                                    alpha_field=alpha_field, u_old=u_old,
+                                                    source_field=source_field, bc_value=bc_value,
+                                                    method=method)
+    except Exception as e:
+        # as `numerics/solvers.py:264-271`: report and drop the species' cached assembler; there
+        # is no CPU path to fall back to here, so the error propagates
+        print(f"  [GPU-ASM] {name}: failed ({e})")
+        _GPU_ASM_CACHE.pop(name, None)
+        raise
     if u_prev is not None:
         u.x.array[:] = under_relax * u.x.array[:] + (1.0 - under_relax) * u_prev
     u.x.scatter_forward()

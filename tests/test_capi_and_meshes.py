@@ -78,3 +78,21 @@ def test_locality_order_is_a_renumbering():
     v1 = np.sort(np.abs(meshes.tet_signed_volume(conn, xyz)))
     v2 = np.sort(np.abs(meshes.tet_signed_volume(c2, x2)))
     np.testing.assert_allclose(v1, v2)
+
+
+def test_solve_transport_drops_cache_entry_and_reraises(monkeypatch):
+    """A failure inside the GPU branch is reported, the species' cached assembler is dropped
+    (`numerics/solvers.py:264-271`) and the error propagates: there is no CPU fallback."""
+    import types
+    from dsv import solvers
+
+    def boom(*a, **k):
+        raise RuntimeError("device lost")
+
+    monkeypatch.setattr(solvers, "_solve_gpu_asm", boom)
+    solvers._GPU_ASM_CACHE["O2"] = {"asm": None}
+    u = types.SimpleNamespace(x=types.SimpleNamespace(array=np.zeros(4),
+                                                      scatter_forward=lambda: None))
+    with pytest.raises(RuntimeError, match="device lost"):
+        solvers.solve_transport("O2", None, None, u, use_gpu=True, de_field=object())
+    assert "O2" not in solvers._GPU_ASM_CACHE
