@@ -105,6 +105,7 @@ struct Partition {
     uint16_t* eidx = nullptr;     // local neighbour index per entry
     float* ewt = nullptr;         // K_vw(D = 1) per entry
     double* ewt64 = nullptr;      // the same weights in FP64 (FP64 residual and apply)
+    int32_t* emap = nullptr;      // CSR position (caller order) of each entry, -1 for padding
 };
 
 struct Workspace;  // solver buffers, defined in solve.hip

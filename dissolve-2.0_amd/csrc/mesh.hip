@@ -168,6 +168,7 @@ void free_partition(Partition& p) {
     dfree(p.eidx);
     dfree(p.ewt);
     dfree(p.ewt64);
+    dfree(p.emap);
     p.n_edge_entries = 0;
     p.nb = 0;
     p.n_entries = 0;
@@ -250,6 +251,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
     std::vector<uint16_t> elperm((size_t)nb * NV, 0), eidx;
     std::vector<float> ewt;
     std::vector<double> ewt64;
+    std::vector<int32_t> emap;
     std::vector<int> eorder;
     for (int b = 0; b < nb; ++b) {
         const int v0 = b * NV, nv = std::min(NV, V - v0);
@@ -327,6 +329,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
             eidx.resize(base + 64 * (size_t)width, 0);
             ewt.resize(base + 64 * (size_t)width, 0.0f);
             ewt64.resize(base + 64 * (size_t)width, 0.0);
+            emap.resize(base + 64 * (size_t)width, -1);
             for (int r = r0; r < r1; ++r) {
                 const int ov = eorder[r];
                 const int c = perm_h[v0 + ov];
@@ -340,6 +343,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
                     eidx[at] = (uint16_t)local(ip[ki[j]]);
                     ewt[at] = (float)kv[j];
                     ewt64[at] = kv[j];
+                    emap[at] = j;
                     ++i;
                 }
             }
@@ -375,6 +379,7 @@ void build_partition(dsv_mesh* m, int NV, int L) {
         p.eidx = upload(eidx);
         p.ewt = upload(ewt);
         p.ewt64 = upload(ewt64);
+        p.emap = upload(emap);
         p.lperm = upload(lperm);
         p.sptr = upload(sptr);
         p.ell = upload(ell);

@@ -109,8 +109,9 @@ template <class T>
 struct SpeciesP {
     const T* D;
     const T* A;
-    T D6;  // fast forms: D / 6
-    T D1;  // stored-geometry form: D
+    T D6;        // fast forms: D / 6
+    T D1;        // stored-geometry form: D (1 when W holds the species' own K(D) weights)
+    const T* W;  // stored-geometry form with per-species weights: K(D_s) per ELL entry
 };
 
 template <class T>
@@ -121,7 +122,8 @@ struct OpParams {
     T inv_dt;
     int lumped;
     int fast;
-    int stored;  // stored-geometry form of the CG kernel (fast form, FP32)
+    int stored;  // stored-geometry form (lumped mass): apply, residual, diagonal, CG
+    int persp;   // stored form with per-species weights (some species has a D field)
 };
 
 // vectors of the apply-type kernels (vertex order of the kernel)
@@ -372,10 +374,10 @@ __device__ __forceinline__ const double* edge_weights<double>(const Geo& g) {
     return g.ewt64;
 }
 
-template <class T, int S>
-__device__ __forceinline__ void edge_rows(const Geo& g, const SV<T, S>* spv, int b, int nv,
-                                          int ls, int& ov, bool& live, SV<T, S>& own,
-                                          T (&acc)[S]) {
+template <class T, int S, bool PERSP = false, bool WSUM = false>
+__device__ __forceinline__ void edge_rows(const Geo& g, const OpParams<T>& prm,
+                                          const SV<T, S>* spv, int b, int nv, int ls, int& ov,
+                                          bool& live, SV<T, S>& own, T (&acc)[S]) {
     const T* __restrict__ ewt = edge_weights<T>(g);
     const int L = g.L;
     const int vps = 64 / L;
@@ -388,25 +390,33 @@ __device__ __forceinline__ void edge_rows(const Geo& g, const SV<T, S>* spv, int
     live = rank < nv;
     const int ovl = (int)g.elperm[b * g.NV + rank];
     ov = live ? ovl : 0;
-    own = spv[ov];
+    if constexpr (!WSUM) own = spv[ov];
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] = T(0);
     for (int t0 = 0; t0 < width; t0 += 4) {
         int ix[4];
-        T wt[4];
+        T wt[PERSP ? S : 1][4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = jbeg + 64 * min(t0 + u, width - 1);
-            const int iv = g.eidx[j];
-            const T wv = ewt[j];
-            ix[u] = t0 + u < width ? iv : ov;
-            wt[u] = t0 + u < width ? wv : T(0);
+            const bool in = t0 + u < width;
+            ix[u] = in ? (int)g.eidx[j] : ov;
+#pragma unroll
+            for (int q = 0; q < (PERSP ? S : 1); ++q) {
+                const T wv = PERSP ? prm.sp[q].W[j] : ewt[j];
+                wt[q][u] = in ? wv : T(0);
+            }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const SV<T, S> nb = spv[ix[u]];
+            if constexpr (WSUM) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] += wt[u] * (nb.v[s] - own.v[s]);
+                for (int s = 0; s < S; ++s) acc[s] += wt[PERSP ? s : 0][u];
+            } else {
+                const SV<T, S> nb = spv[ix[u]];
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[s] += wt[PERSP ? s : 0][u] * (nb.v[s] - own.v[s]);
+            }
         }
     }
 #pragma unroll
@@ -434,13 +444,14 @@ constexpr int MF_THREADS_MAX = 512;  // workgroup sizes 256 and 512
 // matrix-free operator kernel.  MODE: APPLY y = A x; DIAG y = 1 / diag(A); RESID r = b - A x
 // (FP64 partials of r.r and b.b); MASS (generic form only) y = lumped (inv_dt + alpha) mass
 // diagonal, the `mdiag` of the fast form.
-template <class T, int S, int MODE, bool FAST, bool STORED = false>
+template <class T, int S, int MODE, bool FAST, bool STORED = false, bool PERSP = false>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
      double* __restrict__ pout) {
     static_assert(!(FAST && MODE == MF_MASS), "the mass diagonal is built by the generic form");
-    static_assert(!STORED || ((MODE == MF_APPLY || MODE == MF_RESID) && FAST),
-                  "stored geometry: fast-form apply and residual");
+    static_assert(!STORED || (MODE != MF_MASS && FAST),
+                  "stored geometry: apply, residual and diagonal");
+    static_assert(STORED || !PERSP, "per-species weights belong to the stored form");
     constexpr int KIND = MODE == MF_DIAG ? ROW_DIAG : (MODE == MF_MASS ? ROW_MASS : ROW_MATVEC);
     extern __shared__ __align__(32) unsigned char smem_raw[];
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
@@ -471,16 +482,20 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
             SV<T, S> own;
             T acc[S];
             if constexpr (STORED)
-                edge_rows<T, S>(g, spv, b, nv, ls, ov, live, own, acc);
+                edge_rows<T, S, PERSP, KIND == ROW_DIAG>(g, prm, spv, b, nv, ls, ov, live, own,
+                                                         acc);
             else
                 lane_rows<T, S, KIND, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
             if (!live || (ls & (g.L - 1)) != 0) continue;
             const int vtx = v0 + ov;
             if constexpr (STORED) {
-                // D K(D=1) x + mdiag x
+                // rows D1 sum_w w_vw (x_w - x_v) + mdiag x_v; diagonal -D1 sum_w w_vw + mdiag
+                // (K 1 = 0: the diagonal of K is minus the sum of its off-diagonal entries)
                 const SV<T, S> md = reinterpret_cast<const SV<T, S>*>(prm.mdiag)[vtx];
 #pragma unroll
-                for (int s = 0; s < S; ++s) acc[s] = prm.sp[s].D1 * acc[s] + md.v[s] * own.v[s];
+                for (int s = 0; s < S; ++s)
+                    acc[s] = KIND == ROW_DIAG ? md.v[s] - prm.sp[s].D1 * acc[s]
+                                              : prm.sp[s].D1 * acc[s] + md.v[s] * own.v[s];
             } else if constexpr (FAST) {
                 finish_fast<T, S, KIND>(prm, vtx, own, acc);
             }
@@ -525,7 +540,7 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
 // matrix-free Chronopoulos-Gear PCG iteration K_i (parity par = i & 1), or the INIT kernel
 // (r_0 = (T) rin for the solved species, 0 otherwise; x = p = s_{-1} = 0; u_0 = r_0 / diag;
 // w_0 = A u_0).  Partials per workgroup: gamma, delta, rho of the next iteration.
-template <class T, int S, bool FAST, bool INIT, bool STORED>
+template <class T, int S, bool FAST, bool INIT, bool STORED, bool PERSP = false>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
      CGState* __restrict__ st, const T* __restrict__ pin, int nin, T* __restrict__ pout,
@@ -671,7 +686,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
             T acc[S];
             if constexpr (STORED) {
                 static_assert(std::is_same<T, float>::value && FAST, "stored geometry: FP32 fast form");
-                edge_rows<T, S>(g, spv, b, nv, ls, ov, live, own, acc);
+                edge_rows<T, S, PERSP>(g, prm, spv, b, nv, ls, ov, live, own, acc);
             } else {
                 lane_rows<T, S, ROW_MATVEC, FAST>(g, prm, sgeo, spv, b, nv, ls, ov, live, own, acc);
             }
@@ -782,29 +797,33 @@ __device__ __forceinline__ void prefetch_slots(const Geo& g, int b, SlotPre& sp)
 
 // sum over the slot's edges of K_vw(D=1) (u_w - u_v), reduced over the L lanes of the vertex;
 // ov < 0 marks a lane past the block's vertices (it still takes part in the lane reduction)
-template <int S>
-__device__ __forceinline__ void slot_rows(const Geo& g, const SV<float, S>* spv, int jbeg,
-                                          int width, int ov, SV<float, S>& own, float (&acc)[S]) {
+template <int S, bool PERSP>
+__device__ __forceinline__ void slot_rows(const Geo& g, const OpParams<float>& prm,
+                                          const SV<float, S>* spv, int jbeg, int width, int ov,
+                                          SV<float, S>& own, float (&acc)[S]) {
     const int o = ov < 0 ? 0 : ov;
     own = spv[o];
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] = 0.0f;
     for (int t0 = 0; t0 < width; t0 += 4) {
         int ix[4];
-        float wt[4];
+        float wt[PERSP ? S : 1][4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = jbeg + 64 * min(t0 + u, width - 1);
-            const int iv = g.eidx[j];
-            const float wv = g.ewt[j];
-            ix[u] = t0 + u < width ? iv : o;
-            wt[u] = t0 + u < width ? wv : 0.0f;
+            const bool in = t0 + u < width;
+            ix[u] = in ? (int)g.eidx[j] : o;
+#pragma unroll
+            for (int q = 0; q < (PERSP ? S : 1); ++q) {
+                const float wv = PERSP ? prm.sp[q].W[j] : g.ewt[j];
+                wt[q][u] = in ? wv : 0.0f;
+            }
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const SV<float, S> nb = spv[ix[u]];
 #pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] += wt[u] * (nb.v[s] - own.v[s]);
+            for (int s = 0; s < S; ++s) acc[s] += wt[PERSP ? s : 0][u] * (nb.v[s] - own.v[s]);
         }
     }
 #pragma unroll
@@ -814,7 +833,7 @@ __device__ __forceinline__ void slot_rows(const Geo& g, const SV<float, S>* spv,
     }
 }
 
-template <int S>
+template <int S, bool PERSP>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_cg_st(Geo g, OpParams<float> prm, CGVec<float, S> v, CGState* __restrict__ st,
         const float* __restrict__ pin, int nin, float* __restrict__ pout, int par) {
@@ -949,7 +968,7 @@ k_cg_st(Geo g, OpParams<float> prm, CGVec<float, S> v, CGState* __restrict__ st,
         auto rows = [&](int ls, int jbeg, int width, int ov) {
             SV<T, S> own;
             T acc[S];
-            slot_rows<S>(g, spv, jbeg, width, ov, own, acc);
+            slot_rows<S, PERSP>(g, prm, spv, jbeg, width, ov, own, acc);
             if (ov < 0 || (ls & (g.L - 1)) != 0) return;
             const int vtx = v0 + ov;
             const SV<T, S> md = reinterpret_cast<const SV<T, S>*>(prm.mdiag)[vtx];
@@ -1245,6 +1264,17 @@ __global__ void k_unpack(int V, const int32_t* __restrict__ perm, const SV<Ti, S
     for (int s = 0; s < S; ++s)
         if (dst[s]) dst[s][c] = (double)r.v[s];
 }
+// stored-form weights of a species with a D field: entry e takes the CSR value at its position
+// (0 for padding entries)
+template <class T>
+__global__ void k_gather_weights(int64_t n, const int32_t* __restrict__ emap,
+                                 const double* __restrict__ kvals, T* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int32_t j = emap[e];
+    out[e] = j >= 0 ? (T)kvals[j] : T(0);
+}
+
 // per-ELL-entry coefficient of the entry's cell (0 for padding entries)
 template <class T>
 __global__ void k_inc_coef(int64_t n, const int32_t* __restrict__ ell_cell,
@@ -1361,6 +1391,8 @@ struct Workspace {
     void* vb[VB_COUNT] = {};
     double* csr[MAXS] = {};  // CSR64 values per species
     void* wd[6] = {};        // FP64 warm-start directions d_k and A d_k (dsv_solve_warm)
+    void* fw[2][MAXS] = {};  // stored form: per-species weights K(D_s) of sets 0 / 1
+    double* kvals = nullptr; // CSR values of K(D_s) (nnz), assembled to build them
     // operator reuse: the inputs of coefficient sets 0 / 1, of the Dirichlet bits, and whether
     // the Jacobi diagonal (VB_DINV) belongs to set 0
     OpKey key[2];
@@ -1400,6 +1432,9 @@ static void free_ws(Workspace* w) {
     dfree(w->mdiag[1]);
     for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
     for (int i = 0; i < 6; ++i) dfree(w->wd[i]);
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < MAXS; ++q) dfree(w->fw[a][q]);
+    dfree(w->kvals);
     dfree(w->bcbits);
     for (int i = 0; i < VB_COUNT; ++i) dfree(w->vb[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -1538,11 +1573,15 @@ static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char
     const Geo g = make_geo(m, P);
     OpParams<T> p;
     std::memcpy(&p, prm, sizeof(p));
-    if constexpr (MODE == MF_APPLY || MODE == MF_RESID) {
+    if constexpr (MODE == MF_APPLY || MODE == MF_RESID || MODE == MF_DIAG) {
         if (p.stored) {
             const size_t lds_st = checked_lds<T, S>(m, P, true);
-            hipLaunchKernelGGL((k_mf<T, S, MODE, true, true>), dim3(mf_grid(P)), dim3(P.threads),
-                               lds_st, stream, g, coords<T>(m), p, v, pout);
+            if (p.persp)
+                hipLaunchKernelGGL((k_mf<T, S, MODE, true, true, true>), dim3(mf_grid(P)),
+                                   dim3(P.threads), lds_st, stream, g, coords<T>(m), p, v, pout);
+            else
+                hipLaunchKernelGGL((k_mf<T, S, MODE, true, true, false>), dim3(mf_grid(P)),
+                                   dim3(P.threads), lds_st, stream, g, coords<T>(m), p, v, pout);
             return;
         }
     }
@@ -1602,17 +1641,44 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
     }
     h.fast = 0;
     h.stored = 0;
+    h.persp = 0;
     h.mdiag = nullptr;
-    if (fast) {
+    const bool stored = lumped != 0 && m->operator_form == DSV_FORM_STORED && P.eptr;
+    if (fast || stored) {
         if (!w->mdiag[set]) DSV_HIP(hipMalloc(&w->mdiag[set], (size_t)m->V * rec_bytes(sizeof(T), S)));
         MFVec<T, S> v{};
         v.y = (SV<T, S>*)w->mdiag[set];
         unsigned char hb[sizeof(OpParams<double>)] = {};
         std::memcpy(hb, &h, sizeof(h));
         launch_mf<T, S, MF_MASS>(m, P, hb, v, nullptr, st);
-        h.fast = 1;
+        h.fast = fast ? 1 : 0;
         h.mdiag = w->mdiag[set];
-        h.stored = (m->operator_form == DSV_FORM_STORED && P.eptr) ? 1 : 0;
+        h.stored = stored ? 1 : 0;
+    }
+    if (stored && !fast) {
+        // some species has a D field: every species gets its own weights, K(D_s) per ELL entry
+        // for a field (its off-diagonal CSR values, no mass, reaction or Dirichlet rows, gathered
+        // through the entries' CSR positions), the unit weights times D for a scalar
+        h.persp = 1;
+        const size_t ne = (size_t)P.n_edge_entries;
+        for (int s = 0; s < S; ++s) {
+            if (sp[s].D == nullptr) {
+                h.sp[s].W = (const T*)(const void*)(sizeof(T) == 4 ? (const void*)P.ewt
+                                                                   : (const void*)P.ewt64);
+                continue;
+            }
+            if (!w->kvals) w->kvals = dalloc<double>((size_t)m->nnz);
+            if (!w->fw[set][s]) DSV_HIP(hipMalloc(&w->fw[set][s], std::max<size_t>(1, ne) * sizeof(T)));
+            dsv_species ks{};
+            ks.D = sp[s].D;
+            ks.D_len = sp[s].D_len;
+            ks.D_value = sp[s].D_value;
+            assemble_into(m, ks, 0.0, 1, w->kvals, st);
+            hipLaunchKernelGGL((k_gather_weights<T>), dim3(cdiv(ne, 256)), dim3(256), 0, st,
+                               (int64_t)ne, P.emap, w->kvals, (T*)w->fw[set][s]);
+            h.sp[s].W = (const T*)w->fw[set][s];
+            h.sp[s].D1 = T(1);
+        }
     }
     std::memcpy(set ? w->prm2 : w->prm, &h, sizeof(h));
     DSV_HIP(hipGetLastError());
@@ -1644,13 +1710,14 @@ static int cg_grid(dsv_mesh* m, Workspace* w) {
     OpParams<T> p;
     std::memcpy(&p, w->prm, sizeof(p));
     const bool stored = std::is_same<T, float>::value && p.stored;
-    const int key = stored ? 1 : (p.fast ? 2 : 3);
+    const int key = stored ? (p.persp ? 4 : 1) : (p.fast ? 2 : 3);
     if (w->cg_grid_key == key) return w->cg_grid;
     const void* f = nullptr;
     size_t lds = 0;
     if constexpr (std::is_same<T, float>::value) {
         if (stored) {
-            f = reinterpret_cast<const void*>(&k_cg_st<S>);
+            f = p.persp ? reinterpret_cast<const void*>(&k_cg_st<S, true>)
+                        : reinterpret_cast<const void*>(&k_cg_st<S, false>);
             lds = checked_lds<T, S>(m, P, true);
         }
     }
@@ -1684,12 +1751,23 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     if constexpr (std::is_same<T, float>::value) {
         if (p.stored) {
             const size_t lds_st = checked_lds<T, S>(m, P, true);
-            if constexpr (!INIT)
-                hipLaunchKernelGGL((k_cg_st<S>), dim3(n), dim3(P.threads), lds_st, stream, g, p, v,
-                                   w->st, pin, n, pout, par);
-            else
-                hipLaunchKernelGGL((k_cg<T, S, true, INIT, true>), dim3(n), dim3(P.threads), lds_st,
-                                   stream, g, coords<T>(m), p, v, w->st, pin, n, pout, par);
+            if constexpr (!INIT) {
+                if (p.persp)
+                    hipLaunchKernelGGL((k_cg_st<S, true>), dim3(n), dim3(P.threads), lds_st,
+                                       stream, g, p, v, w->st, pin, n, pout, par);
+                else
+                    hipLaunchKernelGGL((k_cg_st<S, false>), dim3(n), dim3(P.threads), lds_st,
+                                       stream, g, p, v, w->st, pin, n, pout, par);
+            } else {
+                if (p.persp)
+                    hipLaunchKernelGGL((k_cg<T, S, true, INIT, true, true>), dim3(n),
+                                       dim3(P.threads), lds_st, stream, g, coords<T>(m), p, v,
+                                       w->st, pin, n, pout, par);
+                else
+                    hipLaunchKernelGGL((k_cg<T, S, true, INIT, true, false>), dim3(n),
+                                       dim3(P.threads), lds_st, stream, g, coords<T>(m), p, v,
+                                       w->st, pin, n, pout, par);
+            }
             return;
         }
     }
