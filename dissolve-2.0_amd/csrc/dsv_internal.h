@@ -149,8 +149,12 @@ struct dsv_mesh {
     int use_graphs = 1;
     int operator_form = DSV_FORM_STORED;  // inner FP32 CG operator
     int op_reuse = 0;        // dsv_set_operator_reuse
+    int narrow = 1;          // dsv_set_species_narrowing
+    int stat_launches = 0;   // counters of the solve in progress (dsv_stats)
+    int stat_narrowings = 0;
     unsigned op_epoch = 0;   // bumped by every dsv_set_operator_reuse call
     dsv::Workspace* ws[3][4] = {};  // [method][n_species]
+    dsv::Workspace* nws[3][4] = {};  // inner CG narrowed to fewer species: [method][n_species]
 };
 
 namespace dsv {

@@ -39,6 +39,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 #include <type_traits>
 
 #include "dsv_internal.h"
@@ -1264,6 +1265,127 @@ __global__ void k_unpack(int V, const int32_t* __restrict__ perm, const SV<Ti, S
     for (int s = 0; s < S; ++s)
         if (dst[s]) dst[s][c] = (double)r.v[s];
 }
+// ---------------------------------------------------------------------------------------
+// Species narrowing of the batched inner CG: the species sel[0..S2) of a workspace of S
+// species continue in a workspace of S2 < S species (kernels of fewer species move fewer bytes
+// and reduce fewer partials).  k_narrow copies their per-vertex operator data and either the
+// FP64 right-hand side (before the first iteration) or, between chunks, the iteration vectors
+// read by the next iteration (parity 0), its partials (zero-padded to the narrowed grid, so the
+// sums, taken in the same per-thread order, are bit-identical) and the CG state.  k_widen copies
+// the correction and the counters back.
+template <class T, int S>
+__device__ __forceinline__ T pick(const SV<T, S>& r, int s) {
+    T x = r.v[0];
+#pragma unroll
+    for (int k = 1; k < S; ++k)
+        if (s == k) x = r.v[k];
+    return x;
+}
+template <class T, int S, int S2>
+__device__ __forceinline__ SV<T, S2> pick_all(const SV<T, S>& r, const int (&sel)[S2]) {
+    SV<T, S2> o;
+#pragma unroll
+    for (int i = 0; i < S2; ++i) o.v[i] = pick<T, S>(r, sel[i]);
+    return o;
+}
+
+constexpr int NARROW_VECS = 5;  // X, P, R[0], W[0], Sd[0]
+
+template <class T, int S, int S2>
+struct NarrowArgs {
+    int sel[S2];
+    const SV<T, S>* dinv;
+    SV<T, S2>* dinv2;
+    const SV<T, S>* mdiag;  // null when the operator form has no mass diagonal
+    SV<T, S2>* mdiag2;
+    const uint8_t* bc;
+    uint8_t* bc2;
+    const SV<double, S>* rin;  // before the first iteration (else null)
+    SV<double, S2>* rin2;
+    const SV<T, S>* vec[NARROW_VECS];  // between chunks (else null)
+    SV<T, S2>* vec2[NARROW_VECS];
+    const T* pin;
+    int nin;
+    T* pin2;
+    int nin2;
+    const CGState* st;  // between chunks (else null)
+    CGState* st2;
+};
+
+template <class T, int S, int S2>
+__global__ void k_narrow(int V, NarrowArgs<T, S, S2> a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nth = gridDim.x * blockDim.x;
+    for (int v = tid; v < V; v += nth) {
+        a.dinv2[v] = pick_all<T, S, S2>(a.dinv[v], a.sel);
+        if (a.mdiag) a.mdiag2[v] = pick_all<T, S, S2>(a.mdiag[v], a.sel);
+        if (a.bc) {
+            const unsigned b = a.bc[v];
+            unsigned b2 = 0u;
+#pragma unroll
+            for (int i = 0; i < S2; ++i) b2 |= ((b >> a.sel[i]) & 1u) << i;
+            a.bc2[v] = (uint8_t)b2;
+        }
+        if (a.rin) a.rin2[v] = pick_all<double, S, S2>(a.rin[v], a.sel);
+        if (a.st)
+#pragma unroll
+            for (int k = 0; k < NARROW_VECS; ++k) a.vec2[k][v] = pick_all<T, S, S2>(a.vec[k][v], a.sel);
+    }
+    if (!a.st) return;
+    // partials q * nin + b with q = c * S + s (c: gamma, delta, rho)
+    for (int j = tid; j < 3 * S2 * a.nin2; j += nth) {
+        const int q2 = j / a.nin2, b = j - q2 * a.nin2;
+        const int c = q2 / S2, i = q2 - c * S2;
+        a.pin2[j] = b < a.nin ? a.pin[(c * S + a.sel[i]) * a.nin + b] : T(0);
+    }
+    if (tid == 0) {
+        const CGState& s = *a.st;
+        CGState o = s;
+        for (int i = 0; i < MAXS; ++i) {
+            const bool in = i < S2;
+            const int f = in ? a.sel[i] : 0;
+            o.tol2[i] = in ? s.tol2[f] : 0.0;
+            o.rr[i] = in ? s.rr[f] : 0.0;
+            o.dead[i] = in ? s.dead[f] : 0;
+            o.iters[i] = in ? s.iters[f] : 0;
+            o.status[i] = in ? s.status[f] : ST_IDLE;
+            for (int p = 0; p < 2; ++p) {
+                o.rz[p][i] = in ? s.rz[p][f] : 0.0;
+                o.alp[p][i] = in ? s.alp[p][f] : 0.0;
+                o.act[p][i] = in ? s.act[p][f] : 0;
+            }
+        }
+        *a.st2 = o;
+    }
+}
+
+template <class T, int S, int S2>
+__global__ void k_widen(int V, int s0, int s1, SV<T, S>* __restrict__ X,
+                        const SV<T, S2>* __restrict__ X2, CGState* __restrict__ st,
+                        const CGState* __restrict__ st2) {
+    const int sel[2] = {s0, s1};
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < V) {
+        SV<T, S> x = X[tid];
+        const SV<T, S2> x2 = X2[tid];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+#pragma unroll
+            for (int i = 0; i < S2; ++i)
+                if (sel[i] == k) x.v[k] = x2.v[i];
+        X[tid] = x;
+    }
+    if (tid == 0) {
+        for (int i = 0; i < S2; ++i) {
+            st->iters[sel[i]] = st2->iters[i];
+            st->status[sel[i]] = st2->status[i];
+            st->rr[sel[i]] = st2->rr[i];
+        }
+        for (int s = 0; s < MAXS; ++s) st->act[0][s] = st->act[1][s] = 0;
+        st->any_active = 0;
+    }
+}
+
 // stored-form weights of a species with a D field: entry e takes the CSR value at its position
 // (0 for padding entries)
 template <class T>
@@ -1446,6 +1568,8 @@ void free_workspaces(dsv_mesh* m) {
         for (int b = 0; b < 4; ++b) {
             free_ws(m->ws[a][b]);
             m->ws[a][b] = nullptr;
+            free_ws(m->nws[a][b]);
+            m->nws[a][b] = nullptr;
         }
     dfree(m->Ae);
 }
@@ -1477,8 +1601,8 @@ static int upd_grid(const dsv_mesh* m) {
 
 static bool is_fp32(int method) { return method == DSV_METHOD_MF32_IR64; }
 
-static Workspace* get_ws(dsv_mesh* m, int method, int S) {
-    Workspace*& w = m->ws[method][S];
+static Workspace* get_ws(dsv_mesh* m, int method, int S, bool narrowed = false) {
+    Workspace*& w = narrowed ? m->nws[method][S] : m->ws[method][S];
     if (w) return w;
     w = new Workspace();
     w->method = method;
@@ -1829,14 +1953,138 @@ static void cg_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stre
     launch_cg<T, S, false>(m, w, parity, stream);
 }
 
+// Poll history of run_cg handed to the narrowed run that continues it (species remapped)
+struct PollSeed {
+    CGState last{};  // state at the most recent completed chunk end
+    double rr0[MAXS] = {0, 0, 0};  // r.r and iteration counts at the chunk end before it
+    int it0[MAXS] = {0, 0, 0};
+    bool have_last = false, have0 = false;
+    int pending = 0;   // iterations queued after `last` ahead of the narrowed run
+    int launched = 0;  // iterations launched so far
+};
+
+// A narrowed run starts only if it is predicted to need at least this many more iterations
+// (its copies cost about one iteration of the wider kernel)
+constexpr int NARROW_MIN = 4;
+
+template <class T, int S>
+static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream,
+                  const PollSeed* seed = nullptr);
+
+// The workspace of the S2 species sel[] of w, with its operator parameters: those species'
+// coefficients, and its own mass diagonal and Dirichlet bits (which k_narrow fills).
+template <class T, int S, int S2>
+static Workspace* narrow_ws(dsv_mesh* m, Workspace* w, const int* sel) {
+    Workspace* w2 = get_ws(m, w->method, S2, true);
+    OpParams<T> p, p2;
+    std::memcpy(&p, w->prm, sizeof(p));
+    p2 = p;
+    for (int i = 0; i < MAXS; ++i) p2.sp[i] = i < S2 ? p.sp[sel[i]] : SpeciesP<T>{};
+    if (p.mdiag && !w2->mdiag[0])
+        DSV_HIP(hipMalloc(&w2->mdiag[0], (size_t)m->V * rec_bytes(sizeof(T), S2)));
+    p2.mdiag = p.mdiag ? w2->mdiag[0] : nullptr;
+    p2.bcbits = p.bcbits ? w2->bcbits : nullptr;
+    std::memset(w2->prm, 0, sizeof(w2->prm));
+    std::memcpy(w2->prm, &p2, sizeof(p2));
+    return w2;
+}
+
+template <class T, int S, int S2>
+static NarrowArgs<T, S, S2> narrow_args(Workspace* w, Workspace* w2, const int* sel) {
+    NarrowArgs<T, S, S2> a{};
+    for (int i = 0; i < S2; ++i) a.sel[i] = sel[i];
+    OpParams<T> p;
+    std::memcpy(&p, w->prm, sizeof(p));
+    a.dinv = (const SV<T, S>*)w->vb[VB_DINV];
+    a.dinv2 = (SV<T, S2>*)w2->vb[VB_DINV];
+    a.mdiag = (const SV<T, S>*)p.mdiag;
+    a.mdiag2 = p.mdiag ? (SV<T, S2>*)w2->mdiag[0] : nullptr;
+    a.bc = p.bcbits;
+    a.bc2 = p.bcbits ? w2->bcbits : nullptr;
+    return a;
+}
+
+template <class T, int S, int S2>
+static void launch_narrow(dsv_mesh* m, const NarrowArgs<T, S, S2>& a, hipStream_t st) {
+    bool ok = a.dinv && a.dinv2 && (!a.mdiag || a.mdiag2) && (!a.bc || a.bc2) && (!a.rin || a.rin2);
+    if (a.st)
+        for (int k = 0; k < NARROW_VECS; ++k) ok = ok && a.vec[k] && a.vec2[k];
+    ok = ok && (!a.st || (a.st2 && a.pin && a.pin2 && a.nin2 >= a.nin));
+    if (!ok) throw std::runtime_error("species narrowing: a workspace vector is missing");
+    hipLaunchKernelGGL((k_narrow<T, S, S2>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V, a);
+    DSV_HIP(hipGetLastError());
+}
+
+template <class T, int S, int S2>
+static void widen(dsv_mesh* m, Workspace* w, Workspace* w2, const int* sel, hipStream_t st) {
+    static_assert(S2 <= 2, "k_widen takes two species");
+    hipLaunchKernelGGL((k_widen<T, S, S2>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V, sel[0],
+                       S2 > 1 ? sel[1] : 0, (SV<T, S>*)w->vb[VB_X], (const SV<T, S2>*)w2->vb[VB_X],
+                       w->st, (const CGState*)w2->st);
+    DSV_HIP(hipGetLastError());
+}
+
+static CGState remap_state(const CGState& s, const int* sel, int S2) {
+    CGState o = s;
+    for (int i = 0; i < MAXS; ++i) {
+        const bool in = i < S2;
+        const int f = in ? sel[i] : 0;
+        o.tol2[i] = in ? s.tol2[f] : 0.0;
+        o.rr[i] = in ? s.rr[f] : 0.0;
+        o.iters[i] = in ? s.iters[f] : 0;
+        o.status[i] = in ? s.status[f] : ST_IDLE;
+    }
+    return o;
+}
+
+// Continues run_cg on w with the kernel of the S2 species sel[] (the chunk queued last on w
+// leaves the next iteration at parity 0).  Returns false, with nothing queued, when the
+// narrowed kernel's persistent grid is smaller than w's: its partials could not take w's.
+template <class T, int S, int S2>
+static bool narrow_mid(dsv_mesh* m, Workspace* w, const int* sel, int max_it, hipStream_t st,
+                       const PollSeed& seed) {
+    if (w->method == DSV_METHOD_CSR64) return false;
+    Workspace* w2 = narrow_ws<T, S, S2>(m, w, sel);
+    const int n = cg_grid<T, S>(m, w), n2 = cg_grid<T, S2>(m, w2);
+    if (n2 < n) return false;
+    NarrowArgs<T, S, S2> a = narrow_args<T, S, S2>(w, w2, sel);
+    const CGVec<T, S> v = cg_vecs<T, S>(w);
+    const CGVec<T, S2> v2 = cg_vecs<T, S2>(w2);
+    const SV<T, S>* in[NARROW_VECS] = {v.X, v.P, v.R[0], v.W[0], v.Sd[0]};
+    SV<T, S2>* out[NARROW_VECS] = {v2.X, v2.P, v2.R[0], v2.W[0], v2.Sd[0]};
+    for (int k = 0; k < NARROW_VECS; ++k) {
+        a.vec[k] = in[k];
+        a.vec2[k] = out[k];
+    }
+    a.pin = (const T*)w->pU[0];
+    a.nin = n;
+    a.pin2 = (T*)w2->pU[0];
+    a.nin2 = n2;
+    a.st = w->st;
+    a.st2 = w2->st;
+    launch_narrow<T, S, S2>(m, a, st);
+    m->stat_narrowings += 1;
+    PollSeed s2 = seed;
+    s2.last = remap_state(seed.last, sel, S2);
+    for (int i = 0; i < MAXS; ++i) {
+        s2.rr0[i] = i < S2 ? seed.rr0[sel[i]] : 0.0;
+        s2.it0[i] = i < S2 ? seed.it0[sel[i]] : 0;
+    }
+    run_cg<T, S2>(m, w2, max_it, st, &s2);
+    widen<T, S, S2>(m, w, w2, sel, st);
+    return true;
+}
+
 // Runs CG iterations until no species is active.  Chunks of an even number of iterations
 // are captured once into a hipGraph (re-captured when the operator parameters change) and
-// replayed; the host polls one state word per chunk.
+// replayed; the host polls one state word per chunk.  Once some species have finished and the
+// rest are predicted to need NARROW_MIN or more iterations, the rest continue with the kernel
+// of fewer species (narrow_mid).
 template <class T, int S>
-static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream) {
+static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, const PollSeed* seed) {
     const int chunk = std::max(2, m->graph_chunk & ~1);
     const int tail = std::max(2, (chunk / 4) & ~1);
-    int launched = 0;
+    int launched = seed ? seed->launched : 0;
     if (m->use_graphs && (!w->graph_valid || w->graph_chunk != chunk ||
                           std::memcmp(w->graph_prm, w->prm, sizeof(w->prm)) != 0)) {
         for (hipGraphExec_t* ge : {&w->graph, &w->graph_tail}) {
@@ -1844,11 +2092,13 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream) {
             *ge = nullptr;
         }
         w->graph_valid = 0;
+        // captured on the workspace's own stream, which has no work of its own queued (a
+        // narrowed workspace is captured while the wider run's chunks are in flight on `stream`)
         for (int which = 0; which < 2; ++which) {
             hipGraph_t gr;
-            DSV_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < (which ? tail : chunk); ++i) cg_iteration<T, S>(m, w, i & 1, stream);
-            DSV_HIP(hipStreamEndCapture(stream, &gr));
+            DSV_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < (which ? tail : chunk); ++i) cg_iteration<T, S>(m, w, i & 1, w->stream);
+            DSV_HIP(hipStreamEndCapture(w->stream, &gr));
             DSV_HIP(hipGraphInstantiate(which ? &w->graph_tail : &w->graph, gr, nullptr, nullptr, 0));
             DSV_HIP(hipGraphDestroy(gr));
         }
@@ -1873,46 +2123,69 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream) {
         DSV_HIP(hipEventRecord(w->ev[slot], stream));
         return len;
     };
-    const int Sn = w->S;
-    CGState last{};  // state after the most recent completed chunk (host copy)
-    bool have_last = false;
-    double rr0[MAXS] = {0, 0, 0};  // r.r and iteration counts at the chunk end before it
-    int it0[MAXS] = {0, 0, 0};
-    bool have0 = false;
-    int len_q[2] = {chunk, 0};
-    enqueue(0, chunk);
+    PollSeed h;  // `last`, the chunk end before it, and iterations queued after `last`
+    if (seed) h = *seed;
+    // iterations still needed after `inflight` more than `last` has, or -1 while unknown
+    auto predict = [&](int inflight) -> double {
+        if (!h.have_last || !h.have0) return -1.0;
+        double remaining = 0.0;
+        for (int s = 0; s < S; ++s) {
+            if (h.last.status[s] != ST_RUNNING) continue;
+            const int dn = h.last.iters[s] - h.it0[s];
+            if (dn <= 0 || !(h.last.rr[s] > 0.0) || !(h.rr0[s] > h.last.rr[s])) return -1.0;
+            const double rate = std::log(h.last.rr[s] / h.rr0[s]) / dn;  // per iteration, < 0
+            remaining = std::max(remaining, std::log(h.last.tol2[s] / h.last.rr[s]) / rate);
+        }
+        return remaining - inflight;
+    };
+    auto next_len = [&](int inflight) {
+        const double r = predict(inflight);
+        return r >= 0.0 && r < chunk ? tail : chunk;
+    };
+    int len_q[2] = {0, 0};
+    len_q[0] = enqueue(0, next_len(h.pending));
+    // the CSR path has its own kernels and vectors: it never narrows
+    bool may_narrow = m->narrow != 0 && w->method != DSV_METHOD_CSR64;
     for (int k = 0;; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
-        int next_len = chunk;
-        if (have_last && have0) {
-            double remaining = 0.0;
-            bool known = true;
-            for (int s = 0; s < Sn; ++s) {
-                if (last.status[s] != ST_RUNNING) continue;
-                const int dn = last.iters[s] - it0[s];
-                if (dn <= 0 || !(last.rr[s] > 0.0) || !(rr0[s] > last.rr[s])) {
-                    known = false;
-                    continue;
-                }
-                const double rate = std::log(last.rr[s] / rr0[s]) / dn;  // per iteration, < 0
-                remaining = std::max(remaining, std::log(last.tol2[s] / last.rr[s]) / rate);
-            }
-            // chunk k (in flight) runs len_q[cur] more iterations before chunk k + 1
-            if (known && remaining - len_q[cur] < chunk) next_len = tail;
-        }
-        if (launched <= max_it + 2 * chunk) len_q[nxt] = enqueue(nxt, next_len);
+        // chunk k (in flight) runs len_q[cur] more iterations before chunk k + 1
+        const int inflight = len_q[cur] + h.pending;
+        len_q[nxt] = launched <= max_it + 2 * chunk ? enqueue(nxt, next_len(inflight)) : 0;
         DSV_HIP(hipEventSynchronize(w->ev[cur]));
-        if (have_last) {
-            for (int s = 0; s < Sn; ++s) {
-                rr0[s] = last.rr[s];
-                it0[s] = last.iters[s];
+        if (h.have_last) {
+            for (int s = 0; s < S; ++s) {
+                h.rr0[s] = h.last.rr[s];
+                h.it0[s] = h.last.iters[s];
             }
-            have0 = true;
+            h.have0 = true;
         }
-        last = w->h_poll[cur];  // copied before the next enqueue reuses this slot
-        have_last = true;
-        if (!last.any_active || launched > max_it + 2 * chunk) break;
+        h.last = w->h_poll[cur];  // copied before the next enqueue reuses this slot
+        h.have_last = true;
+        h.pending = 0;
+        if (!h.last.any_active || launched > max_it + 2 * chunk) break;
+        if constexpr (S > 1) {
+            if (may_narrow) {
+                int sel[MAXS] = {0, 0, 0}, n = 0;
+                for (int s = 0; s < S; ++s)
+                    if (h.last.status[s] == ST_RUNNING) sel[n++] = s;
+                const double r = predict(len_q[nxt]);
+                if (n > 0 && n < S && (r < 0.0 || r >= NARROW_MIN)) {
+                    PollSeed sd = h;
+                    sd.pending = len_q[nxt];
+                    sd.launched = launched;
+                    bool done = false;
+                    if (n == 1) {
+                        done = narrow_mid<T, S, 1>(m, w, sel, max_it, stream, sd);
+                    } else {
+                        if constexpr (S == 3) done = narrow_mid<T, 3, 2>(m, w, sel, max_it, stream, sd);
+                    }
+                    if (done) break;
+                    may_narrow = false;
+                }
+            }
+        }
     }
+    m->stat_launches += launched - (seed ? seed->launched : 0);
     return launched;
 }
 
@@ -1973,10 +2246,53 @@ static void check_solve_args(const dsv_mesh* m, int S, const dsv_species* sp, bo
     }
 }
 
-// inner CG on A d = r64 (d starts at 0), T = CG precision
+template <class T, int S>
+static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* active, int max_it,
+                     hipStream_t st);
+
+// inner CG of the S2 species sel[] of w (the others are not active) in the narrowed workspace
+template <class T, int S, int S2>
+static void inner_narrow(dsv_mesh* m, Workspace* w, const int* sel, const double* tol2,
+                         int max_it, hipStream_t st) {
+    if (w->method == DSV_METHOD_CSR64) throw std::logic_error("inner_narrow: CSR method");
+    Workspace* w2 = narrow_ws<T, S, S2>(m, w, sel);
+    NarrowArgs<T, S, S2> a = narrow_args<T, S, S2>(w, w2, sel);
+    a.rin = (const SV<double, S>*)w->vb[VB_R64];
+    a.rin2 = (SV<double, S2>*)w2->vb[VB_R64];
+    launch_narrow<T, S, S2>(m, a, st);
+    m->stat_narrowings += 1;
+    double t2[MAXS] = {0, 0, 0};
+    int a2[MAXS] = {0, 0, 0};
+    for (int i = 0; i < S2; ++i) {
+        t2[i] = tol2[sel[i]];
+        a2[i] = 1;
+    }
+    inner_cg<T, S2>(m, w2, t2, a2, max_it, st);
+    widen<T, S, S2>(m, w, w2, sel, st);
+}
+
+// inner CG on A d = r64 (d starts at 0), T = CG precision; the matrix-free methods solve only
+// the active species' system when some are not active (inner_narrow)
 template <class T, int S>
 static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* active, int max_it,
                      hipStream_t st) {
+    if constexpr (S > 1) {
+        if (m->narrow && w->method != DSV_METHOD_CSR64) {
+            int sel[MAXS] = {0, 0, 0}, n = 0;
+            for (int s = 0; s < S; ++s)
+                if (active[s]) sel[n++] = s;
+            if (n == 1) {
+                inner_narrow<T, S, 1>(m, w, sel, tol2, max_it, st);
+                return;
+            }
+            if constexpr (S == 3) {
+                if (n == 2) {
+                    inner_narrow<T, 3, 2>(m, w, sel, tol2, max_it, st);
+                    return;
+                }
+            }
+        }
+    }
     CGState h = read_state(w, st);
     for (int s = 0; s < MAXS; ++s) {
         h.tol2[s] = s < S ? tol2[s] : 0.0;
@@ -2020,6 +2336,8 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     Workspace* w = get_ws(m, method, S);
     hipStream_t st = w->stream;
     const auto t0 = std::chrono::steady_clock::now();
+    m->stat_launches = 0;
+    m->stat_narrowings = 0;
     const bool internal = method != DSV_METHOD_CSR64;
     const int32_t* perm = internal ? m->perm : nullptr;
     const unsigned gv = cdiv(m->V, 256);
@@ -2146,6 +2464,8 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
             stats->initial_rel_residual[s] = rel0[s];
         }
         stats->solve_ms = ms;
+        stats->cg_launches = m->stat_launches;
+        stats->narrowings = m->stat_narrowings;
     }
 }
 
@@ -2721,7 +3041,15 @@ int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
         m->use_graphs = use_graphs ? 1 : 0;
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 4; ++b)
-                if (m->ws[a][b]) m->ws[a][b]->graph_valid = 0;
+                for (Workspace* w : {m->ws[a][b], m->nws[a][b]})
+                    if (w) w->graph_valid = 0;
+    });
+}
+
+int dsv_set_species_narrowing(dsv_mesh* m, int32_t enable) {
+    return guard([&] {
+        if (!m) throw ArgError{"NULL mesh"};
+        m->narrow = enable ? 1 : 0;
     });
 }
 

@@ -28,6 +28,7 @@ EXPORTS = (
     "dsv_last_error", "dsv_device_count", "dsv_mesh_create", "dsv_mesh_destroy", "dsv_mesh_info",
     "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs",
     "dsv_apply", "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form",
+    "dsv_set_species_narrowing",
     "dsv_set_operator_reuse", "dsv_time_apply",
     "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_mlp_features",
     "dsv_mlp_forward",
@@ -66,6 +67,7 @@ class Stats(ctypes.Structure):
         ("iterations", ctypes.c_int32 * 3), ("outer", ctypes.c_int32 * 3),
         ("converged", ctypes.c_int32 * 3), ("rel_residual", ctypes.c_double * 3),
         ("initial_rel_residual", ctypes.c_double * 3), ("solve_ms", ctypes.c_double),
+        ("cg_launches", ctypes.c_int32), ("narrowings", ctypes.c_int32),
     ]
 
     def as_dict(self, n):
@@ -74,7 +76,8 @@ class Stats(ctypes.Structure):
             "converged": [bool(c) for c in self.converged[:n]],
             "rel_residual": list(self.rel_residual[:n]),
             "initial_rel_residual": list(self.initial_rel_residual[:n]),
-            "solve_ms": self.solve_ms,
+            "solve_ms": self.solve_ms, "cg_launches": self.cg_launches,
+            "narrowings": self.narrowings,
         }
 
 
@@ -112,6 +115,7 @@ def load():
                                      ctypes.POINTER(Stats)]),
         "dsv_set_solver_options": (ctypes.c_int, [P, D, I32, I32, I32]),
         "dsv_set_operator_form": (ctypes.c_int, [P, I32]),
+        "dsv_set_species_narrowing": (ctypes.c_int, [P, I32]),
         "dsv_set_operator_reuse": (ctypes.c_int, [P, I32]),
         "dsv_time_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, I32, I32,
                                           P, ctypes.POINTER(D)]),

@@ -108,6 +108,11 @@ class Mesh:
         _lib.call("dsv_set_solver_options", self._h, float(inner_rtol), int(max_outer),
                   int(graph_chunk), int(bool(use_graphs)))
 
+    def set_species_narrowing(self, enable=True):
+        """Continue a batched inner CG whose species finish at different iterations with the
+        kernel of fewer species (libdsv dsv_set_species_narrowing; default on)."""
+        _lib.call("dsv_set_species_narrowing", self._h, 1 if enable else 0)
+
     FORM_RECOMPUTE, FORM_STORED = 0, 1
 
     def set_operator_form(self, form):

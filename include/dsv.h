@@ -71,6 +71,9 @@ typedef struct dsv_stats {
     double rel_residual[3];      /* final true FP64 ||b - A x|| / ||b|| */
     double initial_rel_residual[3];
     double solve_ms;             /* wall time of the solve on the stream */
+    int32_t cg_launches;         /* CG iterations launched, all species together (includes
+                                    the no-op ones queued after convergence) */
+    int32_t narrowings;          /* inner solves continued with fewer species */
 } dsv_stats;
 
 const char* dsv_last_error(void);
@@ -178,6 +181,16 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
  * sweeps) and the number of CG iterations per captured hipGraph chunk. */
 int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
                            int32_t graph_chunk, int32_t use_graphs);
+
+/* Species narrowing of the batched matrix-free inner CG (DSV_METHOD_MF32_IR64, DSV_METHOD_MF64;
+ * no counterpart in the reference, which solves each species on its own).  With enable != 0
+ * (the default), an inner solve in which some species are not active, or in which some finish
+ * while the others are predicted to need several more iterations, continues the remaining
+ * species with the kernel of fewer species: it moves fewer bytes and reduces fewer partials per
+ * iteration.  The copies between the two keep every species' iterates; only the order of
+ * later partial sums may differ (the narrowed kernel may run another persistent grid), so the
+ * solutions agree to rounding, not bit for bit. */
+int dsv_set_species_narrowing(dsv_mesh* m, int32_t enable);
 
 /* Operator reuse (no counterpart in the reference, which re-assembles every call).  With
  * reuse != 0, dsv_solve, dsv_apply, dsv_residual_norms and dsv_warm_start keep the operator
