@@ -107,13 +107,27 @@ __global__ void k_bc_values(int32_t n_bc, const int32_t* __restrict__ bc, double
 
 // Dirichlet indices come from the caller's device memory: copy them back and range-check
 // before any kernel indexes with them.
+// Range check of a Dirichlet index list (a device-to-host copy).  Under operator reuse, whose
+// contract is that the arrays named do not change, a list (address, length) passes once per
+// reuse epoch.
 void validate_bc_list(const dsv_mesh* m, const int32_t* bc, int32_t n_bc) {
     if (n_bc <= 0) return;
     if (!bc) throw ArgError{"bc_dofs is NULL but n_bc > 0"};
+    const std::pair<const int32_t*, int32_t> key(bc, n_bc);
+    if (m->op_reuse && m->bc_ok_epoch == m->op_epoch)
+        for (const auto& k : m->bc_ok)
+            if (k == key) return;
     std::vector<int32_t> h(n_bc);
     DSV_HIP(hipMemcpy(h.data(), bc, (size_t)n_bc * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int32_t v : h)
         if (v < 0 || v >= m->V) throw ArgError{"bc dof index out of range"};
+    if (m->op_reuse) {
+        if (m->bc_ok_epoch != m->op_epoch) {
+            m->bc_ok.clear();
+            m->bc_ok_epoch = m->op_epoch;
+        }
+        m->bc_ok.push_back(key);
+    }
 }
 
 void check_species_coefs(const dsv_mesh* m, const dsv_species& s) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/dsv.h"
@@ -153,6 +154,9 @@ struct dsv_mesh {
     int stat_launches = 0;   // counters of the solve in progress (dsv_stats)
     int stat_narrowings = 0;
     unsigned op_epoch = 0;   // bumped by every dsv_set_operator_reuse call
+    // Dirichlet lists (address, length) range-checked in epoch bc_ok_epoch (operator reuse)
+    mutable std::vector<std::pair<const int32_t*, int32_t>> bc_ok;
+    mutable unsigned bc_ok_epoch = ~0u;
     dsv::Workspace* ws[3][4] = {};  // [method][n_species]
     dsv::Workspace* nws[3][4] = {};  // inner CG narrowed to fewer species: [method][n_species]
 };

@@ -541,11 +541,19 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
 // matrix-free Chronopoulos-Gear PCG iteration K_i (parity par = i & 1), or the INIT kernel
 // (r_0 = (T) rin for the solved species, 0 otherwise; x = p = s_{-1} = 0; u_0 = r_0 / diag;
 // w_0 = A u_0).  Partials per workgroup: gamma, delta, rho of the next iteration.
+// The state snapshot the host polls, written by workgroup 0 of a chunk's last iteration into
+// pinned host memory (visible to the host once the event recorded after the chunk completes:
+// events release to system scope)
+__device__ __forceinline__ void publish_state(const CGState* __restrict__ st,
+                                              CGState* __restrict__ poll) {
+    if (poll) *poll = *st;
+}
+
 template <class T, int S, bool FAST, bool INIT, bool STORED, bool PERSP = false>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
      CGState* __restrict__ st, const T* __restrict__ pin, int nin, T* __restrict__ pout,
-     int par) {
+     int par, CGState* __restrict__ poll) {
     extern __shared__ __align__(32) unsigned char smem_raw[];
     Vec4<T>* sgeo = reinterpret_cast<Vec4<T>*>(smem_raw);
     // the stored form stages no coordinates
@@ -580,6 +588,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
                 for (int s = 0; s < S; ++s) st->act[par][s] = 0;
                 st->kk[par] = k;
                 st->any_active = 0;
+                publish_state(st, poll);
             }
             return;
         }
@@ -619,6 +628,7 @@ k_cg(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, CGVec<T, S> v,
             }
             st->kk[par] = k + 1;
             st->any_active = any ? 1 : 0;
+            publish_state(st, poll);
         }
         if (!any) return;
     }
@@ -837,7 +847,8 @@ __device__ __forceinline__ void slot_rows(const Geo& g, const OpParams<float>& p
 template <int S, bool PERSP>
 __global__ void __launch_bounds__(MF_THREADS_MAX)
 k_cg_st(Geo g, OpParams<float> prm, CGVec<float, S> v, CGState* __restrict__ st,
-        const float* __restrict__ pin, int nin, float* __restrict__ pout, int par) {
+        const float* __restrict__ pin, int nin, float* __restrict__ pout, int par,
+        CGState* __restrict__ poll) {
     using T = float;
     extern __shared__ __align__(32) unsigned char smem_raw[];
     SV<T, S>* spv = reinterpret_cast<SV<T, S>*>(smem_raw);
@@ -857,6 +868,7 @@ k_cg_st(Geo g, OpParams<float> prm, CGVec<float, S> v, CGState* __restrict__ st,
             for (int s = 0; s < S; ++s) st->act[par][s] = 0;
             st->kk[par] = k;
             st->any_active = 0;
+            publish_state(st, poll);
         }
         return;
     }
@@ -909,6 +921,7 @@ k_cg_st(Geo g, OpParams<float> prm, CGVec<float, S> v, CGState* __restrict__ st,
             }
             st->kk[par] = k + 1;
             st->any_active = any ? 1 : 0;
+            publish_state(st, poll);
         }
     }
     if (!any) return;
@@ -1500,7 +1513,9 @@ struct Workspace {
     double* pR = nullptr;     // partials of the residual kernel (2S per workgroup)
     double* h_pR = nullptr;   // pinned copy
     CGState* h_poll = nullptr;  // pinned state snapshots after the two chunks in flight
+    CGState* d_poll = nullptr;  // their device address (the MF CG kernels write them)
     hipEvent_t ev[2] = {};
+    hipEvent_t ev_io[2] = {};  // ordering with the caller's stream: its work in, the solution out
     // operator coefficients: [0] kernels of the CG precision, [1] FP64 residual of the
     // refinement path; per-ELL-entry D then alpha per species, and the fast form's mass
     // diagonal (species records)
@@ -1524,8 +1539,8 @@ struct Workspace {
     int cg_grid = 0;
     int cg_grid_key = 0;
     // captured CG chunk and the parameters it was captured with
-    hipGraphExec_t graph = nullptr;
-    hipGraphExec_t graph_tail = nullptr;  // the short chunk used near convergence
+    hipGraphExec_t graph[2] = {};       // per poll slot (the MF kernels publish into it)
+    hipGraphExec_t graph_tail[2] = {};  // the short chunk used near convergence
     int graph_chunk = 0;
     int graph_valid = 0;
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
@@ -1535,14 +1550,18 @@ static size_t rec_bytes(size_t t, int S) { return t * S; }
 
 static void free_ws(Workspace* w) {
     if (!w) return;
-    if (w->graph) (void)hipGraphExecDestroy(w->graph);
-    if (w->graph_tail) (void)hipGraphExecDestroy(w->graph_tail);
+    for (int i = 0; i < 2; ++i) {
+        if (w->graph[i]) (void)hipGraphExecDestroy(w->graph[i]);
+        if (w->graph_tail[i]) (void)hipGraphExecDestroy(w->graph_tail[i]);
+    }
     dfree(w->st);
     if (w->h_st) (void)hipHostFree(w->h_st);
     if (w->h_pR) (void)hipHostFree(w->h_pR);
     if (w->h_poll) (void)hipHostFree(w->h_poll);
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i) {
         if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+        if (w->ev_io[i]) (void)hipEventDestroy(w->ev_io[i]);
+    }
     dfree(w->pA);
     dfree(w->pU[0]);
     dfree(w->pU[1]);
@@ -1615,9 +1634,12 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S, bool narrowed = false) 
         DSV_HIP(hipHostMalloc((void**)&w->h_st, sizeof(CGState), hipHostMallocDefault));
         DSV_HIP(hipHostMalloc((void**)&w->h_pR, sizeof(double) * 2 * MAXS * MAX_PROD,
                               hipHostMallocDefault));
-        DSV_HIP(hipHostMalloc((void**)&w->h_poll, 2 * sizeof(CGState), hipHostMallocDefault));
-        for (int i = 0; i < 2; ++i)
+        DSV_HIP(hipHostMalloc((void**)&w->h_poll, 2 * sizeof(CGState), hipHostMallocCoherent));
+        DSV_HIP(hipHostGetDevicePointer((void**)&w->d_poll, w->h_poll, 0));
+        for (int i = 0; i < 2; ++i) {
             DSV_HIP(hipEventCreateWithFlags(&w->ev[i], hipEventDisableTiming));
+            DSV_HIP(hipEventCreateWithFlags(&w->ev_io[i], hipEventDisableTiming));
+        }
         w->pA = dalloc<double>((size_t)MAXS * MAX_PROD);
         w->pU[0] = dalloc<double>((size_t)3 * MAXS * MAX_PROD);
         w->pU[1] = dalloc<double>((size_t)3 * MAXS * MAX_PROD);
@@ -1861,9 +1883,11 @@ static int cg_grid(dsv_mesh* m, Workspace* w) {
     return n;
 }
 
-// MF CG kernel: INIT (writes the partials K_0 reads, pU[0]) or K_i with parity par
+// MF CG kernel: INIT (writes the partials K_0 reads, pU[0]) or K_i with parity par (which
+// publishes the state to `poll` when it is not null)
 template <class T, int S, bool INIT>
-static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
+static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream,
+                      CGState* poll = nullptr) {
     const Partition& P = m->part;
     const Geo g = make_geo(m, P);
     OpParams<T> p;
@@ -1878,19 +1902,19 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
             if constexpr (!INIT) {
                 if (p.persp)
                     hipLaunchKernelGGL((k_cg_st<S, true>), dim3(n), dim3(P.threads), lds_st,
-                                       stream, g, p, v, w->st, pin, n, pout, par);
+                                       stream, g, p, v, w->st, pin, n, pout, par, poll);
                 else
                     hipLaunchKernelGGL((k_cg_st<S, false>), dim3(n), dim3(P.threads), lds_st,
-                                       stream, g, p, v, w->st, pin, n, pout, par);
+                                       stream, g, p, v, w->st, pin, n, pout, par, poll);
             } else {
                 if (p.persp)
                     hipLaunchKernelGGL((k_cg<T, S, true, INIT, true, true>), dim3(n),
                                        dim3(P.threads), lds_st, stream, g, coords<T>(m), p, v,
-                                       w->st, pin, n, pout, par);
+                                       w->st, pin, n, pout, par, poll);
                 else
                     hipLaunchKernelGGL((k_cg<T, S, true, INIT, true, false>), dim3(n),
                                        dim3(P.threads), lds_st, stream, g, coords<T>(m), p, v,
-                                       w->st, pin, n, pout, par);
+                                       w->st, pin, n, pout, par, poll);
             }
             return;
         }
@@ -1898,10 +1922,10 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream) {
     const size_t lds = checked_lds<T, S>(m, P);
     if (p.fast)
         hipLaunchKernelGGL((k_cg<T, S, true, INIT, false>), dim3(n), dim3(P.threads), lds, stream, g,
-                           coords<T>(m), p, v, w->st, pin, n, pout, par);
+                           coords<T>(m), p, v, w->st, pin, n, pout, par, poll);
     else
         hipLaunchKernelGGL((k_cg<T, S, false, INIT, false>), dim3(n), dim3(P.threads), lds, stream, g,
-                           coords<T>(m), p, v, w->st, pin, n, pout, par);
+                           coords<T>(m), p, v, w->st, pin, n, pout, par, poll);
 }
 
 template <int S, int MODE>
@@ -1941,16 +1965,18 @@ static void csr_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t str
     launch_update<S, false>(m, w, parity, stream);
 }
 
-// one CG iteration of the workspace's method (iteration parity `parity`)
+// one CG iteration of the workspace's method (iteration parity `parity`); the matrix-free
+// kernel publishes its state to `poll` when that is not null
 template <class T, int S>
-static void cg_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream) {
+static void cg_iteration(dsv_mesh* m, Workspace* w, int parity, hipStream_t stream,
+                         CGState* poll = nullptr) {
     if constexpr (std::is_same<T, double>::value) {
         if (w->method == DSV_METHOD_CSR64) {
             csr_iteration<S>(m, w, parity, stream);
             return;
         }
     }
-    launch_cg<T, S, false>(m, w, parity, stream);
+    launch_cg<T, S, false>(m, w, parity, stream, poll);
 }
 
 // Poll history of run_cg handed to the narrowed run that continues it (species remapped)
@@ -2084,24 +2110,32 @@ template <class T, int S>
 static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, const PollSeed* seed) {
     const int chunk = std::max(2, m->graph_chunk & ~1);
     const int tail = std::max(2, (chunk / 4) & ~1);
+    // the MF kernels publish the state the host polls themselves; the CSR path copies it
+    const bool kpoll = w->method != DSV_METHOD_CSR64;
     int launched = seed ? seed->launched : 0;
     if (m->use_graphs && (!w->graph_valid || w->graph_chunk != chunk ||
                           std::memcmp(w->graph_prm, w->prm, sizeof(w->prm)) != 0)) {
-        for (hipGraphExec_t* ge : {&w->graph, &w->graph_tail}) {
-            if (*ge) (void)hipGraphExecDestroy(*ge);
-            *ge = nullptr;
-        }
+        for (int slot = 0; slot < 2; ++slot)
+            for (hipGraphExec_t* ge : {&w->graph[slot], &w->graph_tail[slot]}) {
+                if (*ge) (void)hipGraphExecDestroy(*ge);
+                *ge = nullptr;
+            }
         w->graph_valid = 0;
         // captured on the workspace's own stream, which has no work of its own queued (a
         // narrowed workspace is captured while the wider run's chunks are in flight on `stream`)
-        for (int which = 0; which < 2; ++which) {
-            hipGraph_t gr;
-            DSV_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < (which ? tail : chunk); ++i) cg_iteration<T, S>(m, w, i & 1, w->stream);
-            DSV_HIP(hipStreamEndCapture(w->stream, &gr));
-            DSV_HIP(hipGraphInstantiate(which ? &w->graph_tail : &w->graph, gr, nullptr, nullptr, 0));
-            DSV_HIP(hipGraphDestroy(gr));
-        }
+        for (int slot = 0; slot < 2; ++slot)
+            for (int which = 0; which < 2; ++which) {
+                const int len = which ? tail : chunk;
+                hipGraph_t gr;
+                DSV_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < len; ++i)
+                    cg_iteration<T, S>(m, w, i & 1, w->stream,
+                                       kpoll && i == len - 1 ? &w->d_poll[slot] : nullptr);
+                DSV_HIP(hipStreamEndCapture(w->stream, &gr));
+                DSV_HIP(hipGraphInstantiate(which ? &w->graph_tail[slot] : &w->graph[slot], gr,
+                                            nullptr, nullptr, 0));
+                DSV_HIP(hipGraphDestroy(gr));
+            }
         w->graph_valid = 1;
         w->graph_chunk = chunk;
         std::memcpy(w->graph_prm, w->prm, sizeof(w->prm));
@@ -2113,39 +2147,56 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
     // that few no-op iterations follow convergence.
     auto enqueue = [&](int slot, int len) {
         if (m->use_graphs) {
-            DSV_HIP(hipGraphLaunch(len == chunk ? w->graph : w->graph_tail, stream));
+            DSV_HIP(hipGraphLaunch(len == chunk ? w->graph[slot] : w->graph_tail[slot], stream));
         } else {
-            for (int i = 0; i < len; ++i) cg_iteration<T, S>(m, w, i & 1, stream);
+            for (int i = 0; i < len; ++i)
+                cg_iteration<T, S>(m, w, i & 1, stream,
+                                   kpoll && i == len - 1 ? &w->d_poll[slot] : nullptr);
         }
         launched += len;
-        DSV_HIP(hipMemcpyAsync(&w->h_poll[slot], w->st, sizeof(CGState), hipMemcpyDeviceToHost,
-                               stream));
+        if (!kpoll)
+            DSV_HIP(hipMemcpyAsync(&w->h_poll[slot], w->st, sizeof(CGState), hipMemcpyDeviceToHost,
+                                   stream));
         DSV_HIP(hipEventRecord(w->ev[slot], stream));
         return len;
     };
     PollSeed h;  // `last`, the chunk end before it, and iterations queued after `last`
     if (seed) h = *seed;
-    // iterations still needed after `inflight` more than `last` has, or -1 while unknown
-    auto predict = [&](int inflight) -> double {
-        if (!h.have_last || !h.have0) return -1.0;
-        double remaining = 0.0;
+    // Iterations each running species still needs after `inflight` more than `last` has
+    // (negative: predicted done within them): the fewest and the most over the species;
+    // known = false while some species has no rate yet.
+    struct Pred {
+        bool known;
+        double fewest, most;
+    };
+    auto predict = [&](int inflight) -> Pred {
+        Pred p{h.have_last && h.have0, 1e300, -1e300};
+        if (!p.known) return p;
         for (int s = 0; s < S; ++s) {
             if (h.last.status[s] != ST_RUNNING) continue;
             const int dn = h.last.iters[s] - h.it0[s];
-            if (dn <= 0 || !(h.last.rr[s] > 0.0) || !(h.rr0[s] > h.last.rr[s])) return -1.0;
+            if (dn <= 0 || !(h.last.rr[s] > 0.0) || !(h.rr0[s] > h.last.rr[s])) {
+                p.known = false;
+                return p;
+            }
             const double rate = std::log(h.last.rr[s] / h.rr0[s]) / dn;  // per iteration, < 0
-            remaining = std::max(remaining, std::log(h.last.tol2[s] / h.last.rr[s]) / rate);
+            const double r = std::log(h.last.tol2[s] / h.last.rr[s]) / rate - inflight;
+            p.fewest = std::min(p.fewest, r);
+            p.most = std::max(p.most, r);
         }
-        return remaining - inflight;
+        return p;
     };
+    bool may_narrow = S > 1 && m->narrow != 0 && w->method != DSV_METHOD_CSR64;
+    // Short chunks once the next event is predicted within a full one: the last species
+    // finishing (few no-op iterations follow convergence) or, where narrowing may follow, the
+    // first one finishing (the rest move to the narrower kernel soon after).
     auto next_len = [&](int inflight) {
-        const double r = predict(inflight);
-        return r >= 0.0 && r < chunk ? tail : chunk;
+        const Pred p = predict(inflight);
+        if (!p.known) return chunk;
+        return (may_narrow ? p.fewest : p.most) < chunk ? tail : chunk;
     };
     int len_q[2] = {0, 0};
     len_q[0] = enqueue(0, next_len(h.pending));
-    // the CSR path has its own kernels and vectors: it never narrows
-    bool may_narrow = m->narrow != 0 && w->method != DSV_METHOD_CSR64;
     for (int k = 0;; ++k) {
         const int cur = k & 1, nxt = cur ^ 1;
         // chunk k (in flight) runs len_q[cur] more iterations before chunk k + 1
@@ -2168,8 +2219,8 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
                 int sel[MAXS] = {0, 0, 0}, n = 0;
                 for (int s = 0; s < S; ++s)
                     if (h.last.status[s] == ST_RUNNING) sel[n++] = s;
-                const double r = predict(len_q[nxt]);
-                if (n > 0 && n < S && (r < 0.0 || r >= NARROW_MIN)) {
+                const Pred p = predict(len_q[nxt]);
+                if (n > 0 && n < S && (!p.known || p.most >= NARROW_MIN)) {
                     PollSeed sd = h;
                     sd.pending = len_q[nxt];
                     sd.launched = launched;
@@ -2189,6 +2240,29 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
     return launched;
 }
 
+// Start of an inner solve: tolerances, active species and the iteration limit, written into the
+// CG state on the stream (no host round trip)
+struct CGStart {
+    double tol2[MAXS];
+    int active[MAXS];
+    int max_it;
+};
+__global__ void k_cg_start(CGState* __restrict__ st, CGStart a) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int s = 0; s < MAXS; ++s) {
+        st->tol2[s] = a.tol2[s];
+        st->act[1][s] = a.active[s];
+        st->act[0][s] = 0;
+        st->dead[s] = 0;
+        st->iters[s] = 0;
+        st->status[s] = a.active[s] ? ST_RUNNING : ST_IDLE;
+    }
+    st->k = 0;
+    st->kk[0] = st->kk[1] = 0;
+    st->max_it = a.max_it;
+    st->any_active = 1;
+}
+
 static CGState read_state(Workspace* w, hipStream_t stream) {
     DSV_HIP(hipMemcpyAsync(w->h_st, w->st, sizeof(CGState), hipMemcpyDeviceToHost, stream));
     DSV_HIP(hipStreamSynchronize(stream));
@@ -2203,8 +2277,10 @@ static void write_state(Workspace* w, const CGState& h, hipStream_t stream) {
 
 // ---------------------------------------------------------------------------------------
 // residual r64 = b - A x64 (FP64, vertex order of the method); returns rr and bb per species
+// (with `state`, the CG state left by the previous inner solve comes back in the same round trip)
 template <int S>
-static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb) {
+static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb,
+                       CGState* state = nullptr) {
     int nprod;
     if (w->method == DSV_METHOD_CSR64) {
         CSRVec<S> v{};
@@ -2225,7 +2301,10 @@ static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * 2 * S * nprod, hipMemcpyDeviceToHost,
                            stream));
+    if (state)
+        DSV_HIP(hipMemcpyAsync(w->h_st, w->st, sizeof(CGState), hipMemcpyDeviceToHost, stream));
     DSV_HIP(hipStreamSynchronize(stream));
+    if (state) *state = *w->h_st;
     for (int s = 0; s < S; ++s) {
         double a = 0.0, b = 0.0;
         for (int i = 0; i < nprod; ++i) {
@@ -2293,20 +2372,13 @@ static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* a
             }
         }
     }
-    CGState h = read_state(w, st);
+    CGStart a{};
     for (int s = 0; s < MAXS; ++s) {
-        h.tol2[s] = s < S ? tol2[s] : 0.0;
-        h.act[1][s] = (s < S && active[s]) ? 1 : 0;
-        h.act[0][s] = 0;
-        h.dead[s] = 0;
-        h.iters[s] = 0;
-        h.status[s] = (s < S && active[s]) ? ST_RUNNING : ST_IDLE;
+        a.tol2[s] = s < S ? tol2[s] : 0.0;
+        a.active[s] = (s < S && active[s]) ? 1 : 0;
     }
-    h.k = 0;
-    h.kk[0] = h.kk[1] = 0;
-    h.max_it = max_it;
-    h.any_active = 1;
-    write_state(w, h, st);
+    a.max_it = max_it;
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, st, w->st, a);
     if (w->method == DSV_METHOD_CSR64)
         launch_update<S, true>(m, w, 1, st);  // U_{-1}
     else
@@ -2330,12 +2402,17 @@ template <int S>
 static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
                        double* rr, hipStream_t st);
 
+// Runs on the workspace's stream, ordered after the work queued on `caller` and, at return,
+// ahead of any later work on `caller` (events; the host waits only for the residual norms).
 template <int S>
 static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped, int method,
-                       double rtol, int max_it, dsv_stats* stats, const WarmArgs* warm = nullptr) {
+                       double rtol, int max_it, dsv_stats* stats, hipStream_t caller,
+                       const WarmArgs* warm = nullptr) {
     Workspace* w = get_ws(m, method, S);
     hipStream_t st = w->stream;
     const auto t0 = std::chrono::steady_clock::now();
+    DSV_HIP(hipEventRecord(w->ev_io[0], caller));
+    DSV_HIP(hipStreamWaitEvent(st, w->ev_io[0], 0));
     m->stat_launches = 0;
     m->stat_narrowings = 0;
     const bool internal = method != DSV_METHOD_CSR64;
@@ -2386,10 +2463,20 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     double tol_abs[MAXS] = {0, 0, 0}, rel0[MAXS] = {0, 0, 0}, relf[MAXS] = {0, 0, 0};
     int conv[MAXS] = {0, 0, 0}, outer[MAXS] = {0, 0, 0}, dead[MAXS] = {0, 0, 0};
     int iters_total[MAXS] = {0, 0, 0};
+    int prev_act[MAXS] = {0, 0, 0};  // species of the previous inner solve (none before round 1)
     const int max_rounds = method == DSV_METHOD_MF32_IR64 ? m->max_outer : 4;
     for (int round = 0;; ++round) {
         double rr[MAXS] = {0, 0, 0}, bb[MAXS] = {0, 0, 0};
-        residual64<S>(m, w, st, rr, bb);
+        CGState h2{};
+        residual64<S>(m, w, st, rr, bb, round > 0 ? &h2 : nullptr);
+        if (round > 0) {
+            // the state's iteration counters restart in every inner solve; accumulate them
+            for (int s = 0; s < S; ++s) {
+                if (!prev_act[s]) continue;
+                iters_total[s] += h2.iters[s];
+                if (h2.status[s] == ST_BREAKDOWN) dead[s] = 1;
+            }
+        }
         if (round == 0 && warm && warm->n_hist + warm->n_extra > 0) {
             // x64 and r64 move to the Galerkin start (rr: its updated residual); should that
             // already meet the tolerance, the true residual confirms it
@@ -2439,19 +2526,15 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
                                (const SV<double, S>*)w->vb[VB_X], mask);
         }
         DSV_HIP(hipGetLastError());
-        // the state's iteration counters restart in every inner solve; accumulate them
-        const CGState h2 = read_state(w, st);
-        for (int s = 0; s < S; ++s) {
-            if (!act[s]) continue;
-            iters_total[s] += h2.iters[s];
-            if (h2.status[s] == ST_BREAKDOWN) dead[s] = 1;
-        }
+        for (int s = 0; s < S; ++s) prev_act[s] = act[s];
     }
     double* xo[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
     hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm,
                        (const SV<double, S>*)x64, xo[0], xo[1], xo[2]);
     DSV_HIP(hipGetLastError());
-    DSV_HIP(hipStreamSynchronize(st));
+    // the solution is ordered on the caller's stream (no host wait for the unpack)
+    DSV_HIP(hipEventRecord(w->ev_io[1], st));
+    DSV_HIP(hipStreamWaitEvent(caller, w->ev_io[1], 0));
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) {
@@ -2984,11 +3067,11 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
         if (max_it < 1) throw ArgError{"max_it must be >= 1"};
         check_solve_args(m, n_species, sp, true);
         DSV_HIP(hipSetDevice(m->device));
-        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        const hipStream_t cs = (hipStream_t)stream;
         switch (n_species) {
-            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
-            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
-            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats); break;
+            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs); break;
+            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs); break;
+            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs); break;
         }
     });
 }
@@ -3011,7 +3094,7 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         for (int i = 0; i < n_hist * n_species; ++i)
             if (!hist[i]) throw ArgError{"dsv_solve_warm: NULL history vector"};
         DSV_HIP(hipSetDevice(m->device));
-        DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
+        const hipStream_t cs = (hipStream_t)stream;
         WarmArgs wa;
         wa.n_hist = n_hist;
         wa.hist = hist;
@@ -3021,9 +3104,9 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         wa.r_before = r_before;
         wa.r_after = r_after;
         switch (n_species) {
-            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, &wa); break;
-            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, &wa); break;
-            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, &wa); break;
+            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
         }
     });
 }

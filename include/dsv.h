@@ -10,9 +10,11 @@
  *   - Mesh arrays given to dsv_mesh_create are HOST pointers and are copied.  Every other
  *     array argument is a DEVICE pointer (memory of the device the mesh was created on),
  *     in the caller's vertex / cell numbering.
- *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are stream-ordered;
- *     dsv_solve, dsv_residual_norms and dsv_warm_start synchronise the stream before
- *     returning.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are stream-ordered.
+ *     dsv_solve and dsv_solve_warm run after the work queued on `stream` and leave the
+ *     solution ordered ahead of later work on it (they return once convergence is known,
+ *     with the final copy into x still queued); dsv_residual_norms and dsv_warm_start
+ *     synchronise the stream before returning.
  *   - Cell / vertex coefficient arrays follow the reference's _to_cell_avg rule: length C is
  *     taken per cell, length V is averaged over the 4 vertices of each cell.
  *
@@ -70,7 +72,7 @@ typedef struct dsv_stats {
     int32_t converged[3];        /* ||b - A x||_2 <= rtol ||b||_2 in FP64 */
     double rel_residual[3];      /* final true FP64 ||b - A x|| / ||b|| */
     double initial_rel_residual[3];
-    double solve_ms;             /* wall time of the solve on the stream */
+    double solve_ms;             /* host wall time of the call */
     int32_t cg_launches;         /* CG iterations launched, all species together (includes
                                     the no-op ones queued after convergence) */
     int32_t narrowings;          /* inner solves continued with fewer species */
@@ -197,8 +199,9 @@ int dsv_set_species_narrowing(dsv_mesh* m, int32_t enable);
  * data they build (per-entry coefficients, mass diagonal, Jacobi diagonal, Dirichlet bits) and
  * skip rebuilding it when a later call names the same coefficient and Dirichlet arrays (same
  * pointers and lengths), the same scalar D / alpha values, inv_dt, mass lumping and operator
- * form.  The caller guarantees that the contents of those arrays do not change while reuse is
- * on; every call of this function discards what was built (call it again after changing a
+ * form.  Dirichlet index lists are range-checked once per (address, length) while reuse is on.
+ * The caller guarantees that the contents of those arrays do not change while reuse is on;
+ * every call of this function discards what was built (call it again after changing a
  * coefficient array in place).  Default: off. */
 int dsv_set_operator_reuse(dsv_mesh* m, int32_t reuse);
 
