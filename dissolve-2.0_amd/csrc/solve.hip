@@ -1502,6 +1502,8 @@ static OpKey make_key(const dsv_mesh* m, int S, const dsv_species* sp, double in
     return k;
 }
 
+struct WarmOut;
+
 struct Workspace {
     int method = 0, S = 0, V = 0;
     hipStream_t stream = nullptr;
@@ -1528,6 +1530,9 @@ struct Workspace {
     void* vb[VB_COUNT] = {};
     double* csr[MAXS] = {};  // CSR64 values per species
     void* wd[6] = {};        // FP64 warm-start directions d_k and A d_k (dsv_solve_warm)
+    double* pW = nullptr;    // partials of the warm start's dot and norm kernels
+    WarmOut* wout = nullptr;    // its coefficients and norms (device) ...
+    WarmOut* h_wout = nullptr;  // ... and their pinned copy
     void* fw[2][MAXS] = {};  // stored form: per-species weights K(D_s) of sets 0 / 1
     double* kvals = nullptr; // CSR values of K(D_s) (nnz), assembled to build them
     // operator reuse: the inputs of coefficient sets 0 / 1, of the Dirichlet bits, and whether
@@ -1573,6 +1578,9 @@ static void free_ws(Workspace* w) {
     dfree(w->mdiag[1]);
     for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
     for (int i = 0; i < 6; ++i) dfree(w->wd[i]);
+    dfree(w->pW);
+    dfree(w->wout);
+    if (w->h_wout) (void)hipHostFree(w->h_wout);
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < MAXS; ++q) dfree(w->fw[a][q]);
     dfree(w->kvals);
@@ -2277,10 +2285,8 @@ static void write_state(Workspace* w, const CGState& h, hipStream_t stream) {
 
 // ---------------------------------------------------------------------------------------
 // residual r64 = b - A x64 (FP64, vertex order of the method); returns rr and bb per species
-// (with `state`, the CG state left by the previous inner solve comes back in the same round trip)
 template <int S>
-static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb,
-                       CGState* state = nullptr) {
+static int residual64_launch(dsv_mesh* m, Workspace* w, hipStream_t stream) {
     int nprod;
     if (w->method == DSV_METHOD_CSR64) {
         CSRVec<S> v{};
@@ -2299,6 +2305,14 @@ static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr
         nprod = mf_grid(m->part);
     }
     DSV_HIP(hipGetLastError());
+    return nprod;
+}
+
+// (with `state`, the CG state left by the previous inner solve comes back in the same round trip)
+template <int S>
+static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb,
+                       CGState* state = nullptr) {
+    const int nprod = residual64_launch<S>(m, w, stream);
     DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * 2 * S * nprod, hipMemcpyDeviceToHost,
                            stream));
     if (state)
@@ -2400,7 +2414,7 @@ struct WarmArgs {
 };
 template <int S>
 static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
-                       double* rr, hipStream_t st);
+                       int nres, double* rr, double* bb, hipStream_t st);
 
 // Runs on the workspace's stream, ordered after the work queued on `caller` and, at return,
 // ahead of any later work on `caller` (events; the host waits only for the residual norms).
@@ -2468,7 +2482,17 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     for (int round = 0;; ++round) {
         double rr[MAXS] = {0, 0, 0}, bb[MAXS] = {0, 0, 0};
         CGState h2{};
-        residual64<S>(m, w, st, rr, bb, round > 0 ? &h2 : nullptr);
+        const bool warm0 = round == 0 && warm && warm->n_hist + warm->n_extra > 0;
+        if (warm0) {
+            // x64 and r64 move to the Galerkin start (rr: its residual); should that already
+            // meet the tolerance, the true residual confirms it
+            fused_warm<S>(m, w, sp, *warm, residual64_launch<S>(m, w, st), rr, bb, st);
+            bool met = true;
+            for (int s = 0; s < S; ++s) met &= std::sqrt(rr[s]) <= rtol * std::sqrt(bb[s]);
+            if (met) residual64<S>(m, w, st, rr, bb);
+        } else {
+            residual64<S>(m, w, st, rr, bb, round > 0 ? &h2 : nullptr);
+        }
         if (round > 0) {
             // the state's iteration counters restart in every inner solve; accumulate them
             for (int s = 0; s < S; ++s) {
@@ -2476,14 +2500,6 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
                 iters_total[s] += h2.iters[s];
                 if (h2.status[s] == ST_BREAKDOWN) dead[s] = 1;
             }
-        }
-        if (round == 0 && warm && warm->n_hist + warm->n_extra > 0) {
-            // x64 and r64 move to the Galerkin start (rr: its updated residual); should that
-            // already meet the tolerance, the true residual confirms it
-            fused_warm<S>(m, w, sp, *warm, rr, st);
-            bool met = true;
-            for (int s = 0; s < S; ++s) met &= std::sqrt(rr[s]) <= rtol * std::sqrt(bb[s]);
-            if (met) residual64<S>(m, w, st, rr, bb);
         }
         bool all = true;
         for (int s = 0; s < S; ++s) {
@@ -2594,7 +2610,16 @@ struct ProjVec {
     const SV<double, S>* x0;
     SV<double, S>* x;
     double c[3][MAXS];
+    const double* cdev;  // when set: the coefficients c[k][s] at cdev[k * MAXS + s] (device)
 };
+
+template <int S, int ND>
+__device__ __forceinline__ void proj_coefs(const ProjVec<S>& pv, double (&c)[ND][S]) {
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+#pragma unroll
+        for (int s = 0; s < S; ++s) c[k][s] = pv.cdev ? pv.cdev[k * MAXS + s] : pv.c[k][s];
+}
 
 // partials per species: d_j . A d_k (j <= k), then d_j . r0
 template <int S, int ND>
@@ -2658,6 +2683,8 @@ template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_norms(int V, ProjVec<S> pv,
                                                             double* __restrict__ pout) {
     __shared__ double red[S * 16];
+    double c[ND][S];
+    proj_coefs<S, ND>(pv, c);
     double tot[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) tot[s] = 0.0;
@@ -2667,7 +2694,7 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_norms(int V, ProjVec<S> p
         for (int k = 0; k < ND; ++k) {
             const SV<double, S> Ad = pv.Ad[k][i];
 #pragma unroll
-            for (int s = 0; s < S; ++s) r.v[s] -= pv.c[k][s] * Ad.v[s];
+            for (int s = 0; s < S; ++s) r.v[s] -= c[k][s] * Ad.v[s];
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) tot[s] += r.v[s] * r.v[s];
@@ -2679,6 +2706,8 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_norms(int V, ProjVec<S> p
 template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_update(int V, ProjVec<S> pv,
                                                              SV<double, S>* __restrict__ r) {
+    double c[ND][S];
+    proj_coefs<S, ND>(pv, c);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
         SV<double, S> x = pv.x[i], rv = r[i];
 #pragma unroll
@@ -2686,8 +2715,8 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_update(int V, ProjVec<S> 
             const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                x.v[s] += pv.c[k][s] * d.v[s];
-                rv.v[s] -= pv.c[k][s] * Ad.v[s];
+                x.v[s] += c[k][s] * d.v[s];
+                rv.v[s] -= c[k][s] * Ad.v[s];
             }
         }
         pv.x[i] = x;
@@ -2714,14 +2743,16 @@ __global__ void k_pack_diff(int V, const int32_t* __restrict__ perm, const doubl
 // Solves G c = g for the n x n SPD Gram matrix G (row-major) by LDL^T, dropping a direction
 // whose pivot falls below rel_tol of its own energy d.Ad (numerically dependent on the
 // previous ones, or zero); a dropped direction gets c = 0.
-static void solve_gram(int n, const double* G, const double* g, double* c, double rel_tol) {
+__host__ __device__ static void solve_gram(int n, const double* G, const double* g, double* c,
+                                           double rel_tol) {
     double L[3][3] = {}, Dg[3] = {}, y[3] = {};
     bool keep[3] = {};
     for (int j = 0; j < n; ++j) {
         double dj = G[j * n + j];
         for (int k = 0; k < j; ++k)
             if (keep[k]) dj -= L[j][k] * L[j][k] * Dg[k];
-        keep[j] = G[j * n + j] > 0.0 && std::isfinite(dj) && dj > rel_tol * G[j * n + j];
+        const bool finite = dj - dj == 0.0;  // false for an infinity or a NaN
+        keep[j] = G[j * n + j] > 0.0 && finite && dj > rel_tol * G[j * n + j];
         if (!keep[j]) continue;
         Dg[j] = dj;
         for (int i = j + 1; i < n; ++i) {
@@ -2745,6 +2776,60 @@ static void solve_gram(int n, const double* G, const double* g, double* c, doubl
         for (int i = j + 1; i < n; ++i)
             if (keep[i]) v -= L[i][j] * c[i];
         c[j] = v;
+    }
+}
+
+// Device side of the warm start folded into a solve: the Gram systems, then the acceptance
+// test, with no host round trip in between.
+struct WarmOut {
+    double c[3][MAXS];  // coefficients (0 where the start stays)
+    double rr0[MAXS];   // ||r||^2 of the given start and ||b||^2 (the solve's first residual)
+    double bb[MAXS];
+    double rr1[MAXS];   // ||r||^2 of the start taken
+};
+
+// Sums of the dot partials (PROJ_GRID per quantity), one Gram solve per species (thread 0)
+template <int S, int ND>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ dots,
+                                                            WarmOut* __restrict__ out) {
+    constexpr int NP = ND * (ND + 1) / 2 + ND;
+    __shared__ double red[S * NP * 16];
+    double q[S * NP];
+    consume_partials<S * NP>(dots, PROJ_GRID, q, red);
+    if (threadIdx.x != 0) return;
+    for (int s = 0; s < S; ++s) {
+        double G[ND * ND], g[ND], c[ND];
+        int t = s * NP;
+        for (int j = 0; j < ND; ++j)
+            for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
+        for (int j = 0; j < ND; ++j) g[j] = q[t++];
+        solve_gram(ND, G, g, c, 1e-10);
+        for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
+    }
+}
+
+// Takes the combination of a species only where some direction moved it and its FP64
+// residual fell (the given start is the fallback): sums the norm partials (PROJ_GRID per
+// species) and the residual kernel's (nres per quantity: r.r then b.b per species).
+template <int S, int ND>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_accept(const double* __restrict__ norms,
+                                                             const double* __restrict__ res,
+                                                             int nres, WarmOut* __restrict__ out) {
+    __shared__ double red[2 * S * 16];
+    double n1[S], r0[2 * S];
+    consume_partials<S>(norms, PROJ_GRID, n1, red);
+    consume_partials<2 * S>(res, nres, r0, red);
+    if (threadIdx.x != 0) return;
+    for (int s = 0; s < S; ++s) {
+        bool moved = false;  // every direction dropped: nothing to take
+        for (int k = 0; k < ND; ++k) moved |= out->c[k][s] != 0.0;
+        const double a = sqrt(r0[s]), b = sqrt(n1[s]);
+        const bool take = moved && b - b == 0.0 && b < a;
+        if (!take)
+            for (int k = 0; k < ND; ++k) out->c[k][s] = 0.0;
+        out->rr0[s] = r0[s];
+        out->bb[s] = r0[S + s];
+        out->rr1[s] = take ? n1[s] : r0[s];
     }
 }
 
@@ -2828,12 +2913,17 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
     }
 }
 
+// Galerkin warm start of dsv_solve_warm after the residual kernel of the solve's first round
+// (its partials in w->pR, nres per quantity): rr and bb come back as that residual's norms,
+// rr updated to the start taken; one host round trip.
 template <int S, int ND>
 static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
-                          double* rr, hipStream_t st) {
-    constexpr int NP = ND * (ND + 1) / 2 + ND;
+                          int nres, double* rr, double* bb, hipStream_t st) {
     const unsigned gv = cdiv(m->V, 256);
     const unsigned char* prm64 = w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm;
+    if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * 9 * PROJ_GRID);
+    if (!w->wout) w->wout = dalloc<WarmOut>(1);
+    if (!w->h_wout) DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocDefault));
     ProjVec<S> pv{};
     for (int k = 0; k < ND; ++k) {
         for (int j = 0; j < 2; ++j)
@@ -2866,64 +2956,37 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     }
     pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
     pv.x = (SV<double, S>*)w->vb[VB_X64];
+    pv.cdev = &w->wout->c[0][0];
     hipLaunchKernelGGL((k_proj_dots<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
-                       w->pR);
-    DSV_HIP(hipGetLastError());
-    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * NP * PROJ_GRID,
-                           hipMemcpyDeviceToHost, st));
-    DSV_HIP(hipStreamSynchronize(st));
-    for (int s = 0; s < S; ++s) {
-        double q[NP];
-        for (int j = 0; j < NP; ++j) {
-            double a = 0.0;
-            for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)(s * NP + j) * PROJ_GRID + b];
-            q[j] = a;
-        }
-        double G[ND * ND], g[ND], c[ND];
-        int t = 0;
-        for (int j = 0; j < ND; ++j)
-            for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
-        for (int j = 0; j < ND; ++j) g[j] = q[t++];
-        solve_gram(ND, G, g, c, 1e-10);
-        for (int k = 0; k < ND; ++k) pv.c[k][s] = c[k];
-    }
+                       w->pW);
+    hipLaunchKernelGGL((k_proj_solve<S, ND>), dim3(1), dim3(PROJ_THREADS), 0, st, w->pW, w->wout);
     hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
-                       w->pR);
+                       w->pW);
+    hipLaunchKernelGGL((k_proj_accept<S, ND>), dim3(1), dim3(PROJ_THREADS), 0, st, w->pW, w->pR,
+                       nres, w->wout);
+    hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V,
+                       pv, (SV<double, S>*)w->vb[VB_R64]);
     DSV_HIP(hipGetLastError());
-    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * PROJ_GRID, hipMemcpyDeviceToHost,
-                           st));
+    DSV_HIP(hipMemcpyAsync(w->h_wout, w->wout, sizeof(WarmOut), hipMemcpyDeviceToHost, st));
     DSV_HIP(hipStreamSynchronize(st));
-    bool any = false;
+    const WarmOut& o = *w->h_wout;
     for (int s = 0; s < S; ++s) {
-        double a = 0.0;
-        for (int b = 0; b < PROJ_GRID; ++b) a += w->h_pR[(size_t)s * PROJ_GRID + b];
-        const double r0 = std::sqrt(rr[s]), r1 = std::sqrt(a);
-        bool moved = false;  // every direction dropped: nothing to take
-        for (int k = 0; k < ND; ++k) moved |= pv.c[k][s] != 0.0;
-        const bool take = moved && std::isfinite(r1) && r1 < r0;  // the start is the fallback
-        if (!take)
-            for (int k = 0; k < ND; ++k) pv.c[k][s] = 0.0;
-        any |= take;
-        if (wa.r_before) wa.r_before[s] = r0;
-        if (wa.r_after) wa.r_after[s] = take ? r1 : r0;
+        rr[s] = o.rr1[s];
+        bb[s] = o.bb[s];
+        if (wa.r_before) wa.r_before[s] = std::sqrt(o.rr0[s]);
+        if (wa.r_after) wa.r_after[s] = std::sqrt(o.rr1[s]);
         if (wa.coef)
-            for (int k = 0; k < ND; ++k) wa.coef[k * S + s] = pv.c[k][s];
-        if (take) rr[s] = a;
-    }
-    if (any) {
-        hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
-                           m->V, pv, (SV<double, S>*)w->vb[VB_R64]);
-        DSV_HIP(hipGetLastError());
+            for (int k = 0; k < ND; ++k) wa.coef[k * S + s] = o.c[k][s];
     }
 }
 
 template <int S>
 static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
-                       double* rr, hipStream_t st) {
+                       int nres, double* rr, double* bb, hipStream_t st) {
     switch (wa.n_hist + wa.n_extra) {
-        case 1: fused_warm_nd<S, 1>(m, w, sp, wa, rr, st); break;
-        case 2: fused_warm_nd<S, 2>(m, w, sp, wa, rr, st); break;
-        default: fused_warm_nd<S, 3>(m, w, sp, wa, rr, st); break;
+        case 1: fused_warm_nd<S, 1>(m, w, sp, wa, nres, rr, bb, st); break;
+        case 2: fused_warm_nd<S, 2>(m, w, sp, wa, nres, rr, bb, st); break;
+        default: fused_warm_nd<S, 3>(m, w, sp, wa, nres, rr, bb, st); break;
     }
 }
 
