@@ -1543,6 +1543,9 @@ struct Workspace {
     // persistent grid of the MF CG kernels and the kernel variant it was sized for
     int cg_grid = 0;
     int cg_grid_key = 0;
+    // CG iterations per species of the last solve's first inner solve [0] and of a later one
+    // [1] (0: none), the first chunks' guide for the next solve
+    int prior[2][MAXS] = {};
     // captured CG chunk and the parameters it was captured with
     hipGraphExec_t graph[2] = {};       // per poll slot (the MF kernels publish into it)
     hipGraphExec_t graph_tail[2] = {};  // the short chunk used near convergence
@@ -1995,6 +1998,7 @@ struct PollSeed {
     bool have_last = false, have0 = false;
     int pending = 0;   // iterations queued after `last` ahead of the narrowed run
     int launched = 0;  // iterations launched so far
+    int prior[MAXS] = {0, 0, 0};  // iterations the species took in the previous such solve
 };
 
 // A narrowed run starts only if it is predicted to need at least this many more iterations
@@ -2103,6 +2107,7 @@ static bool narrow_mid(dsv_mesh* m, Workspace* w, const int* sel, int max_it, hi
     for (int i = 0; i < MAXS; ++i) {
         s2.rr0[i] = i < S2 ? seed.rr0[sel[i]] : 0.0;
         s2.it0[i] = i < S2 ? seed.it0[sel[i]] : 0;
+        s2.prior[i] = i < S2 ? seed.prior[sel[i]] : 0;
     }
     run_cg<T, S2>(m, w2, max_it, st, &s2);
     widen<T, S, S2>(m, w, w2, sel, st);
@@ -2155,7 +2160,10 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
     // that few no-op iterations follow convergence.
     auto enqueue = [&](int slot, int len) {
         if (m->use_graphs) {
-            DSV_HIP(hipGraphLaunch(len == chunk ? w->graph[slot] : w->graph_tail[slot], stream));
+            if (len == chunk)
+                DSV_HIP(hipGraphLaunch(w->graph[slot], stream));
+            else
+                for (int i = 0; i < len; i += tail) DSV_HIP(hipGraphLaunch(w->graph_tail[slot], stream));
         } else {
             for (int i = 0; i < len; ++i)
                 cg_iteration<T, S>(m, w, i & 1, stream,
@@ -2171,37 +2179,48 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
     PollSeed h;  // `last`, the chunk end before it, and iterations queued after `last`
     if (seed) h = *seed;
     // Iterations each running species still needs after `inflight` more than `last` has
-    // (negative: predicted done within them): the fewest and the most over the species;
-    // known = false while some species has no rate yet.
+    // (negative: predicted done within them), from its r.r at the last two chunk ends or,
+    // before that, from the iterations it took in the previous such solve: the fewest and the
+    // most over the species; known = false while some species has neither.
     struct Pred {
-        bool known;
+        bool known;       // for every running species
+        bool any;         // for some running species (fewest: over those)
         double fewest, most;
     };
     auto predict = [&](int inflight) -> Pred {
-        Pred p{h.have_last && h.have0, 1e300, -1e300};
+        Pred p{h.have_last, false, 1e300, -1e300};
         if (!p.known) return p;
         for (int s = 0; s < S; ++s) {
             if (h.last.status[s] != ST_RUNNING) continue;
+            double r = 0.0;
             const int dn = h.last.iters[s] - h.it0[s];
-            if (dn <= 0 || !(h.last.rr[s] > 0.0) || !(h.rr0[s] > h.last.rr[s])) {
+            if (h.have0 && dn > 0 && h.last.rr[s] > 0.0 && h.rr0[s] > h.last.rr[s]) {
+                const double rate = std::log(h.last.rr[s] / h.rr0[s]) / dn;  // per iteration, < 0
+                r = std::log(h.last.tol2[s] / h.last.rr[s]) / rate - inflight;
+            } else if (h.prior[s] > 0) {
+                r = h.prior[s] - h.last.iters[s] - inflight;
+            } else {
                 p.known = false;
-                return p;
+                continue;
             }
-            const double rate = std::log(h.last.rr[s] / h.rr0[s]) / dn;  // per iteration, < 0
-            const double r = std::log(h.last.tol2[s] / h.last.rr[s]) / rate - inflight;
+            p.any = true;
             p.fewest = std::min(p.fewest, r);
             p.most = std::max(p.most, r);
         }
         return p;
     };
     bool may_narrow = S > 1 && m->narrow != 0 && w->method != DSV_METHOD_CSR64;
-    // Short chunks once the next event is predicted within a full one: the last species
-    // finishing (few no-op iterations follow convergence) or, where narrowing may follow, the
-    // first one finishing (the rest move to the narrower kernel soon after).
+    // A chunk ends near the next predicted event: the last species finishing (few no-op
+    // iterations follow convergence) or, where narrowing may follow, the first one finishing
+    // (the rest move to the narrower kernel soon after).  Lengths are multiples of the short
+    // chunk up to a full one.
     auto next_len = [&](int inflight) {
         const Pred p = predict(inflight);
-        if (!p.known) return chunk;
-        return (may_narrow ? p.fewest : p.most) < chunk ? tail : chunk;
+        if (!(may_narrow ? p.any : p.known)) return chunk;
+        const double e = may_narrow ? p.fewest : p.most;
+        if (e >= chunk) return chunk;
+        const int n = std::max(1, (int)std::ceil(std::max(e, 0.0) / tail));
+        return std::min(chunk, n * tail);
     };
     int len_q[2] = {0, 0};
     len_q[0] = enqueue(0, next_len(h.pending));
@@ -2341,12 +2360,12 @@ static void check_solve_args(const dsv_mesh* m, int S, const dsv_species* sp, bo
 
 template <class T, int S>
 static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* active, int max_it,
-                     hipStream_t st);
+                     const double* rr0, const int* prior, hipStream_t st);
 
 // inner CG of the S2 species sel[] of w (the others are not active) in the narrowed workspace
 template <class T, int S, int S2>
 static void inner_narrow(dsv_mesh* m, Workspace* w, const int* sel, const double* tol2,
-                         int max_it, hipStream_t st) {
+                         int max_it, const double* rr0, const int* prior, hipStream_t st) {
     if (w->method == DSV_METHOD_CSR64) throw std::logic_error("inner_narrow: CSR method");
     Workspace* w2 = narrow_ws<T, S, S2>(m, w, sel);
     NarrowArgs<T, S, S2> a = narrow_args<T, S, S2>(w, w2, sel);
@@ -2354,33 +2373,37 @@ static void inner_narrow(dsv_mesh* m, Workspace* w, const int* sel, const double
     a.rin2 = (SV<double, S2>*)w2->vb[VB_R64];
     launch_narrow<T, S, S2>(m, a, st);
     m->stat_narrowings += 1;
-    double t2[MAXS] = {0, 0, 0};
-    int a2[MAXS] = {0, 0, 0};
+    double t2[MAXS] = {0, 0, 0}, r2[MAXS] = {0, 0, 0};
+    int a2[MAXS] = {0, 0, 0}, p2[MAXS] = {0, 0, 0};
     for (int i = 0; i < S2; ++i) {
         t2[i] = tol2[sel[i]];
         a2[i] = 1;
+        r2[i] = rr0[sel[i]];
+        p2[i] = prior[sel[i]];
     }
-    inner_cg<T, S2>(m, w2, t2, a2, max_it, st);
+    inner_cg<T, S2>(m, w2, t2, a2, max_it, r2, p2, st);
     widen<T, S, S2>(m, w, w2, sel, st);
 }
 
 // inner CG on A d = r64 (d starts at 0), T = CG precision; the matrix-free methods solve only
-// the active species' system when some are not active (inner_narrow)
+// the active species' system when some are not active (inner_narrow).  rr0: r64.r64 per
+// species (the CG's first r.r); prior: iterations per species of the previous such solve
+// (0 = none), the first chunks' guide.
 template <class T, int S>
 static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* active, int max_it,
-                     hipStream_t st) {
+                     const double* rr0, const int* prior, hipStream_t st) {
     if constexpr (S > 1) {
         if (m->narrow && w->method != DSV_METHOD_CSR64) {
             int sel[MAXS] = {0, 0, 0}, n = 0;
             for (int s = 0; s < S; ++s)
                 if (active[s]) sel[n++] = s;
             if (n == 1) {
-                inner_narrow<T, S, 1>(m, w, sel, tol2, max_it, st);
+                inner_narrow<T, S, 1>(m, w, sel, tol2, max_it, rr0, prior, st);
                 return;
             }
             if constexpr (S == 3) {
                 if (n == 2) {
-                    inner_narrow<T, 3, 2>(m, w, sel, tol2, max_it, st);
+                    inner_narrow<T, 3, 2>(m, w, sel, tol2, max_it, rr0, prior, st);
                     return;
                 }
             }
@@ -2398,7 +2421,16 @@ static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* a
     else
         launch_cg<T, S, true>(m, w, 0, st);
     DSV_HIP(hipGetLastError());
-    run_cg<T, S>(m, w, max_it, st);
+    PollSeed seed;
+    seed.have_last = true;  // the state before the first iteration
+    for (int s = 0; s < MAXS; ++s) {
+        const bool on = s < S && active[s];
+        seed.last.status[s] = on ? ST_RUNNING : ST_IDLE;
+        seed.last.rr[s] = on ? rr0[s] : 0.0;
+        seed.last.tol2[s] = on ? tol2[s] : 0.0;
+        seed.prior[s] = on ? prior[s] : 0;
+    }
+    run_cg<T, S>(m, w, max_it, st, &seed);
 }
 
 // Galerkin warm start folded into a solve (dsv_solve_warm): directions from past solutions
@@ -2496,6 +2528,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         if (round > 0) {
             // the state's iteration counters restart in every inner solve; accumulate them
             for (int s = 0; s < S; ++s) {
+                w->prior[std::min(round - 1, 1)][s] = prev_act[s] ? h2.iters[s] : 0;
                 if (!prev_act[s]) continue;
                 iters_total[s] += h2.iters[s];
                 if (h2.status[s] == ST_BREAKDOWN) dead[s] = 1;
@@ -2519,9 +2552,14 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         int mask = 0;
         for (int s = 0; s < S; ++s) {
             if (conv[s] || dead[s]) continue;
+            // The FP32 sweep's target is floored at half the FP64 tolerance: below that, its
+            // recursive residual parts from the true one (FP32 operator rounding amplified by
+            // the condition number), and on the 806k-tet disc the true residual after the sweep
+            // ends near 0.85 rtol ||b|| for floors 0.3 and 0.5 alike, the latter with 7% fewer
+            // iterations.
             const double rn = std::sqrt(rr[s]);
             const double t = method == DSV_METHOD_MF32_IR64
-                                 ? std::max(m->inner_rtol * rn, 0.3 * tol_abs[s])
+                                 ? std::max(m->inner_rtol * rn, 0.5 * tol_abs[s])
                                  : 0.9 * tol_abs[s];
             tol2[s] = t * t;
             act[s] = 1;
@@ -2532,12 +2570,13 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         int budget = max_it;
         for (int s = 0; s < S; ++s)
             if (act[s]) budget = std::min(budget, max_it - iters_total[s]);
+        const int* prior = w->prior[std::min(round, 1)];
         if (method == DSV_METHOD_MF32_IR64) {
-            inner_cg<float, S>(m, w, tol2, act, std::max(1, budget), st);
+            inner_cg<float, S>(m, w, tol2, act, std::max(1, budget), rr, prior, st);
             hipLaunchKernelGGL((k_add_correction<float, S>), dim3(gv), dim3(256), 0, st, m->V, x64,
                                (const SV<float, S>*)w->vb[VB_X], mask);
         } else {
-            inner_cg<double, S>(m, w, tol2, act, std::max(1, budget), st);
+            inner_cg<double, S>(m, w, tol2, act, std::max(1, budget), rr, prior, st);
             hipLaunchKernelGGL((k_add_correction<double, S>), dim3(gv), dim3(256), 0, st, m->V, x64,
                                (const SV<double, S>*)w->vb[VB_X], mask);
         }

@@ -231,10 +231,13 @@ class Stepper:
                 X, sigma = feats[k]
                 y = ((self.x[k] - 2.0 * self.u[k] + self.u_hist[0][k]) / sigma.double()).float()
                 self.pairs.append((X, y))
-        self.u_hist = [self.u_hist[-1]] + self.u_hist[:-1]
+        # rotate the buffers instead of copying them: the solution becomes u, u becomes
+        # u_{n-1}, and the oldest history buffer holds the next step's start
+        oldest = self.u_hist[-1]
+        self.u_hist = [self.u] + self.u_hist[:-1]
+        self.u, self.x = self.x, oldest
         for k in range(self.S):
-            self.u_hist[0][k].copy_(self.u[k])
-            self.u[k].copy_(self.x[k])
+            self._arr[k].x = ptr(self.x[k]).value
         self.n_steps += 1
         d = st.as_dict(self.S)
         d["step_s"] = time.perf_counter() - t0
