@@ -6,9 +6,9 @@ untimed warm-up steps, then K timed steps):
   plain    -- FP64 CSR re-assembled every step + FP64 Jacobi-PCG (csr64), previous-solution
               start (the reference's operator path and warm start)
   fp32     -- fused FP32 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64), previous start
-  chosen   -- the fp32 path started from the Galerkin warm start over the last three solution
-              increments (previous solution as fallback)
-  learned  -- the fp32 path started from the Galerkin warm start over two increments and the
+  chosen   -- the fp32 path started from the Galerkin warm start over the last H solution
+              increments (--history-dirs, default 6; previous solution as fallback)
+  learned  -- the fp32 path started from the Galerkin warm start over H - 1 increments and the
               learned MLP direction, trained on the GPU on the W warm-up steps
 `value` is the chosen path's steps/s; the timed steps include every warm-start computation
 (features, inference, projection).  The roofline entry is the fused FP32 3-species CG
@@ -83,7 +83,7 @@ def load_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30):
+def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, history_dirs=6):
     import torch
     from dsv.transport import Stepper
     mlp = None
@@ -94,7 +94,7 @@ def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30):
                  grad_phi=w.get("grad_norm"), eps=w.get("eps"), mlp=mlp)
     train = None
     if warm == "learned":
-        st.set_warm_start("extrapolate", history_dirs=2)
+        st.set_warm_start("extrapolate", history_dirs=min(history_dirs, st.MAX_HIST - 1))
         torch.cuda.synchronize()
         t = time.perf_counter()
         loss = st.train_warm_start(warmup, epochs=epochs)
@@ -106,7 +106,7 @@ def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30):
             f"loss {loss[0]:.3g} -> {loss[-1]:.3g}")
     else:
         if warm != "previous":
-            st.set_warm_start(warm, history_dirs=3)
+            st.set_warm_start(warm, history_dirs=history_dirs)
         for i in range(warmup):
             st.step()
     torch.cuda.synchronize()
@@ -220,7 +220,7 @@ def kernels_at_scale(shapes=((256, 256, 1), (256, 256, 2))):
     return out
 
 
-def run_series(mesh, w, n=200, epochs=5):
+def run_series(mesh, w, n=200, epochs=5, history_dirs=6):
     """Receding-interface series: the previous-solution start, the Galerkin extrapolation and
     the learned start (trained on steps 1..n/2) over steps n/2+1..n; prints one JSON line."""
     import torch
@@ -243,10 +243,10 @@ def run_series(mesh, w, n=200, epochs=5):
             loss = st.train_warm_start(half, epochs=epochs, before_step=drv)
             train = {"epochs": epochs, "seconds_incl_steps": round(time.perf_counter() - t, 2),
                      "loss_first_last": [round(loss[0], 4), round(loss[-1], 4)]}
-            st.set_warm_start("learned", history_dirs=2)
+            st.set_warm_start("learned", history_dirs=min(history_dirs, st.MAX_HIST - 1))
         else:
             if warm == "extrapolate":
-                st.set_warm_start("extrapolate", history_dirs=3)
+                st.set_warm_start("extrapolate", history_dirs=history_dirs)
             for _ in range(half):
                 drv()
                 st.step()
@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--no-scale", action="store_true",
                     help="skip the kernel timings on the 14.7M-tet gyroid lattice")
     ap.add_argument("--apply-reps", type=int, default=50)
+    ap.add_argument("--history-dirs", type=int, default=6,
+                    help="past solution increments in the Galerkin warm start (1..6)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--series", action="store_true",
                     help="instead of the headline: the receding-interface time series on the "
@@ -317,7 +319,7 @@ def main():
         profile_run(mesh, w, a.apply_reps)
         return
     if a.series:
-        run_series(mesh, w)
+        run_series(mesh, w, history_dirs=a.history_dirs)
         return
     tuned = None
     if a.partition:
@@ -338,7 +340,8 @@ def main():
                                ("learned", "mf32_ir64", "learned")):
         if name not in paths:
             continue
-        r, st = run_path(mesh, w, method, a.steps, a.warmup, rtol, warm=warm)
+        r, st = run_path(mesh, w, method, a.steps, a.warmup, rtol, warm=warm,
+                         history_dirs=a.history_dirs)
         res[name] = r
         log(f"{name} ({method}, {warm} start): {r['steps_per_s']:.2f} steps/s, "
             f"iterations/step {r['iterations_per_step']}, converged={r['converged']}")
@@ -396,8 +399,9 @@ def main():
                                f"vertices, O2/Cl/OH D=2.0/2.0/5.3, dt={w['dt']}, rtol={rtol:g}",
                    "method": "FP32 fused 3-species Jacobi-PCG (Chronopoulos-Gear, one kernel per "
                              "iteration, stored FP32 geometry) in FP64 refinement, FP64 "
-                             "Galerkin warm start over the last 3 solution increments "
-                             "(previous solution as fallback), hipGraph chunks",
+                             f"Galerkin warm start over the last {a.history_dirs} solution "
+                             "increments (previous solution as fallback), species narrowing, "
+                             "hipGraph chunks",
                    "parallelism": "1 GPU", "launch_shape": tuned},
         "plain_fp64": {"steps_per_s": round(plain["steps_per_s"], 3),
                        "cg_iterations_per_step": its(plain)},

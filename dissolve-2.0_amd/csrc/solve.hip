@@ -53,6 +53,7 @@ void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped,
                    hipStream_t st);
 
 constexpr int MAXS = 3;
+constexpr int MAXD = 6;  // warm-start directions of dsv_solve_warm
 constexpr int MAX_PROD = 2048;  // max workgroups of a partial-producing kernel
 
 enum MFMode { MF_APPLY = 0, MF_DIAG = 1, MF_RESID = 3, MF_MASS = 4 };
@@ -1529,7 +1530,14 @@ struct Workspace {
     unsigned char prm2[sizeof(OpParams<double>)] = {};  // OpParams<double> of the residual
     void* vb[VB_COUNT] = {};
     double* csr[MAXS] = {};  // CSR64 values per species
-    void* wd[6] = {};        // FP64 warm-start directions d_k and A d_k (dsv_solve_warm)
+    void* wd[2 * MAXD] = {};  // FP64 warm-start directions d_k and A d_k (dsv_solve_warm)
+    // the previous dsv_solve_warm call's directions: buffer pair per direction, the arrays
+    // each was formed from, its history count, and the key of the operator applied to them
+    int wmap[MAXD] = {};
+    const double* wpair[MAXD][MAXS][2] = {};
+    int wnhist = 0;
+    int wvalid = 0;
+    OpKey wkey;
     double* pW = nullptr;    // partials of the warm start's dot and norm kernels
     WarmOut* wout = nullptr;    // its coefficients and norms (device) ...
     WarmOut* h_wout = nullptr;  // ... and their pinned copy
@@ -1580,7 +1588,7 @@ static void free_ws(Workspace* w) {
     dfree(w->mdiag[0]);
     dfree(w->mdiag[1]);
     for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
-    for (int i = 0; i < 6; ++i) dfree(w->wd[i]);
+    for (int i = 0; i < 2 * MAXD; ++i) dfree(w->wd[i]);
     dfree(w->pW);
     dfree(w->wout);
     if (w->h_wout) (void)hipHostFree(w->h_wout);
@@ -2461,6 +2469,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     DSV_HIP(hipStreamWaitEvent(st, w->ev_io[0], 0));
     m->stat_launches = 0;
     m->stat_narrowings = 0;
+    m->stat_dir_reused = 0;
     const bool internal = method != DSV_METHOD_CSR64;
     const int32_t* perm = internal ? m->perm : nullptr;
     const unsigned gv = cdiv(m->V, 256);
@@ -2604,6 +2613,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         stats->solve_ms = ms;
         stats->cg_launches = m->stat_launches;
         stats->narrowings = m->stat_narrowings;
+        stats->dirs_reused = m->stat_dir_reused;
     }
 }
 
@@ -2638,17 +2648,18 @@ static void residual_norms_impl(dsv_mesh* m, const dsv_species* sp, double inv_d
 // definite -- enters, and c solves (d_j . A d_k) c_k = d_j . r0 with r0 = b - A x0.
 
 constexpr int PROJ_THREADS = 256;
-constexpr int PROJ_GRID = 256;  // S * 9 * PROJ_GRID partials fit the residual partial buffer
+constexpr int PROJ_GRID = 256;  // dsv_warm_start's S * 9 * PROJ_GRID partials (3 directions at
+                                // most) fit the residual partial buffer
 static_assert(MAXS * 9 * PROJ_GRID <= 2 * MAXS * MAX_PROD, "projection partials");
 
 template <int S>
 struct ProjVec {
-    const SV<double, S>* d[3];
-    const SV<double, S>* Ad[3];
+    const SV<double, S>* d[MAXD];
+    const SV<double, S>* Ad[MAXD];
     const SV<double, S>* r0;
     const SV<double, S>* x0;
     SV<double, S>* x;
-    double c[3][MAXS];
+    double c[MAXD][MAXS];
     const double* cdev;  // when set: the coefficients c[k][s] at cdev[k * MAXS + s] (device)
 };
 
@@ -2660,35 +2671,40 @@ __device__ __forceinline__ void proj_coefs(const ProjVec<S>& pv, double (&c)[ND]
         for (int s = 0; s < S; ++s) c[k][s] = pv.cdev ? pv.cdev[k * MAXS + s] : pv.c[k][s];
 }
 
-// partials per species: d_j . A d_k (j <= k), then d_j . r0
+// partials per species: d_j . A d_k (j <= k), then d_j . r0 (quantity s * NP + q).  One pass
+// over the vertices per species, so that a pass holds NP accumulators (up to 27 for six
+// directions) rather than S * NP.
 template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv,
                                                            double* __restrict__ pout) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
-    __shared__ double red[S * NP * 16];
-    double tot[S * NP];
+    __shared__ double red[NP * 16];
+#pragma unroll 1
+    for (int s = 0; s < S; ++s) {
+        double tot[NP];
 #pragma unroll
-    for (int q = 0; q < S * NP; ++q) tot[q] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-        SV<double, S> d[ND], Ad[ND];
+        for (int q = 0; q < NP; ++q) tot[q] = 0.0;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+            double d[ND], Ad[ND];
 #pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            d[k] = pv.d[k][i];
-            Ad[k] = pv.Ad[k][i];
-        }
-        const SV<double, S> r = pv.r0[i];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            int q = s * NP;
+            for (int k = 0; k < ND; ++k) {
+                d[k] = pick<double, S>(pv.d[k][i], s);
+                Ad[k] = pick<double, S>(pv.Ad[k][i], s);
+            }
+            const double r = pick<double, S>(pv.r0[i], s);
+            int q = 0;
 #pragma unroll
             for (int j = 0; j < ND; ++j)
 #pragma unroll
-                for (int k = j; k < ND; ++k) tot[q++] += d[j].v[s] * Ad[k].v[s];
+                for (int k = j; k < ND; ++k) tot[q++] += d[j] * Ad[k];
 #pragma unroll
-            for (int j = 0; j < ND; ++j) tot[q++] += d[j].v[s] * r.v[s];
+            for (int j = 0; j < ND; ++j) tot[q++] += d[j] * r;
         }
+        block_sum_all<NP>(tot, red);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) pout[(s * NP + q) * gridDim.x + blockIdx.x] = tot[q];
     }
-    write_partials<S * NP>(tot, red, pout);
 }
 
 // x = x0 + sum_k c_k d_k, and partials of ||r0 - sum_k c_k A d_k||^2 per species
@@ -2784,8 +2800,8 @@ __global__ void k_pack_diff(int V, const int32_t* __restrict__ perm, const doubl
 // previous ones, or zero); a dropped direction gets c = 0.
 __host__ __device__ static void solve_gram(int n, const double* G, const double* g, double* c,
                                            double rel_tol) {
-    double L[3][3] = {}, Dg[3] = {}, y[3] = {};
-    bool keep[3] = {};
+    double L[MAXD][MAXD] = {}, Dg[MAXD] = {}, y[MAXD] = {};
+    bool keep[MAXD] = {};
     for (int j = 0; j < n; ++j) {
         double dj = G[j * n + j];
         for (int k = 0; k < j; ++k)
@@ -2821,7 +2837,7 @@ __host__ __device__ static void solve_gram(int n, const double* G, const double*
 // Device side of the warm start folded into a solve: the Gram systems, then the acceptance
 // test, with no host round trip in between.
 struct WarmOut {
-    double c[3][MAXS];  // coefficients (0 where the start stays)
+    double c[MAXD][MAXS];  // coefficients (0 where the start stays)
     double rr0[MAXS];   // ||r||^2 of the given start and ||b||^2 (the solve's first residual)
     double bb[MAXS];
     double rr1[MAXS];   // ||r||^2 of the start taken
@@ -2832,18 +2848,20 @@ template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ dots,
                                                             WarmOut* __restrict__ out) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
-    __shared__ double red[S * NP * 16];
-    double q[S * NP];
-    consume_partials<S * NP>(dots, PROJ_GRID, q, red);
-    if (threadIdx.x != 0) return;
+    __shared__ double red[NP * 16];
+#pragma unroll 1
     for (int s = 0; s < S; ++s) {
-        double G[ND * ND], g[ND], c[ND];
-        int t = s * NP;
-        for (int j = 0; j < ND; ++j)
-            for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
-        for (int j = 0; j < ND; ++j) g[j] = q[t++];
-        solve_gram(ND, G, g, c, 1e-10);
-        for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
+        double q[NP];
+        consume_partials<NP>(dots + (size_t)s * NP * PROJ_GRID, PROJ_GRID, q, red);
+        if (threadIdx.x == 0) {
+            double G[ND * ND], g[ND], c[ND];
+            int t = 0;
+            for (int j = 0; j < ND; ++j)
+                for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
+            for (int j = 0; j < ND; ++j) g[j] = q[t++];
+            solve_gram(ND, G, g, c, 1e-10);
+            for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
+        }
     }
 }
 
@@ -2960,28 +2978,58 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
                           int nres, double* rr, double* bb, hipStream_t st) {
     const unsigned gv = cdiv(m->V, 256);
     const unsigned char* prm64 = w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm;
-    if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * 9 * PROJ_GRID);
+    if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD) * PROJ_GRID);
     if (!w->wout) w->wout = dalloc<WarmOut>(1);
     if (!w->h_wout) DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocDefault));
-    ProjVec<S> pv{};
-    for (int k = 0; k < ND; ++k) {
-        for (int j = 0; j < 2; ++j)
-            if (!w->wd[2 * k + j])
-                DSV_HIP(hipMalloc(&w->wd[2 * k + j], (size_t)m->V * rec_bytes(sizeof(double), S)));
-        auto* d = (SV<double, S>*)w->wd[2 * k];
-        auto* Ad = (SV<double, S>*)w->wd[2 * k + 1];
-        const double* a[3] = {nullptr, nullptr, nullptr};
-        const double* b[3] = {nullptr, nullptr, nullptr};
+    // Direction k is a[s] - b[s] per species: d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
+    // (history), then the explicit ones.  Under operator reuse (whose contract keeps the arrays
+    // named unchanged), a history direction k >= 2 whose pair of arrays is the previous call's
+    // pair k - 1 is that call's direction, and its A d_k is taken from there: with the same FP64
+    // operator, one apply per call for a history shifted by one step.
+    const double* a[MAXD][MAXS] = {};
+    const double* b[MAXD][MAXS] = {};
+    for (int k = 0; k < ND; ++k)
         for (int s = 0; s < S; ++s) {
-            if (k < wa.n_hist) {  // d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
-                a[s] = k == 0 ? sp[s].x : wa.hist[(size_t)(k - 1) * S + s];
-                b[s] = wa.hist[(size_t)k * S + s];
+            if (k < wa.n_hist) {
+                a[k][s] = k == 0 ? sp[s].x : wa.hist[(size_t)(k - 1) * S + s];
+                b[k][s] = wa.hist[(size_t)k * S + s];
             } else {
-                a[s] = wa.extra[(size_t)(k - wa.n_hist) * S + s];
+                a[k][s] = wa.extra[(size_t)(k - wa.n_hist) * S + s];
             }
         }
-        hipLaunchKernelGGL((k_pack_diff<S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, a[0],
-                           a[1], a[2], b[0], b[1], b[2], d);
+    const OpKey& opkey = w->key[w->method == DSV_METHOD_MF32_IR64 ? 1 : 0];
+    const bool can = m->op_reuse && w->wvalid && same_key(opkey, w->wkey);
+    int map[MAXD];  // direction -> buffer pair (wd[2p], wd[2p + 1])
+    bool reuse[MAXD] = {}, busy[MAXD] = {};
+    for (int k = 2; k < ND && can; ++k) {
+        if (k >= wa.n_hist || k - 1 >= w->wnhist) continue;
+        bool same = true;
+        for (int s = 0; s < S; ++s)
+            same = same && a[k][s] == w->wpair[k - 1][s][0] && b[k][s] == w->wpair[k - 1][s][1];
+        if (!same) continue;
+        reuse[k] = true;
+        map[k] = w->wmap[k - 1];
+        busy[map[k]] = true;
+    }
+    ProjVec<S> pv{};
+    for (int k = 0; k < ND; ++k) {
+        if (!reuse[k]) {
+            int p = 0;
+            while (busy[p]) ++p;
+            busy[p] = true;
+            map[k] = p;
+        }
+        const int p = map[k];
+        for (int j = 0; j < 2; ++j)
+            if (!w->wd[2 * p + j])
+                DSV_HIP(hipMalloc(&w->wd[2 * p + j], (size_t)m->V * rec_bytes(sizeof(double), S)));
+        auto* d = (SV<double, S>*)w->wd[2 * p];
+        auto* Ad = (SV<double, S>*)w->wd[2 * p + 1];
+        pv.d[k] = d;
+        pv.Ad[k] = Ad;
+        if (reuse[k]) continue;
+        hipLaunchKernelGGL((k_pack_diff<S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, a[k][0],
+                           a[k][1], a[k][2], b[k][0], b[k][1], b[k][2], d);
         for (int s = 0; s < S; ++s)
             if (sp[s].n_bc > 0)
                 hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0,
@@ -2990,8 +3038,21 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
         v.x = d;
         v.y = Ad;
         launch_mf<double, S, MF_APPLY>(m, m->part, prm64, v, nullptr, st);
-        pv.d[k] = d;
-        pv.Ad[k] = Ad;
+    }
+    m->stat_dir_reused += [&] {
+        int n = 0;
+        for (int k = 0; k < ND; ++k) n += reuse[k];
+        return n;
+    }();
+    w->wvalid = m->op_reuse;
+    w->wkey = opkey;
+    w->wnhist = wa.n_hist;
+    for (int k = 0; k < ND; ++k) {
+        w->wmap[k] = map[k];
+        for (int s = 0; s < S; ++s) {
+            w->wpair[k][s][0] = a[k][s];
+            w->wpair[k][s][1] = b[k][s];
+        }
     }
     pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
     pv.x = (SV<double, S>*)w->vb[VB_X64];
@@ -3025,7 +3086,10 @@ static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const W
     switch (wa.n_hist + wa.n_extra) {
         case 1: fused_warm_nd<S, 1>(m, w, sp, wa, nres, rr, bb, st); break;
         case 2: fused_warm_nd<S, 2>(m, w, sp, wa, nres, rr, bb, st); break;
-        default: fused_warm_nd<S, 3>(m, w, sp, wa, nres, rr, bb, st); break;
+        case 3: fused_warm_nd<S, 3>(m, w, sp, wa, nres, rr, bb, st); break;
+        case 4: fused_warm_nd<S, 4>(m, w, sp, wa, nres, rr, bb, st); break;
+        case 5: fused_warm_nd<S, 5>(m, w, sp, wa, nres, rr, bb, st); break;
+        default: fused_warm_nd<S, 6>(m, w, sp, wa, nres, rr, bb, st); break;
     }
 }
 
@@ -3188,8 +3252,8 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
             throw ArgError{"dsv_solve_warm: method must be matrix-free (MF32_IR64 or MF64)"};
         if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
         if (max_it < 1) throw ArgError{"max_it must be >= 1"};
-        if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > 3)
-            throw ArgError{"dsv_solve_warm: n_hist + n_extra must be in [0, 3]"};
+        if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > MAXD)
+            throw ArgError{"dsv_solve_warm: n_hist + n_extra must be in [0, 6]"};
         if ((n_hist > 0 && !hist) || (n_extra > 0 && !extra))
             throw ArgError{"dsv_solve_warm: NULL hist or extra"};
         check_solve_args(m, n_species, sp, true);

@@ -68,6 +68,7 @@ class Stats(ctypes.Structure):
         ("converged", ctypes.c_int32 * 3), ("rel_residual", ctypes.c_double * 3),
         ("initial_rel_residual", ctypes.c_double * 3), ("solve_ms", ctypes.c_double),
         ("cg_launches", ctypes.c_int32), ("narrowings", ctypes.c_int32),
+        ("dirs_reused", ctypes.c_int32),
     ]
 
     def as_dict(self, n):
@@ -77,7 +78,7 @@ class Stats(ctypes.Structure):
             "rel_residual": list(self.rel_residual[:n]),
             "initial_rel_residual": list(self.initial_rel_residual[:n]),
             "solve_ms": self.solve_ms, "cg_launches": self.cg_launches,
-            "narrowings": self.narrowings,
+            "narrowings": self.narrowings, "dirs_reused": self.dirs_reused,
         }
 
 

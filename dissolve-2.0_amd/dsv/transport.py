@@ -12,8 +12,8 @@ solution): "previous" starts every solve from u_n; "extrapolate" and "learned" s
 Galerkin warm start of libdsv (dsv_warm_start) over u_n + span{directions}, per species --
 the combination minimising the A-energy norm of the error, taken only where its FP64
 residual is below that of u_n (the previous solution stays the fallback and the judge).
-"extrapolate" uses the last history_dirs (default 3) increments u_n - u_{n-1},
-u_{n-1} - u_{n-2}, ...; "learned" uses at most two of them plus the per-vertex MLP's
+"extrapolate" uses the last history_dirs (default 6, at most 6) increments u_n - u_{n-1},
+u_{n-1} - u_{n-2}, ...; "learned" uses at most five of them plus the per-vertex MLP's
 prediction of the part of the next increment that extrapolation misses,
 (u_{n+1} - 2 u_n + u_{n-1}) / (sigma tau), sigma = max |u_n - u_{n-1}| and tau the training
 targets' RMS.  `train_warm_start(n)` runs n steps collecting (features, target) pairs, fits
@@ -32,6 +32,8 @@ from .core import Mesh, ptr, species_array, species_struct, stream_ptr, to_devic
 
 
 class Stepper:
+    MAX_HIST = 6  # past solutions kept (libdsv dsv_solve_warm takes up to 6 directions)
+
     def __init__(self, mesh: Mesh, species, dt, method="mf32_ir64", rtol=1e-8, max_it=20000,
                  lumped=1, phi=None, grad_phi=None, eps=None, mlp=None, reuse_operator=True):
         import torch
@@ -48,7 +50,8 @@ class Stepper:
         self._keep = []
         self.defs = species
         self.u = [to_device(s["u0"], dev, torch.float64).clone() for s in species]
-        self.u_hist = [[u.clone() for u in self.u] for _ in range(3)]  # u_{n-1}, u_{n-2}, u_{n-3}
+        # u_{n-1}, u_{n-2}, ..., u_{n-MAX_HIST}
+        self.u_hist = [[u.clone() for u in self.u] for _ in range(self.MAX_HIST)]
         self.b = [torch.zeros_like(u) for u in self.u]
         self.x = [torch.zeros_like(u) for u in self.u]
         self.f = [to_device(s["f"], dev, torch.float64).clone() if s.get("f") is not None else None
@@ -72,7 +75,7 @@ class Stepper:
                 mask[self.bc[k].long()] = 1
             self.bc_mask.append(mask)
         self.warm = "previous"
-        self.history_dirs = 3
+        self.history_dirs = self.MAX_HIST
         self.collect = False
         self.pairs = []
         self.tau = 1.0
@@ -85,9 +88,9 @@ class Stepper:
         return self.u_hist[0]
 
     def set_warm_start(self, mode, history_dirs=None):
-        """"previous", "extrapolate" or "learned" (needs a trained MLP); history_dirs (1..3) is
-        the number of past increments u_n - u_{n-1}, u_{n-1} - u_{n-2}, ... among the
-        directions (at most 2 are used beside the learned direction)."""
+        """"previous", "extrapolate" or "learned" (needs a trained MLP); history_dirs
+        (1..MAX_HIST) is the number of past increments u_n - u_{n-1}, u_{n-1} - u_{n-2}, ...
+        among the directions (at most MAX_HIST - 1 are used beside the learned direction)."""
         if mode not in ("previous", "extrapolate", "learned"):
             raise ValueError(f"unknown warm start {mode!r}")
         if mode != "previous" and self.method == "csr64":
@@ -95,8 +98,8 @@ class Stepper:
         if mode == "learned" and (self.mlp is None or not self.mlp.trained):
             raise ValueError("the learned warm start needs a trained MLP (train_warm_start)")
         if history_dirs is not None:
-            if history_dirs not in (1, 2, 3):
-                raise ValueError("history_dirs must be 1, 2 or 3")
+            if not 1 <= int(history_dirs) <= self.MAX_HIST:
+                raise ValueError(f"history_dirs must be in 1..{self.MAX_HIST}")
             self.history_dirs = int(history_dirs)
         self.warm = mode
 
@@ -205,7 +208,7 @@ class Stepper:
         if warm is None and self.warm != "previous" and self.n_steps >= 1:
             # Galerkin warm start over past increments (and the MLP direction), folded into
             # the solve: the directions are formed from the stored past solutions in libdsv
-            nh = min(self.history_dirs, 2 if learned else 3, self.n_steps)
+            nh = min(self.history_dirs, self.MAX_HIST - (1 if learned else 0), self.n_steps)
             extra = [self._mlp_direction(feats, k) for k in range(self.S)] if learned else []
             nd = nh + (1 if learned else 0)
             hist = (ctypes.c_void_p * max(1, nh * self.S))(

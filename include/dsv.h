@@ -76,6 +76,8 @@ typedef struct dsv_stats {
     int32_t cg_launches;         /* CG iterations launched, all species together (includes
                                     the no-op ones queued after convergence) */
     int32_t narrowings;          /* inner solves continued with fewer species */
+    int32_t dirs_reused;         /* dsv_solve_warm: directions whose A d came from the
+                                    previous call (history shifted by one step) */
 } dsv_stats;
 
 const char* dsv_last_error(void);
@@ -169,7 +171,11 @@ int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
  * the differences of past solutions -- hist[j*n_species + s] is species s at step n-1-j, the
  * start sp[s].x is step n, so d_0 = x - hist_0, d_j = hist_(j-1) - hist_j (n_hist of them) --
  * followed by n_extra explicit directions extra[k*n_species + s] (NULL = zero);
- * n_hist + n_extra <= 3.  The combination is taken per species only where it lowers the FP64
+ * n_hist + n_extra <= 6.  While operator reuse is on (dsv_set_operator_reuse), a history
+ * direction k >= 2 formed from the same two arrays as the previous call's direction k - 1,
+ * with the same operator, takes its A d from that call (the reuse contract keeps the arrays
+ * unchanged): a caller that shifts its history by one step pays one apply per call for d_0 and
+ * one for d_1.  The combination is taken per species only where it lowers the FP64
  * residual (the start stays the fallback); coef (n_dir*n_species), r_before and r_after
  * (n_species) are as in dsv_warm_start and may be NULL.  Matrix-free methods only
  * (DSV_METHOD_MF32_IR64, DSV_METHOD_MF64).  No counterpart in the reference. */
@@ -200,7 +206,8 @@ int dsv_set_species_narrowing(dsv_mesh* m, int32_t enable);
  * skip rebuilding it when a later call names the same coefficient and Dirichlet arrays (same
  * pointers and lengths), the same scalar D / alpha values, inv_dt, mass lumping and operator
  * form.  Dirichlet index lists are range-checked once per (address, length) while reuse is on.
- * The caller guarantees that the contents of those arrays do not change while reuse is on;
+ * The caller guarantees that the contents of those arrays, and of the past solutions given to
+ * dsv_solve_warm, do not change while reuse is on;
  * every call of this function discards what was built (call it again after changing a
  * coefficient array in place).  Default: off. */
 int dsv_set_operator_reuse(dsv_mesh* m, int32_t reuse);

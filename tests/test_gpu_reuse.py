@@ -38,7 +38,8 @@ def _solve(mesh, sps, u0, bs, t, method):
 def test_stepper_reuse_is_bit_identical_and_follows_refresh(gpu):
     """A stepper that keeps its operator across steps (reuse on) against one that rebuilds it
     every call (reuse off, its own mesh): the same iterations and bit-identical solutions, also
-    after the coefficients are refreshed in place."""
+    after the coefficients are refreshed in place, with the warm start's A d of the shifted
+    history taken from the previous step under reuse."""
     from dsv import workloads
     from dsv.core import Mesh
     from dsv.transport import Stepper
@@ -50,14 +51,21 @@ def test_stepper_reuse_is_bit_identical_and_follows_refresh(gpu):
         st.set_warm_start("extrapolate")
         steppers.append(st)
     a, b = steppers
-    for n in range(6):
-        if n == 3:
+    reused = []
+    for n in range(9):
+        if n == 5:
             for st in steppers:
                 st.refresh_coefficients(alpha=[st.alpha_dev[0] * 2.5, None, None])
         ha, hb = a.step(), b.step()
         assert ha["iterations"] == hb["iterations"], (n, ha, hb)
         for ua, ub in zip(a.u, b.u):
             assert np.array_equal(ua.cpu().numpy(), ub.cpu().numpy()), n
+        assert hb["dirs_reused"] == 0
+        reused.append(ha["dirs_reused"])
+    # the shifted history's A d_k (k >= 2) come from the previous step while the operator
+    # stands: none on the first steps (too short a history) and right after the refresh
+    assert reused[:3] == [0, 0, 0] and reused[5] == 0, reused
+    assert reused[3] > 0 and reused[4] > 0 and reused[8] > 0, reused
 
 
 def test_per_call_apis_never_reuse_a_stale_operator(gpu):
