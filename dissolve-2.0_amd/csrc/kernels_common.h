@@ -25,11 +25,45 @@ __device__ __forceinline__ T dot(const Vec3<T>& a, const Vec3<T>& b) {
     return a.x * b.x + a.y * b.y + a.z * b.z;
 }
 
-// Sum over the 64 lanes of a wavefront; every lane gets the total.
+// Lane exchange by DPP (VALU, no LDS round trip), gfx9-family controls: quad_perm xor 1 and
+// xor 2, row_half_mirror, row_mirror, row_bcast:15 and row_bcast:31.  Lanes of rows left out
+// by ROW_MASK read 0.
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141,
+              DPP_ROW_MIRROR = 0x140, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// v + the value of lane (lane ^ 1), resp. (lane ^ 2): the sum over a vertex's 2 or 4 lanes
+template <class T>
+__device__ __forceinline__ T add_lane_xor1(T v) {
+    return v + dpp_move<DPP_QUAD_XOR1>(v);
+}
+template <class T>
+__device__ __forceinline__ T add_lane_xor2(T v) {
+    return v + dpp_move<DPP_QUAD_XOR2>(v);
+}
+
+// Sum over the 64 lanes of a wavefront; every lane gets the total.  Quad, half-row and row
+// sums by DPP, then rows 0+1 and 2+3 by row broadcasts, lane 63 holding the total, which
+// readlane hands to every lane.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_move<DPP_QUAD_XOR1>(v);
+    v += dpp_move<DPP_QUAD_XOR2>(v);
+    v += dpp_move<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_move<DPP_ROW_MIRROR>(v);
+    v += dpp_move<DPP_ROW_BCAST15, 0xA>(v);
+    v += dpp_move<DPP_ROW_BCAST31, 0xC>(v);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 // Coefficient of cell e for a caller-order coefficient array (COEF_CELL: length C,

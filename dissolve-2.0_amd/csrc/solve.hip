@@ -343,8 +343,8 @@ __device__ __forceinline__ void lane_rows(const Geo& g, const OpParams<T>& prm,
     // the same bits)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if (L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
-        if (L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+        if (L >= 2) acc[s] = add_lane_xor1(acc[s]);
+        if (L >= 4) acc[s] = add_lane_xor2(acc[s]);
     }
 }
 
@@ -423,8 +423,8 @@ __device__ __forceinline__ void edge_rows(const Geo& g, const OpParams<T>& prm,
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if (L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
-        if (L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+        if (L >= 2) acc[s] = add_lane_xor1(acc[s]);
+        if (L >= 4) acc[s] = add_lane_xor2(acc[s]);
     }
 }
 
@@ -840,8 +840,8 @@ __device__ __forceinline__ void slot_rows(const Geo& g, const OpParams<float>& p
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if (g.L >= 2) acc[s] += __shfl_xor(acc[s], 1, 64);
-        if (g.L >= 4) acc[s] += __shfl_xor(acc[s], 2, 64);
+        if (g.L >= 2) acc[s] = add_lane_xor1(acc[s]);
+        if (g.L >= 4) acc[s] = add_lane_xor2(acc[s]);
     }
 }
 
@@ -2671,39 +2671,79 @@ __device__ __forceinline__ void proj_coefs(const ProjVec<S>& pv, double (&c)[ND]
         for (int s = 0; s < S; ++s) c[k][s] = pv.cdev ? pv.cdev[k * MAXS + s] : pv.c[k][s];
 }
 
-// partials per species: d_j . A d_k (j <= k), then d_j . r0 (quantity s * NP + q).  One pass
-// over the vertices per species, so that a pass holds NP accumulators (up to 27 for six
-// directions) rather than S * NP.
-template <int S, int ND>
+// Gram quantities of a species, in this order: d_j . A d_k (j <= k, row by row), then
+// d_j . r0 (NP of them)
+template <int ND>
+constexpr int gram_row(int j) {  // index of d_j . A d_j
+    return j * ND - j * (j - 1) / 2;
+}
+template <int ND>
+constexpr int gram_rows(int j0, int j1) {  // quantities of rows [j0, j1): their G and g entries
+    return gram_row<ND>(j1) - gram_row<ND>(j0) + (j1 - j0);
+}
+
+// partials per species of the Gram rows [J0, J1): d_j . A d_k (k >= j) and d_j . r0, at the
+// species' quantity index (s * NP + q) * gridDim + block.  All species in one pass; with six
+// directions the rows go in three launches so that the accumulators stay in registers.
+template <int S, int ND, int J0, int J1>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv,
                                                            double* __restrict__ pout) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
-    __shared__ double red[NP * 16];
-#pragma unroll 1
-    for (int s = 0; s < S; ++s) {
-        double tot[NP];
+    constexpr int NR = gram_rows<ND>(J0, J1);
+    __shared__ double red[S * NR * 16];
+    double tot[S * NR];
 #pragma unroll
-        for (int q = 0; q < NP; ++q) tot[q] = 0.0;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-            double d[ND], Ad[ND];
+    for (int q = 0; q < S * NR; ++q) tot[q] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        SV<double, S> d[J1 - J0];
 #pragma unroll
-            for (int k = 0; k < ND; ++k) {
-                d[k] = pick<double, S>(pv.d[k][i], s);
-                Ad[k] = pick<double, S>(pv.Ad[k][i], s);
-            }
-            const double r = pick<double, S>(pv.r0[i], s);
-            int q = 0;
+        for (int j = J0; j < J1; ++j) d[j - J0] = pv.d[j][i];
 #pragma unroll
-            for (int j = 0; j < ND; ++j)
+        for (int k = J0; k < ND; ++k) {
+            const SV<double, S> Ad = pv.Ad[k][i];
 #pragma unroll
-                for (int k = j; k < ND; ++k) tot[q++] += d[j] * Ad[k];
+            for (int j = J0; j < J1; ++j)
+                if (j <= k)
 #pragma unroll
-            for (int j = 0; j < ND; ++j) tot[q++] += d[j] * r;
+                    for (int s = 0; s < S; ++s)
+                        tot[s * NR + gram_row<ND>(j) - gram_row<ND>(J0) + (k - j)] +=
+                            d[j - J0].v[s] * Ad.v[s];
         }
-        block_sum_all<NP>(tot, red);
-        if (threadIdx.x == 0)
+        const SV<double, S> r = pv.r0[i];
 #pragma unroll
-            for (int q = 0; q < NP; ++q) pout[(s * NP + q) * gridDim.x + blockIdx.x] = tot[q];
+        for (int j = J0; j < J1; ++j)
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                tot[s * NR + gram_row<ND>(J1) - gram_row<ND>(J0) + (j - J0)] += d[j - J0].v[s] * r.v[s];
+    }
+    block_sum_all<S * NR>(tot, red);
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int q = 0; q < gram_row<ND>(J1) - gram_row<ND>(J0); ++q)
+            pout[(s * NP + gram_row<ND>(J0) + q) * gridDim.x + blockIdx.x] = tot[s * NR + q];
+#pragma unroll
+        for (int j = J0; j < J1; ++j)
+            pout[(s * NP + ND * (ND + 1) / 2 + j) * gridDim.x + blockIdx.x] =
+                tot[s * NR + gram_row<ND>(J1) - gram_row<ND>(J0) + (j - J0)];
+    }
+}
+
+template <int S, int ND>
+static void launch_proj_dots(const dsv_mesh* m, const ProjVec<S>& pv, double* pout,
+                             hipStream_t st) {
+    if constexpr (ND <= 3) {
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+                           m->V, pv, pout);
+    } else {
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, 2>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+                           m->V, pv, pout);
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 2, 4>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+                           m->V, pv, pout);
+        if constexpr (ND > 4)
+            hipLaunchKernelGGL((k_proj_dots<S, ND, 4, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0,
+                               st, m->V, pv, pout);
     }
 }
 
@@ -2795,42 +2835,47 @@ __global__ void k_pack_diff(int V, const int32_t* __restrict__ perm, const doubl
     out[i] = r;
 }
 
-// Solves G c = g for the n x n SPD Gram matrix G (row-major) by LDL^T, dropping a direction
+// Solves G c = g for the N x N SPD Gram matrix G (row-major) by LDL^T, dropping a direction
 // whose pivot falls below rel_tol of its own energy d.Ad (numerically dependent on the
 // previous ones, or zero); a dropped direction gets c = 0.
-__host__ __device__ static void solve_gram(int n, const double* G, const double* g, double* c,
+template <int N>
+__host__ __device__ static void solve_gram(const double* G, const double* g, double* c,
                                            double rel_tol) {
-    double L[MAXD][MAXD] = {}, Dg[MAXD] = {}, y[MAXD] = {};
-    bool keep[MAXD] = {};
-    for (int j = 0; j < n; ++j) {
-        double dj = G[j * n + j];
+    double L[N][N] = {}, Dg[N] = {}, y[N] = {};
+    bool keep[N] = {};
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        double dj = G[j * N + j];
+#pragma unroll
         for (int k = 0; k < j; ++k)
             if (keep[k]) dj -= L[j][k] * L[j][k] * Dg[k];
         const bool finite = dj - dj == 0.0;  // false for an infinity or a NaN
-        keep[j] = G[j * n + j] > 0.0 && finite && dj > rel_tol * G[j * n + j];
-        if (!keep[j]) continue;
-        Dg[j] = dj;
-        for (int i = j + 1; i < n; ++i) {
-            double v = G[i * n + j];
+        keep[j] = G[j * N + j] > 0.0 && finite && dj > rel_tol * G[j * N + j];
+        Dg[j] = keep[j] ? dj : 1.0;
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) {
+            double v = G[i * N + j];
+#pragma unroll
             for (int k = 0; k < j; ++k)
                 if (keep[k]) v -= L[i][k] * L[j][k] * Dg[k];
-            L[i][j] = v / dj;
+            L[i][j] = keep[j] ? v / dj : 0.0;
         }
     }
-    for (int j = 0; j < n; ++j) {
-        if (!keep[j]) continue;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
         double v = g[j];
+#pragma unroll
         for (int k = 0; k < j; ++k)
             if (keep[k]) v -= L[j][k] * y[k];
-        y[j] = v;
+        y[j] = keep[j] ? v : 0.0;
     }
-    for (int j = n - 1; j >= 0; --j) {
-        c[j] = 0.0;
-        if (!keep[j]) continue;
+#pragma unroll
+    for (int j = N - 1; j >= 0; --j) {
         double v = y[j] / Dg[j];
-        for (int i = j + 1; i < n; ++i)
+#pragma unroll
+        for (int i = j + 1; i < N; ++i)
             if (keep[i]) v -= L[i][j] * c[i];
-        c[j] = v;
+        c[j] = keep[j] ? v : 0.0;
     }
 }
 
@@ -2849,20 +2894,29 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __res
                                                             WarmOut* __restrict__ out) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
     __shared__ double red[NP * 16];
+    __shared__ double qs[S][NP];
 #pragma unroll 1
     for (int s = 0; s < S; ++s) {
         double q[NP];
         consume_partials<NP>(dots + (size_t)s * NP * PROJ_GRID, PROJ_GRID, q, red);
-        if (threadIdx.x == 0) {
-            double G[ND * ND], g[ND], c[ND];
-            int t = 0;
-            for (int j = 0; j < ND; ++j)
-                for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
-            for (int j = 0; j < ND; ++j) g[j] = q[t++];
-            solve_gram(ND, G, g, c, 1e-10);
-            for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
-        }
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int j = 0; j < NP; ++j) qs[s][j] = q[j];
     }
+    __syncthreads();
+    if (threadIdx.x >= S) return;
+    const int s = threadIdx.x;  // one species per thread
+    double G[ND * ND], g[ND], c[ND];
+    int t = 0;
+#pragma unroll
+    for (int j = 0; j < ND; ++j)
+#pragma unroll
+        for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = qs[s][t++];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) g[j] = qs[s][t++];
+    solve_gram<ND>(G, g, c, 1e-10);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
 }
 
 // Takes the combination of a species only where some direction moved it and its FP64
@@ -2921,8 +2975,7 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
     pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
     pv.x0 = (const SV<double, S>*)w->vb[VB_X64];
     pv.x = (SV<double, S>*)w->vb[VB_X];
-    hipLaunchKernelGGL((k_proj_dots<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
-                       w->pR);
+    launch_proj_dots<S, ND>(m, pv, w->pR, st);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * NP * PROJ_GRID,
                            hipMemcpyDeviceToHost, st));
@@ -2939,7 +2992,7 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
         for (int j = 0; j < ND; ++j)
             for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
         for (int j = 0; j < ND; ++j) g[j] = q[t++];
-        solve_gram(ND, G, g, c, 1e-10);
+        solve_gram<ND>(G, g, c, 1e-10);
         for (int k = 0; k < ND; ++k) pv.c[k][s] = c[k];
     }
     hipLaunchKernelGGL((k_proj_combine<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V,
@@ -3057,8 +3110,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
     pv.x = (SV<double, S>*)w->vb[VB_X64];
     pv.cdev = &w->wout->c[0][0];
-    hipLaunchKernelGGL((k_proj_dots<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
-                       w->pW);
+    launch_proj_dots<S, ND>(m, pv, w->pW, st);
     hipLaunchKernelGGL((k_proj_solve<S, ND>), dim3(1), dim3(PROJ_THREADS), 0, st, w->pW, w->wout);
     hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
                        w->pW);

@@ -176,3 +176,54 @@ def test_solve_warm_matches_separate_warm_start_and_solve(gpu, method):
         _lib.call("dsv_solve_warm", mesh.handle, 3, species_array(structs), inv_dt, 1,
                   _lib.METHODS["csr64"], 1e-10, 5000, 1, zero, 0, None, stream_ptr(gpu), st,
                   None, None, None)
+
+
+@pytest.mark.parametrize("n_hist,n_extra", [(6, 0), (5, 1), (4, 0)])
+def test_solve_warm_many_directions_match_oracle(gpu, n_hist, n_extra):
+    """dsv_solve_warm with up to six directions (past solutions and explicit ones): the
+    coefficients and both residual norms of its Galerkin start against the oracle's
+    galerkin_start over the same directions (the start's FP64 projection is the oracle's;
+    bars as above), and the solve converges to the direct solution."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import Mesh, species_array, species_struct, stream_ptr, ptr
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("kuhn7_jitter_morton", seed=13)
+    mesh = Mesh(conn, xyz)
+    V = len(xyz)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
+    exact = [OS.direct(A, b) for A, b in zip(mats, bs)]
+    # past solutions drifting smoothly toward the exact one, plus noise
+    past = [[u0[s] - (j + 1) * 0.2 * (exact[s] - u0[s]) * (1 + 0.1 * j) + rng.normal(0, 1e-3, V)
+             for s in range(3)] for j in range(n_hist)]
+    extra = [[rng.normal(0, 1, V) for _ in range(3)] for _ in range(n_extra)]
+    keep = [t(v) for row in past + extra for v in row]
+    hist_t = keep[:3 * n_hist]
+    extra_t = keep[3 * n_hist:]
+    sp = [dict(s, b=t(b), x=t(u)) for s, u, b in zip(sps, u0, bs)]
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                              bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                              b=s["b"], x=s["x"], keep=keep) for s in sp]
+    harr = (ctypes.c_void_p * (3 * n_hist))(*[ptr(v).value for v in hist_t])
+    earr = (ctypes.c_void_p * max(1, 3 * n_extra))(*[ptr(v).value for v in extra_t])
+    nd = n_hist + n_extra
+    coef = (ctypes.c_double * (3 * nd))()
+    r0, r1 = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+    st = _lib.Stats()
+    _lib.call("dsv_solve_warm", mesh.handle, 3, species_array(structs), inv_dt, 1,
+              _lib.METHODS["mf64"], 1e-10, 5000, n_hist, harr, n_extra, earr, stream_ptr(gpu), st,
+              coef, r0, r1)
+    d = st.as_dict(3)
+    assert all(d["converged"]), d
+    for s in range(3):
+        seq = [u0[s]] + [past[j][s] for j in range(n_hist)]
+        dirs = [seq[j] - seq[j + 1] for j in range(n_hist)] + [extra[k][s] for k in range(n_extra)]
+        xr, cr, n0, n1 = OW.galerkin_start(mats[s], bs[s], u0[s], dirs, sps[s].get("bc_dofs"),
+                                           sps[s].get("bc_value", 0.0))
+        assert abs(r0[s] - n0) <= 1e-9 * n0
+        assert abs(r1[s] - n1) <= 1e-9 * n0
+        for k in range(nd):
+            assert abs(coef[k * 3 + s] - cr[k]) <= 1e-8 * max(np.abs(cr).max(), 1e-30), (s, k)
+        x = sp[s]["x"].cpu().numpy()
+        assert np.abs(x - exact[s]).max() <= 1e-7 * np.abs(exact[s]).max()
