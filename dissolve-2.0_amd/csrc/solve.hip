@@ -1539,6 +1539,7 @@ struct Workspace {
     int wvalid = 0;
     OpKey wkey;
     double* pW = nullptr;    // partials of the warm start's dot and norm kernels
+    double* pT = nullptr;    // their totals
     WarmOut* wout = nullptr;    // its coefficients and norms (device) ...
     WarmOut* h_wout = nullptr;  // ... and their pinned copy
     void* fw[2][MAXS] = {};  // stored form: per-species weights K(D_s) of sets 0 / 1
@@ -1590,6 +1591,7 @@ static void free_ws(Workspace* w) {
     for (int s = 0; s < MAXS; ++s) dfree(w->csr[s]);
     for (int i = 0; i < 2 * MAXD; ++i) dfree(w->wd[i]);
     dfree(w->pW);
+    dfree(w->pT);
     dfree(w->wout);
     if (w->h_wout) (void)hipHostFree(w->h_wout);
     for (int a = 0; a < 2; ++a)
@@ -2879,6 +2881,18 @@ __host__ __device__ static void solve_gram(const double* G, const double* g, dou
     }
 }
 
+// Totals of partials, one workgroup per quantity: out[q] = sum over b < nprod of
+// partials[q * nprod + b] (fixed order), so that a single-workgroup consumer reads totals,
+// not every producer's partial.
+__global__ __launch_bounds__(PROJ_THREADS) void k_sum_partials(const double* __restrict__ partials,
+                                                              int nprod, double* __restrict__ out) {
+    __shared__ double red[16];
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nprod; b += blockDim.x) v[0] += partials[(size_t)blockIdx.x * nprod + b];
+    block_sum_all<1>(v, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
 // Device side of the warm start folded into a solve: the Gram systems, then the acceptance
 // test, with no host round trip in between.
 struct WarmOut {
@@ -2888,49 +2902,36 @@ struct WarmOut {
     double rr1[MAXS];   // ||r||^2 of the start taken
 };
 
-// Sums of the dot partials (PROJ_GRID per quantity), one Gram solve per species (thread 0)
+// One Gram solve per species (thread s), from the totals of the dot partials
 template <int S, int ND>
-__global__ __launch_bounds__(PROJ_THREADS) void k_proj_solve(const double* __restrict__ dots,
-                                                            WarmOut* __restrict__ out) {
+__global__ __launch_bounds__(64) void k_proj_solve(const double* __restrict__ tot,
+                                                  WarmOut* __restrict__ out) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
-    __shared__ double red[NP * 16];
-    __shared__ double qs[S][NP];
-#pragma unroll 1
-    for (int s = 0; s < S; ++s) {
-        double q[NP];
-        consume_partials<NP>(dots + (size_t)s * NP * PROJ_GRID, PROJ_GRID, q, red);
-        if (threadIdx.x == 0)
-#pragma unroll
-            for (int j = 0; j < NP; ++j) qs[s][j] = q[j];
-    }
-    __syncthreads();
     if (threadIdx.x >= S) return;
-    const int s = threadIdx.x;  // one species per thread
+    const int s = threadIdx.x;
+    const double* q = tot + s * NP;
     double G[ND * ND], g[ND], c[ND];
     int t = 0;
 #pragma unroll
     for (int j = 0; j < ND; ++j)
 #pragma unroll
-        for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = qs[s][t++];
+        for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
 #pragma unroll
-    for (int j = 0; j < ND; ++j) g[j] = qs[s][t++];
+    for (int j = 0; j < ND; ++j) g[j] = q[t++];
     solve_gram<ND>(G, g, c, 1e-10);
 #pragma unroll
     for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
 }
 
 // Takes the combination of a species only where some direction moved it and its FP64
-// residual fell (the given start is the fallback): sums the norm partials (PROJ_GRID per
-// species) and the residual kernel's (nres per quantity: r.r then b.b per species).
+// residual fell (the given start is the fallback), from the totals of the norm partials and
+// of the residual kernel's (r.r then b.b per species).
 template <int S, int ND>
-__global__ __launch_bounds__(PROJ_THREADS) void k_proj_accept(const double* __restrict__ norms,
-                                                             const double* __restrict__ res,
-                                                             int nres, WarmOut* __restrict__ out) {
-    __shared__ double red[2 * S * 16];
-    double n1[S], r0[2 * S];
-    consume_partials<S>(norms, PROJ_GRID, n1, red);
-    consume_partials<2 * S>(res, nres, r0, red);
+__global__ __launch_bounds__(64) void k_proj_accept(const double* __restrict__ tot,
+                                                   WarmOut* __restrict__ out) {
     if (threadIdx.x != 0) return;
+    const double* n1 = tot;     // ||r0 - sum_k c_k A d_k||^2 per species
+    const double* r0 = tot + S;  // the residual kernel's r.r, then b.b, per species
     for (int s = 0; s < S; ++s) {
         bool moved = false;  // every direction dropped: nothing to take
         for (int k = 0; k < ND; ++k) moved |= out->c[k][s] != 0.0;
@@ -3033,6 +3034,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     const unsigned char* prm64 = w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm;
     if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD) * PROJ_GRID);
     if (!w->wout) w->wout = dalloc<WarmOut>(1);
+    if (!w->pT) w->pT = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD));
     if (!w->h_wout) DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocDefault));
     // Direction k is a[s] - b[s] per species: d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
     // (history), then the explicit ones.  Under operator reuse (whose contract keeps the arrays
@@ -3110,12 +3112,17 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
     pv.x = (SV<double, S>*)w->vb[VB_X64];
     pv.cdev = &w->wout->c[0][0];
+    constexpr int NP = ND * (ND + 1) / 2 + ND;
     launch_proj_dots<S, ND>(m, pv, w->pW, st);
-    hipLaunchKernelGGL((k_proj_solve<S, ND>), dim3(1), dim3(PROJ_THREADS), 0, st, w->pW, w->wout);
+    hipLaunchKernelGGL(k_sum_partials, dim3(S * NP), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID,
+                       w->pT);
+    hipLaunchKernelGGL((k_proj_solve<S, ND>), dim3(1), dim3(64), 0, st, w->pT, w->wout);
     hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
                        w->pW);
-    hipLaunchKernelGGL((k_proj_accept<S, ND>), dim3(1), dim3(PROJ_THREADS), 0, st, w->pW, w->pR,
-                       nres, w->wout);
+    hipLaunchKernelGGL(k_sum_partials, dim3(S), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID, w->pT);
+    hipLaunchKernelGGL(k_sum_partials, dim3(2 * S), dim3(PROJ_THREADS), 0, st, w->pR, nres,
+                       w->pT + S);
+    hipLaunchKernelGGL((k_proj_accept<S, ND>), dim3(1), dim3(64), 0, st, w->pT, w->wout);
     hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V,
                        pv, (SV<double, S>*)w->vb[VB_R64]);
     DSV_HIP(hipGetLastError());
