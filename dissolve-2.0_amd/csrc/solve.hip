@@ -2650,6 +2650,9 @@ static void residual_norms_impl(dsv_mesh* m, const dsv_species* sp, double inv_d
 // definite -- enters, and c solves (d_j . A d_k) c_k = d_j . r0 with r0 = b - A x0.
 
 constexpr int PROJ_THREADS = 256;
+constexpr int PROJ_GRID_F = 1024;  // workgroups of dsv_solve_warm's norm and update kernels (the
+                                  // dot kernels, whose epilogue writes 13-39 partials per
+                                  // workgroup, run PROJ_GRID)
 constexpr int PROJ_GRID = 256;  // dsv_warm_start's S * 9 * PROJ_GRID partials (3 directions at
                                 // most) fit the residual partial buffer
 static_assert(MAXS * 9 * PROJ_GRID <= 2 * MAXS * MAX_PROD, "projection partials");
@@ -2733,18 +2736,18 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv
 }
 
 template <int S, int ND>
-static void launch_proj_dots(const dsv_mesh* m, const ProjVec<S>& pv, double* pout,
+static void launch_proj_dots(const dsv_mesh* m, const ProjVec<S>& pv, double* pout, int grid,
                              hipStream_t st) {
     if constexpr (ND <= 3) {
-        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, ND>), dim3(grid), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
     } else {
-        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, 2>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, 2>), dim3(grid), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
-        hipLaunchKernelGGL((k_proj_dots<S, ND, 2, 4>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st,
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 2, 4>), dim3(grid), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
         if constexpr (ND > 4)
-            hipLaunchKernelGGL((k_proj_dots<S, ND, 4, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0,
+            hipLaunchKernelGGL((k_proj_dots<S, ND, 4, ND>), dim3(grid), dim3(PROJ_THREADS), 0,
                                st, m->V, pv, pout);
     }
 }
@@ -2976,7 +2979,7 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
     pv.r0 = (const SV<double, S>*)w->vb[VB_R64];
     pv.x0 = (const SV<double, S>*)w->vb[VB_X64];
     pv.x = (SV<double, S>*)w->vb[VB_X];
-    launch_proj_dots<S, ND>(m, pv, w->pR, st);
+    launch_proj_dots<S, ND>(m, pv, w->pR, PROJ_GRID, st);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * S * NP * PROJ_GRID,
                            hipMemcpyDeviceToHost, st));
@@ -3032,7 +3035,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
                           int nres, double* rr, double* bb, hipStream_t st) {
     const unsigned gv = cdiv(m->V, 256);
     const unsigned char* prm64 = w->method == DSV_METHOD_MF32_IR64 ? w->prm2 : w->prm;
-    if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD) * PROJ_GRID);
+    if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD) * PROJ_GRID_F);
     if (!w->wout) w->wout = dalloc<WarmOut>(1);
     if (!w->pT) w->pT = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD));
     if (!w->h_wout) DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocDefault));
@@ -3113,17 +3116,18 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     pv.x = (SV<double, S>*)w->vb[VB_X64];
     pv.cdev = &w->wout->c[0][0];
     constexpr int NP = ND * (ND + 1) / 2 + ND;
-    launch_proj_dots<S, ND>(m, pv, w->pW, st);
+    launch_proj_dots<S, ND>(m, pv, w->pW, PROJ_GRID, st);
     hipLaunchKernelGGL(k_sum_partials, dim3(S * NP), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID,
                        w->pT);
     hipLaunchKernelGGL((k_proj_solve<S, ND>), dim3(1), dim3(64), 0, st, w->pT, w->wout);
-    hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V, pv,
-                       w->pW);
-    hipLaunchKernelGGL(k_sum_partials, dim3(S), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID, w->pT);
+    hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st, m->V,
+                       pv, w->pW);
+    hipLaunchKernelGGL(k_sum_partials, dim3(S), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID_F,
+                       w->pT);
     hipLaunchKernelGGL(k_sum_partials, dim3(2 * S), dim3(PROJ_THREADS), 0, st, w->pR, nres,
                        w->pT + S);
     hipLaunchKernelGGL((k_proj_accept<S, ND>), dim3(1), dim3(64), 0, st, w->pT, w->wout);
-    hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID), dim3(PROJ_THREADS), 0, st, m->V,
+    hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st, m->V,
                        pv, (SV<double, S>*)w->vb[VB_R64]);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_wout, w->wout, sizeof(WarmOut), hipMemcpyDeviceToHost, st));
