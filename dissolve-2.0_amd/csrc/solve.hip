@@ -2520,7 +2520,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     double tol_abs[MAXS] = {0, 0, 0}, rel0[MAXS] = {0, 0, 0}, relf[MAXS] = {0, 0, 0};
     int conv[MAXS] = {0, 0, 0}, outer[MAXS] = {0, 0, 0}, dead[MAXS] = {0, 0, 0};
     int iters_total[MAXS] = {0, 0, 0};
-    int prev_act[MAXS] = {0, 0, 0};  // species of the previous inner solve (none before round 1)
+    int prev_act[MAXS] = {0, 0, 0};  // species of the previous inner solve (none in the first sweep)
     const int max_rounds = method == DSV_METHOD_MF32_IR64 ? m->max_outer : 4;
     for (int round = 0;; ++round) {
         double rr[MAXS] = {0, 0, 0}, bb[MAXS] = {0, 0, 0};

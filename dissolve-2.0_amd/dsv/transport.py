@@ -11,7 +11,7 @@ Warm starts (no counterpart in the reference, whose solves start from the previo
 solution): "previous" starts every solve from u_n; "extrapolate" and "learned" start from the
 Galerkin warm start of libdsv (dsv_warm_start) over u_n + span{directions}, per species --
 the combination minimising the A-energy norm of the error, taken only where its FP64
-residual is below that of u_n (the previous solution stays the fallback and the judge).
+residual is below that of u_n (the previous solution stays the fallback and the yardstick).
 "extrapolate" uses the last history_dirs (default 6, at most 6) increments u_n - u_{n-1},
 u_{n-1} - u_{n-2}, ...; "learned" uses at most five of them plus the per-vertex MLP's
 prediction of the part of the next increment that extrapolation misses,

@@ -148,7 +148,7 @@ int dsv_solve(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_
 
 /* ||b_s - A_s x_s||_2 (rnorm, host array of n_species) and ||b_s||_2 (bnorm, may be NULL)
  * for x_s = sp[s].x with its Dirichlet entries replaced by bc_value -- the residual dsv_solve
- * starts from; FP64, matrix-free.  Used to judge candidate initial guesses. */
+ * starts from; FP64, matrix-free.  Used to compare candidate initial guesses. */
 int dsv_residual_norms(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
                        int32_t lumped, double* rnorm, double* bnorm, void* stream);
 

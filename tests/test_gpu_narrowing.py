@@ -38,13 +38,13 @@ def _case(seed=3):
     return conn, xyz, sps, u0, bs, exact, 1 / dt
 
 
-def _solve(gpu, mesh, sps, starts, bs, inv_dt, method, narrow):
+def _solve(gpu, mesh, sps, starts, bs, inv_dt, method, narrow, lumped=1):
     import torch
     from dsv.solvers import solve_batched
     t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
     mesh.set_species_narrowing(narrow)
     sp = [dict(s, b=t(b), x=t(u)) for s, u, b in zip(sps, starts, bs)]
-    st = solve_batched(mesh, sp, inv_dt, rtol=1e-10, method=method)
+    st = solve_batched(mesh, sp, inv_dt, rtol=1e-10, method=method, lumped=lumped)
     return [s["x"].cpu().numpy() for s in sp], st
 
 
@@ -96,6 +96,30 @@ def test_narrowing_before_the_first_iteration(gpu, method, done):
         assert on[1]["iterations"][s] == 0 and np.array_equal(on[0][s], off[0][s])
         assert np.abs(on[0][s] - exact[s]).max() <= 1e-13 * np.abs(exact[s]).max()
     _compare(on, off, exact, [s for s in range(3) if s not in done])
+
+
+def test_narrowing_with_consistent_mass(gpu):
+    """Consistent mass runs the recomputing generic kernels (per-cell D, no stored geometry):
+    narrowing hands over their coefficient arrays the same way."""
+    from dsv.core import Mesh
+    conn, xyz, sps, u0, _, _, inv_dt = _case(seed=17)
+    rng = np.random.default_rng(17)
+    V = len(xyz)
+    bs, exact = [], []
+    for s, u in zip(sps, u0):
+        D = s["D"] if "D" in s else s["D_value"]
+        al = s["alpha"] if "alpha" in s else s.get("alpha_value", 0.0)
+        ip, ix, data = OA.assemble_values(conn, xyz, D, inv_dt, al, 0, s.get("bc_dofs"))
+        b = OA.assemble_rhs(conn, xyz, u, inv_dt, rng.uniform(0, 1, V), 0, s.get("bc_dofs"),
+                            s.get("bc_value", 0.0))
+        bs.append(b)
+        exact.append(OS.direct(OA.to_scipy(ip, ix, data, V), b))
+    mesh = Mesh(conn, xyz)
+    mesh.set_solver_options(graph_chunk=2)
+    for method in ("mf32_ir64", "mf64"):
+        on = _solve(gpu, mesh, sps, u0, bs, inv_dt, method, True, lumped=0)
+        off = _solve(gpu, mesh, sps, u0, bs, inv_dt, method, False, lumped=0)
+        _compare(on, off, exact, range(3))
 
 
 def test_csr_path_never_narrows(gpu):
