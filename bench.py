@@ -401,7 +401,7 @@ def main():
                              "iteration, stored FP32 geometry) in FP64 refinement, FP64 "
                              f"Galerkin warm start over the last {a.history_dirs} solution "
                              "increments (previous solution as fallback), species narrowing, "
-                             "hipGraph chunks",
+                             "8-iteration polled chunks of direct launches",
                    "parallelism": "1 GPU", "launch_shape": tuned},
         "plain_fp64": {"steps_per_s": round(plain["steps_per_s"], 3),
                        "cg_iterations_per_step": its(plain)},

@@ -146,8 +146,8 @@ struct dsv_mesh {
     // solver options
     double inner_rtol = 1e-3;
     int max_outer = 12;
-    int graph_chunk = 16;
-    int use_graphs = 1;
+    int graph_chunk = 8;  // CG iterations per poll chunk
+    int use_graphs = 0;   // direct launches: on MI355X they outrun hipGraph replay here
     int operator_form = DSV_FORM_STORED;  // inner FP32 CG operator
     int op_reuse = 0;        // dsv_set_operator_reuse
     int narrow = 1;          // dsv_set_species_narrowing

@@ -39,6 +39,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <stdexcept>
 #include <type_traits>
 
@@ -76,6 +77,7 @@ struct CGState {
     int kk[2];             // MF: K_i -> i + 1; host sets kk[1] = 0
     int max_it;            // host
     int any_active;        // A_k / K_i (host polls)
+    int seq;               // poll snapshots only: set last, once the rest is written
 };
 
 // species-interleaved vertex record: the S values of one vertex, unpadded (12 B for three
@@ -543,11 +545,16 @@ k_mf(Geo g, const Vec4<T>* __restrict__ xyz, OpParams<T> prm, MFVec<T, S> v,
 // (r_0 = (T) rin for the solved species, 0 otherwise; x = p = s_{-1} = 0; u_0 = r_0 / diag;
 // w_0 = A u_0).  Partials per workgroup: gamma, delta, rho of the next iteration.
 // The state snapshot the host polls, written by workgroup 0 of a chunk's last iteration into
-// pinned host memory (visible to the host once the event recorded after the chunk completes:
-// events release to system scope)
+// coherent pinned host memory: the fields, a system-scope fence, then seq = 1, which the host
+// waits for (it clears seq before queuing the chunk).
 __device__ __forceinline__ void publish_state(const CGState* __restrict__ st,
                                               CGState* __restrict__ poll) {
-    if (poll) *poll = *st;
+    if (!poll) return;
+    CGState snap = *st;
+    snap.seq = 0;
+    *poll = snap;
+    __threadfence_system();
+    *reinterpret_cast<volatile int*>(&poll->seq) = 1;
 }
 
 template <class T, int S, bool FAST, bool INIT, bool STORED, bool PERSP = false>
@@ -1558,6 +1565,7 @@ struct Workspace {
     // captured CG chunk and the parameters it was captured with
     hipGraphExec_t graph[2] = {};       // per poll slot (the MF kernels publish into it)
     hipGraphExec_t graph_tail[2] = {};  // the short chunk used near convergence
+    hipGraphExec_t graph_tail_np = nullptr;  // the short chunk without the state publication
     int graph_chunk = 0;
     int graph_valid = 0;
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
@@ -1571,6 +1579,7 @@ static void free_ws(Workspace* w) {
         if (w->graph[i]) (void)hipGraphExecDestroy(w->graph[i]);
         if (w->graph_tail[i]) (void)hipGraphExecDestroy(w->graph_tail[i]);
     }
+    if (w->graph_tail_np) (void)hipGraphExecDestroy(w->graph_tail_np);
     dfree(w->st);
     if (w->h_st) (void)hipHostFree(w->h_st);
     if (w->h_pR) (void)hipHostFree(w->h_pR);
@@ -2138,25 +2147,29 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
     int launched = seed ? seed->launched : 0;
     if (m->use_graphs && (!w->graph_valid || w->graph_chunk != chunk ||
                           std::memcmp(w->graph_prm, w->prm, sizeof(w->prm)) != 0)) {
-        for (int slot = 0; slot < 2; ++slot)
-            for (hipGraphExec_t* ge : {&w->graph[slot], &w->graph_tail[slot]}) {
-                if (*ge) (void)hipGraphExecDestroy(*ge);
-                *ge = nullptr;
-            }
+        for (hipGraphExec_t* ge : {&w->graph[0], &w->graph_tail[0], &w->graph[1], &w->graph_tail[1],
+                                   &w->graph_tail_np}) {
+            if (*ge) (void)hipGraphExecDestroy(*ge);
+            *ge = nullptr;
+        }
         w->graph_valid = 0;
         // captured on the workspace's own stream, which has no work of its own queued (a
         // narrowed workspace is captured while the wider run's chunks are in flight on `stream`)
-        for (int slot = 0; slot < 2; ++slot)
+        // per poll slot a full and a short chunk whose last kernel publishes into that slot,
+        // and a short chunk that publishes nothing (leading parts of a longer short-chunk run)
+        for (int slot = 0; slot < 3; ++slot)
             for (int which = 0; which < 2; ++which) {
+                if (slot == 2 && which == 0) continue;
                 const int len = which ? tail : chunk;
+                CGState* poll = kpoll && slot < 2 ? &w->d_poll[slot] : nullptr;
                 hipGraph_t gr;
                 DSV_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
                 for (int i = 0; i < len; ++i)
-                    cg_iteration<T, S>(m, w, i & 1, w->stream,
-                                       kpoll && i == len - 1 ? &w->d_poll[slot] : nullptr);
+                    cg_iteration<T, S>(m, w, i & 1, w->stream, i == len - 1 ? poll : nullptr);
                 DSV_HIP(hipStreamEndCapture(w->stream, &gr));
-                DSV_HIP(hipGraphInstantiate(which ? &w->graph_tail[slot] : &w->graph[slot], gr,
-                                            nullptr, nullptr, 0));
+                hipGraphExec_t* ge = slot == 2 ? &w->graph_tail_np
+                                               : (which ? &w->graph_tail[slot] : &w->graph[slot]);
+                DSV_HIP(hipGraphInstantiate(ge, gr, nullptr, nullptr, 0));
                 DSV_HIP(hipGraphDestroy(gr));
             }
         w->graph_valid = 1;
@@ -2169,22 +2182,48 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
     // iterations remain, and queues short chunks once less than a full chunk is predicted, so
     // that few no-op iterations follow convergence.
     auto enqueue = [&](int slot, int len) {
+        if (kpoll) w->h_poll[slot].seq = 0;
         if (m->use_graphs) {
             if (len == chunk)
                 DSV_HIP(hipGraphLaunch(w->graph[slot], stream));
             else
-                for (int i = 0; i < len; i += tail) DSV_HIP(hipGraphLaunch(w->graph_tail[slot], stream));
+                for (int i = 0; i < len; i += tail)  // only the last part publishes
+                    DSV_HIP(hipGraphLaunch(i + tail < len ? w->graph_tail_np : w->graph_tail[slot],
+                                           stream));
         } else {
             for (int i = 0; i < len; ++i)
                 cg_iteration<T, S>(m, w, i & 1, stream,
                                    kpoll && i == len - 1 ? &w->d_poll[slot] : nullptr);
         }
         launched += len;
-        if (!kpoll)
+        if (!kpoll) {
             DSV_HIP(hipMemcpyAsync(&w->h_poll[slot], w->st, sizeof(CGState), hipMemcpyDeviceToHost,
                                    stream));
-        DSV_HIP(hipEventRecord(w->ev[slot], stream));
+            DSV_HIP(hipEventRecord(w->ev[slot], stream));
+        }
         return len;
+    };
+    // Waits for the chunk of `slot`: its event (CSR path), or the seq word its last kernel
+    // publishes (no event in the stream between chunks).  The spin checks the stream now and
+    // then, so that a chunk that ends without publishing is an error, not a hang.
+    auto wait_slot = [&](int slot) {
+        if (!kpoll) {
+            DSV_HIP(hipEventSynchronize(w->ev[slot]));
+            return;
+        }
+        volatile int* seq = &w->h_poll[slot].seq;
+        for (long i = 1;; ++i) {
+            if (*seq != 0) break;
+            if ((i & 1023) == 0) {
+                const hipError_t q = hipStreamQuery(stream);
+                if (q == hipSuccess) {
+                    if (*seq != 0) break;
+                    throw std::runtime_error("CG chunk ended without publishing its state");
+                }
+                if (q != hipErrorNotReady) DSV_HIP(q);
+            }
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
     };
     PollSeed h;  // `last`, the chunk end before it, and iterations queued after `last`
     if (seed) h = *seed;
@@ -2239,7 +2278,7 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
         // chunk k (in flight) runs len_q[cur] more iterations before chunk k + 1
         const int inflight = len_q[cur] + h.pending;
         len_q[nxt] = launched <= max_it + 2 * chunk ? enqueue(nxt, next_len(inflight)) : 0;
-        DSV_HIP(hipEventSynchronize(w->ev[cur]));
+        wait_slot(cur);
         if (h.have_last) {
             for (int s = 0; s < S; ++s) {
                 h.rr0[s] = h.last.rr[s];

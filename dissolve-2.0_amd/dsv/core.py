@@ -104,7 +104,10 @@ class Mesh:
                   int(lanes_per_vertex))
         self.refresh_info()
 
-    def set_solver_options(self, inner_rtol=1e-3, max_outer=12, graph_chunk=16, use_graphs=True):
+    def set_solver_options(self, inner_rtol=1e-3, max_outer=12, graph_chunk=8, use_graphs=False):
+        """Inner tolerance and sweep limit of mf32_ir64, CG iterations per polled chunk, and
+        whether chunks are replayed from captured hipGraphs (the defaults launch directly:
+        on the 806k-tet disc that is about 5% faster per step than graph replay)."""
         _lib.call("dsv_set_solver_options", self._h, float(inner_rtol), int(max_outer),
                   int(graph_chunk), int(bool(use_graphs)))
 

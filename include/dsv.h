@@ -186,7 +186,8 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
                    double* r_after);
 
 /* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
- * sweeps) and the number of CG iterations per captured hipGraph chunk. */
+ * sweeps), the number of CG iterations per polled chunk, and whether chunks are replayed from
+ * captured hipGraphs (use_graphs != 0) or launched directly.  Defaults: 1e-3, 12, 8, 0. */
 int dsv_set_solver_options(dsv_mesh* m, double inner_rtol, int32_t max_outer,
                            int32_t graph_chunk, int32_t use_graphs);
 
