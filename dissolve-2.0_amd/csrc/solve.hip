@@ -1547,6 +1547,9 @@ struct Workspace {
     OpKey wkey;
     double* pW = nullptr;    // partials of the warm start's dot and norm kernels
     double* pT = nullptr;    // their totals
+    double* gsave[2] = {};   // Gram matrices of the last call [gcur] and the one before
+    int gcur = 0;
+    int gnd = 0;             // directions of the saved one
     WarmOut* wout = nullptr;    // its coefficients and norms (device) ...
     WarmOut* h_wout = nullptr;  // ... and their pinned copy
     void* fw[2][MAXS] = {};  // stored form: per-species weights K(D_s) of sets 0 / 1
@@ -1601,6 +1604,8 @@ static void free_ws(Workspace* w) {
     for (int i = 0; i < 2 * MAXD; ++i) dfree(w->wd[i]);
     dfree(w->pW);
     dfree(w->pT);
+    dfree(w->gsave[0]);
+    dfree(w->gsave[1]);
     dfree(w->wout);
     if (w->h_wout) (void)hipHostFree(w->h_wout);
     for (int a = 0; a < 2; ++a)
@@ -2511,6 +2516,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     m->stat_launches = 0;
     m->stat_narrowings = 0;
     m->stat_dir_reused = 0;
+    m->stat_gram_reused = 0;
     const bool internal = method != DSV_METHOD_CSR64;
     const int32_t* perm = internal ? m->perm : nullptr;
     const unsigned gv = cdiv(m->V, 256);
@@ -2655,6 +2661,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         stats->cg_launches = m->stat_launches;
         stats->narrowings = m->stat_narrowings;
         stats->dirs_reused = m->stat_dir_reused;
+        stats->gram_reused = m->stat_gram_reused;
     }
 }
 
@@ -2772,6 +2779,57 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv
             pout[(s * NP + ND * (ND + 1) / 2 + j) * gridDim.x + blockIdx.x] =
                 tot[s * NR + gram_row<ND>(J1) - gram_row<ND>(J0) + (j - J0)];
     }
+}
+
+// Gram quantities when directions 2.. are the previous call's 1.. (shifted history): only
+// rows 0 and 1 of d_j . A d_k and every d_j . r0 are new, 3 ND - 1 per species, in the order
+// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0.  PART 0 computes the rows and
+// d_0 . r0, d_1 . r0; PART 1 the other d_j . r0 (two launches keep the accumulators in
+// registers).  Each sum runs in the same per-thread and per-workgroup order as in
+// k_proj_dots, so the values are the same bits.
+template <int S, int ND, int PART>
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots_shift(int V, ProjVec<S> pv,
+                                                                 double* __restrict__ pout) {
+    constexpr int NQ = 3 * ND - 1;
+    constexpr int NR = PART == 0 ? 2 * ND + 1 : ND - 2;  // quantities of this part
+    constexpr int Q0 = PART == 0 ? 0 : 2 * ND + 1;       // the first one's index in NQ
+    __shared__ double red[S * NR * 16];
+    double tot[S * NR];
+#pragma unroll
+    for (int q = 0; q < S * NR; ++q) tot[q] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        const SV<double, S> r = pv.r0[i];
+        if constexpr (PART == 0) {
+            const SV<double, S> d0 = pv.d[0][i], d1 = pv.d[1][i];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) {
+                const SV<double, S> Ad = pv.Ad[k][i];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    tot[s * NR + k] += d0.v[s] * Ad.v[s];
+                    if (k >= 1) tot[s * NR + ND + k - 1] += d1.v[s] * Ad.v[s];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                tot[s * NR + 2 * ND - 1] += d0.v[s] * r.v[s];
+                tot[s * NR + 2 * ND] += d1.v[s] * r.v[s];
+            }
+        } else {
+#pragma unroll
+            for (int j = 2; j < ND; ++j) {
+                const SV<double, S> dj = pv.d[j][i];
+#pragma unroll
+                for (int s = 0; s < S; ++s) tot[s * NR + j - 2] += dj.v[s] * r.v[s];
+            }
+        }
+    }
+    block_sum_all<S * NR>(tot, red);
+    if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) pout[(s * NQ + Q0 + q) * gridDim.x + blockIdx.x] = tot[s * NR + q];
 }
 
 template <int S, int ND>
@@ -2944,22 +3002,46 @@ struct WarmOut {
     double rr1[MAXS];   // ||r||^2 of the start taken
 };
 
-// One Gram solve per species (thread s), from the totals of the dot partials
-template <int S, int ND>
+// One Gram solve per species (thread s), from the totals of the dot partials: the full
+// quantities (k_proj_dots), or with SHIFT the new rows of k_proj_dots_shift and, for
+// j, k >= 2, the previous call's d_(j-1) . A d_(k-1) from gprev.  The Gram matrix is saved to
+// gout ([species][MAXD][MAXD]) for the next call.
+template <int S, int ND, bool SHIFT>
 __global__ __launch_bounds__(64) void k_proj_solve(const double* __restrict__ tot,
+                                                  const double* __restrict__ gprev,
+                                                  double* __restrict__ gout,
                                                   WarmOut* __restrict__ out) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
+    constexpr int NQ = 3 * ND - 1;
     if (threadIdx.x >= S) return;
     const int s = threadIdx.x;
-    const double* q = tot + s * NP;
     double G[ND * ND], g[ND], c[ND];
-    int t = 0;
+    if constexpr (SHIFT) {
+        const double* q = tot + s * NQ;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) G[k] = G[k * ND] = q[k];
+#pragma unroll
+        for (int k = 1; k < ND; ++k) G[ND + k] = G[k * ND + 1] = q[ND + k - 1];
+#pragma unroll
+        for (int j = 2; j < ND; ++j)
+#pragma unroll
+            for (int k = 2; k < ND; ++k) G[j * ND + k] = gprev[(s * MAXD + j - 1) * MAXD + k - 1];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) g[j] = q[2 * ND - 1 + j];
+    } else {
+        const double* q = tot + s * NP;
+        int t = 0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j)
+#pragma unroll
+            for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) g[j] = q[t++];
+    }
 #pragma unroll
     for (int j = 0; j < ND; ++j)
 #pragma unroll
-        for (int k = j; k < ND; ++k) G[j * ND + k] = G[k * ND + j] = q[t++];
-#pragma unroll
-    for (int j = 0; j < ND; ++j) g[j] = q[t++];
+        for (int k = 0; k < ND; ++k) gout[(s * MAXD + j) * MAXD + k] = G[j * ND + k];
     solve_gram<ND>(G, g, c, 1e-10);
 #pragma unroll
     for (int k = 0; k < ND; ++k) out->c[k][s] = c[k];
@@ -3155,10 +3237,35 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     pv.x = (SV<double, S>*)w->vb[VB_X64];
     pv.cdev = &w->wout->c[0][0];
     constexpr int NP = ND * (ND + 1) / 2 + ND;
-    launch_proj_dots<S, ND>(m, pv, w->pW, PROJ_GRID, st);
-    hipLaunchKernelGGL(k_sum_partials, dim3(S * NP), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID,
-                       w->pT);
-    hipLaunchKernelGGL((k_proj_solve<S, ND>), dim3(1), dim3(64), 0, st, w->pT, w->wout);
+    // the previous call's Gram matrix serves for directions 2.. when they are its 1..
+    for (int i = 0; i < 2; ++i)
+        if (!w->gsave[i]) w->gsave[i] = dalloc<double>((size_t)MAXS * MAXD * MAXD);
+    bool gshift = ND >= 3 && w->gnd >= ND - 1;
+    for (int k = 2; k < ND; ++k) gshift = gshift && reuse[k];
+    const double* gprev = w->gsave[w->gcur];
+    double* gout = w->gsave[w->gcur ^ 1];
+    if constexpr (ND >= 3) {
+        if (gshift) {
+            hipLaunchKernelGGL((k_proj_dots_shift<S, ND, 0>), dim3(PROJ_GRID), dim3(PROJ_THREADS),
+                               0, st, m->V, pv, w->pW);
+            hipLaunchKernelGGL((k_proj_dots_shift<S, ND, 1>), dim3(PROJ_GRID), dim3(PROJ_THREADS),
+                               0, st, m->V, pv, w->pW);
+            hipLaunchKernelGGL(k_sum_partials, dim3(S * (3 * ND - 1)), dim3(PROJ_THREADS), 0, st,
+                               w->pW, PROJ_GRID, w->pT);
+            hipLaunchKernelGGL((k_proj_solve<S, ND, true>), dim3(1), dim3(64), 0, st, w->pT,
+                               gprev, gout, w->wout);
+        }
+    }
+    if (!gshift) {
+        launch_proj_dots<S, ND>(m, pv, w->pW, PROJ_GRID, st);
+        hipLaunchKernelGGL(k_sum_partials, dim3(S * NP), dim3(PROJ_THREADS), 0, st, w->pW,
+                           PROJ_GRID, w->pT);
+        hipLaunchKernelGGL((k_proj_solve<S, ND, false>), dim3(1), dim3(64), 0, st, w->pT, gprev,
+                           gout, w->wout);
+    }
+    w->gcur ^= 1;
+    w->gnd = ND;
+    m->stat_gram_reused += gshift ? 1 : 0;
     hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st, m->V,
                        pv, w->pW);
     hipLaunchKernelGGL(k_sum_partials, dim3(S), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID_F,

@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
         ("converged", ctypes.c_int32 * 3), ("rel_residual", ctypes.c_double * 3),
         ("initial_rel_residual", ctypes.c_double * 3), ("solve_ms", ctypes.c_double),
         ("cg_launches", ctypes.c_int32), ("narrowings", ctypes.c_int32),
-        ("dirs_reused", ctypes.c_int32),
+        ("dirs_reused", ctypes.c_int32), ("gram_reused", ctypes.c_int32),
     ]
 
     def as_dict(self, n):
@@ -79,6 +79,7 @@ class Stats(ctypes.Structure):
             "initial_rel_residual": list(self.initial_rel_residual[:n]),
             "solve_ms": self.solve_ms, "cg_launches": self.cg_launches,
             "narrowings": self.narrowings, "dirs_reused": self.dirs_reused,
+            "gram_reused": self.gram_reused,
         }
 
 

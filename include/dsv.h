@@ -78,6 +78,8 @@ typedef struct dsv_stats {
     int32_t narrowings;          /* inner solves continued with fewer species */
     int32_t dirs_reused;         /* dsv_solve_warm: directions whose A d came from the
                                     previous call (history shifted by one step) */
+    int32_t gram_reused;         /* dsv_solve_warm: 1 when the Gram entries of those
+                                    directions came from the previous call too */
 } dsv_stats;
 
 const char* dsv_last_error(void);

@@ -51,7 +51,7 @@ def test_stepper_reuse_is_bit_identical_and_follows_refresh(gpu):
         st.set_warm_start("extrapolate")
         steppers.append(st)
     a, b = steppers
-    reused = []
+    reused, gram = [], []
     for n in range(9):
         if n == 5:
             for st in steppers:
@@ -60,12 +60,15 @@ def test_stepper_reuse_is_bit_identical_and_follows_refresh(gpu):
         assert ha["iterations"] == hb["iterations"], (n, ha, hb)
         for ua, ub in zip(a.u, b.u):
             assert np.array_equal(ua.cpu().numpy(), ub.cpu().numpy()), n
-        assert hb["dirs_reused"] == 0
+        assert hb["dirs_reused"] == 0 and hb["gram_reused"] == 0
         reused.append(ha["dirs_reused"])
+        gram.append(ha["gram_reused"])
     # the shifted history's A d_k (k >= 2) come from the previous step while the operator
     # stands: none on the first steps (too short a history) and right after the refresh
     assert reused[:3] == [0, 0, 0] and reused[5] == 0, reused
     assert reused[3] > 0 and reused[4] > 0 and reused[8] > 0, reused
+    # with them, the Gram entries of the shifted directions (bit-identical solutions above)
+    assert gram[3] == 1 and gram[8] == 1 and gram[5] == 0, gram
 
 
 def test_per_call_apis_never_reuse_a_stale_operator(gpu):
