@@ -7,7 +7,7 @@ untimed warm-up steps, then K timed steps):
               start (the reference's operator path and warm start)
   fp32     -- fused FP32 3-species Jacobi-PCG inside FP64 refinement (mf32_ir64), previous start
   chosen   -- the fp32 path started from the Galerkin warm start over the last H solution
-              increments (--history-dirs, default 6; previous solution as fallback)
+              increments (--history-dirs, default 8; previous solution as fallback)
   learned  -- the fp32 path started from the Galerkin warm start over H - 1 increments and the
               learned MLP direction, trained on the GPU on the W warm-up steps
 `value` is the chosen path's steps/s; the timed steps include every warm-start computation
@@ -83,7 +83,7 @@ def load_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, history_dirs=6):
+def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, history_dirs=8):
     import torch
     from dsv.transport import Stepper
     mlp = None
@@ -220,7 +220,7 @@ def kernels_at_scale(shapes=((256, 256, 1), (256, 256, 2))):
     return out
 
 
-def run_series(mesh, w, n=200, epochs=5, history_dirs=6):
+def run_series(mesh, w, n=200, epochs=5, history_dirs=8):
     """Receding-interface series: the previous-solution start, the Galerkin extrapolation and
     the learned start (trained on steps 1..n/2) over steps n/2+1..n; prints one JSON line."""
     import torch
@@ -280,8 +280,8 @@ def main():
     ap.add_argument("--no-scale", action="store_true",
                     help="skip the kernel timings on the 14.7M-tet gyroid lattice")
     ap.add_argument("--apply-reps", type=int, default=50)
-    ap.add_argument("--history-dirs", type=int, default=6,
-                    help="past solution increments in the Galerkin warm start (1..6)")
+    ap.add_argument("--history-dirs", type=int, default=8,
+                    help="past solution increments in the Galerkin warm start (1..8)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--series", action="store_true",
                     help="instead of the headline: the receding-interface time series on the "

@@ -54,7 +54,7 @@ void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped,
                    hipStream_t st);
 
 constexpr int MAXS = 3;
-constexpr int MAXD = 6;  // warm-start directions of dsv_solve_warm
+constexpr int MAXD = 8;  // warm-start directions of dsv_solve_warm
 constexpr int MAX_PROD = 2048;  // max workgroups of a partial-producing kernel
 
 enum MFMode { MF_APPLY = 0, MF_DIAG = 1, MF_RESID = 3, MF_MASS = 4 };
@@ -2843,9 +2843,15 @@ static void launch_proj_dots(const dsv_mesh* m, const ProjVec<S>& pv, double* po
                            m->V, pv, pout);
         hipLaunchKernelGGL((k_proj_dots<S, ND, 2, 4>), dim3(grid), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
-        if constexpr (ND > 4)
+        if constexpr (ND > 4 && ND <= 6)
             hipLaunchKernelGGL((k_proj_dots<S, ND, 4, ND>), dim3(grid), dim3(PROJ_THREADS), 0,
                                st, m->V, pv, pout);
+        if constexpr (ND > 6) {
+            hipLaunchKernelGGL((k_proj_dots<S, ND, 4, 6>), dim3(grid), dim3(PROJ_THREADS), 0,
+                               st, m->V, pv, pout);
+            hipLaunchKernelGGL((k_proj_dots<S, ND, 6, ND>), dim3(grid), dim3(PROJ_THREADS), 0,
+                               st, m->V, pv, pout);
+        }
     }
 }
 
@@ -3298,7 +3304,9 @@ static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const W
         case 3: fused_warm_nd<S, 3>(m, w, sp, wa, nres, rr, bb, st); break;
         case 4: fused_warm_nd<S, 4>(m, w, sp, wa, nres, rr, bb, st); break;
         case 5: fused_warm_nd<S, 5>(m, w, sp, wa, nres, rr, bb, st); break;
-        default: fused_warm_nd<S, 6>(m, w, sp, wa, nres, rr, bb, st); break;
+        case 6: fused_warm_nd<S, 6>(m, w, sp, wa, nres, rr, bb, st); break;
+        case 7: fused_warm_nd<S, 7>(m, w, sp, wa, nres, rr, bb, st); break;
+        default: fused_warm_nd<S, 8>(m, w, sp, wa, nres, rr, bb, st); break;
     }
 }
 
@@ -3462,7 +3470,7 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
         if (max_it < 1) throw ArgError{"max_it must be >= 1"};
         if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > MAXD)
-            throw ArgError{"dsv_solve_warm: n_hist + n_extra must be in [0, 6]"};
+            throw ArgError{"dsv_solve_warm: n_hist + n_extra must be in [0, 8]"};
         if ((n_hist > 0 && !hist) || (n_extra > 0 && !extra))
             throw ArgError{"dsv_solve_warm: NULL hist or extra"};
         check_solve_args(m, n_species, sp, true);

@@ -12,8 +12,8 @@ solution): "previous" starts every solve from u_n; "extrapolate" and "learned" s
 Galerkin warm start of libdsv (dsv_warm_start) over u_n + span{directions}, per species --
 the combination minimising the A-energy norm of the error, taken only where its FP64
 residual is below that of u_n (the previous solution stays the fallback and the yardstick).
-"extrapolate" uses the last history_dirs (default 6, at most 6) increments u_n - u_{n-1},
-u_{n-1} - u_{n-2}, ...; "learned" uses at most five of them plus the per-vertex MLP's
+"extrapolate" uses the last history_dirs (default 8, at most 8) increments u_n - u_{n-1},
+u_{n-1} - u_{n-2}, ...; "learned" uses at most seven of them plus the per-vertex MLP's
 prediction of the part of the next increment that extrapolation misses,
 (u_{n+1} - 2 u_n + u_{n-1}) / (sigma tau), sigma = max |u_n - u_{n-1}| and tau the training
 targets' RMS.  `train_warm_start(n)` runs n steps collecting (features, target) pairs, fits
@@ -32,7 +32,7 @@ from .core import Mesh, ptr, species_array, species_struct, stream_ptr, to_devic
 
 
 class Stepper:
-    MAX_HIST = 6  # past solutions kept (libdsv dsv_solve_warm takes up to 6 directions)
+    MAX_HIST = 8  # past solutions kept (libdsv dsv_solve_warm takes up to 8 directions)
 
     def __init__(self, mesh: Mesh, species, dt, method="mf32_ir64", rtol=1e-8, max_it=20000,
                  lumped=1, phi=None, grad_phi=None, eps=None, mlp=None, reuse_operator=True):

@@ -173,7 +173,7 @@ int dsv_warm_start(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
  * the differences of past solutions -- hist[j*n_species + s] is species s at step n-1-j, the
  * start sp[s].x is step n, so d_0 = x - hist_0, d_j = hist_(j-1) - hist_j (n_hist of them) --
  * followed by n_extra explicit directions extra[k*n_species + s] (NULL = zero);
- * n_hist + n_extra <= 6.  While operator reuse is on (dsv_set_operator_reuse), a history
+ * n_hist + n_extra <= 8.  While operator reuse is on (dsv_set_operator_reuse), a history
  * direction k >= 2 formed from the same two arrays as the previous call's direction k - 1,
  * with the same operator, takes its A d from that call (the reuse contract keeps the arrays
  * unchanged): a caller that shifts its history by one step pays one apply per call for d_0 and

@@ -178,9 +178,9 @@ def test_solve_warm_matches_separate_warm_start_and_solve(gpu, method):
                   None, None, None)
 
 
-@pytest.mark.parametrize("n_hist,n_extra", [(6, 0), (5, 1), (4, 0)])
+@pytest.mark.parametrize("n_hist,n_extra", [(8, 0), (7, 1), (6, 0), (5, 1), (4, 0)])
 def test_solve_warm_many_directions_match_oracle(gpu, n_hist, n_extra):
-    """dsv_solve_warm with up to six directions (past solutions and explicit ones): the
+    """dsv_solve_warm with up to eight directions (past solutions and explicit ones): the
     coefficients and both residual norms of its Galerkin start against the oracle's
     galerkin_start over the same directions (the start's FP64 projection is the oracle's;
     bars as above), and the solve converges to the direct solution."""
