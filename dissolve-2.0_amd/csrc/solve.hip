@@ -28,10 +28,11 @@
 // bit-identical to the owner's), updates p_i = u_i + beta p_{i-1} and x_{i+1} = x_i + alpha p_i
 // for the owned ones, and applies w_{i+1} = A u_{i+1}, writing the partials of iteration i+1.
 // r, w and s alternate between two buffers so that halo reads never see this kernel's writes.
-// No kernel uses a fence or a global atomic: every workgroup reduces ALL partials of the
-// previous kernel in the same fixed order, so all derive identical alpha / beta / convergence
-// decisions; workgroup 0 records them for the host.  Chunks of iterations are captured in a
-// hipGraph and replayed until every species is done.
+// No kernel uses a global atomic, and the iterations need no fence: every workgroup reduces
+// ALL partials of the previous kernel in the same fixed order, so all derive identical
+// alpha / beta / convergence decisions; workgroup 0 records them, and the last kernel of a
+// chunk publishes them to the host (one system-scope fence, one thread).  Chunks of
+// iterations are launched (or replayed from captured hipGraphs) until every species is done.
 //
 // CSR CG (method CSR64, the plain FP64 path): standard Jacobi-PCG with two kernels per
 // iteration, the CSR apply (A_k) and the vector update (U_k), with the same fence-free
@@ -2138,11 +2139,11 @@ static bool narrow_mid(dsv_mesh* m, Workspace* w, const int* sel, int max_it, hi
     return true;
 }
 
-// Runs CG iterations until no species is active.  Chunks of an even number of iterations
-// are captured once into a hipGraph (re-captured when the operator parameters change) and
-// replayed; the host polls one state word per chunk.  Once some species have finished and the
-// rest are predicted to need NARROW_MIN or more iterations, the rest continue with the kernel
-// of fewer species (narrow_mid).
+// Runs CG iterations until no species is active.  Chunks of an even number of iterations are
+// launched directly or, with use_graphs, replayed from hipGraphs captured once (re-captured
+// when the operator parameters change); the host polls one published state per chunk.  Once
+// some species have finished and the rest are predicted to need NARROW_MIN or more iterations,
+// the rest continue with the kernel of fewer species (narrow_mid).
 template <class T, int S>
 static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, const PollSeed* seed) {
     const int chunk = std::max(2, m->graph_chunk & ~1);
