@@ -409,10 +409,14 @@ def main():
                                 "cg_iterations_per_step": its(fp32)},
         "chosen_galerkin_extrapolation": {"steps_per_s": round(chosen["steps_per_s"], 3),
                                           "cg_iterations_per_step": its(chosen),
-                                          "cg_iterations_saved_pct_vs_previous_start": saved(chosen)},
+                                          "cg_iterations_saved_pct_vs_previous_start": saved(chosen),
+                                          "max_rel_diff_vs_fp32_previous_start":
+                                              chosen.get("max_rel_diff_vs_fp32_previous_start")},
         "learned_warm_start": {"steps_per_s": round(learned["steps_per_s"], 3),
                                "cg_iterations_per_step": its(learned),
                                "cg_iterations_saved_pct_vs_previous_start": saved(learned),
+                               "max_rel_diff_vs_fp32_previous_start":
+                                   learned.get("max_rel_diff_vs_fp32_previous_start"),
                                "training": learned["train"],
                                "mlp_coef_last_step": learned["coef_last_step"][-1]
                                if learned["coef_last_step"] else None,
