@@ -31,6 +31,25 @@ def test_loader_declares_signatures_and_last_error_is_empty():
     assert lib.dsv_last_error() == b""
 
 
+def test_setters_reject_a_null_mesh():
+    """The option setters validate their mesh before touching any device state."""
+    lib = _lib.load()
+    for name, args in (("dsv_set_species_narrowing", (None, 1)),
+                       ("dsv_set_operator_reuse", (None, 1)),
+                       ("dsv_set_operator_form", (None, 1)),
+                       ("dsv_set_solver_options", (None, 1e-3, 12, 8, 0))):
+        assert getattr(lib, name)(*args) != 0, name
+        assert b"NULL mesh" in lib.dsv_last_error(), name
+
+
+def test_stats_struct_matches_the_header():
+    """dsv_stats as the loader declares it: every field of include/dsv.h, in order."""
+    txt = open(os.path.join(ROOT, "include", "dsv.h")).read()
+    body = txt[txt.index("typedef struct dsv_stats {"):txt.index("} dsv_stats;")]
+    fields = re.findall(r"\b(?:int32_t|double)\s+([a-z_]+)\s*(?:\[\d+\])?;", body)
+    assert fields == [f[0] for f in _lib.Stats._fields_]
+
+
 def test_mesh_without_gpu_fails_loudly():
     import torch
     if torch.cuda.is_available():
