@@ -2698,8 +2698,8 @@ static void residual_norms_impl(dsv_mesh* m, const dsv_species* sp, double inv_d
 
 constexpr int PROJ_THREADS = 256;
 constexpr int PROJ_GRID_F = 1024;  // workgroups of dsv_solve_warm's norm and update kernels (the
-                                  // dot kernels, whose epilogue writes 13-39 partials per
-                                  // workgroup, run PROJ_GRID)
+                                  // dot kernels, whose epilogue writes up to 44 partials per
+                                  // workgroup, run PROJ_GRID per species)
 constexpr int PROJ_GRID = 256;  // dsv_warm_start's S * 9 * PROJ_GRID partials (3 directions at
                                 // most) fit the residual partial buffer
 static_assert(MAXS * 9 * PROJ_GRID <= 2 * MAXS * MAX_PROD, "projection partials");
@@ -2734,125 +2734,94 @@ constexpr int gram_rows(int j0, int j1) {  // quantities of rows [j0, j1): their
     return gram_row<ND>(j1) - gram_row<ND>(j0) + (j1 - j0);
 }
 
-// partials per species of the Gram rows [J0, J1): d_j . A d_k (k >= j) and d_j . r0, at the
-// species' quantity index (s * NP + q) * gridDim + block.  All species in one pass; with six
-// directions the rows go in three launches so that the accumulators stay in registers.
+// Partials of the Gram rows [J0, J1) of species blockIdx.y: d_j . A d_k (k >= j) and
+// d_j . r0, at the species' quantity index (s * NP + q) * gridDim.x + blockIdx.x.  One species
+// per workgroup row keeps the accumulators of eight directions in registers (the three
+// species of a vertex column run on the same XCD, so its records are read from HBM once), and
+// every quantity's sum has the same order whichever rows a launch covers.
 template <int S, int ND, int J0, int J1>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv,
                                                            double* __restrict__ pout) {
     constexpr int NP = ND * (ND + 1) / 2 + ND;
     constexpr int NR = gram_rows<ND>(J0, J1);
-    __shared__ double red[S * NR * 16];
-    double tot[S * NR];
+    constexpr int NG = gram_row<ND>(J1) - gram_row<ND>(J0);  // of them d_j . A d_k
+    __shared__ double red[NR * 16];
+    const int s = blockIdx.y;
+    double tot[NR];
 #pragma unroll
-    for (int q = 0; q < S * NR; ++q) tot[q] = 0.0;
+    for (int q = 0; q < NR; ++q) tot[q] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-        SV<double, S> d[J1 - J0];
+        double d[J1 - J0];
 #pragma unroll
-        for (int j = J0; j < J1; ++j) d[j - J0] = pv.d[j][i];
+        for (int j = J0; j < J1; ++j) d[j - J0] = pv.d[j][i].v[s];
 #pragma unroll
         for (int k = J0; k < ND; ++k) {
-            const SV<double, S> Ad = pv.Ad[k][i];
+            const double Ad = pv.Ad[k][i].v[s];
 #pragma unroll
             for (int j = J0; j < J1; ++j)
-                if (j <= k)
-#pragma unroll
-                    for (int s = 0; s < S; ++s)
-                        tot[s * NR + gram_row<ND>(j) - gram_row<ND>(J0) + (k - j)] +=
-                            d[j - J0].v[s] * Ad.v[s];
+                if (j <= k) tot[gram_row<ND>(j) - gram_row<ND>(J0) + (k - j)] += d[j - J0] * Ad;
         }
-        const SV<double, S> r = pv.r0[i];
+        const double r = pv.r0[i].v[s];
 #pragma unroll
-        for (int j = J0; j < J1; ++j)
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                tot[s * NR + gram_row<ND>(J1) - gram_row<ND>(J0) + (j - J0)] += d[j - J0].v[s] * r.v[s];
+        for (int j = J0; j < J1; ++j) tot[NG + (j - J0)] += d[j - J0] * r;
     }
-    block_sum_all<S * NR>(tot, red);
+    block_sum_all<NR>(tot, red);
     if (threadIdx.x != 0) return;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
+    for (int q = 0; q < NG; ++q)
+        pout[(s * NP + gram_row<ND>(J0) + q) * gridDim.x + blockIdx.x] = tot[q];
 #pragma unroll
-        for (int q = 0; q < gram_row<ND>(J1) - gram_row<ND>(J0); ++q)
-            pout[(s * NP + gram_row<ND>(J0) + q) * gridDim.x + blockIdx.x] = tot[s * NR + q];
-#pragma unroll
-        for (int j = J0; j < J1; ++j)
-            pout[(s * NP + ND * (ND + 1) / 2 + j) * gridDim.x + blockIdx.x] =
-                tot[s * NR + gram_row<ND>(J1) - gram_row<ND>(J0) + (j - J0)];
-    }
+    for (int j = J0; j < J1; ++j)
+        pout[(s * NP + ND * (ND + 1) / 2 + j) * gridDim.x + blockIdx.x] = tot[NG + (j - J0)];
 }
 
 // Gram quantities when directions 2.. are the previous call's 1.. (shifted history): only
 // rows 0 and 1 of d_j . A d_k and every d_j . r0 are new, 3 ND - 1 per species, in the order
-// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0.  PART 0 computes the rows and
-// d_0 . r0, d_1 . r0; PART 1 the other d_j . r0 (two launches keep the accumulators in
-// registers).  Each sum runs in the same per-thread and per-workgroup order as in
-// k_proj_dots, so the values are the same bits.
-template <int S, int ND, int PART>
+// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0; species blockIdx.y.  Each sum
+// runs in the same per-thread and per-workgroup order as in k_proj_dots, so the values are
+// the same bits.
+template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots_shift(int V, ProjVec<S> pv,
                                                                  double* __restrict__ pout) {
     constexpr int NQ = 3 * ND - 1;
-    constexpr int NR = PART == 0 ? 2 * ND + 1 : ND - 2;  // quantities of this part
-    constexpr int Q0 = PART == 0 ? 0 : 2 * ND + 1;       // the first one's index in NQ
-    __shared__ double red[S * NR * 16];
-    double tot[S * NR];
+    __shared__ double red[NQ * 16];
+    const int s = blockIdx.y;
+    double tot[NQ];
 #pragma unroll
-    for (int q = 0; q < S * NR; ++q) tot[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-        const SV<double, S> r = pv.r0[i];
-        if constexpr (PART == 0) {
-            const SV<double, S> d0 = pv.d[0][i], d1 = pv.d[1][i];
+        const double r = pv.r0[i].v[s];
+        const double d0 = pv.d[0][i].v[s], d1 = pv.d[1][i].v[s];
 #pragma unroll
-            for (int k = 0; k < ND; ++k) {
-                const SV<double, S> Ad = pv.Ad[k][i];
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    tot[s * NR + k] += d0.v[s] * Ad.v[s];
-                    if (k >= 1) tot[s * NR + ND + k - 1] += d1.v[s] * Ad.v[s];
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                tot[s * NR + 2 * ND - 1] += d0.v[s] * r.v[s];
-                tot[s * NR + 2 * ND] += d1.v[s] * r.v[s];
-            }
-        } else {
-#pragma unroll
-            for (int j = 2; j < ND; ++j) {
-                const SV<double, S> dj = pv.d[j][i];
-#pragma unroll
-                for (int s = 0; s < S; ++s) tot[s * NR + j - 2] += dj.v[s] * r.v[s];
-            }
+        for (int k = 0; k < ND; ++k) {
+            const double Ad = pv.Ad[k][i].v[s];
+            tot[k] += d0 * Ad;
+            if (k >= 1) tot[ND + k - 1] += d1 * Ad;
         }
+        tot[2 * ND - 1] += d0 * r;
+        tot[2 * ND] += d1 * r;
+#pragma unroll
+        for (int j = 2; j < ND; ++j) tot[2 * ND + 1 + j - 2] += pv.d[j][i].v[s] * r;
     }
-    block_sum_all<S * NR>(tot, red);
+    block_sum_all<NQ>(tot, red);
     if (threadIdx.x != 0) return;
 #pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int q = 0; q < NR; ++q) pout[(s * NQ + Q0 + q) * gridDim.x + blockIdx.x] = tot[s * NR + q];
+    for (int q = 0; q < NQ; ++q) pout[(s * NQ + q) * gridDim.x + blockIdx.x] = tot[q];
 }
 
+// the Gram rows of ND directions (grid x S workgroups): one launch up to six directions, rows
+// [0, 4) and [4, ND) beyond, so that the accumulators stay in registers
 template <int S, int ND>
 static void launch_proj_dots(const dsv_mesh* m, const ProjVec<S>& pv, double* pout, int grid,
                              hipStream_t st) {
-    if constexpr (ND <= 3) {
-        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, ND>), dim3(grid), dim3(PROJ_THREADS), 0, st,
+    if constexpr (ND <= 6) {
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, ND>), dim3(grid, S), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
     } else {
-        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, 2>), dim3(grid), dim3(PROJ_THREADS), 0, st,
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 0, 4>), dim3(grid, S), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
-        hipLaunchKernelGGL((k_proj_dots<S, ND, 2, 4>), dim3(grid), dim3(PROJ_THREADS), 0, st,
+        hipLaunchKernelGGL((k_proj_dots<S, ND, 4, ND>), dim3(grid, S), dim3(PROJ_THREADS), 0, st,
                            m->V, pv, pout);
-        if constexpr (ND > 4 && ND <= 6)
-            hipLaunchKernelGGL((k_proj_dots<S, ND, 4, ND>), dim3(grid), dim3(PROJ_THREADS), 0,
-                               st, m->V, pv, pout);
-        if constexpr (ND > 6) {
-            hipLaunchKernelGGL((k_proj_dots<S, ND, 4, 6>), dim3(grid), dim3(PROJ_THREADS), 0,
-                               st, m->V, pv, pout);
-            hipLaunchKernelGGL((k_proj_dots<S, ND, 6, ND>), dim3(grid), dim3(PROJ_THREADS), 0,
-                               st, m->V, pv, pout);
-        }
     }
 }
 
@@ -3253,10 +3222,8 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     double* gout = w->gsave[w->gcur ^ 1];
     if constexpr (ND >= 3) {
         if (gshift) {
-            hipLaunchKernelGGL((k_proj_dots_shift<S, ND, 0>), dim3(PROJ_GRID), dim3(PROJ_THREADS),
-                               0, st, m->V, pv, w->pW);
-            hipLaunchKernelGGL((k_proj_dots_shift<S, ND, 1>), dim3(PROJ_GRID), dim3(PROJ_THREADS),
-                               0, st, m->V, pv, w->pW);
+            hipLaunchKernelGGL((k_proj_dots_shift<S, ND>), dim3(PROJ_GRID, S),
+                               dim3(PROJ_THREADS), 0, st, m->V, pv, w->pW);
             hipLaunchKernelGGL(k_sum_partials, dim3(S * (3 * ND - 1)), dim3(PROJ_THREADS), 0, st,
                                w->pW, PROJ_GRID, w->pT);
             hipLaunchKernelGGL((k_proj_solve<S, ND, true>), dim3(1), dim3(64), 0, st, w->pT,
