@@ -1546,6 +1546,7 @@ struct Workspace {
     int wnhist = 0;
     int wvalid = 0;
     OpKey wkey;
+    const double* wout_prev[MAXS] = {};  // the solution arrays the last solve wrote
     double* pW = nullptr;    // partials of the warm start's dot and norm kernels
     double* pT = nullptr;    // their totals
     double* gsave[2] = {};   // Gram matrices of the last call [gcur] and the one before
@@ -2493,6 +2494,7 @@ static void inner_cg(dsv_mesh* m, Workspace* w, const double* tol2, const int* a
 struct WarmArgs {
     int n_hist = 0;
     const double* const* hist = nullptr;
+    bool from_hist = false;  // x0 = hist_0, d_k = hist_k - hist_(k+1) (n_hist + 1 arrays)
     int n_extra = 0;
     const double* const* extra = nullptr;
     double* coef = nullptr;
@@ -2502,6 +2504,17 @@ struct WarmArgs {
 template <int S>
 static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
                        int nres, double* rr, double* bb, hipStream_t st);
+
+// The solution arrays a solve is about to write: the warm-start directions of another
+// workspace may have been formed from them, and this workspace's next call must not take a
+// direction formed from them from this call (their contents change).
+static void note_outputs(dsv_mesh* m, Workspace* w, double* const* out) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 4; ++b)
+            if (m->ws[a][b] && m->ws[a][b] != w) m->ws[a][b]->wvalid = 0;
+    if (w)
+        for (int s = 0; s < MAXS; ++s) w->wout_prev[s] = out[s];
+}
 
 // Runs on the workspace's stream, ordered after the work queued on `caller` and, at return,
 // ahead of any later work on `caller` (events; the host waits only for the residual norms).
@@ -2523,6 +2536,8 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     const unsigned gv = cdiv(m->V, 256);
     const double* bs[3] = {sp[0].b, S > 1 ? sp[1].b : nullptr, S > 2 ? sp[2].b : nullptr};
     const double* xs[3] = {sp[0].x, S > 1 ? sp[1].x : nullptr, S > 2 ? sp[2].x : nullptr};
+    if (warm && warm->from_hist)
+        for (int s = 0; s < S; ++s) xs[s] = warm->hist[s];  // the start is step n of the history
     SV<double, S>* x64 = (SV<double, S>*)w->vb[VB_X64];
     hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
                        bs[2], (SV<double, S>*)w->vb[VB_B64]);
@@ -2644,6 +2659,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm,
                        (const SV<double, S>*)x64, xo[0], xo[1], xo[2]);
     DSV_HIP(hipGetLastError());
+    note_outputs(m, w, xo);
     // the solution is ordered on the caller's stream (no host wait for the unpack)
     DSV_HIP(hipEventRecord(w->ev_io[1], st));
     DSV_HIP(hipStreamWaitEvent(caller, w->ev_io[1], 0));
@@ -3117,6 +3133,7 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
             for (int k = 0; k < ND; ++k) coef[k * S + s] = take ? pv.c[k][s] : 0.0;
     }
     if (out[0] || out[1] || out[2]) {
+        note_outputs(m, nullptr, out);
         hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
                            (const SV<double, S>*)pv.x, out[0], out[1], out[2]);
         DSV_HIP(hipGetLastError());
@@ -3137,15 +3154,21 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     if (!w->pT) w->pT = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD));
     if (!w->h_wout) DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocDefault));
     // Direction k is a[s] - b[s] per species: d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
-    // (history), then the explicit ones.  Under operator reuse (whose contract keeps the arrays
-    // named unchanged), a history direction k >= 2 whose pair of arrays is the previous call's
-    // pair k - 1 is that call's direction, and its A d_k is taken from there: with the same FP64
-    // operator, one apply per call for a history shifted by one step.
+    // (history; from_hist: d_k = hist_k - hist_(k+1), x = hist_0), then the explicit ones.
+    // Under operator reuse (whose contract keeps the arrays named unchanged), a history
+    // direction k >= 1 whose pair of arrays is the previous call's pair k - 1 is that call's
+    // direction, and its A d_k is taken from there -- unless one of the two arrays is a
+    // solution the last solve wrote (as the previous start x is when the start is not taken
+    // from the history).  With the same FP64 operator and a history shifted by one step, that
+    // leaves one apply per call (two with the start given in x).
     const double* a[MAXD][MAXS] = {};
     const double* b[MAXD][MAXS] = {};
     for (int k = 0; k < ND; ++k)
         for (int s = 0; s < S; ++s) {
-            if (k < wa.n_hist) {
+            if (k < wa.n_hist && wa.from_hist) {
+                a[k][s] = wa.hist[(size_t)k * S + s];
+                b[k][s] = wa.hist[(size_t)(k + 1) * S + s];
+            } else if (k < wa.n_hist) {
                 a[k][s] = k == 0 ? sp[s].x : wa.hist[(size_t)(k - 1) * S + s];
                 b[k][s] = wa.hist[(size_t)k * S + s];
             } else {
@@ -3154,13 +3177,19 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
         }
     const OpKey& opkey = w->key[w->method == DSV_METHOD_MF32_IR64 ? 1 : 0];
     const bool can = m->op_reuse && w->wvalid && same_key(opkey, w->wkey);
+    auto written = [&](const double* p) {
+        for (int s = 0; s < MAXS; ++s)
+            if (p && p == w->wout_prev[s]) return true;
+        return false;
+    };
     int map[MAXD];  // direction -> buffer pair (wd[2p], wd[2p + 1])
     bool reuse[MAXD] = {}, busy[MAXD] = {};
-    for (int k = 2; k < ND && can; ++k) {
+    for (int k = 1; k < ND && can; ++k) {
         if (k >= wa.n_hist || k - 1 >= w->wnhist) continue;
         bool same = true;
         for (int s = 0; s < S; ++s)
-            same = same && a[k][s] == w->wpair[k - 1][s][0] && b[k][s] == w->wpair[k - 1][s][1];
+            same = same && a[k][s] == w->wpair[k - 1][s][0] && b[k][s] == w->wpair[k - 1][s][1] &&
+                   !written(a[k][s]) && !written(b[k][s]);
         if (!same) continue;
         reuse[k] = true;
         map[k] = w->wmap[k - 1];
@@ -3449,6 +3478,46 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         WarmArgs wa;
         wa.n_hist = n_hist;
         wa.hist = hist;
+        wa.n_extra = n_extra;
+        wa.extra = extra;
+        wa.coef = coef;
+        wa.r_before = r_before;
+        wa.r_after = r_after;
+        switch (n_species) {
+            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+        }
+    });
+}
+
+int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                           int32_t lumped, int32_t method, double rtol, int32_t max_it,
+                           int32_t n_hist, const double* const* hist, int32_t n_extra,
+                           const double* const* extra, void* stream, dsv_stats* stats,
+                           double* coef, double* r_before, double* r_after) {
+    return guard([&] {
+        if (method != DSV_METHOD_MF32_IR64 && method != DSV_METHOD_MF64)
+            throw ArgError{"dsv_solve_warm_history: method must be matrix-free (MF32_IR64 or MF64)"};
+        if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
+        if (max_it < 1) throw ArgError{"max_it must be >= 1"};
+        if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > MAXD)
+            throw ArgError{"dsv_solve_warm_history: n_hist + n_extra must be in [0, 8]"};
+        if (!hist || (n_extra > 0 && !extra))
+            throw ArgError{"dsv_solve_warm_history: NULL hist or extra"};
+        check_solve_args(m, n_species, sp, true);
+        for (int i = 0; i < (n_hist + 1) * n_species; ++i)
+            if (!hist[i]) throw ArgError{"dsv_solve_warm_history: NULL history vector"};
+        for (int s = 0; s < n_species; ++s)
+            for (int i = 0; i < (n_hist + 1) * n_species; ++i)
+                if (hist[i] == sp[s].x)
+                    throw ArgError{"dsv_solve_warm_history: a solution array is also in the history"};
+        DSV_HIP(hipSetDevice(m->device));
+        const hipStream_t cs = (hipStream_t)stream;
+        WarmArgs wa;
+        wa.n_hist = n_hist;
+        wa.hist = hist;
+        wa.from_hist = true;
         wa.n_extra = n_extra;
         wa.extra = extra;
         wa.coef = coef;

@@ -30,7 +30,8 @@ EXPORTS = (
     "dsv_apply", "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form",
     "dsv_set_species_narrowing",
     "dsv_set_operator_reuse", "dsv_time_apply",
-    "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_mlp_features",
+    "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_solve_warm_history",
+    "dsv_mlp_features",
     "dsv_mlp_forward",
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam", "dsv_nodal_gradient",
     "dsv_transport_coefficients",
@@ -130,6 +131,11 @@ def load():
                                           I32, ctypes.POINTER(P), I32, ctypes.POINTER(P), P,
                                           ctypes.POINTER(Stats), ctypes.POINTER(D),
                                           ctypes.POINTER(D), ctypes.POINTER(D)]),
+        "dsv_solve_warm_history": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, D,
+                                                  I32, I32, ctypes.POINTER(P), I32,
+                                                  ctypes.POINTER(P), P, ctypes.POINTER(Stats),
+                                                  ctypes.POINTER(D), ctypes.POINTER(D),
+                                                  ctypes.POINTER(D)]),
         "dsv_nodal_gradient": (ctypes.c_int, [P, P, P, P, P]),
         "dsv_transport_coefficients": (ctypes.c_int, [P, P, P, P, P, P, ctypes.POINTER(CoefParams),
                                                       P, P, P, P, P]),

@@ -9,7 +9,8 @@ re-assembles the FP64 CSR of every species each step (the reference's operator p
 
 Warm starts (no counterpart in the reference, whose solves start from the previous
 solution): "previous" starts every solve from u_n; "extrapolate" and "learned" start from the
-Galerkin warm start of libdsv (dsv_warm_start) over u_n + span{directions}, per species --
+Galerkin warm start of libdsv over u_n + span{directions}, folded into the solve
+(dsv_solve_warm_history, which reads u_n and the past solutions in place), per species --
 the combination minimising the A-energy norm of the error, taken only where its FP64
 residual is below that of u_n (the previous solution stays the fallback and the yardstick).
 "extrapolate" uses the last history_dirs (default 8, at most 8) increments u_n - u_{n-1},
@@ -76,6 +77,9 @@ class Stepper:
             self.bc_mask.append(mask)
         self.warm = "previous"
         self.history_dirs = self.MAX_HIST
+        # the Galerkin solves read the start from the history arrays (dsv_solve_warm_history):
+        # no copy of u_n, and every past direction's A d comes from the previous step
+        self.start_from_history = True
         self.collect = False
         self.pairs = []
         self.tau = 1.0
@@ -202,22 +206,28 @@ class Stepper:
         learned = self.warm == "learned" and self.n_steps >= 1
         if self.mlp is not None and (self.collect or learned) and self.n_steps >= 1:
             feats = [self._features(k) for k in range(self.S)]
-        for k in range(self.S):
-            self.x[k].copy_(self.u[k] if warm is None else warm[k])
+        galerkin = warm is None and self.warm != "previous" and self.n_steps >= 1
+        if not (galerkin and self.start_from_history):
+            for k in range(self.S):
+                self.x[k].copy_(self.u[k] if warm is None else warm[k])
         st = _lib.Stats()
-        if warm is None and self.warm != "previous" and self.n_steps >= 1:
+        if galerkin:
             # Galerkin warm start over past increments (and the MLP direction), folded into
             # the solve: the directions are formed from the stored past solutions in libdsv
             nh = min(self.history_dirs, self.MAX_HIST - (1 if learned else 0), self.n_steps)
             extra = [self._mlp_direction(feats, k) for k in range(self.S)] if learned else []
             nd = nh + (1 if learned else 0)
-            hist = (ctypes.c_void_p * max(1, nh * self.S))(
-                *[ptr(self.u_hist[j][k]).value for j in range(nh) for k in range(self.S)])
+            if self.start_from_history:  # u_n, u_{n-1}, ..., u_{n-nh}
+                steps, fn = [self.u] + self.u_hist[:nh], "dsv_solve_warm_history"
+            else:  # u_{n-1}, ..., u_{n-nh}; the start u_n was copied into x
+                steps, fn = self.u_hist[:nh], "dsv_solve_warm"
+            hist = (ctypes.c_void_p * max(1, len(steps) * self.S))(
+                *[ptr(a[k]).value for a in steps for k in range(self.S)])
             ext = (ctypes.c_void_p * max(1, len(extra)))(*[ptr(e).value for e in extra])
             coef = (ctypes.c_double * (nd * self.S))()
             r0 = (ctypes.c_double * self.S)()
             r1 = (ctypes.c_double * self.S)()
-            _lib.call("dsv_solve_warm", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
+            _lib.call(fn, self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
                       self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it, nh, hist,
                       1 if learned else 0, ext, stream_ptr(self.mesh.device), st, coef, r0, r1)
             self.warm_log.append({

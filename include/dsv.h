@@ -187,6 +187,22 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
                    void* stream, dsv_stats* stats, double* coef, double* r_before,
                    double* r_after);
 
+/* dsv_solve_warm with the start taken from the history (no counterpart in the reference).
+ * hist holds n_hist + 1 steps: hist[j*n_species + s] is species s at step n-j, the start is
+ * hist_0 (step n), and the n_hist history directions are d_j = hist_j - hist_(j+1).  sp[s].x
+ * receives the solution; its content on entry is not read, and it must not be one of the
+ * history arrays.  Otherwise as dsv_solve_warm.  A caller that keeps its last solutions in
+ * rotating buffers needs no copy of the start, and under operator reuse every history
+ * direction but d_0 is the previous call's direction shifted by one (a direction formed
+ * from an array that a solve has written since is not taken): one FP64 apply per call.
+ * The start, the directions and so the results are the same bits as dsv_solve_warm's with
+ * x = hist_0 and the history hist_1 ... */
+int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                           int32_t lumped, int32_t method, double rtol, int32_t max_it,
+                           int32_t n_hist, const double* const* hist, int32_t n_extra,
+                           const double* const* extra, void* stream, dsv_stats* stats,
+                           double* coef, double* r_before, double* r_after);
+
 /* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
  * sweeps), the number of CG iterations per polled chunk, and whether chunks are replayed from
  * captured hipGraphs (use_graphs != 0) or launched directly.  Defaults: 1e-3, 12, 8, 0. */

@@ -227,3 +227,70 @@ def test_solve_warm_many_directions_match_oracle(gpu, n_hist, n_extra):
             assert abs(coef[k * 3 + s] - cr[k]) <= 1e-8 * max(np.abs(cr).max(), 1e-30), (s, k)
         x = sp[s]["x"].cpu().numpy()
         assert np.abs(x - exact[s]).max() <= 1e-7 * np.abs(exact[s]).max()
+
+
+@pytest.mark.parametrize("method", ["mf32_ir64", "mf64"])
+@pytest.mark.parametrize("n_hist,n_extra", [(3, 0), (2, 1), (0, 1)])
+def test_solve_warm_history_matches_solve_warm(gpu, method, n_hist, n_extra):
+    """dsv_solve_warm_history (start = hist_0, directions hist_j - hist_(j+1)) against
+    dsv_solve_warm with x = a copy of hist_0 and the history hist_1 ...: the same start and
+    directions, so the same bits -- solution, coefficients, residual norms, iterations.  The
+    solution array's content on entry (NaN here) is not read; a solution array named in the
+    history and a NULL history are refused."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import Mesh, species_array, species_struct, stream_ptr, ptr
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("kuhn7_jitter_morton", seed=17)
+    mesh = Mesh(conn, xyz)
+    V = len(xyz)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
+    exact = [OS.direct(A, b) for A, b in zip(mats, bs)]
+    start = [t(u0[s]) for s in range(3)]
+    past = [[t(u0[s] - (j + 1) * 0.25 * (exact[s] - u0[s]) + rng.normal(0, 1e-3, V))
+             for s in range(3)] for j in range(n_hist)]
+    extra = [[t(rng.normal(0, 1, V)) for _ in range(3)] for _ in range(n_extra)]
+    keep = [v for row in past + extra for v in row] + start
+    earr = (ctypes.c_void_p * max(1, 3 * n_extra))(*[ptr(v).value for r in extra for v in r])
+    bt = [t(b) for b in bs]
+    outs = []
+    for fn in ("dsv_solve_warm", "dsv_solve_warm_history"):
+        if fn == "dsv_solve_warm":
+            x = [s.clone() for s in start]
+            rows = past
+        else:
+            x = [torch.full((V,), float("nan"), dtype=torch.float64, device=gpu) for _ in range(3)]
+            rows = [start] + past
+        keep.extend(x)
+        structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                                  alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                                  bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                                  b=bt[k], x=x[k], keep=keep) for k, s in enumerate(sps)]
+        harr = (ctypes.c_void_p * max(1, 3 * len(rows)))(*[ptr(v).value for r in rows for v in r])
+        nd = n_hist + n_extra
+        coef = (ctypes.c_double * (3 * nd))()
+        r0, r1 = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+        st = _lib.Stats()
+        _lib.call(fn, mesh.handle, 3, species_array(structs), inv_dt, 1, _lib.METHODS[method],
+                  1e-10, 5000, n_hist, harr if rows else None, n_extra, earr, stream_ptr(gpu), st,
+                  coef, r0, r1)
+        torch.cuda.synchronize()
+        outs.append(([v.cpu().numpy() for v in x], list(coef), list(r0), list(r1), st.as_dict(3)))
+    (xa, ca, r0a, r1a, da), (xb, cb, r0b, r1b, db) = outs
+    assert all(da["converged"]) and all(db["converged"]), (da, db)
+    assert ca == cb and r0a == r0b and r1a == r1b
+    assert da["iterations"] == db["iterations"] and da["outer"] == db["outer"]
+    for s in range(3):
+        assert np.array_equal(xa[s], xb[s])
+        assert np.abs(xb[s] - exact[s]).max() <= 1e-7 * np.abs(exact[s]).max()
+    # refusals: a solution array in the history; no history at all
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                              bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                              b=bt[k], x=start[k], keep=keep) for k, s in enumerate(sps)]
+    harr = (ctypes.c_void_p * 3)(*[ptr(v).value for v in start])
+    for h in (harr, None):
+        with pytest.raises(ValueError):
+            _lib.call("dsv_solve_warm_history", mesh.handle, 3, species_array(structs), inv_dt, 1,
+                      _lib.METHODS[method], 1e-10, 5000, 0, h, 0, None, stream_ptr(gpu), None,
+                      None, None, None)
