@@ -98,6 +98,24 @@ __global__ void k_rhs(int32_t V, const int32_t* __restrict__ conn,
     b[r] = s;
 }
 
+// lumped RHS of up to three species in one pass over the mass diagonal (the same arithmetic
+// as k_rhs per species)
+struct RhsBatch {
+    const double* u[3];
+    const double* f[3];
+    double* b[3];
+};
+__global__ void k_rhs_lumped_batch(int32_t V, int S, const double* __restrict__ lm, RhsBatch a,
+                                   double inv_dt) {
+    int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= V) return;
+    const double mr = lm[r];
+    for (int s = 0; s < S; ++s) {
+        const double fr = a.f[s] ? a.f[s][r] : 0.0;
+        a.b[s][r] = mr * (inv_dt * a.u[s][r] + fr);
+    }
+}
+
 __global__ void k_bc_values(int32_t n_bc, const int32_t* __restrict__ bc, double value,
                             double* __restrict__ b) {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,6 +216,44 @@ int dsv_rhs(dsv_mesh* m, const double* u_old, const double* f, int32_t f_len, do
         if (n_bc > 0)
             hipLaunchKernelGGL(k_bc_values, dim3(cdiv(n_bc, 128)), dim3(128), 0, st, n_bc, bc_dofs,
                                bc_value, b);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_rhs_batch(dsv_mesh* m, int32_t n_species, const dsv_species* sp,
+                  const double* const* u_old, const double* const* f, double inv_dt,
+                  int32_t lumped, double* const* b, void* stream) {
+    return guard([&] {
+        if (!m || !sp || !u_old || !b) throw ArgError{"dsv_rhs_batch: NULL argument"};
+        if (n_species < 1 || n_species > 3) throw ArgError{"n_species must be 1..3"};
+        for (int s = 0; s < n_species; ++s) {
+            if (!u_old[s] || !b[s]) throw ArgError{"dsv_rhs_batch: NULL u_old or b"};
+            if (sp[s].n_bc < 0 || (sp[s].n_bc > 0 && !sp[s].bc_dofs))
+                throw ArgError{"dsv_rhs_batch: invalid bc_dofs"};
+        }
+        DSV_HIP(hipSetDevice(m->device));
+        for (int s = 0; s < n_species; ++s) validate_bc_list(m, sp[s].bc_dofs, sp[s].n_bc);
+        hipStream_t st = (hipStream_t)stream;
+        const unsigned gv = cdiv(m->V, 256);
+        if (lumped) {
+            RhsBatch a{};
+            for (int s = 0; s < n_species; ++s) {
+                a.u[s] = u_old[s];
+                a.f[s] = f ? f[s] : nullptr;
+                a.b[s] = b[s];
+            }
+            hipLaunchKernelGGL(k_rhs_lumped_batch, dim3(gv), dim3(256), 0, st, m->V, n_species,
+                               m->lumped_m, a, inv_dt);
+        } else {
+            for (int s = 0; s < n_species; ++s)
+                hipLaunchKernelGGL(k_rhs, dim3(gv), dim3(256), 0, st, m->V, m->conn, m->detJ,
+                                   m->diag_pos, m->cptr, m->cslot, m->lumped_m, u_old[s],
+                                   f ? f[s] : nullptr, inv_dt, 0, b[s]);
+        }
+        for (int s = 0; s < n_species; ++s)
+            if (sp[s].n_bc > 0)
+                hipLaunchKernelGGL(k_bc_values, dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
+                                   sp[s].n_bc, sp[s].bc_dofs, sp[s].bc_value, b[s]);
         DSV_HIP(hipGetLastError());
     });
 }

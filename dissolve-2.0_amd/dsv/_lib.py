@@ -27,6 +27,7 @@ METHODS = {"csr64": METHOD_CSR64, "mf32_ir64": METHOD_MF32_IR64, "mf64": METHOD_
 EXPORTS = (
     "dsv_last_error", "dsv_device_count", "dsv_mesh_create", "dsv_mesh_destroy", "dsv_mesh_info",
     "dsv_mesh_view", "dsv_mesh_copy_view", "dsv_mesh_set_partition", "dsv_assemble", "dsv_rhs",
+    "dsv_rhs_batch",
     "dsv_apply", "dsv_solve", "dsv_set_solver_options", "dsv_set_operator_form",
     "dsv_set_species_narrowing",
     "dsv_set_operator_reuse", "dsv_time_apply",
@@ -112,6 +113,8 @@ def load():
         "dsv_mesh_set_partition": (ctypes.c_int, [P, I32, I32, I32]),
         "dsv_assemble": (ctypes.c_int, [P, ctypes.POINTER(Species), D, I32, P, P]),
         "dsv_rhs": (ctypes.c_int, [P, P, P, I32, D, I32, P, I32, D, P, P]),
+        "dsv_rhs_batch": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), ctypes.POINTER(P),
+                                         ctypes.POINTER(P), D, I32, ctypes.POINTER(P), P]),
         "dsv_apply": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, ctypes.POINTER(P),
                                      ctypes.POINTER(P), I32, P]),
         "dsv_solve": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, D, I32, P,

@@ -123,15 +123,22 @@ class Stepper:
                                 b=self.b[k], x=self.x[k], keep=keep)
             structs.append(st)
             if "f" in s:
-                import torch
-                self.f[k] = (to_device(s["f"], self.mesh.device, torch.float64).clone()
-                             if s["f"] is not None else None)
+                self.f[k] = (self._vertex_field(s["f"]).clone() if s["f"] is not None else None)
         self._keep = keep
         self._arr = species_array(structs)
         self.defs = species
         # the coefficient arrays above stay unchanged until the next call: the library may
         # keep the operator it builds from them (this call discards any earlier one)
         self.mesh.set_operator_reuse(self.reuse_operator)
+
+    def _vertex_field(self, a):
+        """A per-vertex FP64 source on the device; other lengths raise ValueError (the
+        batched RHS reads V values of each)."""
+        import torch
+        t = to_device(a, self.mesh.device, torch.float64)
+        if t.numel() != self.mesh.n_dofs:
+            raise ValueError(f"f length {t.numel()} unrecognised (expected n_dofs={self.mesh.n_dofs})")
+        return t
 
     def refresh_coefficients(self, alpha=None, f=None):
         """Overwrite, in place, the per-vertex reaction coefficients (alpha[k]) and sources
@@ -146,22 +153,21 @@ class Stepper:
                     raise ValueError(f"species {self.names[k]} has no per-vertex alpha to refresh")
                 self.alpha_dev[k].copy_(to_device(alpha[k], dev, torch.float64))
             if f is not None and f[k] is not None:
-                fk = to_device(f[k], dev, torch.float64)
+                fk = self._vertex_field(f[k])
                 if self.f[k] is None or self.f[k].numel() != fk.numel():
                     self.f[k] = fk.clone()
                 else:
                     self.f[k].copy_(fk)
         self.mesh.set_operator_reuse(self.reuse_operator)
 
-    def rhs(self):
-        m = self.mesh
-        inv_dt = 1.0 / self.dt
-        for k, s in enumerate(self.defs):
-            f = self.f[k]
-            bc = self.bc[k]
-            _lib.call("dsv_rhs", m.handle, ptr(self.u[k]), ptr(f), int(f.numel()) if f is not None else 0,
-                      inv_dt, self.lumped, ptr(bc), int(bc.numel()) if bc is not None else 0,
-                      float(s.get("bc_value", 0.0)), ptr(self.b[k]), stream_ptr(m.device))
+    def rhs(self, stream=None):
+        """b_s = M (u_s / dt + f_s), b_s[bc] = bc value, all species in one call."""
+        P = ctypes.c_void_p * self.S
+        _lib.call("dsv_rhs_batch", self.mesh.handle, self.S, self._arr,
+                  P(*[u.data_ptr() for u in self.u]),
+                  P(*[f.data_ptr() if f is not None else None for f in self.f]),
+                  1.0 / self.dt, self.lumped, P(*[b.data_ptr() for b in self.b]),
+                  stream if stream is not None else stream_ptr(self.mesh.device))
 
     def _features(self, k):
         s = self.defs[k]
@@ -201,7 +207,8 @@ class Stepper:
         """One backward-Euler step of every species; `warm` (list of tensors) replaces the
         initial guess.  Returns the solver statistics."""
         t0 = time.perf_counter()
-        self.rhs()
+        stream = stream_ptr(self.mesh.device)
+        self.rhs(stream)
         feats = None
         learned = self.warm == "learned" and self.n_steps >= 1
         if self.mlp is not None and (self.collect or learned) and self.n_steps >= 1:
@@ -229,7 +236,7 @@ class Stepper:
             r1 = (ctypes.c_double * self.S)()
             _lib.call(fn, self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
                       self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it, nh, hist,
-                      1 if learned else 0, ext, stream_ptr(self.mesh.device), st, coef, r0, r1)
+                      1 if learned else 0, ext, stream, st, coef, r0, r1)
             self.warm_log.append({
                 "step": self.n_steps, "mode": self.warm,
                 "coef": [[coef[k * self.S + s] for s in range(self.S)] for k in range(nd)],
@@ -238,7 +245,7 @@ class Stepper:
         else:
             _lib.call("dsv_solve", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
                       self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it,
-                      stream_ptr(self.mesh.device), st)
+                      stream, st)
         if self.collect and feats is not None:
             for k in range(self.S):
                 X, sigma = feats[k]

@@ -132,6 +132,15 @@ int dsv_rhs(dsv_mesh* m, const double* u_old, const double* f, int32_t f_len, do
             int32_t lumped, const int32_t* bc_dofs, int32_t n_bc, double bc_value, double* b,
             void* stream);
 
+/* dsv_rhs for n_species (1..3) species in one call (the reference calls assemble_rhs once per
+ * species, `dissolve.py:787,895,946` via `numerics/solvers.py:348-353`): b[s] from u_old[s]
+ * and f[s] (per vertex, length V; f or f[s] NULL = absent), with the Dirichlet list and value
+ * of sp[s] (its other fields are not read).  The same values as dsv_rhs, bit for bit; the
+ * lumped mass makes one pass for all species. */
+int dsv_rhs_batch(dsv_mesh* m, int32_t n_species, const dsv_species* sp,
+                  const double* const* u_old, const double* const* f, double inv_dt,
+                  int32_t lumped, double* const* b, void* stream);
+
 /* --- matrix-free apply: CeedDiffusionMassOp.apply (`gpu/ceed_operators.py:341-343`) ---
  * y_s = A_s x_s for n_species species sharing the mesh, the SAME operator as dsv_assemble
  * (including the Dirichlet identity rows).  precision: 64 -> FP64 arithmetic, 32 -> FP32
