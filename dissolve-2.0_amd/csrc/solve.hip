@@ -1275,6 +1275,26 @@ __global__ void k_pack(int V, const int32_t* __restrict__ perm, const double* __
     for (int s = 0; s < S; ++s) r.v[s] = src[s] ? (To)src[s][c] : To(0);
     out[i] = r;
 }
+// the FP64 start of a matrix-free solve: x in internal order with the Dirichlet values of each
+// species (bit s of bcbits) in place
+struct BcVals {
+    double v[3];
+};
+template <int S>
+__global__ void k_pack_start(int V, const int32_t* __restrict__ perm, const double* __restrict__ s0,
+                             const double* __restrict__ s1, const double* __restrict__ s2,
+                             const uint8_t* __restrict__ bcbits, BcVals bv,
+                             SV<double, S>* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int c = perm[i];
+    const unsigned bits = bcbits[i];
+    const double* src[3] = {s0, s1, s2};
+    SV<double, S> r{};
+#pragma unroll
+    for (int s = 0; s < S; ++s) r.v[s] = ((bits >> s) & 1u) ? bv.v[s] : src[s][c];
+    out[i] = r;
+}
 template <class Ti, int S>
 __global__ void k_unpack(int V, const int32_t* __restrict__ perm, const SV<Ti, S>* __restrict__ in,
                          double* __restrict__ d0, double* __restrict__ d1, double* __restrict__ d2) {
@@ -2541,13 +2561,22 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     SV<double, S>* x64 = (SV<double, S>*)w->vb[VB_X64];
     hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
                        bs[2], (SV<double, S>*)w->vb[VB_B64]);
-    hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0], xs[1],
-                       xs[2], x64);
-    for (int s = 0; s < S; ++s)
-        if (sp[s].n_bc > 0)
-            hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0, st,
-                               sp[s].n_bc, sp[s].bc_dofs, internal ? m->iperm : (const int32_t*)nullptr,
-                               s, sp[s].bc_value, x64);
+    if (internal) {
+        // the start with its Dirichlet values, from the Dirichlet bits of the operator set-up
+        prepare_bc(m, w, S, sp, st);
+        BcVals bv{};
+        for (int s = 0; s < S; ++s) bv.v[s] = sp[s].bc_value;
+        hipLaunchKernelGGL((k_pack_start<S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0],
+                           xs[1], xs[2], (const uint8_t*)w->bcbits, bv, x64);
+    } else {
+        hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0],
+                           xs[1], xs[2], x64);
+        for (int s = 0; s < S; ++s)
+            if (sp[s].n_bc > 0)
+                hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0,
+                                   st, sp[s].n_bc, sp[s].bc_dofs, (const int32_t*)nullptr, s,
+                                   sp[s].bc_value, x64);
+    }
     // operator set-up: assembled values / coefficients and the Jacobi diagonal
     if (method == DSV_METHOD_CSR64) {
         SV<double, S>* dinv = (SV<double, S>*)w->vb[VB_DINV];
@@ -2557,7 +2586,6 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
                                w->csr[s], s, dinv);
         }
     } else if (method == DSV_METHOD_MF64) {
-        prepare_bc(m, w, S, sp, st);
         prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
         if (!w->dinv_valid) {
             MFVec<double, S> v{};
@@ -2566,7 +2594,6 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
             w->dinv_valid = m->op_reuse;
         }
     } else {
-        prepare_bc(m, w, S, sp, st);
         prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
         prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1);
         if (!w->dinv_valid) {
@@ -2913,19 +2940,23 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_update(int V, ProjVec<S> 
     }
 }
 
-// d = a - b per species (caller order -> internal order; NULL b: d = a)
+// d = a - b per species (caller order -> internal order; NULL b: d = a), 0 where bit s of
+// bcbits (internal order) marks a Dirichlet vertex of species s
 template <int S>
 __global__ void k_pack_diff(int V, const int32_t* __restrict__ perm, const double* a0,
                             const double* a1, const double* a2, const double* b0, const double* b1,
-                            const double* b2, SV<double, S>* __restrict__ out) {
+                            const double* b2, const uint8_t* __restrict__ bcbits,
+                            SV<double, S>* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     const int c = perm ? perm[i] : i;
+    const unsigned bits = bcbits[i];
     const double* A[3] = {a0, a1, a2};
     const double* B[3] = {b0, b1, b2};
     SV<double, S> r{};
 #pragma unroll
-    for (int s = 0; s < S; ++s) r.v[s] = (A[s] ? A[s][c] : 0.0) - (B[s] ? B[s][c] : 0.0);
+    for (int s = 0; s < S; ++s)
+        r.v[s] = ((bits >> s) & 1u) ? 0.0 : (A[s] ? A[s][c] : 0.0) - (B[s] ? B[s][c] : 0.0);
     out[i] = r;
 }
 
@@ -3213,11 +3244,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
         pv.Ad[k] = Ad;
         if (reuse[k]) continue;
         hipLaunchKernelGGL((k_pack_diff<S>), dim3(gv), dim3(256), 0, st, m->V, m->perm, a[k][0],
-                           a[k][1], a[k][2], b[k][0], b[k][1], b[k][2], d);
-        for (int s = 0; s < S; ++s)
-            if (sp[s].n_bc > 0)
-                hipLaunchKernelGGL((k_set_bc_values<S>), dim3(cdiv(sp[s].n_bc, 128)), dim3(128), 0,
-                                   st, sp[s].n_bc, sp[s].bc_dofs, m->iperm, s, 0.0, d);
+                           a[k][1], a[k][2], b[k][0], b[k][1], b[k][2], w->bcbits, d);
         MFVec<double, S> v{};
         v.x = d;
         v.y = Ad;
