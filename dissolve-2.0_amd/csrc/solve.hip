@@ -2918,28 +2918,6 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_norms(int V, ProjVec<S> p
     write_partials<S>(tot, red, pout);
 }
 
-// in place: x += sum_k c_k d_k, r -= sum_k c_k A d_k (c = 0 leaves a species unchanged)
-template <int S, int ND>
-__global__ __launch_bounds__(PROJ_THREADS) void k_proj_update(int V, ProjVec<S> pv,
-                                                             SV<double, S>* __restrict__ r) {
-    double c[ND][S];
-    proj_coefs<S, ND>(pv, c);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-        SV<double, S> x = pv.x[i], rv = r[i];
-#pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                x.v[s] += c[k][s] * d.v[s];
-                rv.v[s] -= c[k][s] * Ad.v[s];
-            }
-        }
-        pv.x[i] = x;
-        r[i] = rv;
-    }
-}
-
 // d = a - b per species (caller order -> internal order; NULL b: d = a), 0 where bit s of
 // bcbits (internal order) marks a Dirichlet vertex of species s
 template <int S>
@@ -3016,13 +2994,30 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_sum_partials(const double* __r
     if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
 
+// k_sum_partials over two sources in one launch: quantities [0, q1) from p1 (n1 partials
+// each), the following ones from p2 (n2 each), in the same order per quantity
+__global__ __launch_bounds__(PROJ_THREADS) void k_sum_partials2(const double* __restrict__ p1,
+                                                               int n1, int q1,
+                                                               const double* __restrict__ p2,
+                                                               int n2, double* __restrict__ out) {
+    __shared__ double red[16];
+    const bool first = (int)blockIdx.x < q1;
+    const int nprod = first ? n1 : n2;
+    const double* p = first ? p1 + (size_t)blockIdx.x * n1 : p2 + (size_t)(blockIdx.x - q1) * n2;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nprod; b += blockDim.x) v[0] += p[b];
+    block_sum_all<1>(v, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
 // Device side of the warm start folded into a solve: the Gram systems, then the acceptance
 // test, with no host round trip in between.
 struct WarmOut {
-    double c[MAXD][MAXS];  // coefficients (0 where the start stays)
+    double c[MAXD][MAXS];  // coefficients of the Gram solves
     double rr0[MAXS];   // ||r||^2 of the given start and ||b||^2 (the solve's first residual)
     double bb[MAXS];
     double rr1[MAXS];   // ||r||^2 of the start taken
+    int take[MAXS];     // 1 where the combination was taken (else the start stays, c counts 0)
 };
 
 // One Gram solve per species (thread s), from the totals of the dot partials: the full
@@ -3072,23 +3067,47 @@ __global__ __launch_bounds__(64) void k_proj_solve(const double* __restrict__ to
 
 // Takes the combination of a species only where some direction moved it and its FP64
 // residual fell (the given start is the fallback), from the totals of the norm partials and
-// of the residual kernel's (r.r then b.b per species).
+// of the residual kernel's (r.r then b.b per species), then updates in place:
+// x += sum_k c_k d_k, r -= sum_k c_k A d_k for the species taken.  Every workgroup makes the
+// same decision from the same totals; workgroup 0 records it.
 template <int S, int ND>
-__global__ __launch_bounds__(64) void k_proj_accept(const double* __restrict__ tot,
-                                                   WarmOut* __restrict__ out) {
-    if (threadIdx.x != 0) return;
-    const double* n1 = tot;     // ||r0 - sum_k c_k A d_k||^2 per species
+__global__ __launch_bounds__(PROJ_THREADS) void k_proj_accept_update(
+    int V, ProjVec<S> pv, SV<double, S>* __restrict__ r, const double* __restrict__ tot,
+    WarmOut* __restrict__ out) {
+    double c[ND][S];
+    proj_coefs<S, ND>(pv, c);
+    const double* n1 = tot;      // ||r0 - sum_k c_k A d_k||^2 per species
     const double* r0 = tot + S;  // the residual kernel's r.r, then b.b, per species
+#pragma unroll
     for (int s = 0; s < S; ++s) {
         bool moved = false;  // every direction dropped: nothing to take
-        for (int k = 0; k < ND; ++k) moved |= out->c[k][s] != 0.0;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) moved |= c[k][s] != 0.0;
         const double a = sqrt(r0[s]), b = sqrt(n1[s]);
         const bool take = moved && b - b == 0.0 && b < a;
         if (!take)
-            for (int k = 0; k < ND; ++k) out->c[k][s] = 0.0;
-        out->rr0[s] = r0[s];
-        out->bb[s] = r0[S + s];
-        out->rr1[s] = take ? n1[s] : r0[s];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) c[k][s] = 0.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            out->take[s] = take ? 1 : 0;
+            out->rr0[s] = r0[s];
+            out->bb[s] = r0[S + s];
+            out->rr1[s] = take ? n1[s] : r0[s];
+        }
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        SV<double, S> x = pv.x[i], rv = r[i];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                x.v[s] += c[k][s] * d.v[s];
+                rv.v[s] -= c[k][s] * Ad.v[s];
+            }
+        }
+        pv.x[i] = x;
+        r[i] = rv;
     }
 }
 
@@ -3298,13 +3317,10 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     m->stat_gram_reused += gshift ? 1 : 0;
     hipLaunchKernelGGL((k_proj_norms<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st, m->V,
                        pv, w->pW);
-    hipLaunchKernelGGL(k_sum_partials, dim3(S), dim3(PROJ_THREADS), 0, st, w->pW, PROJ_GRID_F,
-                       w->pT);
-    hipLaunchKernelGGL(k_sum_partials, dim3(2 * S), dim3(PROJ_THREADS), 0, st, w->pR, nres,
-                       w->pT + S);
-    hipLaunchKernelGGL((k_proj_accept<S, ND>), dim3(1), dim3(64), 0, st, w->pT, w->wout);
-    hipLaunchKernelGGL((k_proj_update<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st, m->V,
-                       pv, (SV<double, S>*)w->vb[VB_R64]);
+    hipLaunchKernelGGL(k_sum_partials2, dim3(3 * S), dim3(PROJ_THREADS), 0, st, w->pW,
+                       PROJ_GRID_F, S, w->pR, nres, w->pT);
+    hipLaunchKernelGGL((k_proj_accept_update<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st,
+                       m->V, pv, (SV<double, S>*)w->vb[VB_R64], w->pT, w->wout);
     DSV_HIP(hipGetLastError());
     DSV_HIP(hipMemcpyAsync(w->h_wout, w->wout, sizeof(WarmOut), hipMemcpyDeviceToHost, st));
     DSV_HIP(hipStreamSynchronize(st));
@@ -3315,7 +3331,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
         if (wa.r_before) wa.r_before[s] = std::sqrt(o.rr0[s]);
         if (wa.r_after) wa.r_after[s] = std::sqrt(o.rr1[s]);
         if (wa.coef)
-            for (int k = 0; k < ND; ++k) wa.coef[k * S + s] = o.c[k][s];
+            for (int k = 0; k < ND; ++k) wa.coef[k * S + s] = o.take[s] ? o.c[k][s] : 0.0;
     }
 }
 
