@@ -50,12 +50,17 @@ class WarmStartMLP:
 
     # ---- kernels ----
     def features(self, u, u_prev, phi, grad_phi, D, dt, eps, alpha=None, alpha_value=0.0,
-                 bc_mask=None):
-        """(X [n, 7] float32, sigma [1] float32) on the device."""
+                 bc_mask=None, X=None, sigma=None):
+        """(X [n, 7] float32, sigma [1] float32) on the device; X and sigma may be given
+        (contiguous views, e.g. one species' rows of a batch)."""
         import torch
         n = int(u.numel())
-        X = torch.empty((n, NIN), dtype=torch.float32, device=self.device)
-        sigma = torch.empty(1, dtype=torch.float32, device=self.device)
+        if X is None:
+            X = torch.empty((n, NIN), dtype=torch.float32, device=self.device)
+        if sigma is None:
+            sigma = torch.empty(1, dtype=torch.float32, device=self.device)
+        if tuple(X.shape) != (n, NIN) or not X.is_contiguous() or sigma.numel() != 1:
+            raise ValueError("features: X must be a contiguous (n, 7) and sigma a 1-element tensor")
         _lib.call("dsv_mlp_features", n, ptr(u), ptr(u_prev), ptr(phi), ptr(grad_phi), ptr(alpha),
                   float(alpha_value), float(D), float(dt), float(eps), ptr(bc_mask), ptr(X),
                   ptr(sigma), stream_ptr(self.device))

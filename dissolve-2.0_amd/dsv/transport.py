@@ -30,6 +30,7 @@ import numpy as np
 
 from . import _lib
 from .core import Mesh, ptr, species_array, species_struct, stream_ptr, to_device
+from .mlp import NIN
 
 
 class Stepper:
@@ -169,13 +170,21 @@ class Stepper:
                   1.0 / self.dt, self.lumped, P(*[b.data_ptr() for b in self.b]),
                   stream if stream is not None else stream_ptr(self.mesh.device))
 
-    def _features(self, k):
-        s = self.defs[k]
-        alpha = self.alpha_dev[k]
-        return self.mlp.features(self.u[k], self.u_hist[0][k], self.phi, self.grad_phi,
-                                 s.get("D_value", 1.0) if s.get("D") is None else float(np.mean(s["D"])),
-                                 self.dt, self.eps, alpha=alpha,
-                                 alpha_value=s.get("alpha_value", 0.0), bc_mask=self.bc_mask[k])
+    def _features(self):
+        """The MLP features of every species in one (S V, 7) buffer, so that one forward
+        serves them all: (X, sigma [S], [(X_k, sigma_k) per species])."""
+        import torch
+        V = self.u[0].numel()
+        X = torch.empty((self.S * V, NIN), dtype=torch.float32, device=self.mesh.device)
+        sig = torch.empty(self.S, dtype=torch.float32, device=self.mesh.device)
+        per = [(X[k * V:(k + 1) * V], sig[k:k + 1]) for k in range(self.S)]
+        for k, s in enumerate(self.defs):
+            D = s.get("D_value", 1.0) if s.get("D") is None else float(np.mean(s["D"]))
+            self.mlp.features(self.u[k], self.u_hist[0][k], self.phi, self.grad_phi, D, self.dt,
+                              self.eps, alpha=self.alpha_dev[k],
+                              alpha_value=s.get("alpha_value", 0.0), bc_mask=self.bc_mask[k],
+                              X=per[k][0], sigma=per[k][1])
+        return X, sig, per
 
     def residual_norms(self):
         """||b - A x|| per species for the current initial guesses self.x (FP64)."""
@@ -199,9 +208,10 @@ class Stepper:
         return ([[coef[k * self.S + s] for s in range(self.S)] for k in range(nd)],
                 list(r0), list(r1))
 
-    def _mlp_direction(self, feats, k):
-        X, sigma = feats[k]
-        return (sigma.double() * self.tau) * self.mlp.forward(X).double()
+    def _mlp_directions(self, X, sig):
+        """sigma_k tau f(X_k) per species (FP64 rows), from one forward over all species."""
+        out = self.mlp.forward(X).view(self.S, -1).double()
+        return out * (sig.double() * self.tau)[:, None]
 
     def step(self, warm=None):
         """One backward-Euler step of every species; `warm` (list of tensors) replaces the
@@ -212,7 +222,7 @@ class Stepper:
         feats = None
         learned = self.warm == "learned" and self.n_steps >= 1
         if self.mlp is not None and (self.collect or learned) and self.n_steps >= 1:
-            feats = [self._features(k) for k in range(self.S)]
+            X_all, sig_all, feats = self._features()
         galerkin = warm is None and self.warm != "previous" and self.n_steps >= 1
         if not (galerkin and self.start_from_history):
             for k in range(self.S):
@@ -222,7 +232,7 @@ class Stepper:
             # Galerkin warm start over past increments (and the MLP direction), folded into
             # the solve: the directions are formed from the stored past solutions in libdsv
             nh = min(self.history_dirs, self.MAX_HIST - (1 if learned else 0), self.n_steps)
-            extra = [self._mlp_direction(feats, k) for k in range(self.S)] if learned else []
+            extra = list(self._mlp_directions(X_all, sig_all)) if learned else []
             nd = nh + (1 if learned else 0)
             if self.start_from_history:  # u_n, u_{n-1}, ..., u_{n-nh}
                 steps, fn = [self.u] + self.u_hist[:nh], "dsv_solve_warm_history"

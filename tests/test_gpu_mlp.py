@@ -132,3 +132,10 @@ def test_learned_warm_start_saves_iterations(dev):
     assert any(any(e["taken"]) for e in log)
     assert any(c != 0.0 for e in log for c in e["coef"][-1])  # the MLP direction is used
     assert it_learned < it_plain, (it_learned, it_plain, log[-1])
+    # the learned directions come from one forward over every species' rows: the same bits as
+    # one forward per species, sigma_k tau f(X_k)
+    X_all, sig_all, per = learned._features()
+    batched = learned._mlp_directions(X_all, sig_all)
+    for k, (X, sigma) in enumerate(per):
+        one = (sigma.double() * learned.tau) * learned.mlp.forward(X.clone()).double()
+        assert torch.equal(batched[k], one), k
