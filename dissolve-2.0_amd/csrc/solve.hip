@@ -2527,11 +2527,13 @@ static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const W
 
 // The solution arrays a solve is about to write: the warm-start directions of another
 // workspace may have been formed from them, and this workspace's next call must not take a
-// direction formed from them from this call (their contents change).
-static void note_outputs(dsv_mesh* m, Workspace* w, double* const* out) {
+// direction formed from them (their contents change).  A solve that formed directions
+// (`formed`) keeps them and records its outputs for that check; any other solve drops the
+// directions of its workspace too, since they may name arrays it writes.
+static void note_outputs(dsv_mesh* m, Workspace* w, double* const* out, bool formed) {
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 4; ++b)
-            if (m->ws[a][b] && m->ws[a][b] != w) m->ws[a][b]->wvalid = 0;
+            if (m->ws[a][b] && (m->ws[a][b] != w || !formed)) m->ws[a][b]->wvalid = 0;
     if (w)
         for (int s = 0; s < MAXS; ++s) w->wout_prev[s] = out[s];
 }
@@ -2686,7 +2688,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm,
                        (const SV<double, S>*)x64, xo[0], xo[1], xo[2]);
     DSV_HIP(hipGetLastError());
-    note_outputs(m, w, xo);
+    note_outputs(m, w, xo, warm && warm->n_hist + warm->n_extra > 0);
     // the solution is ordered on the caller's stream (no host wait for the unpack)
     DSV_HIP(hipEventRecord(w->ev_io[1], st));
     DSV_HIP(hipStreamWaitEvent(caller, w->ev_io[1], 0));
@@ -3183,7 +3185,7 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
             for (int k = 0; k < ND; ++k) coef[k * S + s] = take ? pv.c[k][s] : 0.0;
     }
     if (out[0] || out[1] || out[2]) {
-        note_outputs(m, nullptr, out);
+        note_outputs(m, nullptr, out, false);
         hipLaunchKernelGGL((k_unpack<double, S>), dim3(gv), dim3(256), 0, st, m->V, m->perm,
                            (const SV<double, S>*)pv.x, out[0], out[1], out[2]);
         DSV_HIP(hipGetLastError());

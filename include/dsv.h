@@ -203,7 +203,8 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
  * history arrays.  Otherwise as dsv_solve_warm.  A caller that keeps its last solutions in
  * rotating buffers needs no copy of the start, and under operator reuse every history
  * direction but d_0 is the previous call's direction shifted by one (a direction formed
- * from an array that a solve has written since is not taken): one FP64 apply per call.
+ * from an array that a solve has written since is not taken, and a solve without directions
+ * on the mesh in between drops them all): one FP64 apply per call.
  * The start, the directions and so the results are the same bits as dsv_solve_warm's with
  * x = hist_0 and the history hist_1 ... */
 int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,

@@ -118,12 +118,14 @@ def test_stepper_start_from_history_is_bit_identical(gpu):
                                                                  hb["dirs_reused"])
 
 
-def test_direction_from_a_written_solution_is_not_reused(gpu):
-    """A direction whose arrays include a solution the previous solve wrote is formed anew.
+@pytest.mark.parametrize("between", [False, True])
+def test_direction_from_a_written_solution_is_not_reused(gpu, between):
+    """A direction whose arrays include a solution a solve wrote since is formed anew.
     A solve with the start in x (X) records d_0 = X - H0; a history-start solve whose history
     then names X as step n-1 has d_1 = X - H0 from the same two arrays, but X now holds that
-    solve's solution.  Against the same two calls on a mesh without reuse: bit-identical, and
-    only d_2 = H0 - H1 (arrays nobody wrote) taken from the previous call."""
+    solve's solution.  Against the same calls on a mesh without reuse: bit-identical, and only
+    d_2 = H0 - H1 (arrays nobody wrote) taken from the previous call -- none at all when a
+    plain solve (into another array) ran in between, which drops the recorded directions."""
     import ctypes
     import torch
     from dsv import _lib
@@ -165,6 +167,10 @@ def test_direction_from_a_written_solution_is_not_reused(gpu):
         _lib.call("dsv_solve_warm", mesh.handle, 3, structs(X), inv_dt, 1,
                   _lib.METHODS["mf32_ir64"], 1e-9, 5000, 2, arr([H0, H1]), 0, None,
                   stream_ptr(gpu), st1, None, None, None)
+        if between:
+            W = [t(u) for u in u0]
+            _lib.call("dsv_solve", mesh.handle, 3, structs(W), inv_dt, 1,
+                      _lib.METHODS["mf32_ir64"], 1e-9, 5000, stream_ptr(gpu), _lib.Stats())
         Z = [torch.full((V,), float("nan"), dtype=torch.float64, device=gpu) for _ in u0]
         _lib.call("dsv_solve_warm_history", mesh.handle, 3, structs(Z), inv_dt, 1,
                   _lib.METHODS["mf32_ir64"], 1e-9, 5000, 3, arr([Y, X, H0, H1]), 0, None,
@@ -176,4 +182,4 @@ def test_direction_from_a_written_solution_is_not_reused(gpu):
     assert da["iterations"] == db["iterations"], (da, db)
     for x, y in zip(za, zb):
         assert np.isfinite(x).all() and np.array_equal(x, y)
-    assert da["dirs_reused"] == 1 and db["dirs_reused"] == 0, (da, db)
+    assert da["dirs_reused"] == (0 if between else 1) and db["dirs_reused"] == 0, (da, db)
