@@ -294,3 +294,49 @@ def test_solve_warm_history_matches_solve_warm(gpu, method, n_hist, n_extra):
             _lib.call("dsv_solve_warm_history", mesh.handle, 3, species_array(structs), inv_dt, 1,
                       _lib.METHODS[method], 1e-10, 5000, 0, h, 0, None, stream_ptr(gpu), None,
                       None, None, None)
+
+
+@pytest.mark.parametrize("pick", [[2], [0, 2]])
+def test_solve_warm_history_fewer_species(gpu, pick):
+    """dsv_solve_warm_history for one and two species against dsv_solve_warm with the start
+    copied into x: the same bits."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import Mesh, species_array, species_struct, stream_ptr, ptr
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("kuhn7_jitter_morton", seed=23)
+    mesh = Mesh(conn, xyz)
+    V, S = len(xyz), len(pick)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
+    exact = [OS.direct(mats[s], bs[s]) for s in pick]
+    start = [t(u0[s]) for s in pick]
+    past = [[t(u0[s] - (j + 1) * 0.3 * (exact[i] - u0[s]) + rng.normal(0, 1e-3, V))
+             for i, s in enumerate(pick)] for j in range(3)]
+    bt = [t(bs[s]) for s in pick]
+    keep = [v for row in past for v in row] + start + bt
+    outs = []
+    for fn in ("dsv_solve_warm", "dsv_solve_warm_history"):
+        if fn == "dsv_solve_warm":
+            x, rows = [v.clone() for v in start], past
+        else:
+            x = [torch.full((V,), float("nan"), dtype=torch.float64, device=gpu) for _ in pick]
+            rows = [start] + past
+        keep.extend(x)
+        structs = [species_struct(mesh, D=sps[s].get("D"), D_value=sps[s].get("D_value", 0.0),
+                                  alpha=sps[s].get("alpha"),
+                                  alpha_value=sps[s].get("alpha_value", 0.0),
+                                  bc_dofs=sps[s].get("bc_dofs"), bc_value=sps[s].get("bc_value", 0.0),
+                                  b=bt[i], x=x[i], keep=keep) for i, s in enumerate(pick)]
+        harr = (ctypes.c_void_p * (S * len(rows)))(*[ptr(v).value for r in rows for v in r])
+        coef = (ctypes.c_double * (3 * S))()
+        st = _lib.Stats()
+        _lib.call(fn, mesh.handle, S, species_array(structs), inv_dt, 1, _lib.METHODS["mf32_ir64"],
+                  1e-10, 5000, 3, harr, 0, None, stream_ptr(gpu), st, coef, None, None)
+        torch.cuda.synchronize()
+        outs.append(([v.cpu().numpy() for v in x], list(coef), st.as_dict(S)))
+    (xa, ca, da), (xb, cb, db) = outs
+    assert all(da["converged"]) and all(db["converged"]) and ca == cb, (da, db)
+    assert da["iterations"] == db["iterations"]
+    for i in range(S):
+        assert np.array_equal(xa[i], xb[i])
+        assert np.abs(xb[i] - exact[i]).max() <= 1e-7 * np.abs(exact[i]).max()
