@@ -116,6 +116,35 @@ __global__ void k_rhs_lumped_batch(int32_t V, int S, const double* __restrict__ 
     }
 }
 
+// Per-cell source -> per-vertex values: the reference's per-cell branch of assemble_rhs
+// (`gpu/gpu_assembler.py:560-571`), f_vert[v] = f[searchsorted(elem_nodes, v)] with CuPy's
+// wrap-around fancy indexing.  elem_nodes (the flat 4C connectivity) is not sorted, so the
+// index is whatever CuPy's per-key bisection lands on; restated here per vertex: 4C when v is
+// above the last entry, else the left bisection over [0, 4C - 1].  Each key is searched on its
+// own, as CuPy's kernel does (numpy's searchsorted reuses the previous key's bounds for
+// increasing keys and can land elsewhere on an unsorted array).
+__global__ void k_cell_source_to_vertex(int32_t C, int32_t V, const int32_t* __restrict__ conn,
+                                        const double* __restrict__ fc, double* __restrict__ fv) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int64_t n = 4 * (int64_t)C;
+    int64_t p;
+    if (v > conn[n - 1]) {
+        p = n;
+    } else {
+        int64_t lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (conn[mid] < v)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        p = hi;
+    }
+    fv[v] = fc[p % C];
+}
+
 __global__ void k_bc_values(int32_t n_bc, const int32_t* __restrict__ bc, double value,
                             double* __restrict__ b) {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,13 +232,18 @@ int dsv_rhs(dsv_mesh* m, const double* u_old, const double* f, int32_t f_len, do
             void* stream) {
     return guard([&] {
         if (!m || !u_old || !b) throw ArgError{"dsv_rhs: NULL argument"};
-        if (f && f_len != m->V)
-            throw ArgError{"f length " + std::to_string(f_len) + " unrecognised (expected n_dofs=" +
-                           std::to_string(m->V) + ")"};
+        if (f && f_len != m->V && f_len != m->C)
+            throw ArgError{"f length " + std::to_string(f_len) + " unrecognised"};
         if (n_bc < 0 || (n_bc > 0 && !bc_dofs)) throw ArgError{"dsv_rhs: invalid bc_dofs"};
         DSV_HIP(hipSetDevice(m->device));
         validate_bc_list(m, bc_dofs, n_bc);
         hipStream_t st = (hipStream_t)stream;
+        if (f && f_len == m->C && f_len != m->V) {
+            if (!m->fvert) m->fvert = dalloc<double>(m->V);
+            hipLaunchKernelGGL(k_cell_source_to_vertex, dim3(cdiv(m->V, 256)), dim3(256), 0, st,
+                               m->C, m->V, m->conn, f, m->fvert);
+            f = m->fvert;
+        }
         hipLaunchKernelGGL(k_rhs, dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V, m->conn, m->detJ,
                            m->diag_pos, m->cptr, m->cslot, m->lumped_m, u_old, f, inv_dt,
                            lumped ? 1 : 0, b);

@@ -131,6 +131,9 @@ struct dsv_mesh {
     int32_t* cslot = nullptr;     // 16C slots sorted by (row, col, slot)
     double* lumped_m = nullptr;   // V  sum over incident cells of |detJ|/24
     double* Ae = nullptr;         // 16C cell matrices (assembly scratch)
+    double* fvert = nullptr;      // V  per-vertex values of a per-cell RHS source (scratch)
+    double* phys[3] = {};         // V+1 each: redistancing phi0 / ping-pong, scaffold offsets
+    double h_avg = 0.0;           // mean longest cell edge (computed once, dsv_mesh_h_avg)
     // internal (Morton) numbering for the matrix-free path
     int32_t* perm = nullptr;      // internal -> caller
     int32_t* iperm = nullptr;     // caller -> internal

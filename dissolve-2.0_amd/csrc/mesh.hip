@@ -418,6 +418,9 @@ static void destroy_mesh(dsv_mesh* m) {
     dfree(m->cptr);
     dfree(m->cslot);
     dfree(m->lumped_m);
+    dfree(m->Ae);
+    dfree(m->fvert);
+    for (auto& q : m->phys) dfree(q);
     dfree(m->perm);
     dfree(m->iperm);
     dfree(m->xyz32);

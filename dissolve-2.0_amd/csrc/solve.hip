@@ -2884,10 +2884,11 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_combine(int V, ProjVec<S>
         for (int k = 0; k < ND; ++k) {
             const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                x.v[s] += pv.c[k][s] * d.v[s];
-                r.v[s] -= pv.c[k][s] * Ad.v[s];
-            }
+            for (int s = 0; s < S; ++s)
+                if (pv.c[k][s] != 0.0) {  // a dropped direction may hold NaN: 0 * NaN = NaN
+                    x.v[s] += pv.c[k][s] * d.v[s];
+                    r.v[s] -= pv.c[k][s] * Ad.v[s];
+                }
         }
         pv.x[i] = x;
 #pragma unroll
@@ -2912,7 +2913,8 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_norms(int V, ProjVec<S> p
         for (int k = 0; k < ND; ++k) {
             const SV<double, S> Ad = pv.Ad[k][i];
 #pragma unroll
-            for (int s = 0; s < S; ++s) r.v[s] -= c[k][s] * Ad.v[s];
+            for (int s = 0; s < S; ++s)
+                if (c[k][s] != 0.0) r.v[s] -= c[k][s] * Ad.v[s];  // skip dropped directions
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) tot[s] += r.v[s] * r.v[s];
@@ -3103,10 +3105,11 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_accept_update(
         for (int k = 0; k < ND; ++k) {
             const SV<double, S> d = pv.d[k][i], Ad = pv.Ad[k][i];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                x.v[s] += c[k][s] * d.v[s];
-                rv.v[s] -= c[k][s] * Ad.v[s];
-            }
+            for (int s = 0; s < S; ++s)
+                if (c[k][s] != 0.0) {  // dropped or rejected: keep x and r exactly (NaN-safe)
+                    x.v[s] += c[k][s] * d.v[s];
+                    rv.v[s] -= c[k][s] * Ad.v[s];
+                }
         }
         pv.x[i] = x;
         r[i] = rv;

@@ -36,7 +36,8 @@ class CSRMatrix:
 
     def getValuesCSR(self):
         a = self._asm
-        return (a.d_indptr.cpu().numpy(), a.d_indices.cpu().numpy(), a.d_data.cpu().numpy())
+        data = a.d_data  # assembles a deferred operator first
+        return (a.d_indptr.cpu().numpy(), a.d_indices.cpu().numpy(), data.cpu().numpy())
 
     def to_scipy(self):
         import scipy.sparse as sp
@@ -78,10 +79,31 @@ class GPUSparseAssembler:
         self.n_cells = m.n_cells
         self.n_dofs = m.n_dofs
         self._nnz = m.nnz
-        self.d_data = torch.zeros(self._nnz, dtype=torch.float64, device=m.device)
+        self._d_data = torch.zeros(self._nnz, dtype=torch.float64, device=m.device)
+        self._pending = None
         self.d_b = torch.zeros(self.n_dofs, dtype=torch.float64, device=m.device)
         self.mat = CSRMatrix(self)
         self.lumped = LUMP_MASS
+
+    @property
+    def d_data(self):
+        """CSR values (nnz, FP64) of the last assembled operator; an operator deferred by a
+        solve (defer_assembly) is assembled here, on first read."""
+        if self._pending is not None:
+            De, inv_dt, alpha, bc_dofs = self._pending
+            self.assemble(De, inv_dt, alpha=alpha, bc_dofs=bc_dofs)
+        return self._d_data
+
+    def defer_assembly(self, De, inv_dt: float, alpha=None, bc_dofs=None):
+        """Records the coefficients of an operator that a solve used without assembling it
+        (the matrix-free and warm-started paths); `d_data` and `mat` assemble it when read."""
+        import torch
+
+        def snap(a):
+            if a is None:
+                return None
+            return a.clone() if isinstance(a, torch.Tensor) else np.array(a, copy=True)
+        self._pending = (snap(De), float(inv_dt), snap(alpha), snap(bc_dofs))
 
     # shared read-only device arrays (`gpu/gpu_assembler.py:388-438`)
     @property
@@ -130,8 +152,9 @@ class GPUSparseAssembler:
         """Fill d_data (and mat) with (1/dt + alpha_e) M_e + De_e K_e, then Dirichlet rows."""
         keep = []
         s = self.species(De, alpha, bc_dofs, keep=keep)
+        self._pending = None
         _lib.call("dsv_assemble", self.mesh.handle, s, float(inv_dt), int(self.lumped),
-                  ptr(self.d_data), stream_ptr(self.mesh.device))
+                  ptr(self._d_data), stream_ptr(self.mesh.device))
 
     def assemble_rhs(self, u_old, inv_dt: float, f=None, bc_dofs=None, bc_value: float = 0.0):
         """b = M u_old/dt + M f on the GPU; b[bc_dofs] = bc_value.  Returns d_b (torch)."""
@@ -143,18 +166,11 @@ class GPUSparseAssembler:
         d_f = None
         if f is not None:
             n = int(f.shape[0])
-            if n == self.n_dofs:
-                d_f = to_device(f, dev, torch.float64)
-            elif n == self.n_cells:
-                # the reference's lossy per-cell branch (`gpu/gpu_assembler.py:560-571`): each
-                # vertex takes f[searchsorted(elem_nodes, vertex)] with wrap-around indexing;
-                # the index map is host logic (numpy searchsorted on the flat connectivity)
-                pos = np.searchsorted(self.mesh.conn.ravel(), np.arange(self.n_dofs, dtype=np.int32))
-                f_host = f.detach().cpu().numpy() if isinstance(f, torch.Tensor) else np.asarray(f)
-                d_f = to_device(np.asarray(f_host, dtype=np.float64)[pos % self.n_cells], dev,
-                                torch.float64)
-            else:
+            if n != self.n_dofs and n != self.n_cells:
                 raise ValueError(f"f length {f.shape[0]} unrecognised")
+            # length V: per vertex; length C: the reference's per-cell branch
+            # (`gpu/gpu_assembler.py:560-571`), mapped to vertices inside dsv_rhs
+            d_f = to_device(f, dev, torch.float64)
         bc = None
         if bc_dofs is not None and len(bc_dofs) > 0:
             bc = to_device(bc_dofs, dev, torch.int32)

@@ -46,17 +46,47 @@ def _arr(f):
     return f.x.array if hasattr(f, "x") else f
 
 
+class WarmStartHistory:
+    """Device history of one species' solve starts, newest first (the Galerkin warm start's
+    directions are the differences of consecutive starts).  It stands in the reference's
+    per-name cache as entry "x" (`numerics/solvers.py:358-360`), whose ``set(0.0)`` the driver
+    calls after redistancing because the warm start has gone stale (`dissolve.py:710-719`):
+    here ``set`` discards the history, so the next solve starts from ``u`` alone."""
+
+    def __init__(self, n_dofs, device, depth):
+        import torch
+        self.bufs = [torch.empty(n_dofs, dtype=torch.float64, device=device)
+                     for _ in range(depth + 1)]
+        self.n = 0
+
+    def set(self, value=0.0):
+        self.n = 0
+
+    def push(self, u_host):
+        """Copies the new start into the oldest buffer and makes it the newest."""
+        import torch
+        new = self.bufs.pop()
+        new.copy_(torch.as_tensor(np.asarray(u_host), dtype=torch.float64))
+        self.bufs.insert(0, new)
+        self.n = min(self.n + 1, len(self.bufs))
+        return new
+
+
 def solve_transport(name: str, a_form, L_form, u, bcs: list = None, ksp_type: str = "gmres",
                     pc_type: str = "bjacobi", rtol: float = 1e-8, max_it: int = 1000,
                     under_relax: float = None, comm=None, extra_petsc_opts: dict = None,
                     ceed_op=None, use_gpu: bool = False, de_field=None, alpha_field=None,
                     u_old=None, source_field=None, bc_value: float = 0.0,
-                    method: str = "csr64") -> dict:
+                    method: str = "mf32_ir64", warm_start: str = "extrapolate",
+                    history_dirs: int = 8) -> dict:
     """Solve one species' backward-Euler step a(u, v) = L(v) on the GPU.
 
     Returns {"converged", "iterations", "time_s"} like the reference, plus "rel_residual".
-    `method` selects the libdsv solver: "csr64" (assembled FP64 CSR, the reference's
-    operator path), "mf64" or "mf32_ir64" (matrix-free)."""
+    `method` selects the libdsv solver: "mf32_ir64" (default: FP32 matrix-free CG inside FP64
+    refinement), "mf64" or "csr64" (assembled FP64 CSR, the reference's operator path).
+    `warm_start` "extrapolate" (default, matrix-free methods) starts from the Galerkin warm
+    start over the last `history_dirs` increments of this species' starts (u itself as the
+    fallback); "previous" starts from u, like the reference."""
     t0 = time.time()
     if bcs is None:
         bcs = []
@@ -71,12 +101,17 @@ def solve_transport(name: str, a_form, L_form, u, bcs: list = None, ksp_type: st
                                   "pc_type in {None, 'bjacobi', 'jacobi'}) is provided")
     if ksp_type not in ("gmres", "fgmres", "cg", "bcgs"):
         raise ValueError(f"unsupported ksp_type {ksp_type!r}")
+    if method not in _lib.METHODS:
+        raise ValueError(f"unknown method {method!r}")
+    if warm_start not in ("extrapolate", "previous"):
+        raise ValueError(f"unknown warm_start {warm_start!r}")
     try:
         converged, iterations, rel = _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type,
                                                     pc_type or "bjacobi", rtol, max_it, comm,
                                                     alpha_field=alpha_field, u_old=u_old,
                                                     source_field=source_field, bc_value=bc_value,
-                                                    method=method)
+                                                    method=method, warm_start=warm_start,
+                                                    history_dirs=history_dirs)
     except Exception as e:
         # as `numerics/solvers.py:264-271`: report and drop the species' cached assembler; there
         # is no CPU path to fall back to here, so the error propagates
@@ -92,38 +127,53 @@ def solve_transport(name: str, a_form, L_form, u, bcs: list = None, ksp_type: st
 
 def _solve_gpu_asm(name, de_field, L_form, u, bcs, ksp_type, pc_type, rtol, max_it, comm,
                    alpha_field=None, u_old=None, source_field=None, bc_value: float = 0.0,
-                   method: str = "csr64"):
-    """Per-name cached assembler; assemble, RHS, warm start from u, solve, read back
-    (`numerics/solvers.py:287-372`)."""
+                   method: str = "mf32_ir64", warm_start: str = "extrapolate",
+                   history_dirs: int = 8):
+    """Per-name cached assembler and warm-start history; RHS, start from u (or the Galerkin
+    warm start over the history), solve, read back (`numerics/solvers.py:287-372`)."""
+    import ctypes
     import torch
     mesh = _mesh_of(de_field)
     inv_dt = float(getattr(de_field, "_solver_inv_dt", 1.0))
     bc_dofs = getattr(de_field, "_solver_bc_dofs", None)
     alpha_arr = _arr(alpha_field)
+    depth = max(1, min(int(history_dirs), 8))
     cache = _GPU_ASM_CACHE.get(name)
     if cache is None:
         asm = GPUSparseAssembler(mesh, comm=comm, ref_asm=_SHARED_GEOM)
-        cache = {"asm": asm, "x": torch.zeros(asm.n_dofs, dtype=torch.float64, device=mesh.device)}
+        cache = {"asm": asm, "x": WarmStartHistory(asm.n_dofs, mesh.device, depth),
+                 "sol": torch.zeros(asm.n_dofs, dtype=torch.float64, device=mesh.device)}
         _GPU_ASM_CACHE[name] = cache
-    asm = cache["asm"]
+    elif len(cache["x"].bufs) != depth + 1:
+        cache["x"] = WarmStartHistory(cache["asm"].n_dofs, mesh.device, depth)
+    asm, hist, sol = cache["asm"], cache["x"], cache["sol"]
     if u_old is None:
         raise NotImplementedError("the DOLFINx RHS branch needs L_form; pass u_old")
     d_b = asm.assemble_rhs(_arr(u_old), inv_dt, f=_arr(source_field), bc_dofs=bc_dofs,
                            bc_value=bc_value)
-    x = cache["x"]
-    x.copy_(torch.as_tensor(np.asarray(u.x.array), dtype=torch.float64))
+    start = hist.push(u.x.array)
     keep = []
     mesh.fresh_operator()
     s = asm.species(_arr(de_field), alpha_arr, bc_dofs, bc_value, keep=keep)
     s.b = ptr(d_b).value
-    s.x = ptr(x).value
+    s.x = ptr(sol).value
+    arr = species_array([s])
     st = _lib.Stats()
-    _lib.call("dsv_solve", mesh.handle, 1, species_array([s]), inv_dt, int(asm.lumped),
-              _lib.METHODS[method], float(rtol), int(max_it), stream_ptr(mesh.device), st)
-    if method == "csr64":
-        # keep the assembled operator visible as asm.mat, like the reference's cached Mat
-        asm.assemble(_arr(de_field), inv_dt, alpha=alpha_arr, bc_dofs=bc_dofs)
-    u.x.array[:] = x.cpu().numpy()
+    n_hist = hist.n - 1 if (warm_start == "extrapolate" and method != "csr64") else 0
+    stream = stream_ptr(mesh.device)
+    if n_hist > 0:
+        steps = hist.bufs[:n_hist + 1]
+        harr = (ctypes.c_void_p * len(steps))(*[ptr(t).value for t in steps])
+        _lib.call("dsv_solve_warm_history", mesh.handle, 1, arr, inv_dt, int(asm.lumped),
+                  _lib.METHODS[method], float(rtol), int(max_it), n_hist, harr, 0, None, stream,
+                  st, None, None, None)
+    else:
+        sol.copy_(start)
+        _lib.call("dsv_solve", mesh.handle, 1, arr, inv_dt, int(asm.lumped),
+                  _lib.METHODS[method], float(rtol), int(max_it), stream, st)
+    # asm.mat is assembled from these coefficients only if the caller reads it
+    asm.defer_assembly(_arr(de_field), inv_dt, alpha=alpha_arr, bc_dofs=bc_dofs)
+    u.x.array[:] = sol.cpu().numpy()
     return bool(st.converged[0]), int(st.iterations[0]), float(st.rel_residual[0])
 
 
@@ -196,5 +246,9 @@ class MatrixFreeOperator:
         self.mat = self
 
     def mult(self, A, x, y):
+        import torch
         out = self.ceed_op.apply(x, None)
-        y[:] = out.cpu().numpy() if hasattr(y, "shape") and not hasattr(y, "device") else out
+        if isinstance(y, torch.Tensor):
+            y.copy_(out)
+        else:
+            y[:] = out.cpu().numpy()

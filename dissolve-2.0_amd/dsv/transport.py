@@ -259,8 +259,11 @@ class Stepper:
         if self.collect and feats is not None:
             for k in range(self.S):
                 X, sigma = feats[k]
-                y = ((self.x[k] - 2.0 * self.u[k] + self.u_hist[0][k]) / sigma.double()).float()
-                self.pairs.append((X, y))
+                # sigma = max |u_n - u_{n-1}| is 0 for a species left unchanged by its previous
+                # solve: such a pair has no target scale and is dropped when the pairs are fitted
+                y = ((self.x[k] - 2.0 * self.u[k] + self.u_hist[0][k]) /
+                     sigma.double().clamp_min(1e-30)).float()
+                self.pairs.append((X, y, sigma))
         # rotate the buffers instead of copying them: the solution becomes u, u becomes
         # u_{n-1}, and the oldest history buffer holds the next step's start
         oldest = self.u_hist[-1]
@@ -273,6 +276,38 @@ class Stepper:
         d["step_s"] = time.perf_counter() - t0
         self.history.append(d)
         return d
+
+    def fp64_reference(self, rtol=1e-13, max_it=100000):
+        """The last step's systems solved again by the plain FP64 path (assembled FP64 CSR,
+        FP64 Jacobi-PCG) to `rtol`, starting from the step's solutions: a yardstick for the
+        solutions of the faster paths (their error is max |u - ref| / max |ref| per species).
+        Returns (list of FP64 device tensors, solver statistics)."""
+        import torch
+        if self.n_steps == 0:
+            raise ValueError("no step taken yet")
+        keep = []
+        refs, structs = [], []
+        for k, s in enumerate(self.defs):
+            x = self.u[k].clone()
+            b = self.b[k].clone()
+            keep += [x, b]
+            refs.append(x)
+            structs.append(species_struct(self.mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                                          alpha=self.alpha_dev[k],
+                                          alpha_value=s.get("alpha_value", 0.0),
+                                          bc_dofs=self.bc[k], bc_value=s.get("bc_value", 0.0),
+                                          b=b, x=x, keep=keep))
+        st = _lib.Stats()
+        _lib.call("dsv_solve", self.mesh.handle, self.S, species_array(structs), 1.0 / self.dt,
+                  self.lumped, _lib.METHODS["csr64"], float(rtol), int(max_it),
+                  stream_ptr(self.mesh.device), st)
+        torch.cuda.synchronize(self.mesh.device)
+        return refs, st.as_dict(self.S)
+
+    def step_error(self, refs):
+        """max |u_s - ref_s| / max |ref_s| per species for the last step's solutions."""
+        return [float((u - r).abs().max() / r.abs().max().clamp_min(1e-300))
+                for u, r in zip(self.u, refs)]
 
     def train_warm_start(self, n_steps, epochs=20, batch=16384, before_step=None):
         """Run n_steps steps (with the current warm start; `before_step()` is called before each
@@ -290,10 +325,18 @@ class Stepper:
         self.collect = False
         if not self.pairs:
             raise ValueError("no training pairs (needs at least 2 steps)")
-        X = torch.cat([p[0] for p in self.pairs])
-        y = torch.cat([p[1] for p in self.pairs])
+        sig = torch.cat([p[2] for p in self.pairs]).cpu()
+        use = [p for p, sg in zip(self.pairs, sig.tolist())
+               if sg > 0.0 and np.isfinite(sg) and bool(torch.isfinite(p[1]).all())]
+        self.pairs = []
+        if not use:
+            raise ValueError("no usable training pairs (every species unchanged or non-finite)")
+        X = torch.cat([p[0] for p in use])
+        y = torch.cat([p[1] for p in use])
         self.tau = max(float(torch.sqrt(torch.mean(y.double() ** 2))), 1e-30)
         hist = self.mlp.fit(X, (y / self.tau).float(), epochs=epochs, batch=batch)
-        self.pairs = []
+        if not (np.isfinite(self.tau) and np.all(np.isfinite(hist))):
+            self.mlp.trained = False
+            raise ValueError(f"the warm-start fit diverged (tau {self.tau}, loss {hist[-1]})")
         self.warm = "learned"
         return hist

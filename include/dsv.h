@@ -126,8 +126,11 @@ int dsv_assemble(dsv_mesh* m, const dsv_species* sp, double inv_dt, int32_t lump
                  double* values, void* stream);
 
 /* --- RHS: GPUSparseAssembler.assemble_rhs (`gpu/gpu_assembler.py:534-603`) ---
- * b = M (inv_dt u_old + f) with f per vertex (f_len == V) or absent (f == NULL), then
- * b[bc] = bc_value. */
+ * b = M (inv_dt u_old + f) with f per vertex (f_len == V), per cell (f_len == C, V != C) or
+ * absent (f == NULL), then b[bc] = bc_value.  A per-cell f takes the reference's per-cell
+ * branch (`:560-571`): vertex v gets f[p mod C], p = CuPy's searchsorted of v in the flat
+ * caller connectivity (4C entries, not sorted): 4C if v > conn[4C-1], else the left bisection
+ * over [0, 4C-1], each vertex searched on its own. */
 int dsv_rhs(dsv_mesh* m, const double* u_old, const double* f, int32_t f_len, double inv_dt,
             int32_t lumped, const int32_t* bc_dofs, int32_t n_bc, double bc_value, double* b,
             void* stream);
@@ -289,6 +292,72 @@ int dsv_transport_coefficients(dsv_mesh* m, const double* phi, const double* gra
                                const double* F_film, const double* Cl, const double* O2,
                                const dsv_coef_params* params, double* alpha_o2, double* f_oh,
                                double* film_factor, double* medium_factor, void* stream);
+
+/* --- the per-step physics around the solves: Zn2+ coefficients, film, under-relaxation,
+ * interface velocity, level-set advance and redistancing (device arrays of length V in the
+ * caller's vertex numbering) --- */
+
+typedef struct dsv_film_params {
+    double kf;    /* film formation rate */
+    double kd;    /* film dissolution coefficient */
+    double Fmax;  /* film saturation */
+} dsv_film_params;
+
+/* Zn2+ coefficients of the Zn species block (`dissolve.py:740-778`), per vertex:
+ *   alpha_zn = [phi <= 0] kf (1 - F / Fmax),  src_zn = [phi <= 0] kd F Cl^2
+ * (F_film, Cl NULL -> 0; alpha_zn, src_zn may be NULL), and the scaffold Dirichlet vertices
+ * {v : phi_v > 0} in ascending order into scaffold (device, capacity V) with their number in
+ * *n_scaffold (host; the call then synchronises the stream).  scaffold and n_scaffold are both
+ * NULL or both given. */
+int dsv_zinc_coefficients(dsv_mesh* m, const double* phi, const double* F_film, const double* Cl,
+                          const dsv_film_params* params, double* alpha_zn, double* src_zn,
+                          int32_t* scaffold, int32_t* n_scaffold, void* stream);
+
+/* The film update of the film block (`dissolve.py:797-833`), the lumped pointwise solution of
+ * (1/dt + alpha) F = F_old / dt + src:
+ *   F_new = clip((F_old inv_dt + src) / (inv_dt + alpha), 0, Fmax),
+ *   alpha = [phi <= 0] (kf Zn / Fmax + kd Cl^2),  src = [phi <= 0] kf Zn.
+ * F_new may alias F_old. */
+int dsv_film_update(dsv_mesh* m, const double* phi, const double* Zn, const double* Cl,
+                    const double* F_old, const dsv_film_params* params, double inv_dt,
+                    double* F_new, void* stream);
+
+/* u = omega u_new + (1 - omega) u_prev for n values, solve_transport's under-relaxation
+ * (`numerics/solvers.py:278-279`); u may alias either input. */
+int dsv_under_relax(int32_t n, const double* u_new, const double* u_prev, double omega,
+                    double* u, void* stream);
+
+typedef struct dsv_velocity_params {
+    double h;         /* mean cell size: the interface band is |phi| < 3 h */
+    double k_orr;     /* ORR rate */
+    double zn_solid;  /* molar density of solid Zn, [Zn]_sol */
+    double zn_sat;    /* saturation concentration [Zn]_sat (the Zn Dirichlet value) */
+} dsv_velocity_params;
+
+/* compute_interface_velocity_paper (`physics/interface_velocity.py:622-785`, the default branch:
+ * nodal O2, no exact-ORR override, no probe): per vertex, inside |phi| < 3 h,
+ *   v = min(max(v_Zn, v_ORR, v_O2), 0),  n = grad phi / sqrt(|grad phi|^2 + 1e-12),
+ *   v_Zn = -max(DeZn, 0) |grad Zn . n| / (zn_solid - zn_sat)  (zn_solid if that is ~0),
+ *   v_O2 = -2 max(DeO2, 0) |grad O2 . n| / zn_solid,  v_ORR = -(2 / zn_solid) k_orr max(O2, 0),
+ * and 0 outside; the gradients are the nodal gradients of dsv_nodal_gradient. */
+int dsv_interface_velocity(dsv_mesh* m, const double* phi, const double* Zn, const double* O2,
+                           const double* DeZn, const double* DeO2,
+                           const dsv_velocity_params* params, double* v, void* stream);
+
+/* The algebraic level-set update phi = phi_old + v dt (`dissolve.py:684-690`); phi may alias
+ * phi_old. */
+int dsv_levelset_advance(dsv_mesh* m, const double* phi_old, const double* v, double dt,
+                         double* phi, void* stream);
+
+/* Mean over cells of the longest edge (DOLFINx mesh.h of a tetrahedron), the h_avg of
+ * `domain/mesh_utils.py:559-575`; computed once per mesh, synchronous. */
+int dsv_mesh_h_avg(dsv_mesh* m, double* h_avg);
+
+/* Sussman redistancing, `redistance` (`domain/mesh_utils.py:450-542`), in place: num_iters
+ * Jacobi pseudo-time iterations phi += dtau sign (1 - |grad phi|) outside the band
+ * |phi0| < 3 h that stays frozen, dtau = 0.5 h / max(|grad phi|, 0.5),
+ * sign = phi0 / sqrt(phi0^2 + (1.5 h)^2), phi0 the input; h <= 0 takes dsv_mesh_h_avg. */
+int dsv_redistance(dsv_mesh* m, double* phi, int32_t num_iters, double h, void* stream);
 
 /* --- learned warm start (no counterpart in the reference, whose solves start from the
  * previous solution): a per-vertex MLP 7 -> 64 -> 64 -> 1 with tanh activations, FP32

@@ -150,11 +150,38 @@ def apply_dirichlet_rows(bc_dofs, indptr, indices, data):
     return data
 
 
+def searchsorted_left_per_key(a, keys):
+    """CuPy's ``searchsorted(a, keys)`` (side 'left', array taken as increasing), the call of
+    the reference's per-cell RHS branch (`gpu/gpu_assembler.py:565-569`; CuPy is
+    ``cupy-cuda12x``, version unpinned, `INSTALL.md:373,839`).  CuPy searches every key on its
+    own: len(a) when key > a[-1], else the left bisection over [0, len(a) - 1]
+    (mid = lo + (hi - lo) // 2; a[mid] < key -> lo = mid + 1, else hi = mid; answer hi).  On a
+    sorted array that is the usual insertion point; on the unsorted flat connectivity it is
+    whatever the bisection path lands on.  numpy's ``searchsorted`` differs there: for
+    increasing keys it starts each bisection from the previous key's answer.  Vectorised over
+    the keys (every key takes the same number of halvings of its own interval)."""
+    a = np.asarray(a)
+    keys = np.asarray(keys)
+    n = len(a)
+    lo = np.zeros(len(keys), dtype=np.int64)
+    hi = np.full(len(keys), n - 1, dtype=np.int64)
+    active = lo < hi
+    while active.any():
+        mid = lo + (hi - lo) // 2
+        go_right = active & (a[np.minimum(mid, n - 1)] < keys)
+        go_left = active & ~go_right
+        lo = np.where(go_right, mid + 1, lo)
+        hi = np.where(go_left, mid, hi)
+        active = lo < hi
+    return np.where(keys > a[n - 1], n, hi)
+
+
 def assemble_rhs(conn, xyz, u_old, inv_dt, f=None, lumped=1, bc_dofs=None, bc_value=0.0):
     """b = M (inv_dt u_old + f), lumped (dJ/24 per vertex) or consistent, then b[bc] = bc_value
     (`gpu/gpu_assembler.py:176-242,534-603`).  A per-cell f follows the reference's
-    per-cell branch literally (`:560-571`): f_vert = f[searchsorted(conn.ravel(), arange(V))]
-    with out-of-range indices wrapped as CuPy's fancy indexing does."""
+    per-cell branch (`:560-571`): f_vert = f[searchsorted(conn.ravel(), arange(V))] with CuPy's
+    per-key search (``searchsorted_left_per_key``) and out-of-range indices wrapped as CuPy's
+    fancy indexing does."""
     n_cells, n_dofs = len(conn), len(xyz)
     u_old = np.asarray(u_old, dtype=np.float64)
     if u_old.shape[0] != n_dofs:
@@ -164,7 +191,7 @@ def assemble_rhs(conn, xyz, u_old, inv_dt, f=None, lumped=1, bc_dofs=None, bc_va
     elif len(f) == n_dofs:
         fv = np.asarray(f, dtype=np.float64)
     elif len(f) == n_cells:
-        pos = np.searchsorted(conn.ravel(), np.arange(n_dofs, dtype=np.int32))
+        pos = searchsorted_left_per_key(conn.ravel(), np.arange(n_dofs, dtype=np.int32))
         fv = np.asarray(f, dtype=np.float64)[pos % n_cells]
     else:
         raise ValueError(f"f length {len(f)} unrecognised")

@@ -80,12 +80,16 @@ def under_relax(u_new, u_prev, omega):
 def _c_lib():
     import ctypes
     import os
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liboracle_pcg.so")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "liboracle.so")
     lib = ctypes.CDLL(path)
     lib.oracle_pcg.restype = ctypes.c_int
     P = ctypes.c_void_p
-    lib.oracle_pcg.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.c_double, ctypes.c_int,
-                               ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    I, Dbl = ctypes.c_int, ctypes.c_double
+    lib.oracle_pcg.argtypes = [I, P, P, P, P, P, Dbl, I, I, ctypes.POINTER(Dbl)]
+    lib.oracle_assemble.restype = None
+    lib.oracle_assemble.argtypes = [I, P, P, P, P, Dbl, I, P, P, P, I, P, I, P, I]
+    lib.oracle_rhs.restype = None
+    lib.oracle_rhs.argtypes = [I, I, P, P, P, P, Dbl, I, P, I, Dbl, P, I]
     return lib
 
 
@@ -103,3 +107,54 @@ def pcg_jacobi_c(indptr, indices, data, b, x0, rtol, max_it=10000, threads=0):
     it = lib.oracle_pcg(len(bb), c(ip), c(ix), c(a), c(bb), c(x), float(rtol), int(max_it),
                         int(threads), ctypes.byref(rel))
     return x, it, rel.value
+
+
+class CStep:
+    """One species' backward-Euler step in the oracle's C restatement (oracle/assemble.c and
+    oracle/pcg.c): assembly through the scatter map, RHS, Jacobi-PCG.  The pattern and the
+    scatter map are built once (numpy), like the reference's assembler set-up
+    (`gpu/gpu_assembler.py:369-450`); assemble / rhs / solve are the per-step work."""
+
+    def __init__(self, conn, xyz):
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.C, self.V = len(conn), len(xyz)
+        self.indptr, self.indices = assembly.pattern(self.conn, self.V)
+        self.smap = np.ascontiguousarray(assembly.scatter_map(self.conn, self.indptr, self.indices),
+                                         dtype=np.int32)
+        self.values = np.zeros(len(self.indices))
+        self.lib = _c_lib()
+
+    @staticmethod
+    def _p(a):
+        import ctypes
+        return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+    def assemble(self, De, inv_dt, alpha=None, lumped=1, bc_dofs=None, threads=1):
+        De_c = np.ascontiguousarray(assembly.cell_average(De, self.conn, self.C, self.V))
+        al_c = np.ascontiguousarray(assembly.cell_average(alpha, self.conn, self.C, self.V))
+        bc = None if bc_dofs is None or len(bc_dofs) == 0 else \
+            np.ascontiguousarray(bc_dofs, dtype=np.int32)
+        p = self._p
+        self.lib.oracle_assemble(self.C, p(self.conn), p(self.xyz), p(De_c), p(al_c),
+                                 float(inv_dt), int(lumped), p(self.smap), p(self.indptr),
+                                 p(self.indices), len(self.indices), p(bc),
+                                 0 if bc is None else len(bc), p(self.values), int(threads))
+        return self.values
+
+    def rhs(self, u_old, inv_dt, f=None, lumped=1, bc_dofs=None, bc_value=0.0, threads=1):
+        u = np.ascontiguousarray(u_old, dtype=np.float64)
+        fv = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        if fv is not None and len(fv) != self.V:
+            raise ValueError("CStep.rhs takes a per-vertex f")
+        bc = None if bc_dofs is None or len(bc_dofs) == 0 else \
+            np.ascontiguousarray(bc_dofs, dtype=np.int32)
+        b = np.zeros(self.V)
+        p = self._p
+        self.lib.oracle_rhs(self.C, self.V, p(self.conn), p(self.xyz), p(u), p(fv), float(inv_dt),
+                            int(lumped), p(bc), 0 if bc is None else len(bc), float(bc_value),
+                            p(b), int(threads))
+        return b
+
+    def solve(self, b, x0, rtol, max_it=20000, threads=1):
+        return pcg_jacobi_c(self.indptr, self.indices, self.values, b, x0, rtol, max_it, threads)

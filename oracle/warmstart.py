@@ -11,7 +11,8 @@ Pinning (tests/test_oracle_warmstart.py): the exact solution increment as the on
 gives the exact solution; the residual of the result is orthogonal to every direction
 (Galerkin condition); the A-energy error of the result is below that of any other point of
 x0 + span{d_k} (brute force over random combinations); duplicated and zero directions are
-dropped without changing the result; a direction orthogonal to r0 leaves x0 in place.
+dropped without changing the result; a direction orthogonal to r0 leaves x0 in place; a
+direction holding NaN is dropped and the others are still used.
 """
 import numpy as np
 
@@ -53,7 +54,8 @@ def galerkin_start(A, b, x0, dirs, bc_dofs=None, bc_value=0.0):
     G = D.T @ AD
     G = 0.5 * (G + G.T)
     c = solve_gram(G, D.T @ r0)
-    x = x0 + D @ c
+    used = c != 0.0  # a dropped direction (c = 0) adds nothing, even if it holds NaN or Inf
+    x = x0 + D[:, used] @ c[used]
     n0, n1 = np.linalg.norm(r0), np.linalg.norm(b - A @ x)
     if not (np.isfinite(n1) and n1 < n0):
         return x0, np.zeros_like(c), n0, n0

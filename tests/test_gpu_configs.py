@@ -4,9 +4,11 @@
   (663,552 tets): pattern digest bit-exact, matrix and RHS norms within 1e-12, and the step
   solution of every method at rtol 1e-10 within 1e-8 of the golden fixture (oracle direct /
   rtol-1e-13 solve, tests/golden/).
-* 806k-tet graded disc, three species: three batched steps of the chosen path (FP32 CG in FP64
-  refinement, Galerkin warm start); the last step's solutions satisfy the independently
-  assembled oracle system to rtol 1e-8 and agree with the plain FP64 CSR path.
+* 806k-tet graded disc, three species: step 1 of every method at rtol 1e-10 within 1e-8 of the
+  golden sparse-LU solution (tests/golden/disc3_step1.npz); five steps of the FP32 path
+  (Galerkin warm start and previous-solution start) within 1e-8 at every step of the plain
+  FP64 path's rtol-1e-13 solution of the same system, the last step satisfying the
+  independently assembled oracle system.
 * 14.7M-tet gyroid, three species: the stored-geometry FP32 apply agrees with the FP64 and the
   recomputing FP32 apply, and one batched step converges.
 """
@@ -89,32 +91,83 @@ def test_step_solution(gpu, name, method):
     assert np.abs(x - ref).max() <= 1e-8 * np.abs(ref).max(), (method, st)
 
 
-def test_disc3_batched_steps_satisfy_the_oracle_system(gpu):
+_DISC = {}
+
+
+def _disc3():
     from dsv.core import Mesh
+    if not _DISC:
+        import hashlib
+        w = workloads.disc3()
+        fx = np.load(os.path.join(HERE, "golden", "disc3_step1.npz"), allow_pickle=False)
+        h = hashlib.sha256()
+        h.update(np.ascontiguousarray(w["conn"], dtype=np.int32).tobytes())
+        h.update(np.ascontiguousarray(w["xyz"], dtype=np.float64).tobytes())
+        assert h.hexdigest() == str(fx["mesh_sha256"]), "disc3 mesh differs from the fixture's"
+        _DISC.update(w=w, fx=fx, mesh=Mesh(w["conn"], w["xyz"]))
+    return _DISC["w"], _DISC["mesh"], _DISC["fx"]
+
+
+def _fixture_error(fx, k, name, x):
+    """max over the fixture's vertices of |x - x*| / max |x*| (x* the direct solve)."""
+    ref = fx["x_" + name]
+    return float(np.abs(x[::int(fx["stride"])] - ref).max() / float(fx["max_" + name]))
+
+
+@pytest.mark.parametrize("method", ["csr64", "mf64", "mf32_ir64"])
+def test_disc3_step1_matches_direct_solve(gpu, method):
+    """The first three-species step of the 806k-tet disc at rtol 1e-10 against the golden
+    fixture (sparse-LU solve of the oracle system): every species within 1e-8 of max |x*|.  The
+    same step at rtol 1e-8 misses that bar (O2 about 9e-7, Cl 2e-8 on the CPU), which is why the
+    bar is tested at 1e-10."""
+    from dsv.transport import Stepper
+    w, mesh, fx = _disc3()
+    print(f"[config] disc3 step 1 {method}: {len(w['conn'])} tets", flush=True)
+    st = Stepper(mesh, w["species"], w["dt"], method=method, rtol=1e-10)
+    h = st.step()
+    assert all(h["converged"]), h
+    for k, s in enumerate(w["species"]):
+        err = _fixture_error(fx, k, s["name"], st.u[k].cpu().numpy())
+        print(f"  {s['name']}: error vs direct solve {err:.2e}", flush=True)
+        assert err <= 1e-8, (s["name"], err, h)
+
+
+@pytest.mark.parametrize("warm", ["extrapolate", "previous"])
+def test_disc3_steps_within_1e8_of_fp64(gpu, warm):
+    """Five steps of the FP32 path (Galerkin warm start from the second step, or the previous
+    solution) at rtol 1e-10: at every step each species' solution is within 1e-8 of the same
+    system solved by the plain FP64 path to rtol 1e-13, which is itself within 1e-9 of the
+    sparse-LU fixture at step 1; the last step satisfies the independently assembled oracle
+    system to rtol 1e-10."""
     from dsv.transport import Stepper
     from oracle import assembly as OA
-    w = workloads.disc3()
-    print(f"[config] disc3: {len(w['conn'])} tets", flush=True)
-    mesh = Mesh(w["conn"], w["xyz"])
-    chosen = Stepper(mesh, w["species"], w["dt"], method="mf32_ir64", rtol=w["rtol"])
-    chosen.set_warm_start("extrapolate")
-    plain = Stepper(mesh, w["species"], w["dt"], method="csr64", rtol=w["rtol"])
-    for _ in range(3):
-        hc, hp = chosen.step(), plain.step()
-        assert all(hc["converged"]) and all(hp["converged"]), (hc, hp)
+    w, mesh, fx = _disc3()
+    print(f"[config] disc3 5 steps {warm}", flush=True)
+    st = Stepper(mesh, w["species"], w["dt"], method="mf32_ir64", rtol=1e-10)
+    if warm != "previous":
+        st.set_warm_start(warm)
+    for n in range(5):
+        h = st.step()
+        assert all(h["converged"]), (n, h)
+        refs, hr = st.fp64_reference(1e-13)
+        assert all(hr["converged"]), hr
+        if n == 0:
+            for k, s in enumerate(w["species"]):
+                assert _fixture_error(fx, k, s["name"], refs[k].cpu().numpy()) <= 1e-9
+        errs = st.step_error(refs)
+        print(f"  step {n + 1}: iterations {h['iterations']} error vs FP64 {errs}", flush=True)
+        assert max(errs) <= 1e-8, (n, errs, h)
     inv_dt = 1.0 / w["dt"]
     for k, s in enumerate(w["species"]):
-        u_old = chosen.u_prev[k].cpu().numpy()
-        x = chosen.u[k].cpu().numpy()
+        u_old = st.u_prev[k].cpu().numpy()
+        x = st.u[k].cpu().numpy()
         ip, ix, data = OA.assemble_values(w["conn"], w["xyz"], s["D_value"], inv_dt, s["alpha"],
                                           1, s["bc_dofs"])
         A = OA.to_scipy(ip, ix, data, len(w["xyz"]))
         b = OA.assemble_rhs(w["conn"], w["xyz"], u_old, inv_dt, s["f"], 1, s["bc_dofs"],
                             s["bc_value"])
         rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
-        assert rel <= 1.01e-8, (s["name"], rel)
-        xp = plain.u[k].cpu().numpy()
-        assert np.abs(x - xp).max() <= 1e-5 * np.abs(xp).max(), s["name"]
+        assert rel <= 1.01e-10, (s["name"], rel)
 
 
 def test_gyroid_apply_forms_and_one_step(gpu):

@@ -91,10 +91,11 @@ def test_assembly_values(torch_dev, name, lumped, coef):
     np.testing.assert_array_equal(ix2, ix)
 
 
+@pytest.mark.parametrize("name", ["disc_small", "kuhn5_shuffled"])
 @pytest.mark.parametrize("lumped", [1, 0])
-def test_rhs(torch_dev, lumped):
+def test_rhs(torch_dev, lumped, name):
     from dsv.gpu_assembler import GPUSparseAssembler
-    m, conn, xyz = _m("disc_small")
+    m, conn, xyz = _m(name)
     rng = np.random.default_rng(4)
     V = len(xyz)
     u, f = rng.uniform(0, 1, V), rng.uniform(-1, 1, V)
@@ -104,10 +105,13 @@ def test_rhs(torch_dev, lumped):
     got = asm.assemble_rhs(u, 200.0, f=f, bc_dofs=bc, bc_value=0.3).cpu().numpy()
     ref = OA.assemble_rhs(conn, xyz, u, 200.0, f, lumped, bc, 0.3)
     assert _relerr(got, ref) < 1e-12
-    # per-cell f: the reference's lossy branch
+    # per-cell f: the reference's lossy branch (CuPy's per-key search of every vertex in the
+    # unsorted flat connectivity, on the GPU), with and without Dirichlet rows
     fc = rng.uniform(0, 1, len(conn))
     got = asm.assemble_rhs(u, 200.0, f=fc).cpu().numpy()
     assert _relerr(got, OA.assemble_rhs(conn, xyz, u, 200.0, fc, lumped)) < 1e-12
+    got = asm.assemble_rhs(u, 200.0, f=fc, bc_dofs=bc, bc_value=0.3).cpu().numpy()
+    assert _relerr(got, OA.assemble_rhs(conn, xyz, u, 200.0, fc, lumped, bc, 0.3)) < 1e-12
     with pytest.raises(ValueError):
         asm.assemble_rhs(u[:-1], 200.0)
     with pytest.raises(ValueError):

@@ -340,3 +340,62 @@ def test_solve_warm_history_fewer_species(gpu, pick):
     for i in range(S):
         assert np.array_equal(xa[i], xb[i])
         assert np.abs(xb[i] - exact[i]).max() <= 1e-7 * np.abs(exact[i]).max()
+
+
+@pytest.mark.parametrize("method", ["mf32_ir64", "mf64"])
+def test_nan_direction_keeps_start_finite(gpu, method):
+    """An explicit direction holding NaN for one species (and one of Inf for another) is
+    dropped: the coefficients and residual norms of the start match the oracle's galerkin_start
+    over the same directions (which drops them too and still uses the history), the start and
+    the solution stay finite, and the solve converges to the direct solution."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import Mesh, species_array, species_struct, stream_ptr, ptr
+    conn, xyz, sps, u0, mats, bs, inv_dt, rng = _case("kuhn7_jitter_morton", seed=29)
+    mesh = Mesh(conn, xyz)
+    V = len(xyz)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=gpu)
+    exact = [OS.direct(A, b) for A, b in zip(mats, bs)]
+    n_hist = 2
+    past = [[u0[s] - (j + 1) * 0.3 * (exact[s] - u0[s]) + rng.normal(0, 1e-3, V)
+             for s in range(3)] for j in range(n_hist)]
+    extra = [rng.normal(0, 1, V) for _ in range(3)]
+    extra[1][[3, 50, 200]] = np.nan
+    extra[2][7] = np.inf
+    rows = [[t(u0[s]) for s in range(3)]] + [[t(v) for v in r] for r in past]
+    ext = [t(v) for v in extra]
+    bt = [t(b) for b in bs]
+    x = [torch.full((V,), float("nan"), dtype=torch.float64, device=gpu) for _ in range(3)]
+    keep = [v for r in rows for v in r] + ext + bt + x
+    structs = [species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                              alpha=s.get("alpha"), alpha_value=s.get("alpha_value", 0.0),
+                              bc_dofs=s.get("bc_dofs"), bc_value=s.get("bc_value", 0.0),
+                              b=bt[k], x=x[k], keep=keep) for k, s in enumerate(sps)]
+    harr = (ctypes.c_void_p * (3 * len(rows)))(*[ptr(v).value for r in rows for v in r])
+    earr = (ctypes.c_void_p * 3)(*[ptr(v).value for v in ext])
+    nd = n_hist + 1
+    coef = (ctypes.c_double * (3 * nd))()
+    r0, r1 = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+    st = _lib.Stats()
+    _lib.call("dsv_solve_warm_history", mesh.handle, 3, species_array(structs), inv_dt, 1,
+              _lib.METHODS[method], 1e-10, 5000, n_hist, harr, 1, earr, stream_ptr(gpu), st,
+              coef, r0, r1)
+    torch.cuda.synchronize()
+    d = st.as_dict(3)
+    assert all(d["converged"]), d
+    for s in range(3):
+        seq = [u0[s]] + [past[j][s] for j in range(n_hist)]
+        dirs = [seq[j] - seq[j + 1] for j in range(n_hist)] + [extra[s]]
+        xr, cr, n0, n1 = OW.galerkin_start(mats[s], bs[s], u0[s], dirs, sps[s].get("bc_dofs"),
+                                           sps[s].get("bc_value", 0.0))
+        assert np.isfinite(r1[s]) and abs(r0[s] - n0) <= 1e-9 * n0
+        assert abs(r1[s] - n1) <= 1e-9 * n0
+        for k in range(nd):
+            assert abs(coef[k * 3 + s] - cr[k]) <= 1e-8 * max(np.abs(cr).max(), 1e-30), (s, k)
+        if s > 0:
+            assert coef[n_hist * 3 + s] == 0.0 and cr[n_hist] == 0.0
+            assert n1 < n0  # the history directions are still taken
+        xs = x[s].cpu().numpy()
+        assert np.all(np.isfinite(xs))
+        assert np.abs(xs - exact[s]).max() <= 1e-7 * np.abs(exact[s]).max()

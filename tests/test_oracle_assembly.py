@@ -152,3 +152,75 @@ def test_dirichlet_rows_and_rhs():
         A.assemble_rhs(conn, xyz, u[:-1], 5.0)
     # total mass = volume
     assert abs(m.sum() - np.abs(meshes.tet_signed_volume(conn, xyz)).sum()) < 1e-14
+
+
+def _jittered(n=4, seed=11):
+    conn, xyz = meshes.kuhn_cube(n)
+    rng = np.random.default_rng(seed)
+    interior = ~meshes.boundary_vertex_mask(xyz)
+    return conn, meshes.jitter(xyz, 1.0 / n, 0.2, rng, interior), rng
+
+
+def test_rhs_equals_mass_matrix_times_fields():
+    # b(u, f) = M (inv_dt u + f) with M the consistent (lumped) mass matrix assembled by
+    # assemble_values(De=0, inv_dt=1, alpha=None), whose element matrices the unit-tet closed
+    # forms above pin; non-linear u and f exercise every term of both RHS branches
+    conn, xyz, rng = _jittered()
+    V = len(xyz)
+    u = np.sin(3.0 * xyz[:, 0]) * np.exp(xyz[:, 1]) + rng.uniform(0, 1, V)
+    f = xyz[:, 2] ** 3 - 2.0 * xyz[:, 0] * xyz[:, 1] + rng.normal(size=V)
+    inv_dt = 7.0
+    for lumped in (0, 1):
+        ip, ix, data = A.assemble_values(conn, xyz, 0.0, 1.0, None, lumped)
+        M = A.to_scipy(ip, ix, data, V)
+        for uu, ff in ((u, f), (np.zeros(V), f), (u, None)):
+            ref = M @ (inv_dt * uu + (ff if ff is not None else 0.0))
+            b = A.assemble_rhs(conn, xyz, uu, inv_dt, ff, lumped)
+            np.testing.assert_allclose(b, ref, rtol=1e-13, atol=1e-15 * np.abs(ref).max())
+    # the lumped mass is the row sum of the consistent one
+    ip, ix, data = A.assemble_values(conn, xyz, 0.0, 1.0, None, 0)
+    Mc = A.to_scipy(ip, ix, data, V)
+    np.testing.assert_allclose(np.asarray(Mc.sum(axis=1)).ravel(), A.lumped_mass(conn, xyz),
+                               rtol=1e-13)
+
+
+def _cupy_searchsorted_stdlib(a, key):
+    """CuPy's per-key search written with the standard library's bisect_left (CPython's
+    textbook bisection: mid = (lo + hi) // 2, a[mid] < key -> lo = mid + 1, else hi = mid)."""
+    import bisect
+    if key > a[-1]:
+        return len(a)
+    return bisect.bisect_left(a, key, 0, len(a) - 1)
+
+
+def test_per_key_searchsorted_matches_stdlib_bisect():
+    conn, xyz, rng = _jittered(3, seed=2)
+    keys = np.arange(len(xyz), dtype=np.int32)
+    # sorted input: the usual insertion point
+    srt = np.sort(conn.ravel())
+    np.testing.assert_array_equal(A.searchsorted_left_per_key(srt, keys), np.searchsorted(srt, keys))
+    # unsorted flat connectivity, as the reference passes it, in several orders
+    for trial in range(4):
+        c = conn[rng.permutation(len(conn))] if trial else conn
+        flat = c.ravel()
+        lst = flat.tolist()
+        ref = np.array([_cupy_searchsorted_stdlib(lst, int(k)) for k in keys])
+        np.testing.assert_array_equal(A.searchsorted_left_per_key(flat, keys), ref)
+    # keys above every entry go to len(a), as do keys above the last entry of an unsorted array
+    a = np.array([5, 1, 9, 3])
+    np.testing.assert_array_equal(A.searchsorted_left_per_key(a, np.array([10, 4, 0])),
+                                  [_cupy_searchsorted_stdlib(a.tolist(), k) for k in (10, 4, 0)])
+
+
+def test_per_cell_rhs_source_uses_per_key_search():
+    # the per-cell branch maps vertex v to f[p mod C] with p the per-key search of v
+    conn, xyz, rng = _jittered(3, seed=4)
+    conn = conn[rng.permutation(len(conn))]
+    C, V = len(conn), len(xyz)
+    fc = rng.normal(size=C)
+    lst = conn.ravel().tolist()
+    fv = np.array([fc[_cupy_searchsorted_stdlib(lst, v) % C] for v in range(V)])
+    u = rng.uniform(size=V)
+    for lumped in (0, 1):
+        np.testing.assert_array_equal(A.assemble_rhs(conn, xyz, u, 3.0, fc, lumped),
+                                      A.assemble_rhs(conn, xyz, u, 3.0, fv, lumped))

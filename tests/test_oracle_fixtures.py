@@ -46,3 +46,29 @@ def test_cube20_fixture_solution_is_a_backward_euler_step():
     x = fx["x"]
     assert np.linalg.norm(b - A @ x) <= 1e-12 * np.linalg.norm(b)
     assert np.all(x[s["bc_dofs"]] == 1.0)
+
+
+def test_disc3_fixture_matches_an_independent_cg_solve():
+    """The disc3 step-1 fixture (sparse LU) against the oracle's C Jacobi-PCG of the same
+    re-assembled systems to rtol 1e-12, at the fixture's vertices: within 1e-9 of max |x|; the
+    mesh is the one the generator builds (digest)."""
+    from oracle import assembly as OA
+    from oracle import solve as OS
+    fx = _load("disc3_step1")
+    w = workloads.disc3()
+    assert MF.mesh_digest(w["conn"], w["xyz"]) == str(fx["mesh_sha256"])
+    stride = int(fx["stride"])
+    inv_dt = 1.0 / w["dt"]
+    for s in w["species"]:
+        ip, ix, data = OA.assemble_values(w["conn"], w["xyz"], s["D_value"], inv_dt, s["alpha"],
+                                          1, s["bc_dofs"])
+        b = OA.assemble_rhs(w["conn"], w["xyz"], s["u0"], inv_dt, s["f"], 1, s["bc_dofs"],
+                            s["bc_value"])
+        x0 = np.array(s["u0"], float)
+        if s["bc_dofs"] is not None:
+            x0[s["bc_dofs"]] = s["bc_value"]
+        x, it, rel = OS.pcg_jacobi_c(ip, ix, data, b, x0, 1e-12, 20000, 8)
+        assert rel <= 1e-12
+        ref = fx["x_" + s["name"]]
+        assert np.abs(x[::stride] - ref).max() <= 1e-9 * float(fx["max_" + s["name"]]), s["name"]
+        assert abs(np.abs(x).max() - float(fx["max_" + s["name"]])) <= 1e-9

@@ -87,3 +87,28 @@ def test_c_pcg_matches_numpy_pcg():
 def test_under_relaxation():
     a, b = np.array([1.0, 2.0]), np.array([3.0, 6.0])
     np.testing.assert_allclose(S.under_relax(a, b, 0.3), 0.3 * a + 0.7 * b)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+@pytest.mark.parametrize("lumped", [0, 1])
+def test_c_step_matches_numpy_oracle(threads, lumped):
+    """oracle/assemble.c (assembly through the scatter map, RHS) against the numpy oracle, per
+    cell and per vertex coefficients, with Dirichlet rows; then its PCG step against splu."""
+    conn, xyz = _mesh(6, seed=3)
+    rng = np.random.default_rng(9)
+    C, V = len(conn), len(xyz)
+    cs = S.CStep(conn, xyz)
+    bc = np.arange(0, V, 11, dtype=np.int32)
+    for D, al in ((rng.lognormal(0, 0.5, C), rng.uniform(0, 2, V)), (1.3, None)):
+        vals = cs.assemble(D, 50.0, al, lumped, bc, threads).copy()
+        ip, ix, ref = A.assemble_values(conn, xyz, D, 50.0, al, lumped, bc)
+        assert np.abs(vals - ref).max() <= 1e-12 * np.abs(ref).max()
+    u, f = rng.uniform(0, 1, V), rng.normal(size=V)
+    b = cs.rhs(u, 50.0, f, lumped, bc, 0.4, threads)
+    bref = A.assemble_rhs(conn, xyz, u, 50.0, f, lumped, bc, 0.4)
+    assert np.abs(b - bref).max() <= 1e-12 * np.abs(bref).max()
+    x0 = u.copy()
+    x0[bc] = 0.4
+    x, it, rel = cs.solve(b, x0, 1e-12, threads=threads)
+    xd = S.direct(A.to_scipy(ip, ix, ref, V), bref)
+    assert rel <= 1e-12 and np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()

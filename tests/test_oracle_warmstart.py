@@ -91,3 +91,19 @@ def test_solve_gram_matches_numpy():
     G = M @ M.T + 0.1 * np.eye(3)
     g = rng.normal(0, 1, 3)
     np.testing.assert_allclose(W.solve_gram(G, g), np.linalg.solve(G, g), rtol=1e-12)
+
+
+def test_nan_direction_is_dropped_and_the_others_used(system):
+    # a direction holding NaN gets c = 0 and the result equals the start over the others
+    Am, b, u0, xs, bc, free, rng = system
+    x0 = u0.copy()
+    x0[bc] = 0.7
+    good = [0.5 * (xs - x0) + rng.normal(0, 1e-2, len(u0)), rng.normal(0, 1, len(u0))]
+    bad = rng.normal(0, 1, len(u0))
+    bad[free[:3]] = np.nan
+    x, c, n0, n1 = W.galerkin_start(Am, b, u0, [good[0], bad, good[1]], bc, 0.7)
+    xr, cr, m0, m1 = W.galerkin_start(Am, b, u0, good, bc, 0.7)
+    assert c[1] == 0.0 and np.all(np.isfinite(x))
+    np.testing.assert_allclose([c[0], c[2]], cr, rtol=1e-10)
+    np.testing.assert_allclose(x, xr, rtol=0, atol=1e-12 * np.abs(xr).max())
+    assert n1 < n0 and abs(n1 - m1) <= 1e-10 * n0
