@@ -164,6 +164,40 @@ def error_replay(mesh, w, method, steps, warmup, rtol, warm, history_dirs, final
             "replay_bit_identical": same}
 
 
+def run_coupled(mesh, w, steps, warmup, rtol, levelset=False):
+    """The reference-order four-species step (dsv.coupled.CoupledStepper: Zn, Cl and O2
+    batched, O2 under-relaxed, the film, OH from the new fields; the diffusivities follow the
+    film) on the same mesh: plain FP64 path vs the chosen path, K timed steps after W warm-up
+    steps each."""
+    import torch
+    from dsv import workloads
+    from dsv.coupled import CoupledStepper
+    cw = workloads.coupled_disc(w["conn"], w["xyz"])
+    out = {"workload": f"{w['name']} mesh, Zn/Cl/film/O2/OH in the order of the reference's "
+                       "species blocks, film-dependent diffusivities, O2 omega = 0.3"
+                       + (", interface velocity + level-set advance + redistancing every step"
+                          if levelset else ""),
+           "rtol": rtol}
+    for name, method, warm in (("plain_fp64", "csr64", "previous"),
+                               ("chosen", "mf32_ir64", "extrapolate")):
+        cs = CoupledStepper(mesh, cw, method=method, rtol=rtol, warm=warm, levelset=levelset)
+        for _ in range(warmup):
+            cs.step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hs = [cs.step() for _ in range(steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        out[name] = {"steps_per_s": round(steps / dt, 3),
+                     "cg_iterations_per_step": round(sum(sum(h["iterations"]) for h in hs) / steps, 1),
+                     "iterations_last_step": dict(zip(hs[-1]["names"], hs[-1]["iterations"])),
+                     "converged": all(all(h["converged"]) for h in hs)}
+        log(f"coupled {name}{' + level set' if levelset else ''}: {out[name]}")
+        del cs
+    out["chosen_vs_plain"] = round(out["chosen"]["steps_per_s"] / out["plain_fp64"]["steps_per_s"], 3)
+    return out
+
+
 def cpu_baseline(w, all_threads):
     """The oracle's C restatement of one 3-species step of the same mesh from u0 (bounded
     sample): per species assembly through the scatter map, RHS, Jacobi-PCG to the same rtol,
@@ -393,6 +427,8 @@ def main():
     ap.add_argument("--bar-rtol", type=float, default=5e-11,
                     help="rtol of the *_bar paths: the tolerance at which every species' step "
                          "solution is within 1e-8 of the FP64 solution")
+    ap.add_argument("--no-coupled", action="store_true",
+                    help="skip the reference-order four-species step legs")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the untimed error replays against the FP64 solutions")
     ap.add_argument("--error-every", type=int, default=1,
@@ -482,6 +518,11 @@ def main():
         log(f"error replay {name}: max per species {errors[name]['max_per_species']}, "
             f"bit-identical replay {errors[name]['replay_bit_identical']} "
             f"({time.perf_counter() - t:.1f}s)")
+
+    coupled = None
+    if not a.no_coupled and w["name"] == "disc3":
+        coupled = {"static": run_coupled(mesh, w, a.steps, a.warmup, rtol),
+                   "with_interface": run_coupled(mesh, w, a.steps, a.warmup, rtol, levelset=True)}
 
     # roofline of the fused FP32 3-species CG iteration (the chosen path's dominant kernel)
     ms_iter, ms_apply = time_kernels(mesh, w, a.apply_reps)
@@ -584,6 +625,7 @@ def main():
                           "(untimed replay of the timed run)",
             "bar": 1e-8, "legs": errors},
         "at_error_bar": bar,
+        "reference_order_step": coupled,
         "iterations_last_step": chosen["iterations_per_step"][-1],
         "outer_last_step": chosen["outer_per_step"][-1],
         "converged": all(r["converged"] for r in res.values()),

@@ -35,7 +35,8 @@ EXPORTS = (
     "dsv_mlp_features",
     "dsv_mlp_forward",
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam", "dsv_nodal_gradient",
-    "dsv_transport_coefficients",
+    "dsv_transport_coefficients", "dsv_zinc_coefficients", "dsv_film_update", "dsv_under_relax",
+    "dsv_interface_velocity", "dsv_levelset_advance", "dsv_mesh_h_avg", "dsv_redistance",
 )
 
 
@@ -62,6 +63,17 @@ class CoefParams(ctypes.Structure):
     """dsv_coef_params (include/dsv.h)."""
     _fields_ = [("eps", ctypes.c_double), ("k_orr", ctypes.c_double), ("kd", ctypes.c_double),
                 ("Fmax", ctypes.c_double), ("film_floor", ctypes.c_double)]
+
+
+class FilmParams(ctypes.Structure):
+    """dsv_film_params (include/dsv.h)."""
+    _fields_ = [("kf", ctypes.c_double), ("kd", ctypes.c_double), ("Fmax", ctypes.c_double)]
+
+
+class VelocityParams(ctypes.Structure):
+    """dsv_velocity_params (include/dsv.h)."""
+    _fields_ = [("h", ctypes.c_double), ("k_orr", ctypes.c_double),
+                ("zn_solid", ctypes.c_double), ("zn_sat", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -142,6 +154,15 @@ def load():
         "dsv_nodal_gradient": (ctypes.c_int, [P, P, P, P, P]),
         "dsv_transport_coefficients": (ctypes.c_int, [P, P, P, P, P, P, ctypes.POINTER(CoefParams),
                                                       P, P, P, P, P]),
+        "dsv_zinc_coefficients": (ctypes.c_int, [P, P, P, P, ctypes.POINTER(FilmParams), P, P, P,
+                                                 ctypes.POINTER(I32), P]),
+        "dsv_film_update": (ctypes.c_int, [P, P, P, P, P, ctypes.POINTER(FilmParams), D, P, P]),
+        "dsv_under_relax": (ctypes.c_int, [I32, P, P, D, P, P]),
+        "dsv_interface_velocity": (ctypes.c_int, [P, P, P, P, P, P, ctypes.POINTER(VelocityParams),
+                                                  P, P]),
+        "dsv_levelset_advance": (ctypes.c_int, [P, P, P, D, P, P]),
+        "dsv_mesh_h_avg": (ctypes.c_int, [P, ctypes.POINTER(D)]),
+        "dsv_redistance": (ctypes.c_int, [P, P, I32, D, P]),
         "dsv_mlp_features": (ctypes.c_int, [I32, P, P, P, P, P, D, D, D, D, P, P, P, P]),
         "dsv_mlp_forward": (ctypes.c_int, [P, P, I32, P, P]),
         "dsv_mlp_grad_workspace": (ctypes.c_size_t, [I32]),

@@ -121,6 +121,34 @@ def three_species(conn, xyz, phi, bc_dofs, dt, eps, k_orr=1.0, c0=(1.0, 1.0, 0.1
                 k_orr=k_orr, grad_norm=ng)
 
 
+# nondimensional parameters of the four-species coupled step (same roles as the reference's
+# `config/parameters.py` kf, kd, Fmax, film porosity / constrictivity / tortuosity, D_i0, Znbc)
+COUPLED_PARAMS = dict(kf=5.0, kd=2.0, Fmax=1.0, film_floor=0.25 * 1.0 / 2.0 ** 2,
+                      D0=(0.7, 2.0, 5.3, 2.0), Znbc=1.0, o2_bc_value=1.0, k_orr=1.0, omega=0.3,
+                      orr_eps_factor=2.0)
+
+
+def coupled_disc(conn, xyz, z_if=0.19, dt=0.005, name="coupled"):
+    """The four-species step of the reference's species blocks (`dissolve.py:740-954`) on a disc
+    mesh: Zn (scaffold Dirichlet at phi > 0), Cl, the film, O2 (ORR sink, bottom Dirichlet,
+    under-relaxation 0.3) and OH (source from the new film, Cl and O2), with the effective
+    diffusivities of the film (`physics/interface_velocity.py:90-155`).  phi = z - z_if (the
+    metal layer on top), the ORR band eps = 2 h_avg (`dissolve.py:371`), no film at the start.
+    Returns dict(conn, xyz, dt, params, state) with state = phi, grad_phi, Zn, Cl, O2, OH, F
+    (per-vertex arrays) and o2_bc (the bottom-face vertices)."""
+    p = dict(COUPLED_PARAMS)
+    phi = xyz[:, 2] - z_if
+    h = h_avg(conn, xyz)
+    p["eps"] = p["orr_eps_factor"] * h
+    p["h"] = h
+    medium = 1.0 - smoothed_heaviside(phi, p["eps"])
+    st = dict(phi=phi, grad_phi=nodal_grad_norm(conn, xyz, phi),
+              Zn=np.where(phi > 0.0, p["Znbc"], 0.0), Cl=medium.copy(), O2=medium.copy(),
+              OH=0.1 * medium, F=np.zeros(len(xyz)),
+              o2_bc=np.nonzero(xyz[:, 2] <= 1e-12)[0].astype(np.int32))
+    return dict(name=name, conn=conn, xyz=xyz, dt=dt, params=p, state=st)
+
+
 def disc3(seed=0, scale=1.47, xy_factor=0.94, z_if=0.19):
     """seed 0: 805,903 tets, 165,863 vertices."""
     conn, xyz = meshes.graded_disc(seed=seed, scale=scale, xy_factor=xy_factor)
