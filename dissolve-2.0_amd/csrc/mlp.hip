@@ -15,6 +15,8 @@
 // order; a second kernel sums the per-workgroup gradients in workgroup order.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <unordered_map>
 
 #include "dsv_internal.h"
 #include "kernels_common.h"
@@ -230,6 +232,31 @@ __global__ void k_mlp_features(int n, const double* __restrict__ u, const double
     x[6] = bcmask && bcmask[i] ? 1.f : 0.f;
 }
 
+// Static scratch of dsv_mlp_grad_noalloc: the partial gradients of at most NOALLOC_WG
+// workgroups (4.85 MB), held by every GPU the library runs on until the process ends.  Every
+// call writes all the partials it reads, so nothing needs clearing; calls that overlap on one
+// GPU share this array and race.
+constexpr int NOALLOC_WG = 256;
+constexpr int NOALLOC_MAX_N = NOALLOC_WG * GRAD_WG;  // 16384 samples
+static __device__ __align__(256) float scratch[(size_t)NOALLOC_WG * (MLP_NPARAM + 1)];
+
+// Device address of `scratch` on the current device, looked up once per device (the array's
+// name is a host-side stand-in, not a device address).
+static float* scratch_on_current_device() {
+    static std::mutex mu;
+    static std::unordered_map<int, float*> addr;
+    int dev = 0;
+    DSV_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    DSV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw ArgError{"dsv_mlp_grad_noalloc: no device address for the scratch array"};
+    addr[dev] = static_cast<float*>(p);
+    return static_cast<float*>(p);
+}
+
 }  // namespace dsv
 
 using namespace dsv;
@@ -266,6 +293,25 @@ int dsv_mlp_grad(const float* params, const float* X, const float* y, int32_t n,
                            (float*)workspace);
         hipLaunchKernelGGL(k_mlp_grad_reduce, dim3(cdiv(MLP_NPARAM + 1, 256)), dim3(256), 0, st,
                            (const float*)workspace, nwg, n, grad, loss);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_mlp_grad_noalloc(const float* params, const float* X, const float* y, int32_t n,
+                         float* grad, float* loss, void* stream) {
+    return guard([&] {
+        if (!params || !X || !y || !grad || n < 1)
+            throw ArgError{"dsv_mlp_grad_noalloc: bad arguments"};
+        if (n > NOALLOC_MAX_N)
+            throw ArgError{"dsv_mlp_grad_noalloc: n = " + std::to_string(n) +
+                           " exceeds the static scratch (at most " +
+                           std::to_string(NOALLOC_MAX_N) + " samples); use dsv_mlp_grad"};
+        float* work = scratch_on_current_device();
+        const int nwg = (int)cdiv(n, GRAD_WG);  // <= NOALLOC_WG, the same as dsv_mlp_grad's
+        hipStream_t st = (hipStream_t)stream;
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(GRAD_WG), 0, st, params, X, y, n, work);
+        hipLaunchKernelGGL(k_mlp_grad_reduce, dim3(cdiv(MLP_NPARAM + 1, 256)), dim3(256), 0, st,
+                           (const float*)work, nwg, n, grad, loss);
         DSV_HIP(hipGetLastError());
     });
 }

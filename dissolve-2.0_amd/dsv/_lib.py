@@ -34,7 +34,7 @@ EXPORTS = (
     "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_solve_warm_history",
     "dsv_mlp_features",
     "dsv_mlp_forward",
-    "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_adam", "dsv_nodal_gradient",
+    "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_grad_noalloc", "dsv_mlp_adam", "dsv_nodal_gradient",
     "dsv_transport_coefficients", "dsv_zinc_coefficients", "dsv_film_update", "dsv_under_relax",
     "dsv_interface_velocity", "dsv_levelset_advance", "dsv_mesh_h_avg", "dsv_redistance",
 )
@@ -167,6 +167,7 @@ def load():
         "dsv_mlp_forward": (ctypes.c_int, [P, P, I32, P, P]),
         "dsv_mlp_grad_workspace": (ctypes.c_size_t, [I32]),
         "dsv_mlp_grad": (ctypes.c_int, [P, P, P, I32, P, P, P, ctypes.c_size_t, P]),
+        "dsv_mlp_grad_noalloc": (ctypes.c_int, [P, P, P, I32, P, P, P]),
         "dsv_mlp_adam": (ctypes.c_int, [P, P, P, P, I32, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_float, ctypes.c_float, P]),
     }

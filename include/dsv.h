@@ -381,6 +381,14 @@ size_t dsv_mlp_grad_workspace(int32_t n);
  * deterministic (fixed summation order, no atomics). */
 int dsv_mlp_grad(const float* params, const float* X, const float* y, int32_t n, float* grad,
                  float* loss, void* workspace, size_t workspace_bytes, void* stream);
+/* dsv_mlp_grad that allocates nothing: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving or training
+ * loops, harnesses that forbid hipMalloc).  The partial gradients go to a static device array
+ * of the library (4.85 MB per GPU, held until the process ends) instead of a workspace, so
+ * n is limited to 16384 samples (DSV_E_ARG beyond that).  Results are the bits of dsv_mlp_grad
+ * for the same inputs.  Calls that overlap on one GPU share that array and race. */
+int dsv_mlp_grad_noalloc(const float* params, const float* X, const float* y, int32_t n,
+                         float* grad, float* loss, void* stream);
 /* One Adam step (step >= 1 counts from 1) on all DSV_MLP_NPARAM parameters. */
 int dsv_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr,
                  float beta1, float beta2, float eps, void* stream);
