@@ -253,6 +253,32 @@ def workload_text(w, C, V, rtol):
             f"rtol={rtol:g}")
 
 
+def assembly_bytes(C, V, nnz, s):
+    """Algorithmic bytes of one FP64 CSR assembly of species s: connectivity 16 C, FP64
+    coordinates 24 V, the values written 8 nnz, and 8 bytes per value of a coefficient field."""
+    def coef(a):
+        return 0 if a is None else 8 * len(a)
+    return 16 * C + 24 * V + 8 * nnz + coef(s.get("D")) + coef(s.get("alpha"))
+
+
+def time_assembly(mesh, w, reps):
+    """ms per launch of dsv_assemble's kernel for the first species with a reaction field (no
+    Dirichlet rows), HIP events around `reps` graph-replayed launches."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import species_array, species_struct
+    s = next((x for x in w["species"] if x.get("alpha") is not None), w["species"][0])
+    keep = []
+    st = species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0), alpha=s.get("alpha"),
+                        keep=keep)
+    stream = torch.cuda.Stream()
+    ms = ctypes.c_double()
+    _lib.call("dsv_time_apply", mesh.handle, 1, species_array([st]), 1.0 / w["dt"], 1, 64, 2,
+              reps, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(ms))
+    return ms.value, s
+
+
 def copy_floor(nbytes, reps):
     """A plain device copy moving the same bytes as one apply (nbytes/2 read, nbytes/2 written),
     `reps` copies captured in one hipGraph and replayed between two HIP events: what a kernel
@@ -318,6 +344,11 @@ def profile_run(mesh, w, reps):
     torch.cuda.synchronize()
     log(f"fused CG iteration {ms_iter * 1e3:.1f} us, apply {ms_apply * 1e3:.1f} us "
         f"({reps} launches each plus 3 warm-up)")
+    ms_asm, s_asm = time_assembly(mesh, w, reps)
+    torch.cuda.synchronize()
+    log(f"FP64 CSR assembly ({s_asm['name']}) {ms_asm * 1e3:.1f} us, "
+        f"{assembly_bytes(len(w['conn']), len(w['xyz']), mesh.nnz, s_asm)} algorithmic bytes "
+        f"({reps} launches plus 3 warm-up)")
 
 
 def kernels_at_scale(shapes=((256, 256, 1), (256, 256, 2))):
@@ -534,6 +565,10 @@ def main():
         f"matrix-free apply alone {ms_apply * 1e3:.1f} us, {apply_gbs:.0f} GB/s")
     traffic = measured_traffic()
     copy_ref = copy_floor(abytes, a.apply_reps)
+    ms_asm, s_asm = time_assembly(mesh, w, a.apply_reps)
+    asm_bytes = assembly_bytes(C, V, mesh.nnz, s_asm)
+    asm_gbs = asm_bytes / (ms_asm * 1e-3) / 1e9
+    log(f"FP64 CSR assembly ({s_asm['name']}): {ms_asm * 1e3:.1f} us/launch, {asm_gbs:.0f} GB/s")
 
     scale = None
     if not a.no_scale and a.workload == "disc3":
@@ -642,6 +677,11 @@ def main():
                       "us_per_launch": round(ms_apply * 1e3, 2), "achieved_gbs": round(apply_gbs, 1),
                       "frac": round(apply_gbs / HBM_PEAK_GBS, 4),
                       "copy_of_same_bytes": copy_ref},
+        "assembly_hbm": {"kernel": "k_assemble_rows (FP64 CSR assembly, one species, the plain "
+                                   "path's per-step operator)",
+                         "species": s_asm["name"], "algorithmic_bytes_per_launch": asm_bytes,
+                         "us_per_launch": round(ms_asm * 1e3, 2), "achieved_gbs": round(asm_gbs, 1),
+                         "frac": round(asm_gbs / HBM_PEAK_GBS, 4)},
         "kernels_at_scale": scale,
         "cpu_baseline": cpu,
     }

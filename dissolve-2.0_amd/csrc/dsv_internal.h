@@ -130,7 +130,13 @@ struct dsv_mesh {
     int32_t* cptr = nullptr;      // nnz+1 contribution ranges per CSR entry
     int32_t* cslot = nullptr;     // 16C slots sorted by (row, col, slot)
     double* lumped_m = nullptr;   // V  sum over incident cells of |detJ|/24
-    double* Ae = nullptr;         // 16C cell matrices (assembly scratch)
+    int32_t* vinc_ptr = nullptr;  // V+1 incident-cell ranges per vertex (row assembly)
+    int32_t* vinc = nullptr;      // 4C  cell * 4 + local corner, ascending cell per vertex
+    int32_t* rblk = nullptr;      // n_rblk+1 first rows of the assembly workgroups
+    int32_t n_rblk = 0;
+    uint16_t* loc = nullptr;      // 16C LDS location of every contribution (assembly)
+    double* vrec0 = nullptr;      // 4V  caller-order (x, y, z, 0) FP64 records (assembly)
+    int32_t* rdesc = nullptr;     // 4 n_rblk  workgroup descriptors of the assembly
     double* fvert = nullptr;      // V  per-vertex values of a per-cell RHS source (scratch)
     double* phys[3] = {};         // V+1 each: redistancing phi0 / ping-pong, scaffold offsets
     double h_avg = 0.0;           // mean longest cell edge (computed once, dsv_mesh_h_avg)
@@ -170,6 +176,8 @@ namespace dsv {
 void build_partition(dsv_mesh* m, int NV, int L);
 void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped, double* values,
                    hipStream_t st);
+double time_assembly(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped, int n_rep,
+                     hipStream_t stream);
 void free_partition(Partition& p);
 void free_workspaces(dsv_mesh* m);
 void ensure_geometry_views(dsv_mesh* m);

@@ -418,7 +418,12 @@ static void destroy_mesh(dsv_mesh* m) {
     dfree(m->cptr);
     dfree(m->cslot);
     dfree(m->lumped_m);
-    dfree(m->Ae);
+    dfree(m->vinc_ptr);
+    dfree(m->vinc);
+    dfree(m->rblk);
+    dfree(m->loc);
+    dfree(m->vrec0);
+    dfree(m->rdesc);
     dfree(m->fvert);
     for (auto& q : m->phys) dfree(q);
     dfree(m->perm);

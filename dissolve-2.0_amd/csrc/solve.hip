@@ -1647,7 +1647,6 @@ void free_workspaces(dsv_mesh* m) {
             free_ws(m->nws[a][b]);
             m->nws[a][b] = nullptr;
         }
-    dfree(m->Ae);
 }
 
 static int max_lds_per_block(int device) {
@@ -3671,10 +3670,16 @@ int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
         check_solve_args(m, n_species, sp, false);
         if (!ms || n_rep < 1) throw ArgError{"dsv_time_apply: bad arguments"};
         if (precision != 32 && precision != 64) throw ArgError{"precision must be 32 or 64"};
-        if (mode != 0 && mode != 1) throw ArgError{"mode must be 0 (apply) or 1 (CG iteration)"};
+        if (mode < 0 || mode > 2)
+            throw ArgError{"mode must be 0 (apply), 1 (CG iteration) or 2 (CSR assembly)"};
         DSV_HIP(hipSetDevice(m->device));
         DSV_HIP(hipStreamSynchronize((hipStream_t)stream));
         hipStream_t s = (hipStream_t)stream;
+        if (mode == 2) {
+            if (sp[0].n_bc > 0) throw ArgError{"dsv_time_apply: mode 2 times no Dirichlet rows"};
+            *ms = time_assembly(m, sp[0], inv_dt, lumped, n_rep, s);
+            return;
+        }
         switch (n_species * 100 + precision) {
             case 132: *ms = time_impl<float, 1>(m, sp, inv_dt, lumped, mode, n_rep, s); break;
             case 232: *ms = time_impl<float, 2>(m, sp, inv_dt, lumped, mode, n_rep, s); break;

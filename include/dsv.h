@@ -255,8 +255,9 @@ int dsv_set_operator_form(dsv_mesh* m, int32_t form);
  * warm-up launches on `stream`, then n_rep launches captured in one hipGraph and replayed on
  * the library's stream of the mesh between two HIP events.  mode 0 times the plain apply
  * y = A x (x = sp[s].x, or the lumped mass when NULL); mode 1 times one fused CG iteration as
- * dsv_solve launches it (vector updates, w = A u and the partials of the next iteration).  The
- * FP32 kernels use the operator form set by dsv_set_operator_form. */
+ * dsv_solve launches it (vector updates, w = A u and the partials of the next iteration); mode 2
+ * times dsv_assemble's kernel for sp[0] (FP64 CSR values, no Dirichlet rows: sp[0].n_bc must be
+ * 0) on `stream`.  The FP32 kernels use the operator form set by dsv_set_operator_form. */
 int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
                    int32_t lumped, int32_t precision, int32_t mode, int32_t n_rep, void* stream,
                    double* ms);
