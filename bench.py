@@ -58,9 +58,11 @@ def cg_iteration_bytes(C, V, species):
     return apply_bytes(C, V, species) + 36 * V * len(species)
 
 
-def measured_traffic(path=os.path.join(ROOT, "profiles", "k_cg_disc3_traffic.md")):
+def measured_traffic(workload="disc3"):
     """HBM bytes per launch of the fused CG iteration from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel factors), or None."""
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel factors), or
+    None."""
+    path = os.path.join(ROOT, "profiles", f"k_cg_{workload}_traffic.md")
     try:
         with open(path) as f:
             for line in f:
@@ -563,7 +565,7 @@ def main():
     apply_gbs = abytes / (ms_apply * 1e-3) / 1e9
     log(f"fused FP32 CG iteration: {ms_iter * 1e3:.1f} us/launch, {achieved:.0f} GB/s; "
         f"matrix-free apply alone {ms_apply * 1e3:.1f} us, {apply_gbs:.0f} GB/s")
-    traffic = measured_traffic()
+    traffic = measured_traffic(w["name"])
     copy_ref = copy_floor(abytes, a.apply_reps)
     ms_asm, s_asm = time_assembly(mesh, w, a.apply_reps)
     asm_bytes = assembly_bytes(C, V, mesh.nnz, s_asm)
@@ -677,7 +679,7 @@ def main():
                       "us_per_launch": round(ms_apply * 1e3, 2), "achieved_gbs": round(apply_gbs, 1),
                       "frac": round(apply_gbs / HBM_PEAK_GBS, 4),
                       "copy_of_same_bytes": copy_ref},
-        "assembly_hbm": {"kernel": "k_assemble_rows (FP64 CSR assembly, one species, the plain "
+        "assembly_hbm": {"kernel": "k_assemble_seg (FP64 CSR assembly, one species, the plain "
                                    "path's per-step operator)",
                          "species": s_asm["name"], "algorithmic_bytes_per_launch": asm_bytes,
                          "us_per_launch": round(ms_asm * 1e3, 2), "achieved_gbs": round(asm_gbs, 1),

@@ -84,7 +84,10 @@ __global__ __launch_bounds__(SEG_THREADS) void k_assemble_seg(
     __shared__ double T[4 * SEG_PAIRS];               // T[j * SEG_PAIRS + pair]
     __shared__ int32_t cp[4 * SEG_PAIRS + 1];         // the block's contribution ranges
     __shared__ uint16_t lc[4 * SEG_PAIRS];            // the block's contribution locations
-    const int4 d = desc[blockIdx.x];
+    // XCD-aware order: workgroup b runs on XCD b mod 8, and XCD x takes a contiguous run of
+    // row blocks, so that neighbouring blocks (which share cells and vertices) meet in one L2
+    const int nb = gridDim.x, x = blockIdx.x & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int4 d = desc[x * q8 + min(x, r8) + (int)(blockIdx.x >> 3)];
     const int l = threadIdx.x;
     const int4* conn4 = reinterpret_cast<const int4*>(conn);
     // the pairs' incidence entries, and (independent of them) the block's contribution lists,
