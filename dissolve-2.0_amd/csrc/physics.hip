@@ -277,6 +277,116 @@ __global__ __launch_bounds__(HAVG_THREADS) void k_cell_diameter_partials(
     }
 }
 
+// ---- scaffold volumes: per-workgroup partial sums of a per-cell quantity times |det J| / 6,
+// summed on the host in workgroup order (deterministic)
+constexpr int VOL_THREADS = 256;
+constexpr int VOL_MAX_WG = 1024;
+
+__device__ __forceinline__ double smoothed_heaviside(double p, double eps) {
+    return p < -eps ? 0.0
+                    : (p > eps ? 1.0 : 0.5 * (1.0 + p / eps + sin(M_PI * p / eps) / M_PI));
+}
+
+// Volume of the corner tetrahedron cut off at corner a by the plane phi = 0, as a fraction of
+// the cell: the product of the edge parameters a / (a - v_k) over the other three corners
+__device__ __forceinline__ double corner_fraction(const double (&v)[4], int a) {
+    double r = 1.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k != a) r *= v[a] / (v[a] - v[k]);
+    return r;
+}
+
+__device__ __forceinline__ double tet_volume(const Vec3<double>& a, const Vec3<double>& b,
+                                             const Vec3<double>& c, const Vec3<double>& d) {
+    const Vec3<double> u{b.x - a.x, b.y - a.y, b.z - a.z}, v{c.x - a.x, c.y - a.y, c.z - a.z},
+        w{d.x - a.x, d.y - a.y, d.z - a.z};
+    return fabs(dot(u, cross(v, w))) / 6.0;
+}
+
+// Exact fraction of the cell where the linear field with vertex values v is > 0 (marching
+// tetrahedra in the reference cell, affine invariant): 0 / 1 for one-signed cells, a corner
+// tetrahedron for one positive (one negative) vertex, and for two and two the prism between
+// the cut and the positive edge split into three tetrahedra
+__device__ double solid_fraction(const double (&v)[4]) {
+    int npos = 0, first_pos = -1, first_neg = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (v[k] > 0.0) {
+            ++npos;
+            if (first_pos < 0) first_pos = k;
+        } else if (first_neg < 0) {
+            first_neg = k;
+        }
+    }
+    double f;
+    if (npos == 0) {
+        f = 0.0;
+    } else if (npos == 4) {
+        f = 1.0;
+    } else if (npos == 1) {
+        f = corner_fraction(v, first_pos);
+    } else if (npos == 3) {
+        f = 1.0 - corner_fraction(v, first_neg);
+    } else {
+        // positives first, each group in index order
+        double pv[2], nv[2];
+        int ip = 0, in = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (v[k] > 0.0)
+                pv[ip++] = v[k];
+            else
+                nv[in++] = v[k];
+        }
+        const double pa = pv[0], pb = pv[1], na = nv[0], nb = nv[1];
+        const Vec3<double> A{0.0, 0.0, 0.0}, B{1.0, 0.0, 0.0};
+        const Vec3<double> q1{0.0, pa / (pa - na), 0.0};
+        const Vec3<double> q2{0.0, 0.0, pa / (pa - nb)};
+        const Vec3<double> q3{1.0 - pb / (pb - na), pb / (pb - na), 0.0};
+        const Vec3<double> q4{1.0 - pb / (pb - nb), 0.0, pb / (pb - nb)};
+        f = 6.0 * (tet_volume(A, q1, q2, B) + tet_volume(q1, q2, B, q3) + tet_volume(q2, B, q3, q4));
+    }
+    return f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+}
+
+// mode 0: the smoothed Heaviside H_eps(phi) at the 4-point degree-2 rule (barycentric
+// (alpha, beta, beta, beta), weights 1/4); mode 1: the exact solid fraction.  frac (optional)
+// receives the per-cell value.
+__global__ __launch_bounds__(VOL_THREADS) void k_scaffold_volume(
+    int32_t C, const int32_t* __restrict__ conn, const double* __restrict__ detJ,
+    const double* __restrict__ phi, int mode, double eps, double* __restrict__ frac,
+    double* __restrict__ partials) {
+    __shared__ double red[VOL_THREADS / 64];
+    const double qa = (5.0 + 3.0 * sqrt(5.0)) / 20.0, qb = (5.0 - sqrt(5.0)) / 20.0;
+    double s = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)VOL_THREADS + threadIdx.x; e < C;
+         e += (int64_t)gridDim.x * VOL_THREADS) {
+        const int4 n = reinterpret_cast<const int4*>(conn)[e];
+        const double v[4] = {phi[n.x], phi[n.y], phi[n.z], phi[n.w]};
+        double f;
+        if (mode == 0) {
+            const double sum = ((v[0] + v[1]) + v[2]) + v[3];
+            double h = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) h += smoothed_heaviside((qa - qb) * v[k] + qb * sum, eps);
+            f = h / 4.0;
+        } else {
+            f = solid_fraction(v);
+        }
+        if (frac) frac[e] = f;
+        s += f * (detJ[e] / 6.0);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < VOL_THREADS / 64; ++w) t += red[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
 static double* physics_scratch(dsv_mesh* m, int which) {
     // 0, 1: V doubles each (redistancing phi0 and ping-pong); 2: the scaffold offsets
     if (!m->phys[which]) m->phys[which] = dalloc<double>((size_t)m->V + 1);
@@ -392,6 +502,28 @@ int dsv_mesh_h_avg(dsv_mesh* m, double* h_avg) {
             m->h_avg = s / m->C;
         }
         *h_avg = m->h_avg;
+    });
+}
+
+int dsv_scaffold_volume(dsv_mesh* m, const double* phi, int32_t exact, double eps, double* frac,
+                        double* volume, void* stream) {
+    return guard([&] {
+        if (!m || !phi || !volume) throw ArgError{"dsv_scaffold_volume: NULL argument"};
+        if (!exact && !(eps > 0.0)) throw ArgError{"dsv_scaffold_volume: eps must be > 0"};
+        DSV_HIP(hipSetDevice(m->device));
+        hipStream_t st = (hipStream_t)stream;
+        const int nb = (int)std::min<int64_t>(VOL_MAX_WG, cdiv(m->C, VOL_THREADS));
+        double* part = physics_scratch(m, 1);  // V + 1 >= nb doubles? checked below
+        if ((size_t)nb > (size_t)m->V + 1) throw ArgError{"dsv_scaffold_volume: mesh too small"};
+        hipLaunchKernelGGL(k_scaffold_volume, dim3(nb), dim3(VOL_THREADS), 0, st, m->C, m->conn,
+                           m->detJ, phi, exact ? 1 : 0, eps, frac, part);
+        DSV_HIP(hipGetLastError());
+        std::vector<double> h(nb);
+        DSV_HIP(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+        DSV_HIP(hipStreamSynchronize(st));
+        double s = 0.0;
+        for (double x : h) s += x;
+        *volume = s;
     });
 }
 

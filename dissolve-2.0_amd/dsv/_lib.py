@@ -37,6 +37,7 @@ EXPORTS = (
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_grad_noalloc", "dsv_mlp_adam", "dsv_nodal_gradient",
     "dsv_transport_coefficients", "dsv_zinc_coefficients", "dsv_film_update", "dsv_under_relax",
     "dsv_interface_velocity", "dsv_levelset_advance", "dsv_mesh_h_avg", "dsv_redistance",
+    "dsv_scaffold_volume",
 )
 
 
@@ -163,6 +164,7 @@ def load():
         "dsv_levelset_advance": (ctypes.c_int, [P, P, P, D, P, P]),
         "dsv_mesh_h_avg": (ctypes.c_int, [P, ctypes.POINTER(D)]),
         "dsv_redistance": (ctypes.c_int, [P, P, I32, D, P]),
+        "dsv_scaffold_volume": (ctypes.c_int, [P, P, I32, D, P, ctypes.POINTER(D), P]),
         "dsv_mlp_features": (ctypes.c_int, [I32, P, P, P, P, P, D, D, D, D, P, P, P, P]),
         "dsv_mlp_forward": (ctypes.c_int, [P, P, I32, P, P]),
         "dsv_mlp_grad_workspace": (ctypes.c_size_t, [I32]),

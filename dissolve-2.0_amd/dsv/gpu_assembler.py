@@ -190,4 +190,9 @@ class GPUSparseAssembler:
         return nodal_gradient(self.mesh, u_vert)
 
     def scaffold_volume(self, phi_arr, epsilon):
-        raise NotImplementedError("scaffold_volume serves the level-set path, outside this library")
+        """Integral of the smoothed Heaviside H_eps(phi) at the 4-point degree-2 rule, on the GPU
+        (`gpu/gpu_assembler.py:604-643`); a Python float."""
+        from .physics import scaffold_volume
+        if hasattr(phi_arr, "x"):
+            phi_arr = phi_arr.x.array
+        return scaffold_volume(self.mesh, phi_arr, float(epsilon))

@@ -11,7 +11,11 @@ the reference's function names where it has them:
   v_field, params, h, t, gpu_asm)` -- `physics/interface_velocity.py:622-785` (default branch);
 * `levelset_advance(mesh, phi_old, v, dt)` -- `dissolve.py:684-690`;
 * `mesh_h_avg(mesh)` and `redistance(phi, domain, num_iters, dt_redist, h, gpu_asm)` --
-  `domain/mesh_utils.py:450-575`.
+  `domain/mesh_utils.py:450-575`;
+* `scaffold_volume(mesh, phi, eps)`, `compute_scaffold_volume(phi, domain, epsilon, gpu_asm)`
+  and `compute_scaffold_volume_exact(phi, domain)` -- the smoothed and the exact (marching
+  tetrahedra) volume of the scaffold (`gpu/gpu_assembler.py:604-643`,
+  `domain/mesh_utils.py:578-630`, `analysis/mechanics_fe.py:45-82`).
 
 Fields are torch tensors on the mesh's device; the reference-signature functions also take
 objects with `.x.array` (DOLFINx Functions, dsv.fem.Function), copied in and out like the
@@ -149,3 +153,48 @@ def redistance(phi, domain=None, num_iters: int = 8, dt_redist: float = None, h:
     t = _dev(mesh, phi, "phi").clone()
     redistance_device(mesh, t, num_iters)
     _put(phi, t)
+
+
+def scaffold_volume(mesh, phi, eps=None, exact=False, with_fraction=False):
+    """Volume of {phi > 0}: the smoothed-Heaviside quadrature with half-width eps, or the exact
+    marching-tetrahedra volume (exact=True); with_fraction also returns the per-cell values."""
+    import torch
+    if not exact and not (eps is not None and eps > 0):
+        raise ValueError("the smoothed volume needs eps > 0")
+    d_phi = _dev(mesh, phi, "phi")
+    frac = torch.empty(mesh.n_cells, dtype=torch.float64, device=mesh.device) \
+        if with_fraction else None
+    vol = ctypes.c_double()
+    _lib.call("dsv_scaffold_volume", mesh.handle, ptr(d_phi), 1 if exact else 0,
+              float(eps) if eps is not None else 0.0, ptr(frac), ctypes.byref(vol),
+              stream_ptr(mesh.device))
+    return (vol.value, frac) if with_fraction else vol.value
+
+
+def compute_scaffold_volume(phi, domain=None, epsilon: float = None, gpu_asm=None) -> float:
+    """Reference signature (`domain/mesh_utils.py:578-605`); epsilon defaults to 0.5 h_min, h_min
+    the smallest cell diameter (longest edge, `domain/mesh_utils.py:545-556`)."""
+    from .coefficients import _mesh_of
+    mesh = gpu_asm.mesh if gpu_asm is not None else (
+        getattr(domain, "dsv_mesh", domain) if domain is not None else _mesh_of(phi))
+    if epsilon is None:
+        epsilon = 0.5 * mesh_h_min(mesh)
+    return scaffold_volume(mesh, phi, epsilon)
+
+
+def compute_scaffold_volume_exact(phi, domain=None) -> float:
+    """Reference signature (`domain/mesh_utils.py:608-630`)."""
+    from .coefficients import _mesh_of
+    mesh = getattr(domain, "dsv_mesh", domain) if domain is not None else _mesh_of(phi)
+    return scaffold_volume(mesh, phi, exact=True)
+
+
+def mesh_h_min(mesh):
+    """Smallest longest-edge over the cells (host, from the mesh's copy of the coordinates)."""
+    import numpy as np
+    p = mesh.xyz[mesh.conn]
+    L = np.zeros(len(mesh.conn))
+    for a in range(4):
+        for b in range(a + 1, 4):
+            L = np.maximum(L, np.linalg.norm(p[:, a] - p[:, b], axis=1))
+    return float(L.min())

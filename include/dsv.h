@@ -360,6 +360,18 @@ int dsv_mesh_h_avg(dsv_mesh* m, double* h_avg);
  * sign = phi0 / sqrt(phi0^2 + (1.5 h)^2), phi0 the input; h <= 0 takes dsv_mesh_h_avg. */
 int dsv_redistance(dsv_mesh* m, double* phi, int32_t num_iters, double h, void* stream);
 
+/* Scaffold volume of the level set phi (per-vertex P1 field, device).  exact == 0:
+ * GPUSparseAssembler.scaffold_volume (`gpu/gpu_assembler.py:604-643`), the integral of the
+ * smoothed Heaviside H_eps(phi) = 0 / 1 / (1 + phi/eps + sin(pi phi/eps)/pi)/2 at the 4-point
+ * degree-2 rule (barycentric (a, b, b, b), a = (5 + 3 sqrt 5)/20, b = (5 - sqrt 5)/20, weights
+ * 1/4).  exact != 0: compute_scaffold_volume_exact (`domain/mesh_utils.py:608-630`), the exact
+ * volume of {phi > 0} by marching tetrahedra (`analysis/mechanics_fe.py:45-82`, the cell
+ * fraction clipped to [0, 1]).  frac (device, length C, may be NULL) receives the per-cell
+ * quadrature mean of H_eps, resp. the cell's solid fraction; *volume (host) the sum over cells of
+ * that value times |det J| / 6, in a fixed order.  Synchronises the stream. */
+int dsv_scaffold_volume(dsv_mesh* m, const double* phi, int32_t exact, double eps, double* frac,
+                        double* volume, void* stream);
+
 /* --- learned warm start (no counterpart in the reference, whose solves start from the
  * previous solution): a per-vertex MLP 7 -> 64 -> 64 -> 1 with tanh activations, FP32
  * parameters packed as W1[64][7], b1[64], W2[64][64], b2[64], w3[64], b3.  Device pointers. */

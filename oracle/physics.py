@@ -11,6 +11,11 @@ like the rest of oracle/): numpy restatements of
   exact-ORR override, no probe, no R_diff) (`physics/interface_velocity.py:622-785`);
 * `h_avg` -- the mean longest cell edge (`domain/mesh_utils.py:559-575`, DOLFINx mesh.h);
 * `redistance` -- Sussman pseudo-time redistancing (`domain/mesh_utils.py:450-542`);
+* `scaffold_volume_smoothed` -- the smoothed-Heaviside volume at the 4-point degree-2 rule
+  (`gpu/gpu_assembler.py:604-643`);
+* `solid_fraction`, `scaffold_volume_exact` -- the exact per-cell fraction of {phi > 0} by
+  marching tetrahedra and the exact volume (`analysis/mechanics_fe.py:30-82`,
+  `domain/mesh_utils.py:608-630`);
 * `coupled_step` -- one time step of the reference's species blocks in their order
   (`dissolve.py:740-954`): Zn, Cl, film, O2 (under-relaxed), OH from the new fields.
 
@@ -20,8 +25,12 @@ effective diffusivities are D0 without film, D0 times the floor at saturation an
 scaffold; for linear fields on jittered and graded meshes the velocity equals the closed form
 of its three rates (the nodal gradients are exact), with the band and the sign cap; a
 signed-distance plane is a fixed point of redistancing and phi = 2 (z - c) moves by exactly
--(h/4) sign per iteration outside the band; h_avg against a per-cell loop; the coupled step
-against the species' own backward-Euler steps done one by one.
+-(h/4) sign per iteration outside the band; h_avg against a per-cell loop; the exact fraction
+against the convex-hull volume of the cut polytope (scipy) for every sign pattern, and a plane
+through a Kuhn cube gives the exact volume above it; the smoothed volume is that of the
+positive quadrature points when |phi| > eps at all of them, and the exact integral of a linear
+field when eps is large; the coupled step against the species' own backward-Euler steps done
+one by one.
 """
 import numpy as np
 
@@ -127,3 +136,73 @@ def coupled_step(conn, xyz, st, p, dt, rtol=None):
     out = dict(st)
     out.update(Zn=Zn, Cl=Cl, F=F, O2=O2, OH=OH, O2_raw=O2n)
     return out
+
+
+KEAST_A = (5.0 + 3.0 * np.sqrt(5.0)) / 20.0
+KEAST_B = (5.0 - np.sqrt(5.0)) / 20.0
+
+
+def smoothed_heaviside(p, eps):
+    return np.where(p < -eps, 0.0, np.where(p > eps, 1.0,
+                    0.5 * (1.0 + p / eps + np.sin(np.pi * p / eps) / np.pi)))
+
+
+def cell_volumes(conn, xyz):
+    p = np.asarray(xyz, float)[np.asarray(conn)]
+    return np.abs(np.einsum("ij,ij->i", p[:, 1] - p[:, 0],
+                            np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]))) / 6.0
+
+
+def scaffold_volume_smoothed(conn, xyz, phi, eps):
+    pv = np.asarray(phi, float)[np.asarray(conn)]
+    pq = (KEAST_A - KEAST_B) * pv + KEAST_B * pv.sum(axis=1, keepdims=True)
+    cell_h = smoothed_heaviside(pq, eps).mean(axis=1)
+    return float((cell_h * cell_volumes(conn, xyz)).sum()), cell_h
+
+
+def _corner(v, a):
+    r = np.ones(len(v))
+    for k in range(4):
+        m = a != k
+        r[m] *= v[m, a[m]] / (v[m, a[m]] - v[m, k])
+    return r
+
+
+def _tetvol(a, b, c, d):
+    return np.abs(np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a))) / 6.0
+
+
+def solid_fraction(phiv):
+    """Per-cell fraction where the linear field with vertex values phiv (n x 4) is > 0."""
+    v = np.asarray(phiv, float)
+    n = len(v)
+    pos = v > 0.0
+    npos = pos.sum(axis=1)
+    f = np.zeros(n)
+    f[npos == 4] = 1.0
+    m = npos == 1
+    if m.any():
+        f[m] = _corner(v[m], np.argmax(pos[m], axis=1))
+    m = npos == 3
+    if m.any():
+        f[m] = 1.0 - _corner(v[m], np.argmin(pos[m], axis=1))
+    m = npos == 2
+    if m.any():
+        w = v[m]
+        order = np.argsort(~(w > 0.0), axis=1, kind="stable")
+        r = np.arange(len(w))
+        pa, pb, na, nb = (w[r, order[:, k]] for k in range(4))
+        z = np.zeros(len(w))
+        A = np.stack([z, z, z], 1)
+        B = np.stack([z + 1, z, z], 1)
+        q1 = np.stack([z, pa / (pa - na), z], 1)
+        q2 = np.stack([z, z, pa / (pa - nb)], 1)
+        q3 = np.stack([1 - pb / (pb - na), pb / (pb - na), z], 1)
+        q4 = np.stack([1 - pb / (pb - nb), z, pb / (pb - nb)], 1)
+        f[m] = 6.0 * (_tetvol(A, q1, q2, B) + _tetvol(q1, q2, B, q3) + _tetvol(q2, B, q3, q4))
+    return np.clip(f, 0.0, 1.0)
+
+
+def scaffold_volume_exact(conn, xyz, phi):
+    frac = solid_fraction(np.asarray(phi, float)[np.asarray(conn)])
+    return float((frac * cell_volumes(conn, xyz)).sum()), frac

@@ -126,3 +126,29 @@ def test_coupled_step_with_interface_update(case):
         gref = OC.grad_norm(OC.nodal_gradient(conn, xyz, phi))
         assert _rel(cs.grad_phi.cpu().numpy(), gref) <= 1e-12
     assert float(np.abs(cs.phi.cpu().numpy() - w["state"]["phi"]).max()) > 0
+
+
+def test_scaffold_volumes(case):
+    """dsv_scaffold_volume against the oracle: the smoothed-Heaviside quadrature volume
+    (GPUSparseAssembler.scaffold_volume) and the exact marching-tetrahedra volume
+    (compute_scaffold_volume_exact) of a wavy level set, per cell and in total."""
+    from dsv import fem, physics
+    from dsv.gpu_assembler import GPUSparseAssembler
+    mesh, conn, xyz, w = case
+    phi = xyz[:, 2] - 0.11 + 0.03 * np.sin(9 * xyz[:, 0]) * np.cos(7 * xyz[:, 1])
+    eps = 0.02
+    vol, frac = physics.scaffold_volume(mesh, phi, eps, with_fraction=True)
+    rvol, rfrac = OP.scaffold_volume_smoothed(conn, xyz, phi, eps)
+    assert abs(vol - rvol) <= 1e-12 * rvol and _rel(frac.cpu().numpy(), rfrac) <= 1e-12
+    assert abs(GPUSparseAssembler(mesh).scaffold_volume(phi, eps) - rvol) <= 1e-12 * rvol
+    vol, frac = physics.scaffold_volume(mesh, phi, exact=True, with_fraction=True)
+    rvol, rfrac = OP.scaffold_volume_exact(conn, xyz, phi)
+    f = frac.cpu().numpy()
+    assert abs(vol - rvol) <= 1e-12 * rvol and np.abs(f - rfrac).max() <= 1e-12
+    assert ((f > 0) & (f < 1)).sum() > 100  # cut cells of every kind
+    Vs = fem.FunctionSpace(mesh)
+    pf = fem.Function(Vs)
+    pf.x.array[:] = phi
+    assert abs(physics.compute_scaffold_volume_exact(pf, mesh) - rvol) <= 1e-12 * rvol
+    with pytest.raises(ValueError):
+        physics.scaffold_volume(mesh, phi, eps=0.0)

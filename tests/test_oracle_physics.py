@@ -148,3 +148,57 @@ def test_coupled_step_equals_species_steps_in_order():
         assert np.abs(new[name] - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1e-30), name
     assert np.all(new["Zn"][st["phi"] > 0] == p["Znbc"])
     assert new["F"].max() > 0  # the film forms where Zn and Cl meet
+
+
+def _hull_fraction(P, v):
+    """Exact fraction of the tet P (4 x 3) where the linear field with vertex values v is > 0:
+    the convex hull volume of the positive corners and the edge crossings (scipy qhull)."""
+    from scipy.spatial import ConvexHull
+    pts = [P[k] for k in range(4) if v[k] > 0]
+    for a in range(4):
+        for b in range(a + 1, 4):
+            if (v[a] > 0) != (v[b] > 0):
+                t = v[a] / (v[a] - v[b])
+                pts.append(P[a] + t * (P[b] - P[a]))
+    if len(pts) < 4:
+        return 0.0
+    tet = abs(np.linalg.det(np.column_stack([P[1] - P[0], P[2] - P[0], P[3] - P[0]]))) / 6
+    return ConvexHull(np.array(pts)).volume / tet
+
+
+def test_solid_fraction_against_convex_hull_for_every_sign_pattern():
+    rng = np.random.default_rng(7)
+    for trial in range(400):
+        P = rng.normal(size=(4, 3))
+        v = rng.normal(size=4)
+        npos = trial % 5                       # force every count of positive vertices
+        v = np.abs(v)
+        v[rng.permutation(4)[:4 - npos]] *= -1
+        f = OP.solid_fraction(v[None, :])[0]
+        assert abs(f - _hull_fraction(P, v)) < 1e-9, (v, f)
+    # one-signed cells
+    assert OP.solid_fraction(np.array([[1.0, 2, 3, 4], [-1.0, -2, -3, -4]])).tolist() == [1.0, 0.0]
+
+
+def test_exact_and_smoothed_volumes_of_a_plane():
+    conn, xyz = meshes.kuhn_cube(5)
+    for c in (0.37, 0.5, 0.91):
+        phi = xyz[:, 2] - c                     # the plane cut is exact on every cell
+        vol, frac = OP.scaffold_volume_exact(conn, xyz, phi)
+        assert abs(vol - (1.0 - c)) < 1e-13
+        assert np.all((frac >= 0) & (frac <= 1))
+    # |phi| > eps at every quadrature point: H is the indicator, the smoothed volume the exact
+    # volume of the positive cells
+    phi = xyz[:, 2] - 0.5 + 0.01
+    eps = 1e-3
+    pv = phi[conn]
+    pq = (OP.KEAST_A - OP.KEAST_B) * pv + OP.KEAST_B * pv.sum(1, keepdims=True)
+    assert np.abs(pq).min() > eps
+    vs, cell_h = OP.scaffold_volume_smoothed(conn, xyz, phi, eps)
+    ref = ((pq > 0).mean(1) * OP.cell_volumes(conn, xyz)).sum()
+    assert abs(vs - ref) < 1e-14
+    # with eps large, H = (1 + phi/eps + sin(pi phi/eps)/pi)/2 = 1/2 + phi/eps + O((phi/eps)^3),
+    # whose integral over the unit cube the Keast rule gives exactly: 1/2 + (1/2 - 0.49)/eps
+    eps = 1e5
+    vs, _ = OP.scaffold_volume_smoothed(conn, xyz, phi, eps)
+    assert abs(vs - (0.5 + 0.01 / eps)) < 1e-14
