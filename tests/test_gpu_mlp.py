@@ -140,3 +140,106 @@ def test_learned_warm_start_saves_iterations(dev):
     for k, (X, sigma) in enumerate(per):
         one = (sigma.double() * learned.tau) * learned.mlp.forward(X.clone()).double()
         assert torch.equal(batched[k], one), k
+
+
+def _start_from(st, n_hist, extra, rtol, max_it=20000):
+    """Solve the stepper's current systems (right-hand sides already formed) from the Galerkin
+    start over its last n_hist increments plus the explicit directions `extra`, into fresh
+    output arrays.  With rtol = 1 the start itself meets the tolerance and comes back unchanged
+    by any CG iteration.  Returns (solutions, stats, coefficients per species)."""
+    import ctypes
+    import torch
+    from dsv import _lib
+    from dsv.core import ptr, species_array, species_struct, stream_ptr
+    keep, structs, xs = [], [], []
+    for k, s in enumerate(st.defs):
+        x = torch.empty_like(st.u[k])
+        xs.append(x)
+        structs.append(species_struct(st.mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                                      alpha=st.alpha_dev[k], alpha_value=s.get("alpha_value", 0.0),
+                                      bc_dofs=st.bc[k], bc_value=s.get("bc_value", 0.0),
+                                      b=st.b[k], x=x, keep=keep))
+    S = st.S
+    steps = [st.u] + st.u_hist[:n_hist]
+    hist = (ctypes.c_void_p * (len(steps) * S))(*[ptr(a[k]).value for a in steps for k in range(S)])
+    ext = (ctypes.c_void_p * max(1, len(extra)))(*[ptr(e).value for e in extra])
+    nd = n_hist + (1 if extra else 0)
+    coef = (ctypes.c_double * (nd * S))()
+    stats = _lib.Stats()
+    _lib.call("dsv_solve_warm_history", st.mesh.handle, S, species_array(structs), 1.0 / st.dt,
+              st.lumped, _lib.METHODS[st.method], float(rtol), int(max_it), n_hist, hist,
+              1 if extra else 0, ext, stream_ptr(st.mesh.device), stats, coef, None, None)
+    torch.cuda.synchronize(st.mesh.device)
+    return xs, stats.as_dict(S), [[coef[j * S + s] for j in range(nd)] for s in range(S)]
+
+
+@pytest.mark.parametrize("n_hist", [2, 7])
+def test_mlp_direction_isolated_on_the_same_system(dev, n_hist):
+    """What the trained network adds, on one and the same system of a learned trajectory: the
+    Galerkin start over n_hist increments with and without the MLP direction.  Galerkin
+    minimality over nested subspaces fixes the order of their A-energy errors (with the
+    direction never worse, to the Gram solve's rounding); both then converge to the bar.  The
+    iteration counts of the two solves are the network's own gain (printed; the bench reports
+    the same comparison over whole trajectories)."""
+    import torch
+    from dsv import workloads
+    from dsv.ceed_operators import apply_batched
+    from dsv.core import Mesh
+    from dsv.mlp import WarmStartMLP
+    from dsv.transport import Stepper
+    w = workloads.disc3(seed=1, scale=6.0)
+    mesh = Mesh(w["conn"], w["xyz"])
+    st = Stepper(mesh, w["species"], w["dt"], method="mf32_ir64", rtol=1e-10, phi=w["phi"],
+                 grad_phi=w["grad_norm"], eps=w["eps"], mlp=WarmStartMLP(mesh.device))
+    st.set_warm_start("extrapolate", history_dirs=n_hist)
+    st.train_warm_start(9, epochs=30)
+    st.set_warm_start("learned", history_dirs=n_hist)
+    for _ in range(2):
+        st.step()
+    # the next step's systems, untouched by the comparison below
+    st.rhs()
+    X_all, sig_all, _ = st._features()
+    extra = list(st._mlp_directions(X_all, sig_all))
+    with_mlp, _, c_with = _start_from(st, n_hist, extra, 1.0)
+    without, _, c_without = _start_from(st, n_hist, [], 1.0)
+    # the tight FP64 solution of the same systems, from u_n
+    refs = [u.clone() for u in st.u]
+    from dsv import _lib
+    from dsv.core import species_array, species_struct, stream_ptr
+    keep, structs = [], []
+    for k, s in enumerate(st.defs):
+        structs.append(species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0),
+                                      alpha=st.alpha_dev[k], alpha_value=s.get("alpha_value", 0.0),
+                                      bc_dofs=st.bc[k], bc_value=s.get("bc_value", 0.0),
+                                      b=st.b[k], x=refs[k], keep=keep))
+    stats = _lib.Stats()
+    _lib.call("dsv_solve", mesh.handle, st.S, species_array(structs), 1.0 / st.dt, st.lumped,
+              _lib.METHODS["csr64"], 1e-13, 100000, stream_ptr(mesh.device), stats)
+    torch.cuda.synchronize()
+    assert all(stats.as_dict(st.S)["converged"])
+    # A-energy errors of the two starts
+    spec = [dict(D=s.get("D"), D_value=s.get("D_value", 0.0), alpha=st.alpha_dev[k],
+                 alpha_value=s.get("alpha_value", 0.0), bc_dofs=st.bc[k])
+            for k, s in enumerate(st.defs)]
+    e_with = [a - r for a, r in zip(with_mlp, refs)]
+    e_without = [a - r for a, r in zip(without, refs)]
+    Ae_with = apply_batched(mesh, spec, 1.0 / st.dt, e_with, lumped=st.lumped, precision=64)
+    Ae_without = apply_batched(mesh, spec, 1.0 / st.dt, e_without, lumped=st.lumped,
+                               precision=64)
+    E_with = [float(torch.dot(e, a)) for e, a in zip(e_with, Ae_with)]
+    E_without = [float(torch.dot(e, a)) for e, a in zip(e_without, Ae_without)]
+    for k in range(st.S):
+        assert E_with[k] >= 0.0 and E_without[k] >= 0.0
+        if any(c != 0.0 for c in c_with[k]) and any(c != 0.0 for c in c_without[k]):
+            assert E_with[k] <= E_without[k] * (1.0 + 1e-6) + 1e-300, (k, E_with, E_without)
+    # the two solves of the same systems, to the bar
+    sol_with, s_with, _ = _start_from(st, n_hist, extra, 1e-10)
+    sol_without, s_without, _ = _start_from(st, n_hist, [], 1e-10)
+    for sols, s in ((sol_with, s_with), (sol_without, s_without)):
+        assert all(s["converged"])
+        for k in range(st.S):
+            err = float((sols[k] - refs[k]).abs().max() / refs[k].abs().max())
+            assert err <= 1e-8, (k, err)
+    print(f"n_hist={n_hist}: start energy with/without the MLP direction per species "
+          f"{[round(a / b, 4) if b > 0 else None for a, b in zip(E_with, E_without)]}; "
+          f"CG iterations with {s_with['iterations']} without {s_without['iterations']}")
