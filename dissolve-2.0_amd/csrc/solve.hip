@@ -42,6 +42,7 @@
 #include <cstring>
 #include <atomic>
 #include <stdexcept>
+#include <string>
 #include <type_traits>
 
 #include "dsv_internal.h"
@@ -79,6 +80,16 @@ struct CGState {
     int max_it;            // host
     int any_active;        // A_k / K_i (host polls)
     int seq;               // poll snapshots only: set last, once the rest is written
+};
+
+// The residual kernel's norms and, on request, the CG state, published into pinned host
+// memory by k_resid_publish: the host spins on `seq` (set last) instead of copying them back
+// and synchronising the stream.
+struct ResidPub {
+    double rr[MAXS];
+    double bb[MAXS];
+    CGState state;
+    int seq;
 };
 
 // species-interleaved vertex record: the S values of one vertex, unpadded (12 B for three
@@ -1295,6 +1306,29 @@ __global__ void k_pack_start(int V, const int32_t* __restrict__ perm, const doub
     for (int s = 0; s < S; ++s) r.v[s] = ((bits >> s) & 1u) ? bv.v[s] : src[s][c];
     out[i] = r;
 }
+// k_pack of the right-hand sides and k_pack_start of the start in one pass
+template <int S>
+__global__ void k_pack_bx(int V, const int32_t* __restrict__ perm, const double* __restrict__ b0,
+                          const double* __restrict__ b1, const double* __restrict__ b2,
+                          const double* __restrict__ s0, const double* __restrict__ s1,
+                          const double* __restrict__ s2, const uint8_t* __restrict__ bcbits,
+                          BcVals bv, SV<double, S>* __restrict__ bout,
+                          SV<double, S>* __restrict__ xout) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int c = perm[i];
+    const unsigned bits = bcbits[i];
+    const double* bsrc[3] = {b0, b1, b2};
+    const double* src[3] = {s0, s1, s2};
+    SV<double, S> rb, rx;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        rb.v[s] = bsrc[s][c];
+        rx.v[s] = ((bits >> s) & 1u) ? bv.v[s] : src[s][c];
+    }
+    bout[i] = rb;
+    xout[i] = rx;
+}
 template <class Ti, int S>
 __global__ void k_unpack(int V, const int32_t* __restrict__ perm, const SV<Ti, S>* __restrict__ in,
                          double* __restrict__ d0, double* __restrict__ d1, double* __restrict__ d2) {
@@ -1573,7 +1607,10 @@ struct Workspace {
     int gcur = 0;
     int gnd = 0;             // directions of the saved one
     WarmOut* wout = nullptr;    // its coefficients and norms (device) ...
-    WarmOut* h_wout = nullptr;  // ... and their pinned copy
+    WarmOut* h_wout = nullptr;  // ... published by the acceptance kernel into pinned memory
+    WarmOut* d_wout_pub = nullptr;  // (h_wout's device address)
+    ResidPub* h_rpub = nullptr;     // the residual norms and CG state, published (pinned)
+    ResidPub* d_rpub = nullptr;     // (h_rpub's device address)
     void* fw[2][MAXS] = {};  // stored form: per-species weights K(D_s) of sets 0 / 1
     double* kvals = nullptr; // CSR values of K(D_s) (nnz), assembled to build them
     // operator reuse: the inputs of coefficient sets 0 / 1, of the Dirichlet bits, and whether
@@ -1609,6 +1646,7 @@ static void free_ws(Workspace* w) {
     if (w->h_st) (void)hipHostFree(w->h_st);
     if (w->h_pR) (void)hipHostFree(w->h_pR);
     if (w->h_poll) (void)hipHostFree(w->h_poll);
+    if (w->h_rpub) (void)hipHostFree(w->h_rpub);
     for (int i = 0; i < 2; ++i) {
         if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
         if (w->ev_io[i]) (void)hipEventDestroy(w->ev_io[i]);
@@ -1692,6 +1730,8 @@ static Workspace* get_ws(dsv_mesh* m, int method, int S, bool narrowed = false) 
                               hipHostMallocDefault));
         DSV_HIP(hipHostMalloc((void**)&w->h_poll, 2 * sizeof(CGState), hipHostMallocCoherent));
         DSV_HIP(hipHostGetDevicePointer((void**)&w->d_poll, w->h_poll, 0));
+        DSV_HIP(hipHostMalloc((void**)&w->h_rpub, sizeof(ResidPub), hipHostMallocCoherent));
+        DSV_HIP(hipHostGetDevicePointer((void**)&w->d_rpub, w->h_rpub, 0));
         for (int i = 0; i < 2; ++i) {
             DSV_HIP(hipEventCreateWithFlags(&w->ev[i], hipEventDisableTiming));
             DSV_HIP(hipEventCreateWithFlags(&w->ev_io[i], hipEventDisableTiming));
@@ -2159,6 +2199,23 @@ static bool narrow_mid(dsv_mesh* m, Workspace* w, const int* sel, int max_it, hi
     return true;
 }
 
+// Spins until a kernel sets the published word `seq` in pinned host memory.  The spin checks
+// the stream now and then, so that work that ends without publishing is an error, not a hang.
+static void wait_published(volatile int* seq, hipStream_t stream, const char* what) {
+    for (long i = 1;; ++i) {
+        if (*seq != 0) break;
+        if ((i & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(stream);
+            if (q == hipSuccess) {
+                if (*seq != 0) break;
+                throw std::runtime_error(std::string(what) + " ended without publishing its result");
+            }
+            if (q != hipErrorNotReady) DSV_HIP(q);
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+}
+
 // Runs CG iterations until no species is active.  Chunks of an even number of iterations are
 // launched directly or, with use_graphs, replayed from hipGraphs captured once (re-captured
 // when the operator parameters change); the host polls one published state per chunk.  Once
@@ -2237,19 +2294,7 @@ static int run_cg(dsv_mesh* m, Workspace* w, int max_it, hipStream_t stream, con
             DSV_HIP(hipEventSynchronize(w->ev[slot]));
             return;
         }
-        volatile int* seq = &w->h_poll[slot].seq;
-        for (long i = 1;; ++i) {
-            if (*seq != 0) break;
-            if ((i & 1023) == 0) {
-                const hipError_t q = hipStreamQuery(stream);
-                if (q == hipSuccess) {
-                    if (*seq != 0) break;
-                    throw std::runtime_error("CG chunk ended without publishing its state");
-                }
-                if (q != hipErrorNotReady) DSV_HIP(q);
-            }
-        }
-        std::atomic_thread_fence(std::memory_order_acquire);
+        wait_published(&w->h_poll[slot].seq, stream, "a CG chunk");
     };
     PollSeed h;  // `last`, the chunk end before it, and iterations queued after `last`
     if (seed) h = *seed;
@@ -2378,6 +2423,39 @@ static void write_state(Workspace* w, const CGState& h, hipStream_t stream) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Totals of the residual kernel's partials (r.r, then b.b, of every species; nprod each), one
+// wavefront per quantity in a fixed order (lane-strided sums, then the DPP wave sum), and with
+// `st` a snapshot of the CG state, written into the pinned slot `pub`; `seq` is set last.
+// One workgroup of 64 * 2S threads.
+template <int S>
+__global__ __launch_bounds__(64 * 2 * MAXS) void k_resid_publish(const double* __restrict__ pR,
+                                                                int nprod,
+                                                                const CGState* __restrict__ st,
+                                                                ResidPub* __restrict__ pub) {
+    __shared__ double tot[2 * MAXS];
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (q < 2 * S) {  // wave-uniform
+        double a = 0.0;
+        for (int i = lane; i < nprod; i += 64) a += pR[(size_t)q * nprod + i];
+        a = wave_sum(a);
+        if (lane == 0) tot[q] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int s = 0; s < S; ++s) {
+            pub->rr[s] = tot[s];
+            pub->bb[s] = tot[S + s];
+        }
+        if (st) {
+            CGState snap = *st;
+            snap.seq = 0;
+            pub->state = snap;
+        }
+        __threadfence_system();
+        *reinterpret_cast<volatile int*>(&pub->seq) = 1;
+    }
+}
+
 // residual r64 = b - A x64 (FP64, vertex order of the method); returns rr and bb per species
 template <int S>
 static int residual64_launch(dsv_mesh* m, Workspace* w, hipStream_t stream) {
@@ -2407,20 +2485,17 @@ template <int S>
 static void residual64(dsv_mesh* m, Workspace* w, hipStream_t stream, double* rr, double* bb,
                        CGState* state = nullptr) {
     const int nprod = residual64_launch<S>(m, w, stream);
-    DSV_HIP(hipMemcpyAsync(w->h_pR, w->pR, sizeof(double) * 2 * S * nprod, hipMemcpyDeviceToHost,
-                           stream));
-    if (state)
-        DSV_HIP(hipMemcpyAsync(w->h_st, w->st, sizeof(CGState), hipMemcpyDeviceToHost, stream));
-    DSV_HIP(hipStreamSynchronize(stream));
-    if (state) *state = *w->h_st;
+    volatile int* seq = &w->h_rpub->seq;
+    *seq = 0;  // every earlier publication has been waited for
+    hipLaunchKernelGGL((k_resid_publish<S>), dim3(1), dim3(64 * 2 * S), 0, stream, w->pR, nprod,
+                       state ? (const CGState*)w->st : nullptr, w->d_rpub);
+    DSV_HIP(hipGetLastError());
+    wait_published(seq, stream, "the residual");
+    const ResidPub& p = *w->h_rpub;
+    if (state) *state = p.state;
     for (int s = 0; s < S; ++s) {
-        double a = 0.0, b = 0.0;
-        for (int i = 0; i < nprod; ++i) {
-            a += w->h_pR[s * nprod + i];
-            b += w->h_pR[(S + s) * nprod + i];
-        }
-        rr[s] = a;
-        bb[s] = b;
+        rr[s] = p.rr[s];
+        bb[s] = p.bb[s];
     }
 }
 
@@ -2560,16 +2635,18 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     if (warm && warm->from_hist)
         for (int s = 0; s < S; ++s) xs[s] = warm->hist[s];  // the start is step n of the history
     SV<double, S>* x64 = (SV<double, S>*)w->vb[VB_X64];
-    hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
-                       bs[2], (SV<double, S>*)w->vb[VB_B64]);
     if (internal) {
-        // the start with its Dirichlet values, from the Dirichlet bits of the operator set-up
+        // the right-hand sides, and the start with its Dirichlet values from the Dirichlet
+        // bits of the operator set-up, in one pass
         prepare_bc(m, w, S, sp, st);
         BcVals bv{};
         for (int s = 0; s < S; ++s) bv.v[s] = sp[s].bc_value;
-        hipLaunchKernelGGL((k_pack_start<S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0],
-                           xs[1], xs[2], (const uint8_t*)w->bcbits, bv, x64);
+        hipLaunchKernelGGL((k_pack_bx<S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
+                           bs[2], xs[0], xs[1], xs[2], (const uint8_t*)w->bcbits, bv,
+                           (SV<double, S>*)w->vb[VB_B64], x64);
     } else {
+        hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0],
+                           bs[1], bs[2], (SV<double, S>*)w->vb[VB_B64]);
         hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, xs[0],
                            xs[1], xs[2], x64);
         for (int s = 0; s < S; ++s)
@@ -3021,6 +3098,7 @@ struct WarmOut {
     double bb[MAXS];
     double rr1[MAXS];   // ||r||^2 of the start taken
     int take[MAXS];     // 1 where the combination was taken (else the start stays, c counts 0)
+    int seq;            // the pinned copy only: set last by the acceptance kernel
 };
 
 // One Gram solve per species (thread s), from the totals of the dot partials: the full
@@ -3073,14 +3151,17 @@ __global__ __launch_bounds__(64) void k_proj_solve(const double* __restrict__ to
 // of the residual kernel's (r.r then b.b per species), then updates in place:
 // x += sum_k c_k d_k, r -= sum_k c_k A d_k for the species taken.  Every workgroup makes the
 // same decision from the same totals; workgroup 0 records it.
+// With `pub` (pinned host memory), workgroup 0 also publishes the outcome there, the
+// coefficients zero for the species not taken, and sets its `seq` last.
 template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_accept_update(
     int V, ProjVec<S> pv, SV<double, S>* __restrict__ r, const double* __restrict__ tot,
-    WarmOut* __restrict__ out) {
+    WarmOut* __restrict__ out, WarmOut* __restrict__ pub) {
     double c[ND][S];
     proj_coefs<S, ND>(pv, c);
     const double* n1 = tot;      // ||r0 - sum_k c_k A d_k||^2 per species
     const double* r0 = tot + S;  // the residual kernel's r.r, then b.b, per species
+    const bool rec = blockIdx.x == 0 && threadIdx.x == 0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         bool moved = false;  // every direction dropped: nothing to take
@@ -3091,12 +3172,24 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_accept_update(
         if (!take)
 #pragma unroll
             for (int k = 0; k < ND; ++k) c[k][s] = 0.0;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (rec) {
             out->take[s] = take ? 1 : 0;
             out->rr0[s] = r0[s];
             out->bb[s] = r0[S + s];
             out->rr1[s] = take ? n1[s] : r0[s];
+            if (pub) {
+                pub->take[s] = take ? 1 : 0;
+                pub->rr0[s] = r0[s];
+                pub->bb[s] = r0[S + s];
+                pub->rr1[s] = take ? n1[s] : r0[s];
+#pragma unroll
+                for (int k = 0; k < ND; ++k) pub->c[k][s] = c[k][s];
+            }
         }
+    }
+    if (rec && pub) {
+        __threadfence_system();
+        *reinterpret_cast<volatile int*>(&pub->seq) = 1;
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
         SV<double, S> x = pv.x[i], rv = r[i];
@@ -3206,7 +3299,10 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     if (!w->pW) w->pW = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD) * PROJ_GRID_F);
     if (!w->wout) w->wout = dalloc<WarmOut>(1);
     if (!w->pT) w->pT = dalloc<double>((size_t)MAXS * (MAXD * (MAXD + 1) / 2 + MAXD));
-    if (!w->h_wout) DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocDefault));
+    if (!w->h_wout) {
+        DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocCoherent));
+        DSV_HIP(hipHostGetDevicePointer((void**)&w->d_wout_pub, w->h_wout, 0));
+    }
     // Direction k is a[s] - b[s] per species: d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
     // (history; from_hist: d_k = hist_k - hist_(k+1), x = hist_0), then the explicit ones.
     // Under operator reuse (whose contract keeps the arrays named unchanged), a history
@@ -3323,11 +3419,12 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
                        pv, w->pW);
     hipLaunchKernelGGL(k_sum_partials2, dim3(3 * S), dim3(PROJ_THREADS), 0, st, w->pW,
                        PROJ_GRID_F, S, w->pR, nres, w->pT);
+    volatile int* seq = &w->h_wout->seq;
+    *seq = 0;  // every earlier publication has been waited for
     hipLaunchKernelGGL((k_proj_accept_update<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st,
-                       m->V, pv, (SV<double, S>*)w->vb[VB_R64], w->pT, w->wout);
+                       m->V, pv, (SV<double, S>*)w->vb[VB_R64], w->pT, w->wout, w->d_wout_pub);
     DSV_HIP(hipGetLastError());
-    DSV_HIP(hipMemcpyAsync(w->h_wout, w->wout, sizeof(WarmOut), hipMemcpyDeviceToHost, st));
-    DSV_HIP(hipStreamSynchronize(st));
+    wait_published(seq, st, "the warm start");
     const WarmOut& o = *w->h_wout;
     for (int s = 0; s < S; ++s) {
         rr[s] = o.rr1[s];

@@ -239,8 +239,8 @@ class Stepper:
             else:  # u_{n-1}, ..., u_{n-nh}; the start u_n was copied into x
                 steps, fn = self.u_hist[:nh], "dsv_solve_warm"
             hist = (ctypes.c_void_p * max(1, len(steps) * self.S))(
-                *[ptr(a[k]).value for a in steps for k in range(self.S)])
-            ext = (ctypes.c_void_p * max(1, len(extra)))(*[ptr(e).value for e in extra])
+                *[a[k].data_ptr() for a in steps for k in range(self.S)])
+            ext = (ctypes.c_void_p * max(1, len(extra)))(*[e.data_ptr() for e in extra])
             coef = (ctypes.c_double * (nd * self.S))()
             r0 = (ctypes.c_double * self.S)()
             r1 = (ctypes.c_double * self.S)()
