@@ -2871,20 +2871,22 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv
     double tot[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) tot[q] = 0.0;
+    // d_j . A d_k = A d_j . d_k on the free rows (the directions vanish at Dirichlet rows), so
+    // rows [J0, J1) read their own A d_j and the directions, not every A d_k.  Entry (j, k >= j)
+    // is always summed as A d_j . d_k: the same bits as the shifted rows of k_proj_dots_shift.
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-        double d[J1 - J0];
+        double Ad[J1 - J0];
 #pragma unroll
-        for (int j = J0; j < J1; ++j) d[j - J0] = pv.d[j][i].v[s];
-#pragma unroll
-        for (int k = J0; k < ND; ++k) {
-            const double Ad = pv.Ad[k][i].v[s];
-#pragma unroll
-            for (int j = J0; j < J1; ++j)
-                if (j <= k) tot[gram_row<ND>(j) - gram_row<ND>(J0) + (k - j)] += d[j - J0] * Ad;
-        }
+        for (int j = J0; j < J1; ++j) Ad[j - J0] = pv.Ad[j][i].v[s];
         const double r = pv.r0[i].v[s];
 #pragma unroll
-        for (int j = J0; j < J1; ++j) tot[NG + (j - J0)] += d[j - J0] * r;
+        for (int k = J0; k < ND; ++k) {
+            const double d = pv.d[k][i].v[s];
+#pragma unroll
+            for (int j = J0; j < J1; ++j)
+                if (j <= k) tot[gram_row<ND>(j) - gram_row<ND>(J0) + (k - j)] += Ad[j - J0] * d;
+            if (k < J1) tot[NG + (k - J0)] += d * r;
+        }
     }
     block_sum_all<NR>(tot, red);
     if (threadIdx.x != 0) return;
@@ -2898,9 +2900,9 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv
 
 // Gram quantities when directions 2.. are the previous call's 1.. (shifted history): only
 // rows 0 and 1 of d_j . A d_k and every d_j . r0 are new, 3 ND - 1 per species, in the order
-// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0; species blockIdx.y.  Each sum
-// runs in the same per-thread and per-workgroup order as in k_proj_dots, so the values are
-// the same bits.
+// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0; species blockIdx.y.  The rows
+// are summed as A d_0 . d_k and A d_1 . d_k (ND + 3 vectors read), each sum in the same
+// per-thread and per-workgroup order as in k_proj_dots, so the values are the same bits.
 template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots_shift(int V, ProjVec<S> pv,
                                                                  double* __restrict__ pout) {
@@ -2912,17 +2914,14 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots_shift(int V, ProjVec
     for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
         const double r = pv.r0[i].v[s];
-        const double d0 = pv.d[0][i].v[s], d1 = pv.d[1][i].v[s];
+        const double Ad0 = pv.Ad[0][i].v[s], Ad1 = pv.Ad[1][i].v[s];
 #pragma unroll
         for (int k = 0; k < ND; ++k) {
-            const double Ad = pv.Ad[k][i].v[s];
-            tot[k] += d0 * Ad;
-            if (k >= 1) tot[ND + k - 1] += d1 * Ad;
+            const double d = pv.d[k][i].v[s];
+            tot[k] += Ad0 * d;
+            if (k >= 1) tot[ND + k - 1] += Ad1 * d;
+            tot[2 * ND - 1 + k] += d * r;
         }
-        tot[2 * ND - 1] += d0 * r;
-        tot[2 * ND] += d1 * r;
-#pragma unroll
-        for (int j = 2; j < ND; ++j) tot[2 * ND + 1 + j - 2] += pv.d[j][i].v[s] * r;
     }
     block_sum_all<NQ>(tot, red);
     if (threadIdx.x != 0) return;
