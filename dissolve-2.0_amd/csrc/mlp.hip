@@ -64,18 +64,110 @@ __device__ __forceinline__ float mlp_forward(const float* sp, const float (&x)[M
     return out;
 }
 
+// ---------------------------------------------------------------------------------------
+// Inference on the matrix cores.  A wavefront takes 32 samples at a time and computes both
+// layers as transposed products H^T = W X^T with v_mfma_f32_32x32x2f32 (FP32 in, FP32
+// accumulate): rows are hidden units, columns the 32 samples.  In the 32 x 32 result, lane l
+// holds column n = l % 32 and, in register r, row 8 (r / 4) + 4 (l / 32) + r % 4.  Layer 2
+// sums over the hidden units in exactly that order: at k-step (T, r) the two lane halves
+// supply the units they already hold, and the A operand reads the matching columns of W2, so
+// the first layer's outputs feed the second without moving.  Layer 1 pads its 7 inputs to 8.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// tanh(a) = 1 - 2 / (1 + e^(2a)), with the hardware exponential and reciprocal (absolute error
+// about 1e-7; +-1 at overflow of the exponential)
+__device__ __forceinline__ float tanh_fast(float a) {
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * a));
+}
+
+// LDS layout of the inference kernel: W1 rows padded to 9 and W2 rows to 65 floats, so that the
+// 32 rows one operand load touches fall into distinct banks
+constexpr int FW_W1 = 0;                       // [64][9]
+constexpr int FW_W2 = FW_W1 + MLP_H * 9;       // [64][65]
+constexpr int FW_B1 = FW_W2 + MLP_H * 65;
+constexpr int FW_B2 = FW_B1 + MLP_H;
+constexpr int FW_W3 = FW_B2 + MLP_H;
+constexpr int FW_B3 = FW_W3 + MLP_H;
+constexpr int FW_N = FW_B3 + 1;
+
+__device__ __forceinline__ int unit_of(int T, int r, int half) {
+    return 32 * T + 8 * (r >> 2) + 4 * half + (r & 3);
+}
+
 __global__ void __launch_bounds__(256)
 k_mlp_forward(const float* __restrict__ params, const float* __restrict__ X, int n,
               float* __restrict__ out) {
-    __shared__ float sp[MLP_NPARAM];
-    load_params(params, sp);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float x[MLP_IN];
+    __shared__ float sw[FW_N];
+    for (int i = threadIdx.x; i < MLP_H * 9; i += blockDim.x) {
+        const int m = i / 9, k = i % 9;
+        sw[FW_W1 + i] = k < MLP_IN ? params[OFF_W1 + m * MLP_IN + k] : 0.0f;
+    }
+    for (int i = threadIdx.x; i < MLP_H * 65; i += blockDim.x) {
+        const int m = i / 65, k = i % 65;
+        sw[FW_W2 + i] = k < MLP_H ? params[OFF_W2 + m * MLP_H + k] : 0.0f;
+    }
+    for (int i = threadIdx.x; i < MLP_H; i += blockDim.x) {
+        sw[FW_B1 + i] = params[OFF_B1 + i];
+        sw[FW_B2 + i] = params[OFF_B2 + i];
+        sw[FW_W3 + i] = params[OFF_W3 + i];
+    }
+    if (threadIdx.x == 0) sw[FW_B3] = params[OFF_B3];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, col = lane & 31, half = lane >> 5;
+    const int wpb = blockDim.x >> 6;
+    const int stride = gridDim.x * wpb * 32;
+    for (int base = (blockIdx.x * wpb + (threadIdx.x >> 6)) * 32; base < n; base += stride) {
+        const int smp = base + col;
+        const bool live = smp < n;
+        // layer 1: rows 32 T + m of W1 X^T, K = 8 in steps of 2 (lane half = k)
+        f32x16 h1[2];
 #pragma unroll
-    for (int k = 0; k < MLP_IN; ++k) x[k] = X[(size_t)i * MLP_IN + k];
-    float h1[MLP_H];
-    out[i] = mlp_forward(sp, x, h1, nullptr);
+        for (int T = 0; T < 2; ++T)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) h1[T][r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int f = 2 * t + half;
+            const float xb = (live && f < MLP_IN) ? X[(size_t)smp * MLP_IN + f] : 0.0f;
+#pragma unroll
+            for (int T = 0; T < 2; ++T)
+                h1[T] = __builtin_amdgcn_mfma_f32_32x32x2f32(sw[FW_W1 + (32 * T + col) * 9 + f],
+                                                             xb, h1[T], 0, 0, 0);
+        }
+#pragma unroll
+        for (int T = 0; T < 2; ++T)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                h1[T][r] = tanh_fast(h1[T][r] + sw[FW_B1 + unit_of(T, r, half)]);
+        // layer 2: K runs over the (T, r) register pairs, unit_of(T, r, half) per lane half
+        f32x16 h2[2];
+#pragma unroll
+        for (int M = 0; M < 2; ++M)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) h2[M][r] = 0.0f;
+#pragma unroll
+        for (int T = 0; T < 2; ++T)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int u = unit_of(T, r, half);
+#pragma unroll
+                for (int M = 0; M < 2; ++M)
+                    h2[M] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                        sw[FW_W2 + (32 * M + col) * 65 + u], h1[T][r], h2[M], 0, 0, 0);
+            }
+        // output: b3 + sum over the units of w3 tanh(layer 2 + b2); the two lane halves hold
+        // the two halves of a sample's units
+        float acc = 0.0f;
+#pragma unroll
+        for (int M = 0; M < 2; ++M)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int u = unit_of(M, r, half);
+                acc += sw[FW_W3 + u] * tanh_fast(h2[M][r] + sw[FW_B2 + u]);
+            }
+        acc += __shfl_xor(acc, 32, 64);
+        if (half == 0 && live) out[smp] = acc + sw[FW_B3];
+    }
 }
 
 // Gradient of L = mean_i (f(x_i) - y_i)^2 (partials per workgroup: NPARAM values + loss sum)
@@ -203,15 +295,24 @@ __global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g, f
 }
 
 // |x| maximum over n values as float bits (non-negative floats order like their bits)
-__global__ void k_absmax_bits(int n, const double* __restrict__ a, const double* __restrict__ b,
-                              unsigned* __restrict__ out) {
+// max |a - b| as FP32 bits (non-negative floats order like their bits): wave and workgroup
+// maxima first, one atomic per workgroup (max is order-independent, so the result is exact)
+__global__ void __launch_bounds__(256) k_absmax_bits(int n, const double* __restrict__ a,
+                                                     const double* __restrict__ b,
+                                                     unsigned* __restrict__ out) {
+    __shared__ unsigned wmax[4];
     unsigned best = 0u;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float d = (float)fabs(a[i] - b[i]);
         best = max(best, __float_as_uint(d));
     }
     for (int off = 32; off > 0; off >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) best = max(best, wmax[w]);
+        atomicMax(out, best);
+    }
 }
 
 __global__ void k_mlp_features(int n, const double* __restrict__ u, const double* __restrict__ u_prev,
@@ -267,8 +368,10 @@ int dsv_mlp_forward(const float* params, const float* X, int32_t n, float* out, 
     return guard([&] {
         if (!params || !X || !out || n < 0) throw ArgError{"dsv_mlp_forward: bad arguments"};
         if (n == 0) return;
-        hipLaunchKernelGGL(k_mlp_forward, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                           params, X, n, out);
+        // four wavefronts of 32-sample blocks per workgroup, at most 1024 workgroups
+        const unsigned grid = std::min<unsigned>(cdiv(n, 128), 1024);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(256), 0, (hipStream_t)stream, params, X,
+                           n, out);
         DSV_HIP(hipGetLastError());
     });
 }
@@ -340,7 +443,7 @@ int dsv_mlp_features(int32_t n, const double* u, const double* u_prev, const dou
         hipStream_t st = (hipStream_t)stream;
         unsigned* bits = reinterpret_cast<unsigned*>(sigma);  // max |du| as float bits
         DSV_HIP(hipMemsetAsync(bits, 0, sizeof(unsigned), st));
-        hipLaunchKernelGGL(k_absmax_bits, dim3(std::min<unsigned>(cdiv(n, 256), 1024)), dim3(256), 0,
+        hipLaunchKernelGGL(k_absmax_bits, dim3(std::min<unsigned>(cdiv(n, 256), 128)), dim3(256), 0,
                            st, n, u, u_prev, bits);
         hipLaunchKernelGGL(k_mlp_features, dim3(cdiv(n, 256)), dim3(256), 0, st, n, u, u_prev, phi,
                            grad_phi, alpha, alpha_value, D, dt, eps, bc_mask, bits, X);

@@ -1598,6 +1598,7 @@ struct Workspace {
     int wmap[MAXD] = {};
     const double* wpair[MAXD][MAXS][2] = {};
     int wnhist = 0;
+    int wnextra = 0;
     int wvalid = 0;
     OpKey wkey;
     const double* wout_prev[MAXS] = {};  // the solution arrays the last solve wrote
@@ -3302,28 +3303,36 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
         DSV_HIP(hipHostMalloc((void**)&w->h_wout, sizeof(WarmOut), hipHostMallocCoherent));
         DSV_HIP(hipHostGetDevicePointer((void**)&w->d_wout_pub, w->h_wout, 0));
     }
-    // Direction k is a[s] - b[s] per species: d_0 = x - u_{n-1}, d_k = u_{n-k} - u_{n-k-1}
-    // (history; from_hist: d_k = hist_k - hist_(k+1), x = hist_0), then the explicit ones.
+    // Internal direction k: the explicit ones first (k < ne), new every call, then the history
+    // increments h = k - ne, a[s] - b[s] per species: d_0 = x - u_{n-1}, d_h = u_{n-h} -
+    // u_{n-h-1} (from_hist: d_h = hist_h - hist_(h+1), x = hist_0).  The caller's order (and
+    // that of `coef`) is the history first, then the explicit ones.
     // Under operator reuse (whose contract keeps the arrays named unchanged), a history
-    // direction k >= 1 whose pair of arrays is the previous call's pair k - 1 is that call's
-    // direction, and its A d_k is taken from there -- unless one of the two arrays is a
+    // direction h >= 1 whose pair of arrays is the previous call's internal pair k - 1 is that
+    // call's direction, and its A d is taken from there -- unless one of the two arrays is a
     // solution the last solve wrote (as the previous start x is when the start is not taken
     // from the history).  With the same FP64 operator and a history shifted by one step, that
-    // leaves one apply per call (two with the start given in x).
+    // leaves one apply per call for the history (two with the start given in x), and with at
+    // most one explicit direction the Gram rows of directions 2.. shift as well.
+    const int ne = wa.n_extra;
     const double* a[MAXD][MAXS] = {};
     const double* b[MAXD][MAXS] = {};
-    for (int k = 0; k < ND; ++k)
+    int api[MAXD];  // internal -> caller's index
+    for (int k = 0; k < ND; ++k) {
+        const int h = k - ne;
+        api[k] = h < 0 ? wa.n_hist + k : h;
         for (int s = 0; s < S; ++s) {
-            if (k < wa.n_hist && wa.from_hist) {
-                a[k][s] = wa.hist[(size_t)k * S + s];
-                b[k][s] = wa.hist[(size_t)(k + 1) * S + s];
-            } else if (k < wa.n_hist) {
-                a[k][s] = k == 0 ? sp[s].x : wa.hist[(size_t)(k - 1) * S + s];
-                b[k][s] = wa.hist[(size_t)k * S + s];
+            if (h < 0) {
+                a[k][s] = wa.extra[(size_t)k * S + s];
+            } else if (wa.from_hist) {
+                a[k][s] = wa.hist[(size_t)h * S + s];
+                b[k][s] = wa.hist[(size_t)(h + 1) * S + s];
             } else {
-                a[k][s] = wa.extra[(size_t)(k - wa.n_hist) * S + s];
+                a[k][s] = h == 0 ? sp[s].x : wa.hist[(size_t)(h - 1) * S + s];
+                b[k][s] = wa.hist[(size_t)h * S + s];
             }
         }
+    }
     const OpKey& opkey = w->key[w->method == DSV_METHOD_MF32_IR64 ? 1 : 0];
     const bool can = m->op_reuse && w->wvalid && same_key(opkey, w->wkey);
     auto written = [&](const double* p) {
@@ -3334,7 +3343,10 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     int map[MAXD];  // direction -> buffer pair (wd[2p], wd[2p + 1])
     bool reuse[MAXD] = {}, busy[MAXD] = {};
     for (int k = 1; k < ND && can; ++k) {
-        if (k >= wa.n_hist || k - 1 >= w->wnhist) continue;
+        // explicit directions and the newest increment are never reused; the previous call's
+        // direction k - 1 must be its history increment h - 1
+        const int h = k - ne;
+        if (h < 1 || ne != w->wnextra || h - 1 >= w->wnhist) continue;
         bool same = true;
         for (int s = 0; s < S; ++s)
             same = same && a[k][s] == w->wpair[k - 1][s][0] && b[k][s] == w->wpair[k - 1][s][1] &&
@@ -3376,6 +3388,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     w->wvalid = m->op_reuse;
     w->wkey = opkey;
     w->wnhist = wa.n_hist;
+    w->wnextra = ne;
     for (int k = 0; k < ND; ++k) {
         w->wmap[k] = map[k];
         for (int s = 0; s < S; ++s) {
@@ -3431,7 +3444,7 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
         if (wa.r_before) wa.r_before[s] = std::sqrt(o.rr0[s]);
         if (wa.r_after) wa.r_after[s] = std::sqrt(o.rr1[s]);
         if (wa.coef)
-            for (int k = 0; k < ND; ++k) wa.coef[k * S + s] = o.take[s] ? o.c[k][s] : 0.0;
+            for (int k = 0; k < ND; ++k) wa.coef[api[k] * S + s] = o.take[s] ? o.c[k][s] : 0.0;
     }
 }
 
