@@ -2819,9 +2819,9 @@ static void residual_norms_impl(dsv_mesh* m, const dsv_species* sp, double inv_d
 // definite -- enters, and c solves (d_j . A d_k) c_k = d_j . r0 with r0 = b - A x0.
 
 constexpr int PROJ_THREADS = 256;
-constexpr int PROJ_GRID_F = 1024;  // workgroups of dsv_solve_warm's norm and update kernels (the
-                                  // dot kernels, whose epilogue writes up to 44 partials per
-                                  // workgroup, run PROJ_GRID per species)
+constexpr int PROJ_GRID_F = 1024;  // workgroups of dsv_solve_warm's norm and update kernels, and
+                                  // the most its dot kernels (whose epilogue writes up to 44
+                                  // partials per workgroup and species) take
 constexpr int PROJ_GRID = 256;  // dsv_warm_start's S * 9 * PROJ_GRID partials (3 directions at
                                 // most) fit the residual partial buffer
 static_assert(MAXS * 9 * PROJ_GRID <= 2 * MAXS * MAX_PROD, "projection partials");
@@ -2901,33 +2901,49 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots(int V, ProjVec<S> pv
 
 // Gram quantities when directions 2.. are the previous call's 1.. (shifted history): only
 // rows 0 and 1 of d_j . A d_k and every d_j . r0 are new, 3 ND - 1 per species, in the order
-// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0; species blockIdx.y.  The rows
+// d_0 . A d_k (k < ND), d_1 . A d_k (1 <= k < ND), d_j . r0 per species.  The rows
 // are summed as A d_0 . d_k and A d_1 . d_k (ND + 3 vectors read), each sum in the same
 // per-thread and per-workgroup order as in k_proj_dots, so the values are the same bits.
 template <int S, int ND>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_dots_shift(int V, ProjVec<S> pv,
                                                                  double* __restrict__ pout) {
+    // every species of a vertex in one thread, so that each record is read once (a workgroup
+    // row per species read each 8-byte value of a record on its own, 2.5x the bytes from HBM
+    // once the vectors outgrow the caches); per species the sums keep k_proj_dots's order
     constexpr int NQ = 3 * ND - 1;
-    __shared__ double red[NQ * 16];
-    const int s = blockIdx.y;
-    double tot[NQ];
+    __shared__ double red[S * NQ * 16];
+    double tot[S * NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
+    for (int q = 0; q < S * NQ; ++q) tot[q] = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
-        const double r = pv.r0[i].v[s];
-        const double Ad0 = pv.Ad[0][i].v[s], Ad1 = pv.Ad[1][i].v[s];
+        const SV<double, S> r = pv.r0[i], Ad0 = pv.Ad[0][i], Ad1 = pv.Ad[1][i];
 #pragma unroll
         for (int k = 0; k < ND; ++k) {
-            const double d = pv.d[k][i].v[s];
-            tot[k] += Ad0 * d;
-            if (k >= 1) tot[ND + k - 1] += Ad1 * d;
-            tot[2 * ND - 1 + k] += d * r;
+            const SV<double, S> d = pv.d[k][i];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                tot[s * NQ + k] += Ad0.v[s] * d.v[s];
+                if (k >= 1) tot[s * NQ + ND + k - 1] += Ad1.v[s] * d.v[s];
+                tot[s * NQ + 2 * ND - 1 + k] += d.v[s] * r.v[s];
+            }
         }
     }
-    block_sum_all<NQ>(tot, red);
-    if (threadIdx.x != 0) return;
+    // block sums as in block_sum_all (wave, then the waves in order), one value at a time so
+    // that the epilogue does not hold every wave sum at once; thread q totals quantity q
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) pout[(s * NQ + q) * gridDim.x + blockIdx.x] = tot[q];
+    for (int q = 0; q < S * NQ; ++q) {
+        const double v = wave_sum(tot[q]);
+        if (lane == 0) red[q * 16 + wid] = v;
+    }
+    __syncthreads();
+    static_assert(S * NQ <= PROJ_THREADS, "one thread per quantity");
+    const int q = threadIdx.x;
+    if (q >= S * NQ) return;
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += red[q * 16 + w];
+    pout[q * gridDim.x + blockIdx.x] = t;
 }
 
 // the Gram rows of ND directions (grid x S workgroups): one launch up to six directions, rows
@@ -3407,20 +3423,24 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     for (int k = 2; k < ND; ++k) gshift = gshift && reuse[k];
     const double* gprev = w->gsave[w->gcur];
     double* gout = w->gsave[w->gcur ^ 1];
+    // Gram workgroups: 256 (the epilogue dominates at a few hundred thousand vertices), up to
+    // PROJ_GRID_F on larger meshes so that enough loads are in flight; the shifted and full
+    // Gram kernels use the same grid, so that each sum runs in the same order either way
+    const int pg = std::min(PROJ_GRID_F, std::max(PROJ_GRID, (int)((cdiv(m->V, 2048) + 7) / 8 * 8)));
     if constexpr (ND >= 3) {
         if (gshift) {
-            hipLaunchKernelGGL((k_proj_dots_shift<S, ND>), dim3(PROJ_GRID, S),
-                               dim3(PROJ_THREADS), 0, st, m->V, pv, w->pW);
+            hipLaunchKernelGGL((k_proj_dots_shift<S, ND>), dim3(pg), dim3(PROJ_THREADS), 0, st,
+                               m->V, pv, w->pW);
             hipLaunchKernelGGL(k_sum_partials, dim3(S * (3 * ND - 1)), dim3(PROJ_THREADS), 0, st,
-                               w->pW, PROJ_GRID, w->pT);
+                               w->pW, pg, w->pT);
             hipLaunchKernelGGL((k_proj_solve<S, ND, true>), dim3(1), dim3(64), 0, st, w->pT,
                                gprev, gout, w->wout);
         }
     }
     if (!gshift) {
-        launch_proj_dots<S, ND>(m, pv, w->pW, PROJ_GRID, st);
-        hipLaunchKernelGGL(k_sum_partials, dim3(S * NP), dim3(PROJ_THREADS), 0, st, w->pW,
-                           PROJ_GRID, w->pT);
+        launch_proj_dots<S, ND>(m, pv, w->pW, pg, st);
+        hipLaunchKernelGGL(k_sum_partials, dim3(S * NP), dim3(PROJ_THREADS), 0, st, w->pW, pg,
+                           w->pT);
         hipLaunchKernelGGL((k_proj_solve<S, ND, false>), dim3(1), dim3(64), 0, st, w->pT, gprev,
                            gout, w->wout);
     }
