@@ -183,3 +183,33 @@ def test_direction_from_a_written_solution_is_not_reused(gpu, between):
     for x, y in zip(za, zb):
         assert np.isfinite(x).all() and np.array_equal(x, y)
     assert da["dirs_reused"] == (0 if between else 1) and db["dirs_reused"] == 0, (da, db)
+
+
+def test_stepper_reuse_is_bit_identical_on_a_large_mesh(gpu):
+    """The same comparison on 614,125 vertices, where the warm start's Gram kernels run on more
+    than 256 workgroups (their grid grows with the mesh): reuse on and off still give the same
+    iterations and bit-identical solutions, with the shifted Gram rows taken under reuse."""
+    from dsv import workloads
+    from dsv.core import Mesh
+    from dsv.transport import Stepper
+    conn, xyz = meshes.kuhn_cube(84)
+    conn, xyz = meshes.locality_order(conn, xyz)
+    assert len(xyz) > 2048 * 256
+    bc = np.nonzero(xyz[:, 2] <= 1e-12)[0].astype(np.int32)
+    w = workloads.three_species(conn, xyz, xyz[:, 2] - 0.8, bc, 0.005, 0.03)
+    steppers = []
+    for reuse in (True, False):
+        st = Stepper(Mesh(w["conn"], w["xyz"]), w["species"], w["dt"], method="mf32_ir64",
+                     rtol=w["rtol"], reuse_operator=reuse)
+        st.set_warm_start("extrapolate")
+        steppers.append(st)
+    a, b = steppers
+    gram = []
+    for n in range(6):
+        ha, hb = a.step(), b.step()
+        assert ha["iterations"] == hb["iterations"], (n, ha, hb)
+        for ua, ub in zip(a.u, b.u):
+            assert np.array_equal(ua.cpu().numpy(), ub.cpu().numpy()), n
+        assert hb["gram_reused"] == 0
+        gram.append(ha["gram_reused"])
+    assert gram[4] == 1 and gram[5] == 1, gram
