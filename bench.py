@@ -395,7 +395,8 @@ def run_series(mesh, w, n=200, epochs=5, history_dirs=8):
     from dsv.transport import Stepper
     out = {"metric": "receding-interface series, 3-species BE steps/s and CG iterations over "
                      f"steps {n // 2 + 1}-{n}, rtol {w['rtol']:g}",
-           "workload": f"{w['name']}: {len(w['conn'])} tets, interface speed 0.008 O2, "
+           "workload": f"{w['name']}: {len(w['conn'])} tets, interface receding by the ORR law "
+                       "v = -0.008 O2 inside |phi| < 3 h (libdsv velocity and level-set advance), "
                        "coefficients recomputed on the GPU every step", "paths": {}}
     for name, method, warm in (("plain_fp64", "csr64", "previous"),
                                ("previous", "mf32_ir64", "previous"),

@@ -174,7 +174,8 @@ __device__ __forceinline__ void vertex_gradients(int32_t r, int32_t V,
 
 // v_n = min(max(v_Zn, v_ORR, v_O2), 0) inside |phi| < 3 h, 0 outside, with
 // n = grad phi / sqrt(|grad phi|^2 + 1e-12), v_Zn = -max(DeZn, 0) |grad Zn . n| / (Zn_sol - Zn_sat),
-// v_O2 = -2 max(DeO2, 0) |grad O2 . n| / Zn_sol, v_ORR = -(2 / Zn_sol) k_orr max(O2, 0)
+// v_O2 = -2 max(DeO2, 0) |grad O2 . n| / Zn_sol, v_ORR = -(2 / Zn_sol) k_orr max(O2, 0); a term
+// whose fields are NULL (Zn and DeZn, or DeO2) is left out of the max
 __global__ void k_interface_velocity(int32_t V, const int32_t* __restrict__ conn,
                                      const double* __restrict__ xyz, const double* __restrict__ detJ,
                                      const int32_t* __restrict__ diag_pos,
@@ -191,17 +192,16 @@ __global__ void k_interface_velocity(int32_t V, const int32_t* __restrict__ conn
         vel[r] = 0.0;
         return;
     }
-    const double* const u[3] = {phi, Zn, O2};
+    const double* const u[3] = {phi, Zn ? Zn : phi, O2};  // (an omitted Zn term reads phi)
     Vec3<double> g[3];
     vertex_gradients<3>(r, V, conn, xyz, detJ, diag_pos, cptr, cslot, u, g);
     const double gmag = sqrt(g[0].x * g[0].x + g[0].y * g[0].y + g[0].z * g[0].z + 1e-12);
     const double nx = g[0].x / gmag, ny = g[0].y / gmag, nz = g[0].z / gmag;
     const double dzn = g[1].x * nx + g[1].y * ny + g[1].z * nz;
     const double do2 = g[2].x * nx + g[2].y * ny + g[2].z * nz;
-    const double dezn = np_max(DeZn[r], 0.0), deo2 = np_max(DeO2[r], 0.0);
     const double o2 = np_max(O2[r], 0.0);
-    const double v_zn = -dezn * fabs(dzn) / denom_zn;
-    const double v_o2d = -2.0 * deo2 * fabs(do2) / p.zn_solid;
+    const double v_zn = Zn ? -np_max(DeZn[r], 0.0) * fabs(dzn) / denom_zn : -INFINITY;
+    const double v_o2d = DeO2 ? -2.0 * np_max(DeO2[r], 0.0) * fabs(do2) / p.zn_solid : -INFINITY;
     const double v_orr = -(2.0 / p.zn_solid) * p.k_orr * o2;
     vel[r] = np_min(np_max(v_zn, np_max(v_orr, v_o2d)), 0.0);
 }
@@ -458,8 +458,9 @@ int dsv_interface_velocity(dsv_mesh* m, const double* phi, const double* Zn, con
                            const double* DeZn, const double* DeO2,
                            const dsv_velocity_params* params, double* v, void* stream) {
     return guard([&] {
-        if (!m || !phi || !Zn || !O2 || !DeZn || !DeO2 || !params || !v)
-            throw ArgError{"dsv_interface_velocity: NULL argument"};
+        if (!m || !phi || !O2 || !params || !v || (!Zn != !DeZn))
+            throw ArgError{"dsv_interface_velocity: NULL argument (Zn and DeZn are NULL together "
+                           "or neither)"};
         if (!(params->zn_solid != 0.0)) throw ArgError{"dsv_interface_velocity: zn_solid == 0"};
         double denom = params->zn_solid - params->zn_sat;
         if (fabs(denom) < 1e-30) denom = params->zn_solid;

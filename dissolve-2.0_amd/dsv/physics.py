@@ -99,7 +99,10 @@ def mesh_h_avg(mesh):
 
 
 def interface_velocity(mesh, phi, Zn, O2, DeZn, DeO2, h, k_orr, zn_solid, zn_sat, out=None):
-    """The recession velocity per vertex (device tensor)."""
+    """The recession velocity per vertex (device tensor).  Zn and DeZn None (together) leave the
+    Zn-dissolution rate out, DeO2 None the O2-diffusion rate: with both, the ORR law alone."""
+    if (Zn is None) != (DeZn is None):
+        raise ValueError("Zn and DeZn are given together or not at all")
     d = [_dev(mesh, a, n) for a, n in ((phi, "phi"), (Zn, "Zn"), (O2, "O2"), (DeZn, "DeZn"),
                                        (DeO2, "DeO2"))]
     out = _empty(mesh) if out is None else out

@@ -15,8 +15,9 @@ cosine delta of `physics/weak_forms.py:18-24`.
                     D = 2.0 / 2.0 / 5.3, dt = 0.005, ORR sink on O2, OH source on phi < 0,
                     O2 Dirichlet (bulk value) on the bottom face.
 * gyroid3        -- gyroid lattice 160^3 Kuhn (14.7M tets), same three species.
-* RecedingInterface -- the disc3 time series with the interface receding at a speed
-                    proportional to local O2, coefficients recomputed on the GPU every step.
+* RecedingInterface -- the disc3 time series with the interface receding by the ORR law in
+                    the |phi| < 3 h band (speed proportional to local O2), coefficients
+                    recomputed on the GPU every step.
 """
 from __future__ import annotations
 
@@ -172,29 +173,43 @@ def gyroid3(n=160, level=0.3):
 
 class RecedingInterface:
     """The receding-interface time series on a three-species workload: before each step the
-    level set moves with a normal speed proportional to the local O2 concentration,
-    phi <- phi - dt kappa O2 (the metal, phi > 0, dissolves where oxygen reaches it), and the
-    step's coefficients follow on the GPU: |grad phi| (libdsv dsv_nodal_gradient), the ORR sink
-    alpha_O2 = delta_eps(phi) |grad phi| k_orr and the OH source 4 delta_eps |grad phi| k_orr O2
-    (dsv_transport_coefficients, the species blocks' `dissolve.py:866-875,923-936` with no film).
+    level set recedes with the ORR law alone, v = -(2 / zn_solid) k_orr max(O2, 0) inside the
+    band |phi| < 3 h (h the mean cell size) and 0 outside (libdsv dsv_interface_velocity with
+    the Zn-dissolution and O2-diffusion rates left out, `physics/interface_velocity.py:741,749`),
+    zn_solid chosen so that v = -speed O2 in the band; then phi += v dt (dsv_levelset_advance,
+    `dissolve.py:684-690`), |grad phi| (dsv_nodal_gradient), and the step's coefficients: the
+    ORR sink alpha_O2 = delta_eps(phi) |grad phi| k_orr and the OH source
+    4 delta_eps |grad phi| k_orr O2 (dsv_transport_coefficients, the species blocks'
+    `dissolve.py:866-875,923-936` with no film).  Every field stays on the device and every
+    update is a libdsv kernel.
 
     Call the instance before each stepper.step(); it updates stepper.phi and stepper.grad_phi
     in place (the learned warm start's features) and refreshes the stepper's alpha_O2 and f_OH.
     """
 
-    def __init__(self, stepper, w, kappa=0.008):
+    def __init__(self, stepper, w, speed=0.008):
+        import torch
+        from .physics import mesh_h_avg
         self.st = stepper
         self.w = w
-        self.kappa = float(kappa)
         if stepper.phi is None or stepper.grad_phi is None:
             raise ValueError("the stepper needs phi and grad_phi")
+        if not speed > 0.0:
+            raise ValueError("speed must be positive")
+        self.k_orr = float(w["k_orr"])
+        self.zn_solid = 2.0 * self.k_orr / float(speed)
+        self.h = mesh_h_avg(stepper.mesh)
+        self.v = torch.empty_like(stepper.phi)
 
     def __call__(self):
         from .coefficients import grad_norm, transport_coefficients
+        from .physics import interface_velocity, levelset_advance
         st, w = self.st, self.w
         mesh = st.mesh
         o2 = st.u[0]
-        st.phi.sub_(st.dt * self.kappa * o2)
+        interface_velocity(mesh, st.phi, None, o2, None, None, self.h, self.k_orr, self.zn_solid,
+                           0.0, out=self.v)
+        levelset_advance(mesh, st.phi, self.v, st.dt, out=st.phi)
         grad_norm(mesh, st.phi, out=st.grad_phi)
         c = transport_coefficients(mesh, st.phi, st.grad_phi, w["eps"], w["k_orr"], O2=o2)
         st.refresh_coefficients(alpha=[c["alpha_o2"], None, None], f=[None, None, c["f_oh"]])

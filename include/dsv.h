@@ -340,7 +340,9 @@ typedef struct dsv_velocity_params {
  *   v = min(max(v_Zn, v_ORR, v_O2), 0),  n = grad phi / sqrt(|grad phi|^2 + 1e-12),
  *   v_Zn = -max(DeZn, 0) |grad Zn . n| / (zn_solid - zn_sat)  (zn_solid if that is ~0),
  *   v_O2 = -2 max(DeO2, 0) |grad O2 . n| / zn_solid,  v_ORR = -(2 / zn_solid) k_orr max(O2, 0),
- * and 0 outside; the gradients are the nodal gradients of dsv_nodal_gradient. */
+ * and 0 outside; the gradients are the nodal gradients of dsv_nodal_gradient.  Zn and DeZn NULL
+ * (together) leave v_Zn out, DeO2 NULL leaves v_O2 out: with both, v = min(v_ORR, 0) in the band,
+ * the ORR law alone (`physics/interface_velocity.py:741,749`). */
 int dsv_interface_velocity(dsv_mesh* m, const double* phi, const double* Zn, const double* O2,
                            const double* DeZn, const double* DeO2,
                            const dsv_velocity_params* params, double* v, void* stream);

@@ -78,10 +78,14 @@ def interface_velocity(conn, xyz, phi, Zn, O2, DeZn, DeO2, h, k_orr, zn_solid=ZN
     gphi = OC.nodal_gradient(conn, xyz, phi)
     gmag = np.sqrt(gphi[:, 0] ** 2 + gphi[:, 1] ** 2 + gphi[:, 2] ** 2 + 1e-12)
     n = gphi / gmag[:, None]
-    dzn = (OC.nodal_gradient(conn, xyz, Zn) * n).sum(1)
-    do2 = (OC.nodal_gradient(conn, xyz, O2) * n).sum(1)
-    v_zn = -np.maximum(DeZn, 0.0) * np.abs(dzn) / denom
-    v_o2d = -2.0 * np.maximum(DeO2, 0.0) * np.abs(do2) / zn_solid
+    # a term whose fields are None (Zn and DeZn, or DeO2) is left out of the max
+    v_zn = v_o2d = np.full(len(phi), -np.inf)
+    if Zn is not None:
+        dzn = (OC.nodal_gradient(conn, xyz, Zn) * n).sum(1)
+        v_zn = -np.maximum(DeZn, 0.0) * np.abs(dzn) / denom
+    if DeO2 is not None:
+        do2 = (OC.nodal_gradient(conn, xyz, O2) * n).sum(1)
+        v_o2d = -2.0 * np.maximum(DeO2, 0.0) * np.abs(do2) / zn_solid
     v_orr = -(2.0 / zn_solid) * k_orr * np.maximum(O2, 0.0)
     v = np.minimum(np.maximum(v_zn, np.maximum(v_orr, v_o2d)), 0.0)
     return np.where(band, v, 0.0)

@@ -74,6 +74,13 @@ def test_interface_velocity_advance_h_avg_and_redistance(case):
     v = physics.interface_velocity(mesh, phi, Zn, O2, DeZn, DeO2, *args).cpu().numpy()
     ref = OP.interface_velocity(conn, xyz, phi, Zn, O2, DeZn, DeO2, *args)
     assert (ref != 0).sum() > 50 and _rel(v, ref) <= 1e-12
+    # the rates left out: no Zn term, no O2-diffusion term, the ORR law alone
+    for zn, dezn, deo2 in ((None, None, DeO2), (Zn, DeZn, None), (None, None, None)):
+        got = physics.interface_velocity(mesh, phi, zn, O2, dezn, deo2, *args).cpu().numpy()
+        want = OP.interface_velocity(conn, xyz, phi, zn, O2, dezn, deo2, *args)
+        assert (want != 0).sum() > 50 and _rel(got, want) <= 1e-12, (zn is None, deo2 is None)
+    with pytest.raises(ValueError):
+        physics.interface_velocity(mesh, phi, Zn, O2, None, DeO2, *args)
     p2 = physics.levelset_advance(mesh, phi, v, 0.005).cpu().numpy()
     assert _rel(p2, phi + v * 0.005) <= 1e-15
     for iters in (1, 2, 5):
@@ -152,3 +159,34 @@ def test_scaffold_volumes(case):
     assert abs(physics.compute_scaffold_volume_exact(pf, mesh) - rvol) <= 1e-12 * rvol
     with pytest.raises(ValueError):
         physics.scaffold_volume(mesh, phi, eps=0.0)
+
+
+def test_receding_interface_driver_matches_oracle(gpu):
+    """The receding-interface series' interface step: the ORR law alone in the |phi| < 3 h band
+    (libdsv's velocity with the Zn and O2-diffusion rates left out), phi += v dt, then
+    |grad phi| and the refreshed ORR sink and OH source, against the oracle on the same fields."""
+    from dsv.core import Mesh
+    from dsv.transport import Stepper
+    from oracle import coefficients as OC
+    w = workloads.disc3(seed=1, scale=6.0)
+    conn, xyz = w["conn"], w["xyz"]
+    mesh = Mesh(conn, xyz)
+    st = Stepper(mesh, w["species"], w["dt"], method="mf32_ir64", rtol=1e-10, phi=w["phi"],
+                 grad_phi=w["grad_norm"], eps=w["eps"])
+    st.step()  # O2 away from its initial field
+    drv = workloads.RecedingInterface(st, w, speed=0.5)
+    phi0, o2 = st.phi.cpu().numpy(), st.u[0].cpu().numpy()
+    drv()
+    h = OP.h_avg(conn, xyz)
+    v = OP.interface_velocity(conn, xyz, phi0, None, o2, None, None, h, w["k_orr"],
+                              2.0 * w["k_orr"] / 0.5, 0.0)
+    band = np.abs(phi0) < 3 * h
+    assert band.sum() > 100 and np.all(v[~band] == 0.0) and v.min() < 0.0
+    phi1 = phi0 + w["dt"] * v
+    assert _rel(st.phi.cpu().numpy(), phi1) <= 1e-12
+    gn = OC.grad_norm(OC.nodal_gradient(conn, xyz, phi1))
+    assert _rel(st.grad_phi.cpu().numpy(), gn) <= 1e-12
+    alpha, f_oh, _, _ = OC.transport_coefficients(phi1, gn, None, None, o2, w["eps"], w["k_orr"],
+                                                  0.0, 1.0, 1.0)
+    assert _rel(st.alpha_dev[0].cpu().numpy(), alpha) <= 1e-12
+    assert _rel(st.f[2].cpu().numpy(), f_oh) <= 1e-12

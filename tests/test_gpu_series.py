@@ -1,6 +1,7 @@
 """The learned warm start over the receding-interface time series on the full 806k-tet disc:
-200 backward-Euler steps of O2 / Cl / OH while the interface recedes at a speed proportional to
-local O2 (coefficients recomputed on the GPU every step); the MLP is trained on the GPU on steps
+200 backward-Euler steps of O2 / Cl / OH while the interface recedes by the ORR law in the
+|phi| < 3 h band, at a speed proportional to local O2 (coefficients recomputed on the GPU every
+step); the MLP is trained on the GPU on steps
 1-100 of the solver's own outputs and used on steps 101-200.  Against the plain
 previous-solution start over steps 101-200: far fewer CG iterations; at rtol 1e-10 every
 checked step's solutions lie within 1e-8 of the plain FP64 path's rtol-1e-13 solution of the

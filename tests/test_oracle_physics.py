@@ -82,6 +82,17 @@ def test_interface_velocity_closed_form_for_linear_fields(k):
     assert np.all(v[np.abs(phi) >= 3 * h] == 0)
     # no band: all zero
     assert np.all(OP.interface_velocity(conn, xyz, phi + 100, Zn, O2, DeZn, DeO2, h, k_orr) == 0)
+    # left-out rates: without Zn the max runs over the ORR and O2-diffusion rates, without both
+    # it is the ORR law alone (the same closed forms)
+    band = np.abs(phi) < 3 * h
+    v2 = OP.interface_velocity(conn, xyz, phi, None, O2, None, DeO2, h, k_orr, zs, zsat)
+    ref2 = np.where(band, np.minimum(np.maximum(v_orr, v_o2), 0.0), 0.0)
+    assert np.abs(v2 - ref2).max() <= 1e-9 * np.abs(ref2).max()
+    v3 = OP.interface_velocity(conn, xyz, phi, None, O2, None, None, h, k_orr, zs, zsat)
+    assert np.array_equal(v3, np.where(band, np.minimum(v_orr, 0.0), 0.0))
+    v4 = OP.interface_velocity(conn, xyz, phi, Zn, O2, DeZn, None, h, k_orr, zs, zsat)
+    ref4 = np.where(band, np.minimum(np.maximum(v_zn, v_orr), 0.0), 0.0)
+    assert np.abs(v4 - ref4).max() <= 1e-9 * np.abs(ref4).max()
 
 
 def test_h_avg_bruteforce():
