@@ -86,11 +86,32 @@ __global__ void k_transport_coefficients(int32_t V, const double* __restrict__ p
         f_oh[v] = in_medium * p.kd * Fv * cl * cl + 4.0 * dl * p.k_orr * o2;
     }
     if (film || medium) {
-        const double sat = fmin(fmax(Fv / fmax(p.Fmax, 1e-20), 0.0), 1.0);
+        const double sat = np_min(np_max(Fv / fmax(p.Fmax, 1e-20), 0.0), 1.0);  // numpy clip
         const double ff = (1.0 - sat) + sat * p.film_floor;
         if (film) film[v] = ff;
         if (medium) medium[v] = ff * in_medium;
     }
+}
+
+// Effective diffusivities: DeZn = D0[0] film_factor, DeCl / DeOH / DeO2 = D0[1..3]
+// medium_factor, the factors as in k_transport_coefficients
+struct Diff4 {
+    double d[4];
+};
+__global__ void k_effective_diffusion(int32_t V, const double* __restrict__ phi,
+                                      const double* __restrict__ F, dsv_coef_params p, Diff4 D0,
+                                      double* __restrict__ DeZn, double* __restrict__ DeCl,
+                                      double* __restrict__ DeOH, double* __restrict__ DeO2) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double Fv = F ? F[v] : 0.0;
+    const double sat = np_min(np_max(Fv / fmax(p.Fmax, 1e-20), 0.0), 1.0);  // numpy clip
+    const double ff = (1.0 - sat) + sat * p.film_floor;
+    const double medium = ff * (phi[v] <= 0.0 ? 1.0 : 0.0);
+    if (DeZn) DeZn[v] = ff * D0.d[0];
+    if (DeCl) DeCl[v] = medium * D0.d[1];
+    if (DeOH) DeOH[v] = medium * D0.d[2];
+    if (DeO2) DeO2[v] = medium * D0.d[3];
 }
 
 }  // namespace dsv
@@ -126,6 +147,21 @@ int dsv_transport_coefficients(dsv_mesh* m, const double* phi, const double* gra
         hipLaunchKernelGGL(k_transport_coefficients, dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
                            phi, grad_norm, F_film, Cl, O2, *params, alpha_o2, f_oh, film_factor,
                            medium_factor);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_effective_diffusion(dsv_mesh* m, const double* phi, const double* F_film,
+                            const dsv_coef_params* params, const double* D0, double* DeZn,
+                            double* DeCl, double* DeOH, double* DeO2, void* stream) {
+    return guard([&] {
+        if (!m || !phi || !params || !D0)
+            throw ArgError{"dsv_effective_diffusion: NULL mesh, phi, params or D0"};
+        Diff4 d{{D0[0], D0[1], D0[2], D0[3]}};
+        DSV_HIP(hipSetDevice(m->device));
+        hipLaunchKernelGGL(k_effective_diffusion, dim3(cdiv(m->V, 256)), dim3(256), 0,
+                           (hipStream_t)stream, m->V, phi, F_film, *params, d, DeZn, DeCl, DeOH,
+                           DeO2);
         DSV_HIP(hipGetLastError());
     });
 }

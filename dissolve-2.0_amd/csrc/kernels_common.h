@@ -77,4 +77,12 @@ __device__ __forceinline__ double cell_coef(const double* __restrict__ p, int mo
     return (((p[n0] + p[n1]) + p[n2]) + p[n3]) * 0.25;
 }
 
+// numpy's maximum / minimum: NaN if either operand is NaN (fmax / fmin drop a NaN operand)
+__device__ __forceinline__ double np_max(double a, double b) {
+    return (a != a || b != b) ? a + b : (a >= b ? a : b);
+}
+__device__ __forceinline__ double np_min(double a, double b) {
+    return (a != a || b != b) ? a + b : (a <= b ? a : b);
+}
+
 }  // namespace dsv

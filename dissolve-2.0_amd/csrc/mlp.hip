@@ -358,6 +358,19 @@ static float* scratch_on_current_device() {
     return static_cast<float*>(p);
 }
 
+// The learned warm start's directions in FP64: dirs[s][i] = out[s V + i] (sigma_s tau), from one
+// forward pass over the rows of all species
+struct DirOut {
+    double* d[3];
+};
+__global__ void k_mlp_directions(int32_t V, const float* __restrict__ out,
+                                 const float* __restrict__ sigma, double tau, DirOut dirs) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (i >= V) return;
+    dirs.d[s][i] = (double)out[(size_t)s * V + i] * ((double)sigma[s] * tau);
+}
+
 }  // namespace dsv
 
 using namespace dsv;
@@ -427,6 +440,22 @@ int dsv_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t s
         const float c2 = 1.f - std::pow(beta2, (float)step);
         hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(MLP_NPARAM, 256)), dim3(256), 0, (hipStream_t)stream,
                            params, grad, m, v, MLP_NPARAM, lr, beta1, beta2, eps, c1, c2);
+        DSV_HIP(hipGetLastError());
+    });
+}
+
+int dsv_mlp_directions(int32_t n_species, int32_t V, const float* out, const float* sigma,
+                       double tau, double* const* dirs, void* stream) {
+    return guard([&] {
+        if (n_species < 1 || n_species > 3 || V < 1 || !out || !sigma || !dirs)
+            throw ArgError{"dsv_mlp_directions: bad arguments"};
+        DirOut d{};
+        for (int s = 0; s < n_species; ++s) {
+            if (!dirs[s]) throw ArgError{"dsv_mlp_directions: NULL direction"};
+            d.d[s] = dirs[s];
+        }
+        hipLaunchKernelGGL(k_mlp_directions, dim3(cdiv(V, 256), n_species), dim3(256), 0,
+                           (hipStream_t)stream, V, out, sigma, tau, d);
         DSV_HIP(hipGetLastError());
     });
 }

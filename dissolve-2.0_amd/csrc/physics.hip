@@ -16,14 +16,6 @@
 
 namespace dsv {
 
-// numpy's maximum / minimum: NaN if either operand is NaN
-__device__ __forceinline__ double np_max(double a, double b) {
-    return (a != a || b != b) ? a + b : (a >= b ? a : b);
-}
-__device__ __forceinline__ double np_min(double a, double b) {
-    return (a != a || b != b) ? a + b : (a <= b ? a : b);
-}
-
 // ---- Zn2+ coefficients: alpha_zn = [phi <= 0] kf (1 - F / Fmax), src_zn = [phi <= 0] kd F Cl^2
 __global__ void k_zinc_coefficients(int32_t V, const double* __restrict__ phi,
                                     const double* __restrict__ F, const double* __restrict__ Cl,

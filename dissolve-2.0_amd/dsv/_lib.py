@@ -33,9 +33,10 @@ EXPORTS = (
     "dsv_set_operator_reuse", "dsv_time_apply",
     "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_solve_warm_history",
     "dsv_mlp_features",
-    "dsv_mlp_forward",
+    "dsv_mlp_forward", "dsv_mlp_directions",
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_grad_noalloc", "dsv_mlp_adam", "dsv_nodal_gradient",
-    "dsv_transport_coefficients", "dsv_zinc_coefficients", "dsv_film_update", "dsv_under_relax",
+    "dsv_transport_coefficients", "dsv_effective_diffusion", "dsv_zinc_coefficients",
+    "dsv_film_update", "dsv_under_relax",
     "dsv_interface_velocity", "dsv_levelset_advance", "dsv_mesh_h_avg", "dsv_redistance",
     "dsv_scaffold_volume",
 )
@@ -155,6 +156,8 @@ def load():
         "dsv_nodal_gradient": (ctypes.c_int, [P, P, P, P, P]),
         "dsv_transport_coefficients": (ctypes.c_int, [P, P, P, P, P, P, ctypes.POINTER(CoefParams),
                                                       P, P, P, P, P]),
+        "dsv_effective_diffusion": (ctypes.c_int, [P, P, P, ctypes.POINTER(CoefParams),
+                                                   ctypes.POINTER(D), P, P, P, P, P]),
         "dsv_zinc_coefficients": (ctypes.c_int, [P, P, P, P, ctypes.POINTER(FilmParams), P, P, P,
                                                  ctypes.POINTER(I32), P]),
         "dsv_film_update": (ctypes.c_int, [P, P, P, P, P, ctypes.POINTER(FilmParams), D, P, P]),
@@ -167,6 +170,7 @@ def load():
         "dsv_scaffold_volume": (ctypes.c_int, [P, P, I32, D, P, ctypes.POINTER(D), P]),
         "dsv_mlp_features": (ctypes.c_int, [I32, P, P, P, P, P, D, D, D, D, P, P, P, P]),
         "dsv_mlp_forward": (ctypes.c_int, [P, P, I32, P, P]),
+        "dsv_mlp_directions": (ctypes.c_int, [I32, I32, P, P, D, ctypes.POINTER(P), P]),
         "dsv_mlp_grad_workspace": (ctypes.c_size_t, [I32]),
         "dsv_mlp_grad": (ctypes.c_int, [P, P, P, I32, P, P, P, ctypes.c_size_t, P]),
         "dsv_mlp_grad_noalloc": (ctypes.c_int, [P, P, P, I32, P, P, P]),

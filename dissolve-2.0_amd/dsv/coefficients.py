@@ -96,25 +96,40 @@ def orr_sink_and_oh_source(mesh, phi, normgradphi, epsilon, k_orr, kd=0.0, F_fil
     return d["alpha_o2"], d["f_oh"]
 
 
+def effective_diffusion(mesh, phi, F_film, Fmax, film_floor, D0, out=None):
+    """(DeZn, DeCl, DeOH, DeO2) device tensors (libdsv dsv_effective_diffusion): D0 = (D_Zn0,
+    D_Cl0, D_OH0, D_O20), DeZn = D_Zn0 f, the others D_i0 f [phi <= 0], with the film factor
+    f = (1 - s) + s film_floor, s = clip(F / Fmax, 0, 1); `out` may give the four tensors."""
+    import torch
+    out = out if out is not None else tuple(
+        torch.empty(mesh.n_dofs, dtype=torch.float64, device=mesh.device) for _ in range(4))
+    p = _lib.CoefParams(1.0, 0.0, 0.0, float(Fmax), float(film_floor))
+    d0 = (ctypes.c_double * 4)(*[float(x) for x in D0])
+    keep = [_dev(mesh, a) for a in (phi, F_film)]
+    _lib.call("dsv_effective_diffusion", mesh.handle, ptr(keep[0]), ptr(keep[1]),
+              ctypes.byref(p), d0, *[ptr(t) for t in out], stream_ptr(mesh.device))
+    return out
+
+
 def compute_effective_diffusion(phi, F_film, params, DeZn, DeCl, DeOH, DeO2):
     """D_Zn = D_Zn0 f, D_i = D_i0 f [phi <= 0] for Cl, OH, O2 with the film factor
-    f = (1 - s) + s eps delta / tau^2, s = clip(F / Fmax, 0, 1) (reference signature and checks)."""
+    f = (1 - s) + s eps delta / tau^2, s = clip(F / Fmax, 0, 1) (reference signature and checks;
+    the checks run on the host copies)."""
     mesh = _mesh_of(phi)
     floor = params.film_porosity * params.film_constrictivity / (params.film_tortuosity ** 2)
-    d = transport_coefficients(mesh, phi, np.zeros(mesh.n_dofs), 1.0, F_film=F_film,
-                               Fmax=params.Fmax, film_floor=floor,
-                               outputs=("film_factor", "medium_factor"))
-    results = {DeZn: params.diff_zn * d["film_factor"], DeCl: params.diff_cl * d["medium_factor"],
-               DeOH: params.diff_oh * d["medium_factor"], DeO2: params.diff_o2 * d["medium_factor"]}
-    for fn, arr in results.items():
+    fields = (DeZn, DeCl, DeOH, DeO2)
+    out = effective_diffusion(mesh, phi, F_film, params.Fmax, floor,
+                              (params.diff_zn, params.diff_cl, params.diff_oh, params.diff_o2))
+    host = [t.cpu().numpy() for t in out]
+    for fn, arr in zip(fields, host):
         name = getattr(fn, "name", "De")
-        if not bool(arr.isfinite().all()):
+        if not np.all(np.isfinite(arr)):
             raise RuntimeError(f"[bounds] {name} contains NaN/Inf after effective diffusion update.")
-        if bool((arr < 0).any()):
+        if np.any(arr < 0):
             raise RuntimeError(f"[bounds] {name} has {int((arr < 0).sum())} negative values.")
-    for fn, arr in results.items():
+    for fn, arr, t in zip(fields, host, out):
         if hasattr(fn, "x"):
-            fn.x.array[:] = arr.cpu().numpy()
+            fn.x.array[:] = arr
             fn.x.scatter_forward()
         else:
-            fn.copy_(arr)
+            fn.copy_(t)

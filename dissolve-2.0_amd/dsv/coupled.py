@@ -26,7 +26,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _lib
-from .coefficients import transport_coefficients, grad_norm
+from .coefficients import effective_diffusion, transport_coefficients, grad_norm
 from .core import ptr, species_array, species_struct, stream_ptr, to_device
 from .physics import (film_update, interface_velocity, mesh_h_avg, redistance_device,
                       under_relax, zinc_coefficients, levelset_advance)
@@ -114,12 +114,8 @@ class CoupledStepper:
 
     def _effective_diffusion(self):
         p = self.p
-        c = transport_coefficients(self.mesh, self.phi, self.grad_phi, 1.0, F_film=self.F,
-                                   Fmax=p["Fmax"], film_floor=p["film_floor"],
-                                   outputs=("film_factor", "medium_factor"))
-        self.De[0].copy_(c["film_factor"]).mul_(p["D0"][0])
-        for k in (1, 2, 3):
-            self.De[k].copy_(c["medium_factor"]).mul_(p["D0"][k])
+        effective_diffusion(self.mesh, self.phi, self.F, p["Fmax"], p["film_floor"], p["D0"],
+                            out=self.De)
 
     def _interface_step(self):
         """Velocity, advance, redistancing and |grad phi| (`dissolve.py:660-730`)."""

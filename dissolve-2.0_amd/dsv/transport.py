@@ -209,9 +209,18 @@ class Stepper:
                 list(r0), list(r1))
 
     def _mlp_directions(self, X, sig):
-        """sigma_k tau f(X_k) per species (FP64 rows), from one forward over all species."""
-        out = self.mlp.forward(X).view(self.S, -1).double()
-        return out * (sig.double() * self.tau)[:, None]
+        """sigma_k tau f(X_k) per species (FP64 device vectors, libdsv dsv_mlp_directions), from
+        one forward over all species."""
+        import torch
+        V = self.u[0].numel()
+        if getattr(self, "_dirs", None) is None:
+            self._dirs = [torch.empty(V, dtype=torch.float64, device=self.mesh.device)
+                          for _ in range(self.S)]
+        out = self.mlp.forward(X)
+        P = ctypes.c_void_p * self.S
+        _lib.call("dsv_mlp_directions", self.S, V, ptr(out), ptr(sig), float(self.tau),
+                  P(*[d.data_ptr() for d in self._dirs]), stream_ptr(self.mesh.device))
+        return self._dirs
 
     def step(self, warm=None):
         """One backward-Euler step of every species; `warm` (list of tensors) replaces the

@@ -294,6 +294,17 @@ int dsv_transport_coefficients(dsv_mesh* m, const double* phi, const double* gra
                                const dsv_coef_params* params, double* alpha_o2, double* f_oh,
                                double* film_factor, double* medium_factor, void* stream);
 
+/* compute_effective_diffusion (`physics/interface_velocity.py:90-155`), per vertex, with
+ * film_factor and medium_factor as above (params: Fmax, film_floor; eps, k_orr, kd unused):
+ *   DeZn = D0[0] film_factor,  DeCl = D0[1] medium_factor,  DeOH = D0[2] medium_factor,
+ *   DeO2 = D0[3] medium_factor.
+ * D0 is a host array of four (the reference's argument order Zn, Cl, OH, O2); F_film may be NULL
+ * (taken as 0) and every output may be NULL.  Unlike the reference, it raises no bounds errors
+ * itself: the Python wrapper checks the fields the reference checks. */
+int dsv_effective_diffusion(dsv_mesh* m, const double* phi, const double* F_film,
+                            const dsv_coef_params* params, const double* D0, double* DeZn,
+                            double* DeCl, double* DeOH, double* DeO2, void* stream);
+
 /* --- the per-step physics around the solves: Zn2+ coefficients, film, under-relaxation,
  * interface velocity, level-set advance and redistancing (device arrays of length V in the
  * caller's vertex numbering) --- */
@@ -390,6 +401,11 @@ int dsv_mlp_features(int32_t n, const double* u, const double* u_prev, const dou
                      void* stream);
 /* out[i] = f(X[i]) for n samples */
 int dsv_mlp_forward(const float* params, const float* X, int32_t n, float* out, void* stream);
+/* The learned directions in FP64 from one forward pass over n_species blocks of V rows:
+ * dirs[s][i] = out[s V + i] * (sigma[s] * tau), sigma the species' dsv_mlp_features scales
+ * (device FP32) and tau the training targets' RMS; no counterpart in the reference. */
+int dsv_mlp_directions(int32_t n_species, int32_t V, const float* out, const float* sigma,
+                       double tau, double* const* dirs, void* stream);
 /* Bytes of device workspace dsv_mlp_grad needs for n samples. */
 size_t dsv_mlp_grad_workspace(int32_t n);
 /* grad = dL/dparams and *loss = L of L = mean_i (f(X[i]) - y[i])^2 (loss may be NULL);

@@ -93,6 +93,20 @@ def test_reference_surface_functions(gpu):
                                                 None, 1.0, 0.0, 0.0, 1.0, 0.0625)
     for fn, d0, f in zip(out, (1.1, 2.0, 5.3, 2.0), (film, med, med, med)):
         assert np.allclose(fn.x.array, d0 * f, rtol=1e-14, atol=0)
+    # the device form against the oracle's effective_diffusion, with and without a film
+    from dsv.coefficients import effective_diffusion
+    from oracle import physics as OP
+    D0 = (0.7, 2.0, 5.3, 2.0)
+    for Fa in (F.x.array, None):
+        got = effective_diffusion(m, phi.x.array, Fa, 1.0, 0.0625, D0)
+        ref = OP.effective_diffusion(phi.x.array, np.zeros(len(xyz)) if Fa is None else Fa, 1.0,
+                                     0.0625, D0)
+        for g, r in zip(got, ref):
+            assert np.array_equal(g.cpu().numpy(), r)
+    # the reference's bounds check: a non-finite film gives non-finite diffusivities
+    F.x.array[3] = np.nan
+    with pytest.raises(RuntimeError, match="NaN/Inf"):
+        compute_effective_diffusion(phi, F, params, *out)
 
 
 def test_argument_errors(gpu):
