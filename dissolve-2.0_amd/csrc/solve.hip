@@ -123,8 +123,10 @@ struct alignas(4 * sizeof(T)) Vec4 {
 // into the per-vertex diagonal `mdiag`.
 template <class T>
 struct SpeciesP {
-    const T* D;
-    const T* A;
+    const T* D;  // per ELL entry (generic form); NULL for a scalar D (D1) or when only the
+                 // stored form is prepared (OpParams::no_generic)
+    const T* A;  // per ELL entry; NULL for a scalar reaction coefficient (A0)
+    T A0;        // the reaction coefficient where A is NULL
     T D6;        // fast forms: D / 6
     T D1;        // stored-geometry form: D (1 when W holds the species' own K(D) weights)
     const T* W;  // stored-geometry form with per-species weights: K(D_s) per ELL entry
@@ -140,6 +142,8 @@ struct OpParams {
     int fast;
     int stored;  // stored-geometry form (lumped mass): apply, residual, diagonal, CG
     int persp;   // stored form with per-species weights (some species has a D field)
+    int no_generic;  // a D field's per-entry values were not built: generic kernels (other than
+                     // the mass diagonal) must not run with these parameters
 };
 
 // vectors of the apply-type kernels (vertex order of the kernel)
@@ -326,11 +330,11 @@ __device__ __forceinline__ void lane_rows(const Geo& g, const OpParams<T>& prm,
                 const T gd = dot(cs, cs) * ri;
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const T mf = inv_dt + prm.sp[s].A[j];
+                    const T mf = inv_dt + (prm.sp[s].A ? prm.sp[s].A[j] : prm.sp[s].A0);
                     if constexpr (KIND == ROW_MASS)
                         acc[s] += mf * adet * T(5.0 / 120.0);
                     else
-                        acc[s] += prm.sp[s].D[j] * T(1.0 / 6.0) * gd +
+                        acc[s] += (prm.sp[s].D ? prm.sp[s].D[j] : prm.sp[s].D1) * T(1.0 / 6.0) * gd +
                                   (prm.lumped ? mf * adet * T(5.0 / 120.0) : mf * adet * T(2.0 / 120.0));
                 }
             } else {
@@ -342,8 +346,9 @@ __device__ __forceinline__ void lane_rows(const Geo& g, const OpParams<T>& prm,
                     if constexpr (FAST) {
                         acc[s] += gsum;
                     } else {
-                        const T mf = inv_dt + prm.sp[s].A[j];
-                        const T kr = -prm.sp[s].D[j] * T(1.0 / 6.0) * gsum;
+                        const T mf = inv_dt + (prm.sp[s].A ? prm.sp[s].A[j] : prm.sp[s].A0);
+                        const T kr = -(prm.sp[s].D ? prm.sp[s].D[j] : prm.sp[s].D1) * T(1.0 / 6.0) *
+                                     gsum;
                         if (prm.lumped)
                             acc[s] += kr + mf * adet * T(5.0 / 120.0) * u0;
                         else
@@ -1613,7 +1618,7 @@ struct Workspace {
     ResidPub* h_rpub = nullptr;     // the residual norms and CG state, published (pinned)
     ResidPub* d_rpub = nullptr;     // (h_rpub's device address)
     void* fw[2][MAXS] = {};  // stored form: per-species weights K(D_s) of sets 0 / 1
-    double* kvals = nullptr; // CSR values of K(D_s) (nnz), assembled to build them
+    double* kvs[MAXS] = {};  // CSR values of K(D_s) (nnz) per species, assembled to build them
     // operator reuse: the inputs of coefficient sets 0 / 1, of the Dirichlet bits, and whether
     // the Jacobi diagonal (VB_DINV) belongs to set 0
     OpKey key[2];
@@ -1671,7 +1676,7 @@ static void free_ws(Workspace* w) {
     if (w->h_wout) (void)hipHostFree(w->h_wout);
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < MAXS; ++q) dfree(w->fw[a][q]);
-    dfree(w->kvals);
+    for (auto& k : w->kvs) dfree(k);
     dfree(w->bcbits);
     for (int i = 0; i < VB_COUNT; ++i) dfree(w->vb[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -1828,6 +1833,8 @@ static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char
             return;
         }
     }
+    if (MODE != MF_MASS && p.no_generic)
+        throw std::logic_error("generic operator kernel on parameters prepared for the stored form");
     const size_t lds = checked_lds<T, S>(m, P);
     if constexpr (MODE != MF_MASS) {
         if (p.fast) {
@@ -1843,11 +1850,14 @@ static void launch_mf(const dsv_mesh* m, const Partition& P, const unsigned char
 // Builds the operator parameters of precision T in coefficient set `set`: per-ELL-entry cell
 // coefficients (D and alpha, cell-averaged like the assembly) and, for the fast form (lumped
 // mass, scalar D for every species), the mass diagonal computed by the generic MASS kernel.
+// Operator parameters of set `set` (0: the CG precision; 1: the FP64 residual of mf32_ir64).
+// Returns true when it assembled the K(D_s) of the species with a D field into w->kvs (which the
+// set-1 call of the same solve then gathers again, k_ready, instead of assembling them twice).
 template <class T, int S>
-static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_species* sp,
-                       double inv_dt, int lumped, hipStream_t st, int set) {
+static bool prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_species* sp,
+                       double inv_dt, int lumped, hipStream_t st, int set, bool k_ready = false) {
     const OpKey key = make_key(m, S, sp, inv_dt, lumped);
-    if (m->op_reuse && same_key(key, w->key[set])) return;  // prm / prm2 still hold the set
+    if (m->op_reuse && same_key(key, w->key[set])) return false;  // prm / prm2 still hold the set
     w->key[set] = key;
     w->key[set].valid = m->op_reuse;
     if (set == 0) w->dinv_valid = 0;
@@ -1857,15 +1867,23 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
     h.bcbits = w->bcbits;
     bool fast = lumped != 0;
     for (int s = 0; s < S; ++s) fast = fast && sp[s].D == nullptr;
+    const bool stored = lumped != 0 && m->operator_form == DSV_FORM_STORED && P.eptr;
+    // Where only stored-form kernels run on these parameters (the FP32 CG, and the FP64 residual
+    // and apply of set 1), a D field needs no per-entry values: the generic kernels are then
+    // refused (no_generic), except the mass diagonal, which reads the reaction alone.
+    const bool d_entries = !(stored && (std::is_same<T, float>::value || set == 1));
     const size_t n = (size_t)P.n_entries;
     for (int s = 0; s < S; ++s) {
         const double* src[2] = {sp[s].D, sp[s].alpha};
         const int32_t len[2] = {sp[s].D_len, sp[s].alpha_len};
-        const double val[2] = {sp[s].D_value, sp[s].alpha ? 0.0 : sp[s].alpha_value};
         const T** out[2] = {&h.sp[s].D, &h.sp[s].A};
         for (int c = 0; c < 2; ++c) {
             *out[c] = nullptr;
-            if (c == 0 && fast) continue;  // the fast form takes D as a scalar
+            if (!src[c]) continue;  // a scalar: D1 / A0 in the kernels
+            if (c == 0 && !d_entries) {
+                h.no_generic = 1;
+                continue;
+            }
             void*& buf = w->coef[set][c][s];
             size_t& cap = w->coef_cap[set][c][s];
             if (cap < n * sizeof(T)) {
@@ -1875,10 +1893,11 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
                 cap = n * sizeof(T);
             }
             hipLaunchKernelGGL((k_inc_coef<T>), dim3(cdiv(n, 256)), dim3(256), 0, st, (int64_t)n,
-                               P.ell_cell, m->conn, src[c], coef_mode(src[c], len[c], m), val[c],
+                               P.ell_cell, m->conn, src[c], coef_mode(src[c], len[c], m), 0.0,
                                (T*)buf);
             *out[c] = (const T*)buf;
         }
+        h.sp[s].A0 = (T)(sp[s].alpha ? 0.0 : sp[s].alpha_value);
         h.sp[s].D6 = (T)(sp[s].D_value / 6.0);
         h.sp[s].D1 = (T)sp[s].D_value;
     }
@@ -1886,7 +1905,6 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
     h.stored = 0;
     h.persp = 0;
     h.mdiag = nullptr;
-    const bool stored = lumped != 0 && m->operator_form == DSV_FORM_STORED && P.eptr;
     if (fast || stored) {
         if (!w->mdiag[set]) DSV_HIP(hipMalloc(&w->mdiag[set], (size_t)m->V * rec_bytes(sizeof(T), S)));
         MFVec<T, S> v{};
@@ -1898,6 +1916,7 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
         h.mdiag = w->mdiag[set];
         h.stored = stored ? 1 : 0;
     }
+    bool assembled = false;
     if (stored && !fast) {
         // some species has a D field: every species gets its own weights, K(D_s) per ELL entry
         // for a field (its off-diagonal CSR values, no mass, reaction or Dirichlet rows, gathered
@@ -1910,21 +1929,25 @@ static void prepare_op(dsv_mesh* m, Workspace* w, const Partition& P, const dsv_
                                                                    : (const void*)P.ewt64);
                 continue;
             }
-            if (!w->kvals) w->kvals = dalloc<double>((size_t)m->nnz);
+            if (!w->kvs[s]) w->kvs[s] = dalloc<double>((size_t)m->nnz);
             if (!w->fw[set][s]) DSV_HIP(hipMalloc(&w->fw[set][s], std::max<size_t>(1, ne) * sizeof(T)));
-            dsv_species ks{};
-            ks.D = sp[s].D;
-            ks.D_len = sp[s].D_len;
-            ks.D_value = sp[s].D_value;
-            assemble_into(m, ks, 0.0, 1, w->kvals, st);
+            if (!k_ready) {
+                dsv_species ks{};
+                ks.D = sp[s].D;
+                ks.D_len = sp[s].D_len;
+                ks.D_value = sp[s].D_value;
+                assemble_into(m, ks, 0.0, 1, w->kvs[s], st);
+                assembled = true;
+            }
             hipLaunchKernelGGL((k_gather_weights<T>), dim3(cdiv(ne, 256)), dim3(256), 0, st,
-                               (int64_t)ne, P.emap, w->kvals, (T*)w->fw[set][s]);
+                               (int64_t)ne, P.emap, w->kvs[s], (T*)w->fw[set][s]);
             h.sp[s].W = (const T*)w->fw[set][s];
             h.sp[s].D1 = T(1);
         }
     }
     std::memcpy(set ? w->prm2 : w->prm, &h, sizeof(h));
     DSV_HIP(hipGetLastError());
+    return assembled;
 }
 
 template <class T, int S>
@@ -2016,6 +2039,8 @@ static void launch_cg(dsv_mesh* m, Workspace* w, int par, hipStream_t stream,
             return;
         }
     }
+    if (p.no_generic)
+        throw std::logic_error("generic CG kernel on parameters prepared for the stored form");
     const size_t lds = checked_lds<T, S>(m, P);
     if (p.fast)
         hipLaunchKernelGGL((k_cg<T, S, true, INIT, false>), dim3(n), dim3(P.threads), lds, stream, g,
@@ -2673,8 +2698,8 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
             w->dinv_valid = m->op_reuse;
         }
     } else {
-        prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
-        prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1);
+        const bool k_ready = prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
+        prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1, k_ready);
         if (!w->dinv_valid) {
             MFVec<float, S> v{};
             v.y = (SV<float, S>*)w->vb[VB_DINV];
