@@ -7,11 +7,13 @@
 //     (`physics/interface_velocity.py:90-155`).
 //
 // The reference scatters every cell's weighted gradient to its 4 vertices with FP64 atomics.
-// Here each vertex gathers over its incident cells in ascending cell order -- the cells of the
-// vertex's diagonal CSR entry, already sorted by the set-up -- so the result is bit-reproducible.
-// A cell's gradient is J^-T (u1 - u0, u2 - u0, u3 - u0) = (du1 c1 + du2 c2 + du3 c3) / det J with
-// c1 = e2 x e3, c2 = e3 x e1, c3 = e1 x e2 (e_k = x_k - x_0), recomputed from the FP64
-// coordinates; the weight is |det J| / 6.
+// Here the gradient is regrouped by graph edge (the stencil of kernels_common.h): the set-up
+// sums, per CSR entry (r, j), w_c grad N_j^c over the cells holding both vertices in ascending
+// cell order (the entry's contribution list, sorted by the pattern build), and every later
+// gradient is one pass over the CSR rows, u_j - u_r times three FP64 weights per entry, with no
+// atomics, so every result is bit-reproducible.  A cell's shape gradients are c_k / det J for
+// k = 1..3 and -(c1 + c2 + c3) / det J for corner 0, with c1 = e2 x e3, c2 = e3 x e1,
+// c3 = e1 x e2 (e_k = x_k - x_0), from the FP64 coordinates.
 #include <cmath>
 
 #include "dsv_internal.h"
@@ -19,47 +21,94 @@
 
 namespace dsv {
 
-__global__ void k_nodal_gradient(int32_t V, const int32_t* __restrict__ conn,
-                                 const double* __restrict__ xyz, const double* __restrict__ detJ,
-                                 const int32_t* __restrict__ diag_pos,
-                                 const int32_t* __restrict__ cptr, const int32_t* __restrict__ cslot,
-                                 const double* __restrict__ u, double* __restrict__ grad,
-                                 double* __restrict__ gnorm) {
+// Set-up of the gradient stencil, one row per thread: G_rj for every entry of row r (0 on the
+// diagonal) and W_r from the diagonal entry's cells.  A contribution slot is cell * 16 +
+// (row corner) * 4 + (column corner).
+__global__ void k_grad_stencil(int32_t V, int64_t nnz, const int32_t* __restrict__ conn,
+                               const double* __restrict__ xyz, const double* __restrict__ detJ,
+                               const int32_t* __restrict__ indptr,
+                               const int32_t* __restrict__ diag_pos,
+                               const int32_t* __restrict__ cptr, const int32_t* __restrict__ cslot,
+                               double* __restrict__ G, double* __restrict__ W) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= V) return;
     const double* X = xyz;
     const double* Y = xyz + V;
     const double* Z = xyz + 2 * (size_t)V;
-    double gx = 0.0, gy = 0.0, gz = 0.0, ws = 0.0;
-    const int32_t p = diag_pos[r];
-    for (int32_t k = cptr[p]; k < cptr[p + 1]; ++k) {
-        const int32_t c = cslot[k] >> 4;
-        const int32_t n0 = conn[4 * (size_t)c], n1 = conn[4 * (size_t)c + 1];
-        const int32_t n2 = conn[4 * (size_t)c + 2], n3 = conn[4 * (size_t)c + 3];
-        const Vec3<double> x0{X[n0], Y[n0], Z[n0]};
-        const Vec3<double> e1{X[n1] - x0.x, Y[n1] - x0.y, Z[n1] - x0.z};
-        const Vec3<double> e2{X[n2] - x0.x, Y[n2] - x0.y, Z[n2] - x0.z};
-        const Vec3<double> e3{X[n3] - x0.x, Y[n3] - x0.y, Z[n3] - x0.z};
-        const Vec3<double> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
-        const double det = dot(e1, c1);
-        const double u0 = u[n0];
-        const double d1 = u[n1] - u0, d2 = u[n2] - u0, d3 = u[n3] - u0;
-        const double w = detJ[c] / 6.0;
-        const double s = w / det;
-        gx += s * (d1 * c1.x + d2 * c2.x + d3 * c3.x);
-        gy += s * (d1 * c1.y + d2 * c2.y + d3 * c3.y);
-        gz += s * (d1 * c1.z + d2 * c2.z + d3 * c3.z);
-        ws += w;
+    const int32_t pd = diag_pos[r];
+    for (int32_t p = indptr[r]; p < indptr[r + 1]; ++p) {
+        double gx = 0.0, gy = 0.0, gz = 0.0, ws = 0.0;
+        for (int32_t k = cptr[p]; k < cptr[p + 1]; ++k) {
+            const int32_t c = cslot[k] >> 4, b = cslot[k] & 3;
+            const double w = detJ[c] / 6.0;
+            ws += w;
+            if (p == pd) continue;
+            const int32_t n0 = conn[4 * (size_t)c], n1 = conn[4 * (size_t)c + 1];
+            const int32_t n2 = conn[4 * (size_t)c + 2], n3 = conn[4 * (size_t)c + 3];
+            const Vec3<double> x0{X[n0], Y[n0], Z[n0]};
+            const Vec3<double> e1{X[n1] - x0.x, Y[n1] - x0.y, Z[n1] - x0.z};
+            const Vec3<double> e2{X[n2] - x0.x, Y[n2] - x0.y, Z[n2] - x0.z};
+            const Vec3<double> e3{X[n3] - x0.x, Y[n3] - x0.y, Z[n3] - x0.z};
+            const Vec3<double> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
+            const double s = w / dot(e1, c1);
+            const Vec3<double> gn = b == 1 ? c1 : b == 2 ? c2 : b == 3 ? c3
+                : Vec3<double>{-(c1.x + c2.x + c3.x), -(c1.y + c2.y + c3.y), -(c1.z + c2.z + c3.z)};
+            gx += s * gn.x;
+            gy += s * gn.y;
+            gz += s * gn.z;
+        }
+        G[p] = gx;
+        G[nnz + p] = gy;
+        G[2 * nnz + p] = gz;
+        if (p == pd) W[r] = ws;
     }
-    gx /= ws;
-    gy /= ws;
-    gz /= ws;
-    if (grad) {
-        grad[3 * (size_t)r] = gx;
-        grad[3 * (size_t)r + 1] = gy;
-        grad[3 * (size_t)r + 2] = gz;
+}
+
+// grad u and |grad u| = sqrt(|grad u|^2 + 1e-12): the quad's lanes 0, 1, 2 store the
+// components, lane 3 the norm
+__global__ __launch_bounds__(256) void k_nodal_gradient(int32_t V, int64_t nnz,
+                                                        const int32_t* __restrict__ indptr,
+                                                        const int32_t* __restrict__ indices,
+                                                        const double* __restrict__ G,
+                                                        const double* __restrict__ W,
+                                                        const double* __restrict__ u,
+                                                        double* __restrict__ grad,
+                                                        double* __restrict__ gnorm) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r = (int32_t)(t / GS_LANES);
+    const int lane = (int)(t % GS_LANES);
+    if (r >= V) return;
+    const double* const uu[1] = {u};
+    Vec3<double> g[1];
+    stencil_gradients<1>(r, lane, nnz, indptr, indices, G, W, uu, g);
+    if (lane < 3) {
+        if (grad) grad[3 * (size_t)r + lane] = lane == 0 ? g[0].x : lane == 1 ? g[0].y : g[0].z;
+    } else if (gnorm) {
+        gnorm[r] = sqrt(g[0].x * g[0].x + g[0].y * g[0].y + g[0].z * g[0].z + 1e-12);
     }
-    if (gnorm) gnorm[r] = sqrt(gx * gx + gy * gy + gz * gz + 1e-12);
+}
+
+void ensure_grad_stencil(dsv_mesh* m) {
+    if (m->gsten) return;
+    double* G = dalloc<double>(3 * (size_t)m->nnz);
+    double* W = nullptr;
+    try {
+        W = dalloc<double>((size_t)m->V);
+    } catch (...) {
+        dfree(G);
+        throw;
+    }
+    hipLaunchKernelGGL(k_grad_stencil, dim3(cdiv(m->V, 128)), dim3(128), 0, 0, m->V, m->nnz,
+                       m->conn, m->xyz, m->detJ, m->indptr, m->diag_pos, m->cptr, m->cslot, G, W);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        dfree(G);
+        dfree(W);
+        DSV_HIP(e);
+    }
+    m->gsten = G;
+    m->gsw = W;
 }
 
 // delta_eps(phi) = (1 + cos(pi phi / eps)) / (2 eps) for |phi| < eps, else 0
@@ -127,8 +176,9 @@ int dsv_nodal_gradient(dsv_mesh* m, const double* u, double* grad, double* grad_
         if (!grad && !grad_norm) throw ArgError{"dsv_nodal_gradient: no output"};
         DSV_HIP(hipSetDevice(m->device));
         hipStream_t st = (hipStream_t)stream;
-        hipLaunchKernelGGL(k_nodal_gradient, dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
-                           m->conn, m->xyz, m->detJ, m->diag_pos, m->cptr, m->cslot, u, grad,
+        ensure_grad_stencil(m);
+        hipLaunchKernelGGL(k_nodal_gradient, dim3(cdiv((int64_t)m->V * GS_LANES, 256)), dim3(256),
+                           0, st, m->V, m->nnz, m->indptr, m->indices, m->gsten, m->gsw, u, grad,
                            grad_norm);
         DSV_HIP(hipGetLastError());
     });

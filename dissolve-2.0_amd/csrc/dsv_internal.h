@@ -139,6 +139,8 @@ struct dsv_mesh {
     int32_t* rdesc = nullptr;     // 4 n_rblk  workgroup descriptors of the assembly
     double* fvert = nullptr;      // V  per-vertex values of a per-cell RHS source (scratch)
     double* phys[3] = {};         // V+1 each: redistancing phi0 / ping-pong, scaffold offsets
+    double* gsten = nullptr;      // 3 nnz  nodal-gradient edge stencil G_rj (x, y, z) (lazy)
+    double* gsw = nullptr;        // V  its cell-volume weights W_r (lazy)
     double h_avg = 0.0;           // mean longest cell edge (computed once, dsv_mesh_h_avg)
     // internal (Morton) numbering for the matrix-free path
     int32_t* perm = nullptr;      // internal -> caller
@@ -181,4 +183,5 @@ double time_assembly(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumpe
 void free_partition(Partition& p);
 void free_workspaces(dsv_mesh* m);
 void ensure_geometry_views(dsv_mesh* m);
+void ensure_grad_stencil(dsv_mesh* m);
 }  // namespace dsv

@@ -77,6 +77,50 @@ __device__ __forceinline__ double cell_coef(const double* __restrict__ p, int mo
     return (((p[n0] + p[n1]) + p[n2]) + p[n3]) * 0.25;
 }
 
+// Edge stencil of the volume-weighted nodal gradient.  The cell gradient of a P1 field is
+// sum_j u_j grad N_j^c, and sum_j grad N_j^c = 0, so the average over the cells c holding r
+// with weights w_c = |det J_c| / 6 regroups by graph edge:
+//   grad u(r) = sum_{j != r} G_rj (u_j - u_r) / W_r,
+//   G_rj = sum_{c holding r and j} w_c grad N_j^c,   W_r = sum_{c holding r} w_c.
+// G is stored per CSR entry as three FP64 arrays (x, y, z, each nnz long, 0 on the diagonal)
+// and W per vertex.  GS_LANES lanes share a row, striding over its entries; the quad's DPP sum
+// leaves the row's totals in every lane of the quad.
+constexpr int GS_LANES = 4;
+template <int NF>
+__device__ __forceinline__ void stencil_gradients(int32_t r, int lane, int64_t nnz,
+                                                  const int32_t* __restrict__ indptr,
+                                                  const int32_t* __restrict__ indices,
+                                                  const double* __restrict__ G,
+                                                  const double* __restrict__ W,
+                                                  const double* const (&u)[NF],
+                                                  Vec3<double> (&g)[NF]) {
+    double ur[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        ur[f] = u[f][r];
+        g[f] = Vec3<double>{0.0, 0.0, 0.0};
+    }
+    const int32_t p1 = indptr[r + 1];
+    for (int32_t p = indptr[r] + lane; p < p1; p += GS_LANES) {
+        const int32_t j = indices[p];
+        const double gx = G[p], gy = G[nnz + p], gz = G[2 * nnz + p];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const double d = u[f][j] - ur[f];
+            g[f].x += gx * d;
+            g[f].y += gy * d;
+            g[f].z += gz * d;
+        }
+    }
+    const double w = W[r];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        g[f].x = add_lane_xor2(add_lane_xor1(g[f].x)) / w;
+        g[f].y = add_lane_xor2(add_lane_xor1(g[f].y)) / w;
+        g[f].z = add_lane_xor2(add_lane_xor1(g[f].z)) / w;
+    }
+}
+
 // numpy's maximum / minimum: NaN if either operand is NaN (fmax / fmin drop a NaN operand)
 __device__ __forceinline__ double np_max(double a, double b) {
     return (a != a || b != b) ? a + b : (a >= b ? a : b);

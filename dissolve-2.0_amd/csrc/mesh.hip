@@ -411,6 +411,8 @@ static void destroy_mesh(dsv_mesh* m) {
     dfree(m->detJ);
     dfree(m->Gflat);
     dfree(m->JinvT);
+    dfree(m->gsten);
+    dfree(m->gsw);
     dfree(m->indptr);
     dfree(m->indices);
     dfree(m->diag_pos);

@@ -6,9 +6,9 @@
 //     nodal-O2 branch), the level-set advance (`dissolve.py:684-690`) and Sussman redistancing
 //     (`domain/mesh_utils.py:450-542`) with the mean cell diameter (`:559-575`).
 //
-// Everything is elementwise except the gradients, which gather over each vertex's incident
-// cells in ascending cell order like k_nodal_gradient (coef.hip): no atomics, so every result is
-// bit-reproducible.  The scaffold list is a stream compaction in three fixed-order passes.
+// Everything is elementwise except the gradients, which run over the CSR rows with the nodal
+// gradient's edge stencil (kernels_common.h, built once per mesh by coef.hip): no atomics, so
+// every result is bit-reproducible.  The scaffold list is a stream compaction in three fixed-order passes.
 #include <cmath>
 
 #include "dsv_internal.h"
@@ -117,76 +117,29 @@ __global__ void k_under_relax(int32_t n, const double* __restrict__ u_new,
 // Gradients of up to three P1 fields at vertex r: the volume-weighted average of the incident
 // cells' constant gradients (the same arithmetic as k_nodal_gradient), sharing each cell's
 // geometry between the fields.
-template <int NF>
-__device__ __forceinline__ void vertex_gradients(int32_t r, int32_t V,
-                                                 const int32_t* __restrict__ conn,
-                                                 const double* __restrict__ xyz,
-                                                 const double* __restrict__ detJ,
-                                                 const int32_t* __restrict__ diag_pos,
-                                                 const int32_t* __restrict__ cptr,
-                                                 const int32_t* __restrict__ cslot,
-                                                 const double* const (&u)[NF],
-                                                 Vec3<double> (&g)[NF]) {
-    const double* X = xyz;
-    const double* Y = xyz + V;
-    const double* Z = xyz + 2 * (size_t)V;
-    double ws = 0.0;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) g[f] = Vec3<double>{0.0, 0.0, 0.0};
-    const int32_t p = diag_pos[r];
-    for (int32_t k = cptr[p]; k < cptr[p + 1]; ++k) {
-        const int32_t c = cslot[k] >> 4;
-        const int32_t n0 = conn[4 * (size_t)c], n1 = conn[4 * (size_t)c + 1];
-        const int32_t n2 = conn[4 * (size_t)c + 2], n3 = conn[4 * (size_t)c + 3];
-        const Vec3<double> x0{X[n0], Y[n0], Z[n0]};
-        const Vec3<double> e1{X[n1] - x0.x, Y[n1] - x0.y, Z[n1] - x0.z};
-        const Vec3<double> e2{X[n2] - x0.x, Y[n2] - x0.y, Z[n2] - x0.z};
-        const Vec3<double> e3{X[n3] - x0.x, Y[n3] - x0.y, Z[n3] - x0.z};
-        const Vec3<double> c1 = cross(e2, e3), c2 = cross(e3, e1), c3 = cross(e1, e2);
-        const double det = dot(e1, c1);
-        const double w = detJ[c] / 6.0;
-        const double s = w / det;
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const double u0 = u[f][n0];
-            const double d1 = u[f][n1] - u0, d2 = u[f][n2] - u0, d3 = u[f][n3] - u0;
-            g[f].x += s * (d1 * c1.x + d2 * c2.x + d3 * c3.x);
-            g[f].y += s * (d1 * c1.y + d2 * c2.y + d3 * c3.y);
-            g[f].z += s * (d1 * c1.z + d2 * c2.z + d3 * c3.z);
-        }
-        ws += w;
-    }
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        g[f].x /= ws;
-        g[f].y /= ws;
-        g[f].z /= ws;
-    }
-}
-
 // v_n = min(max(v_Zn, v_ORR, v_O2), 0) inside |phi| < 3 h, 0 outside, with
 // n = grad phi / sqrt(|grad phi|^2 + 1e-12), v_Zn = -max(DeZn, 0) |grad Zn . n| / (Zn_sol - Zn_sat),
 // v_O2 = -2 max(DeO2, 0) |grad O2 . n| / Zn_sol, v_ORR = -(2 / Zn_sol) k_orr max(O2, 0); a term
 // whose fields are NULL (Zn and DeZn, or DeO2) is left out of the max
-__global__ void k_interface_velocity(int32_t V, const int32_t* __restrict__ conn,
-                                     const double* __restrict__ xyz, const double* __restrict__ detJ,
-                                     const int32_t* __restrict__ diag_pos,
-                                     const int32_t* __restrict__ cptr,
-                                     const int32_t* __restrict__ cslot, const double* phi,
-                                     const double* Zn, const double* O2,
-                                     const double* __restrict__ DeZn,
-                                     const double* __restrict__ DeO2, dsv_velocity_params p,
-                                     double denom_zn, double* __restrict__ vel) {
-    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_interface_velocity(
+    int32_t V, int64_t nnz, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const double* __restrict__ G, const double* __restrict__ W, const double* phi,
+    const double* Zn, const double* O2, const double* __restrict__ DeZn,
+    const double* __restrict__ DeO2, dsv_velocity_params p, double denom_zn,
+    double* __restrict__ vel) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r = (int32_t)(t / GS_LANES);
+    const int lane = (int)(t % GS_LANES);
     if (r >= V) return;
     const double ph = phi[r];
-    if (!(fabs(ph) < 3.0 * p.h)) {
-        vel[r] = 0.0;
+    if (!(fabs(ph) < 3.0 * p.h)) {  // the row's four lanes leave together
+        if (lane == 0) vel[r] = 0.0;
         return;
     }
     const double* const u[3] = {phi, Zn ? Zn : phi, O2};  // (an omitted Zn term reads phi)
     Vec3<double> g[3];
-    vertex_gradients<3>(r, V, conn, xyz, detJ, diag_pos, cptr, cslot, u, g);
+    stencil_gradients<3>(r, lane, nnz, indptr, indices, G, W, u, g);
+    if (lane != 0) return;
     const double gmag = sqrt(g[0].x * g[0].x + g[0].y * g[0].y + g[0].z * g[0].z + 1e-12);
     const double nx = g[0].x / gmag, ny = g[0].y / gmag, nz = g[0].z / gmag;
     const double dzn = g[1].x * nx + g[1].y * ny + g[1].z * nz;
@@ -209,23 +162,23 @@ __global__ void k_levelset_advance(int32_t V, const double* __restrict__ phi_old
 // One Sussman pseudo-time iteration, Jacobi-style (every vertex from the same phi_in):
 // outside the frozen band |phi0| < 3 h, phi_out = phi_in + dtau sign (1 - |grad phi_in|) with
 // dtau = 0.5 h / max(|grad phi_in|, 0.5) and sign = phi0 / sqrt(phi0^2 + (1.5 h)^2)
-__global__ void k_redistance_iter(int32_t V, const int32_t* __restrict__ conn,
-                                  const double* __restrict__ xyz, const double* __restrict__ detJ,
-                                  const int32_t* __restrict__ diag_pos,
-                                  const int32_t* __restrict__ cptr,
-                                  const int32_t* __restrict__ cslot, const double* phi_in,
-                                  const double* __restrict__ phi0, double h,
-                                  double* __restrict__ phi_out) {
-    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_redistance_iter(
+    int32_t V, int64_t nnz, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const double* __restrict__ G, const double* __restrict__ W, const double* phi_in,
+    const double* __restrict__ phi0, double h, double* __restrict__ phi_out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t r = (int32_t)(t / GS_LANES);
+    const int lane = (int)(t % GS_LANES);
     if (r >= V) return;
     const double p0 = phi0[r];
-    if (fabs(p0) < 3.0 * h) {
-        phi_out[r] = phi_in[r];
+    if (fabs(p0) < 3.0 * h) {  // the row's four lanes leave together
+        if (lane == 0) phi_out[r] = phi_in[r];
         return;
     }
     const double* const u[1] = {phi_in};
     Vec3<double> g[1];
-    vertex_gradients<1>(r, V, conn, xyz, detJ, diag_pos, cptr, cslot, u, g);
+    stencil_gradients<1>(r, lane, nnz, indptr, indices, G, W, u, g);
+    if (lane != 0) return;
     const double gm = sqrt(g[0].x * g[0].x + g[0].y * g[0].y + g[0].z * g[0].z + 1e-12);
     const double eps_sign = 1.5 * h;
     const double sgn = p0 / sqrt(p0 * p0 + eps_sign * eps_sign);
@@ -457,9 +410,10 @@ int dsv_interface_velocity(dsv_mesh* m, const double* phi, const double* Zn, con
         double denom = params->zn_solid - params->zn_sat;
         if (fabs(denom) < 1e-30) denom = params->zn_solid;
         DSV_HIP(hipSetDevice(m->device));
-        hipLaunchKernelGGL(k_interface_velocity, dim3(cdiv(m->V, 128)), dim3(128), 0,
-                           (hipStream_t)stream, m->V, m->conn, m->xyz, m->detJ, m->diag_pos,
-                           m->cptr, m->cslot, phi, Zn, O2, DeZn, DeO2, *params, denom, v);
+        ensure_grad_stencil(m);
+        hipLaunchKernelGGL(k_interface_velocity, dim3(cdiv((int64_t)m->V * GS_LANES, 256)),
+                           dim3(256), 0, (hipStream_t)stream, m->V, m->nnz, m->indptr, m->indices,
+                           m->gsten, m->gsw, phi, Zn, O2, DeZn, DeO2, *params, denom, v);
         DSV_HIP(hipGetLastError());
     });
 }
@@ -531,6 +485,7 @@ int dsv_redistance(dsv_mesh* m, double* phi, int32_t num_iters, double h, void* 
             h = ha;
         }
         if (num_iters == 0) return;
+        ensure_grad_stencil(m);
         hipStream_t st = (hipStream_t)stream;
         double* phi0 = physics_scratch(m, 0);
         double* tmp = physics_scratch(m, 1);
@@ -540,9 +495,9 @@ int dsv_redistance(dsv_mesh* m, double* phi, int32_t num_iters, double h, void* 
         for (int it = 0; it < num_iters; ++it) {
             // ping-pong so that the last iteration writes phi
             double* out = ((num_iters - 1 - it) % 2 == 0) ? phi : tmp;
-            hipLaunchKernelGGL(k_redistance_iter, dim3(cdiv(m->V, 128)), dim3(128), 0, st, m->V,
-                               m->conn, m->xyz, m->detJ, m->diag_pos, m->cptr, m->cslot, in, phi0,
-                               h, out);
+            hipLaunchKernelGGL(k_redistance_iter, dim3(cdiv((int64_t)m->V * GS_LANES, 256)),
+                               dim3(256), 0, st, m->V, m->nnz, m->indptr, m->indices, m->gsten,
+                               m->gsw, in, phi0, h, out);
             in = out;
         }
         DSV_HIP(hipGetLastError());

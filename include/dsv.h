@@ -266,8 +266,10 @@ int dsv_time_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
 
 /* Nodal gradient of a P1 field u (device, V): per vertex, the average of the constant gradients
  * of its incident cells weighted by cell volume |det J| / 6 -- GPUSparseAssembler.nodal_gradient
- * (`gpu/gpu_assembler.py:251-307,646-671`), gathered per vertex in ascending cell order instead
- * of scattered with atomics.  grad (V x 3, interleaved gx, gy, gz) and grad_norm
+ * (`gpu/gpu_assembler.py:251-307,646-671`).  Instead of scattering with atomics, the mesh keeps
+ * an edge stencil (three FP64 weights per CSR entry and one per vertex, 24 nnz + 8 V bytes,
+ * built on the first gradient call of any kind) and every call is one pass over the CSR rows,
+ * bit-reproducible.  grad (V x 3, interleaved gx, gy, gz) and grad_norm
  * (sqrt(gx^2 + gy^2 + gz^2 + 1e-12), compute_norm_grad_phi, `physics/interface_velocity.py:65-87`)
  * may each be NULL, not both. */
 int dsv_nodal_gradient(dsv_mesh* m, const double* u, double* grad, double* grad_norm,
