@@ -464,6 +464,9 @@ def main():
                          "solution is within 1e-8 of the FP64 solution")
     ap.add_argument("--no-coupled", action="store_true",
                     help="skip the reference-order four-species step legs")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="instead of the headline: only the reference-order four-species step "
+                         "legs (static, then with the interface update), one JSON line")
     ap.add_argument("--no-errors", action="store_true",
                     help="skip the untimed error replays against the FP64 solutions")
     ap.add_argument("--error-every", type=int, default=1,
@@ -505,6 +508,11 @@ def main():
     gc.freeze()
     if a.series:
         run_series(mesh, w, history_dirs=a.history_dirs)
+        return
+    if a.coupled_only:
+        print(json.dumps({"static": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"]),
+                          "with_interface": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"],
+                                                        levelset=True)}), flush=True)
         return
     tuned = None
     if a.partition:
