@@ -56,6 +56,12 @@ class Stepper:
         self.u_hist = [[u.clone() for u in self.u] for _ in range(self.MAX_HIST)]
         self.b = [torch.zeros_like(u) for u in self.u]
         self.x = [torch.zeros_like(u) for u in self.u]
+        # device addresses of the rotating buffers, kept beside them so that a step builds its
+        # pointer arrays without asking torch again
+        self._p_u = [t.data_ptr() for t in self.u]
+        self._p_hist = [[t.data_ptr() for t in h] for h in self.u_hist]
+        self._p_x = [t.data_ptr() for t in self.x]
+        self._p_b = (ctypes.c_void_p * self.S)(*[t.data_ptr() for t in self.b])
         self.f = [to_device(s["f"], dev, torch.float64).clone() if s.get("f") is not None else None
                   for s in species]
         self.bc = [to_device(s["bc_dofs"], dev, torch.int32) if s.get("bc_dofs") is not None
@@ -84,8 +90,23 @@ class Stepper:
         self.collect = False
         self.pairs = []
         self.tau = 1.0
-        self.warm_log = []
+        self._warm_raw = []   # per Galerkin step: (step, mode, nd, coef, r0, r1) ctypes arrays
+        self._warm_log = []
         self.n_steps = 0
+
+    @property
+    def warm_log(self):
+        """Per Galerkin-started step: {"step", "mode", "coef" [direction][species],
+        "r_previous" (residual norm of u_n), "r_start" (of the start taken), "taken"}; built from
+        the library's outputs when read."""
+        S = self.S
+        for n, mode, nd, coef, r0, r1 in self._warm_raw[len(self._warm_log):]:
+            self._warm_log.append({
+                "step": n, "mode": mode,
+                "coef": [[coef[k * S + s] for s in range(S)] for k in range(nd)],
+                "r_previous": list(r0), "r_start": list(r1),
+                "taken": [r1[k] < r0[k] for k in range(S)]})
+        return self._warm_log
 
     @property
     def u_prev(self):
@@ -164,10 +185,9 @@ class Stepper:
     def rhs(self, stream=None):
         """b_s = M (u_s / dt + f_s), b_s[bc] = bc value, all species in one call."""
         P = ctypes.c_void_p * self.S
-        _lib.call("dsv_rhs_batch", self.mesh.handle, self.S, self._arr,
-                  P(*[u.data_ptr() for u in self.u]),
+        _lib.call("dsv_rhs_batch", self.mesh.handle, self.S, self._arr, P(*self._p_u),
                   P(*[f.data_ptr() if f is not None else None for f in self.f]),
-                  1.0 / self.dt, self.lumped, P(*[b.data_ptr() for b in self.b]),
+                  1.0 / self.dt, self.lumped, self._p_b,
                   stream if stream is not None else stream_ptr(self.mesh.device))
 
     def _features(self):
@@ -244,11 +264,11 @@ class Stepper:
             extra = list(self._mlp_directions(X_all, sig_all)) if learned else []
             nd = nh + (1 if learned else 0)
             if self.start_from_history:  # u_n, u_{n-1}, ..., u_{n-nh}
-                steps, fn = [self.u] + self.u_hist[:nh], "dsv_solve_warm_history"
+                steps, fn = [self._p_u] + self._p_hist[:nh], "dsv_solve_warm_history"
             else:  # u_{n-1}, ..., u_{n-nh}; the start u_n was copied into x
-                steps, fn = self.u_hist[:nh], "dsv_solve_warm"
+                steps, fn = self._p_hist[:nh], "dsv_solve_warm"
             hist = (ctypes.c_void_p * max(1, len(steps) * self.S))(
-                *[a[k].data_ptr() for a in steps for k in range(self.S)])
+                *[p for a in steps for p in a])
             ext = (ctypes.c_void_p * max(1, len(extra)))(*[e.data_ptr() for e in extra])
             coef = (ctypes.c_double * (nd * self.S))()
             r0 = (ctypes.c_double * self.S)()
@@ -256,11 +276,7 @@ class Stepper:
             _lib.call(fn, self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
                       self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it, nh, hist,
                       1 if learned else 0, ext, stream, st, coef, r0, r1)
-            self.warm_log.append({
-                "step": self.n_steps, "mode": self.warm,
-                "coef": [[coef[k * self.S + s] for s in range(self.S)] for k in range(nd)],
-                "r_previous": list(r0), "r_start": list(r1),
-                "taken": [r1[k] < r0[k] for k in range(self.S)]})
+            self._warm_raw.append((self.n_steps, self.warm, nd, coef, r0, r1))
         else:
             _lib.call("dsv_solve", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
                       self.lumped, _lib.METHODS[self.method], self.rtol, self.max_it,
@@ -275,11 +291,13 @@ class Stepper:
                 self.pairs.append((X, y, sigma))
         # rotate the buffers instead of copying them: the solution becomes u, u becomes
         # u_{n-1}, and the oldest history buffer holds the next step's start
-        oldest = self.u_hist[-1]
+        oldest, p_oldest = self.u_hist[-1], self._p_hist[-1]
         self.u_hist = [self.u] + self.u_hist[:-1]
+        self._p_hist = [self._p_u] + self._p_hist[:-1]
         self.u, self.x = self.x, oldest
+        self._p_u, self._p_x = self._p_x, p_oldest
         for k in range(self.S):
-            self._arr[k].x = ptr(self.x[k]).value
+            self._arr[k].x = self._p_x[k]
         self.n_steps += 1
         d = st.as_dict(self.S)
         d["step_s"] = time.perf_counter() - t0
