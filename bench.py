@@ -506,14 +506,6 @@ def main():
     # stepping's own objects (a full collection of the whole process pauses 20-50 ms)
     gc.collect()
     gc.freeze()
-    if a.series:
-        run_series(mesh, w, history_dirs=a.history_dirs)
-        return
-    if a.coupled_only:
-        print(json.dumps({"static": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"]),
-                          "with_interface": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"],
-                                                        levelset=True)}), flush=True)
-        return
     tuned = None
     if a.partition:
         tuned = {"vertices_per_block": nv, "threads": th, "lanes_per_vertex": lanes}
@@ -525,6 +517,14 @@ def main():
                 f"{row[4] * 1e3:.1f} us")
         tuned = {"vertices_per_block": best[0], "threads": best[1], "lanes_per_vertex": best[2]}
         log(f"tuned: {tuned}")
+    if a.series:
+        run_series(mesh, w, history_dirs=a.history_dirs)
+        return
+    if a.coupled_only:
+        print(json.dumps({"static": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"]),
+                          "with_interface": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"],
+                                                        levelset=True)}), flush=True)
+        return
     res, finals = {}, {}
     paths = a.paths.split(",")
     hd = a.history_dirs

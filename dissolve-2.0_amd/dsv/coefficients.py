@@ -73,15 +73,24 @@ def compute_norm_grad_phi(phi, normgradphi, gpu_asm=None):
 
 
 def transport_coefficients(mesh, phi, gnorm, eps, k_orr=0.0, kd=0.0, F_film=None, Cl=None,
-                           O2=None, Fmax=1.0, film_floor=1.0, outputs=("alpha_o2", "f_oh")):
+                           O2=None, Fmax=1.0, film_floor=1.0, outputs=("alpha_o2", "f_oh"),
+                           out=None):
     """Dict of the requested per-vertex fields (device tensors): alpha_o2, f_oh, film_factor,
-    medium_factor (libdsv dsv_transport_coefficients)."""
+    medium_factor (libdsv dsv_transport_coefficients); `out` may map some of the requested
+    names to contiguous FP64 device tensors of length n_dofs to fill instead."""
     import torch
     names = ("alpha_o2", "f_oh", "film_factor", "medium_factor")
     bad = [o for o in outputs if o not in names]
     if bad:
         raise ValueError(f"unknown outputs {bad}")
-    out = {o: torch.empty(mesh.n_dofs, dtype=torch.float64, device=mesh.device) for o in outputs}
+    given = dict(out or {})
+    for o, t in given.items():
+        if o not in outputs or not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or \
+                t.numel() != mesh.n_dofs or not t.is_contiguous() or t.device != mesh.device:
+            raise ValueError(f"out[{o!r}] must be a requested output held in a contiguous FP64 "
+                             "device tensor of length n_dofs")
+    out = {o: given[o] if o in given else
+           torch.empty(mesh.n_dofs, dtype=torch.float64, device=mesh.device) for o in outputs}
     p = _lib.CoefParams(float(eps), float(k_orr), float(kd), float(Fmax), float(film_floor))
     keep = [_dev(mesh, a) for a in (phi, gnorm, F_film, Cl, O2)]
     _lib.call("dsv_transport_coefficients", mesh.handle, *[ptr(t) for t in keep], ctypes.byref(p),

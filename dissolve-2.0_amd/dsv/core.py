@@ -33,6 +33,8 @@ def to_device(arr, device, dtype):
     """numpy / torch (any device) -> contiguous torch tensor on `device` with `dtype`."""
     torch = _torch()
     if isinstance(arr, torch.Tensor):
+        if arr.dtype == dtype and arr.device == device and arr.is_contiguous():
+            return arr
         t = arr.to(device=device, dtype=dtype)
     else:
         t = torch.as_tensor(np.asarray(arr), device=device, dtype=dtype)

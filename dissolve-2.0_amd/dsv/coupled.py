@@ -43,6 +43,7 @@ class _History:
         self.S = len(fields)
         self.ring = [[f.clone() for f in fields]] + \
             [[torch.empty_like(f) for f in fields] for _ in range(depth)]
+        self.ptrs = [[f.data_ptr() for f in r] for r in self.ring]  # rotates with the ring
         self.out = [torch.empty_like(f) for f in fields]
         self.n = 1  # valid ring entries
 
@@ -54,6 +55,7 @@ class _History:
         """The oldest ring entry becomes the newest; returns it (to be filled by the caller)."""
         oldest = self.ring.pop()
         self.ring.insert(0, oldest)
+        self.ptrs.insert(0, self.ptrs.pop())
         self.n = min(self.n + 1, len(self.ring))
         return oldest
 
@@ -95,6 +97,23 @@ class CoupledStepper:
         self.scaffold = None
         self.n_steps = 0
         self.history = []
+        # the two solves' species records, built once: every array they name keeps its address
+        # (the scaffold list is re-pointed when the interface moves)
+        p = self.p
+        self._keep = []
+        self._arr1 = species_array([
+            species_struct(mesh, D=self.De[0], alpha=self.alpha_zn, b=self.b[0],
+                           x=self.g1.out[0], bc_value=p["Znbc"], keep=self._keep),
+            species_struct(mesh, D=self.De[1], b=self.b[1], x=self.g1.out[1], keep=self._keep),
+            species_struct(mesh, D=self.De[3], alpha=self.alpha_o2, bc_dofs=self.o2_bc,
+                           bc_value=p["o2_bc_value"], b=self.b[2], x=self.g1.out[2],
+                           keep=self._keep)])
+        self._arr2 = species_array([
+            species_struct(mesh, D=self.De[2], b=self.b[3], x=self.g2.out[0], keep=self._keep)])
+        P3, P1 = ctypes.c_void_p * 3, ctypes.c_void_p * 1
+        self._rhs1 = (P3(*self.g1.ptrs[0]), P3(self.src_zn.data_ptr(), None, None),
+                      P3(*[b.data_ptr() for b in self.b[:3]]))
+        self._rhs2 = (P1(self.f_oh.data_ptr()), P1(self.b[3].data_ptr()))
         self._interface_changed()
 
     # ---- fields by name
@@ -105,12 +124,14 @@ class CoupledStepper:
     def _interface_changed(self):
         """Quantities that depend on phi alone: the ORR sink and the scaffold list."""
         p = self.p
-        c = transport_coefficients(self.mesh, self.phi, self.grad_phi, p["eps"], p["k_orr"],
-                                   outputs=("alpha_o2",))
-        self.alpha_o2.copy_(c["alpha_o2"])
+        transport_coefficients(self.mesh, self.phi, self.grad_phi, p["eps"], p["k_orr"],
+                               outputs=("alpha_o2",), out={"alpha_o2": self.alpha_o2})
         _, _, self.scaffold = zinc_coefficients(self.mesh, self.phi, None, None, p["kf"], p["kd"],
                                                 p["Fmax"], scaffold=True,
                                                 out=(self.alpha_zn, self.src_zn))
+        n = int(self.scaffold.numel())
+        self._arr1[0].bc_dofs = self.scaffold.data_ptr() if n > 0 else None
+        self._arr1[0].n_bc = n
 
     def _effective_diffusion(self):
         p = self.p
@@ -129,15 +150,12 @@ class CoupledStepper:
         grad_norm(self.mesh, self.phi, out=self.grad_phi)
         self._interface_changed()
 
-    def _solve(self, grp, structs, stream):
+    def _solve(self, grp, arr, stream):
         S = grp.S
-        arr = species_array(structs)
         st = _lib.Stats()
         nh = grp.n - 1 if self.warm == "extrapolate" else 0
         if nh > 0:
-            steps = grp.ring[:nh + 1]
-            hist = (ctypes.c_void_p * (len(steps) * S))(
-                *[ptr(a[k]).value for a in steps for k in range(S)])
+            hist = (ctypes.c_void_p * ((nh + 1) * S))(*[q for a in grp.ptrs[:nh + 1] for q in a])
             _lib.call("dsv_solve_warm_history", self.mesh.handle, S, arr, 1.0 / self.dt, 1,
                       _lib.METHODS[self.method], self.rtol, self.max_it, nh, hist, 0, None,
                       stream, st, None, None, None)
@@ -161,19 +179,12 @@ class CoupledStepper:
         # Zn coefficients from the previous film and Cl
         zinc_coefficients(mesh, self.phi, self.F, cl, p["kf"], p["kd"], p["Fmax"], scaffold=False,
                           out=(self.alpha_zn, self.src_zn))
-        keep = []
-        s1 = [species_struct(mesh, D=self.De[0], alpha=self.alpha_zn, bc_dofs=self.scaffold,
-                             bc_value=p["Znbc"], b=self.b[0], x=self.g1.out[0], keep=keep),
-              species_struct(mesh, D=self.De[1], b=self.b[1], x=self.g1.out[1], keep=keep),
-              species_struct(mesh, D=self.De[3], alpha=self.alpha_o2, bc_dofs=self.o2_bc,
-                             bc_value=p["o2_bc_value"], b=self.b[2], x=self.g1.out[2], keep=keep)]
-        P = ctypes.c_void_p * 3
-        arr1 = species_array(s1)
-        _lib.call("dsv_rhs_batch", mesh.handle, 3, arr1, P(zn.data_ptr(), cl.data_ptr(),
-                                                          o2.data_ptr()),
-                  P(self.src_zn.data_ptr(), None, None), 1.0 / self.dt, 1,
-                  P(*[b.data_ptr() for b in self.b[:3]]), stream)
-        r1 = self._solve(self.g1, s1, stream)
+        rhs_u, rhs_f, rhs_b = self._rhs1
+        for k in range(3):
+            rhs_u[k] = self.g1.ptrs[0][k]
+        _lib.call("dsv_rhs_batch", mesh.handle, 3, self._arr1, rhs_u, rhs_f, 1.0 / self.dt, 1,
+                  rhs_b, stream)
+        r1 = self._solve(self.g1, self._arr1, stream)
         # new Zn, Cl; O2 under-relaxed against its value at the start of the step
         new = self.g1.push()
         old = self.g1.ring[1]
@@ -184,16 +195,13 @@ class CoupledStepper:
         film_update(mesh, self.phi, new[0], new[1], self.F, p["kf"], p["kd"], p["Fmax"], self.dt,
                     out=self.F)
         # OH source from the new film, Cl and O2
-        c = transport_coefficients(mesh, self.phi, self.grad_phi, p["eps"], p["k_orr"], p["kd"],
-                                   F_film=self.F, Cl=new[1], O2=new[2], Fmax=p["Fmax"],
-                                   outputs=("f_oh",))
-        self.f_oh.copy_(c["f_oh"])
-        oh = self.g2.current[0]
-        s2 = [species_struct(mesh, D=self.De[2], b=self.b[3], x=self.g2.out[0], keep=keep)]
+        transport_coefficients(mesh, self.phi, self.grad_phi, p["eps"], p["k_orr"], p["kd"],
+                               F_film=self.F, Cl=new[1], O2=new[2], Fmax=p["Fmax"],
+                               outputs=("f_oh",), out={"f_oh": self.f_oh})
         P1 = ctypes.c_void_p * 1
-        _lib.call("dsv_rhs_batch", mesh.handle, 1, species_array(s2), P1(oh.data_ptr()),
-                  P1(self.f_oh.data_ptr()), 1.0 / self.dt, 1, P1(self.b[3].data_ptr()), stream)
-        r2 = self._solve(self.g2, s2, stream)
+        _lib.call("dsv_rhs_batch", mesh.handle, 1, self._arr2, P1(self.g2.ptrs[0][0]),
+                  self._rhs2[0], 1.0 / self.dt, 1, self._rhs2[1], stream)
+        r2 = self._solve(self.g2, self._arr2, stream)
         self.g2.push()[0].copy_(self.g2.out[0])
         self.n_steps += 1
         d = {"iterations": r1["iterations"] + r2["iterations"],

@@ -65,6 +65,17 @@ def test_transport_coefficients_match_oracle(gpu):
     ref = OC.transport_coefficients(phi, gn, F, Cl, O2, **prm)
     for k, r in zip(names, ref):
         assert np.abs(got[k].cpu().numpy() - r).max() <= 1e-13 * max(np.abs(r).max(), 1e-30), k
+    # the same into caller tensors (out=); a tensor of the wrong length is refused
+    import torch
+    buf = {"f_oh": torch.full((len(xyz),), np.nan, dtype=torch.float64, device=m.device)}
+    got1 = transport_coefficients(m, phi, gn, prm["eps"], prm["k_orr"], prm["kd"], F, Cl, O2,
+                                  prm["Fmax"], prm["film_floor"], outputs=names, out=buf)
+    assert got1["f_oh"] is buf["f_oh"]
+    for k in names:
+        assert torch.equal(got1[k], got[k]), k
+    with pytest.raises(ValueError):
+        transport_coefficients(m, phi, gn, prm["eps"], outputs=("f_oh",),
+                               out={"f_oh": buf["f_oh"][:-1]})
     # absent film / chloride / oxygen are zero
     got0 = transport_coefficients(m, phi, gn, prm["eps"], prm["k_orr"], prm["kd"], outputs=names)
     ref0 = OC.transport_coefficients(phi, gn, None, None, None, **prm)
