@@ -87,10 +87,12 @@ def load_workload(name):
 
 
 def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, history_dirs=8,
-             after_step=None):
+             after_step=None, low_modes=None):
     """W warm-up steps (the learned path trains on them), then K timed steps.  after_step(st),
     if given, runs after every timed step inside the timed region (only the untimed error
-    replays pass one)."""
+    replays pass one).  low_modes = (n_modes, degree, passes): the operator's low-mode subspace
+    is built once before the warm-up (Stepper.build_low_modes; set-up, not timed) and corrects
+    every Galerkin start."""
     import torch
     from dsv.transport import Stepper
     mlp = None
@@ -99,6 +101,14 @@ def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, h
         mlp = WarmStartMLP(mesh.device)
     st = Stepper(mesh, w["species"], w["dt"], method=method, rtol=rtol, phi=w.get("phi"),
                  grad_phi=w.get("grad_norm"), eps=w.get("eps"), mlp=mlp)
+    lm = None
+    if low_modes and low_modes[0] > 0 and warm != "previous":
+        n, deg, passes = low_modes
+        info = st.build_low_modes(n, deg, passes)
+        lm = {"n_modes": n, "degree": deg, "passes": passes,
+              "setup_seconds": round(info["seconds"], 3),
+              "lowest_ritz": [round(r[0], 6) for r in info["ritz"]],
+              "largest_ritz": [round(r[-1], 6) for r in info["ritz"]]}
     train = None
     if warm == "learned":
         st.set_warm_start("extrapolate", history_dirs=min(history_dirs, st.MAX_HIST - 1))
@@ -134,13 +144,14 @@ def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, h
         "outer_per_step": [h["outer"] for h in its],
         "rel_residual_last": its[-1]["rel_residual"],
         "train": train,
+        "low_modes": lm,
         "taken_per_step": [e["taken"] for e in log_tail],
         "coef_last_step": log_tail[-1]["coef"] if log_tail else None,
     }, st
 
 
 def error_replay(mesh, w, method, steps, warmup, rtol, warm, history_dirs, final_u, every=1,
-                 epochs=30):
+                 epochs=30, low_modes=None):
     """Replays a path untimed and solves each checked step's three systems again with the plain
     FP64 path to rtol 1e-13 (Stepper.fp64_reference, started from the step's solution).  The
     solver is deterministic (no atomics, fixed reduction orders), so the replay retraces the
@@ -159,7 +170,7 @@ def error_replay(mesh, w, method, steps, warmup, rtol, warm, history_dirs, final
             errs.append([float(f"{e:.3e}") for e in st.step_error(refs)])
 
     r, st = run_path(mesh, w, method, steps, warmup, rtol, warm=warm, epochs=epochs,
-                     history_dirs=history_dirs, after_step=check)
+                     history_dirs=history_dirs, after_step=check, low_modes=low_modes)
     same = all(torch.equal(a, b) for a, b in zip(st.u, final_u))
     S = len(st.u)
     return {"per_step": errs, "steps_checked": len(errs),
@@ -450,14 +461,22 @@ def main():
     ap.add_argument("--no-scale", action="store_true",
                     help="skip the kernel timings on the 14.7M-tet gyroid lattice")
     ap.add_argument("--apply-reps", type=int, default=50)
-    ap.add_argument("--history-dirs", type=int, default=8,
-                    help="past solution increments in the Galerkin warm start (1..8)")
+    ap.add_argument("--history-dirs", type=int, default=None,
+                    help="past solution increments in the Galerkin warm start (1..8); default 6 "
+                         "with the low-mode correction, 8 without it (the series, the leg "
+                         "without it), each its best")
+    ap.add_argument("--low-modes", type=int, default=64,
+                    help="low-mode subspace per species of the Galerkin paths (0: none; a "
+                         "multiple of 4, at most 64)")
+    ap.add_argument("--low-mode-degree", type=int, default=80)
+    ap.add_argument("--low-mode-passes", type=int, default=4)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--series", action="store_true",
                     help="instead of the headline: the receding-interface time series on the "
                          "disc (200 steps; the MLP trains on steps 1-100), one JSON line")
     ap.add_argument("--paths",
-                    default="plain,fp32,chosen,learned,extrapolate_h7,plain_bar,chosen_bar",
+                    default="plain,fp32,chosen,chosen_no_low_modes,learned,extrapolate_no_mlp,"
+                            "plain_bar,chosen_bar",
                     help="comma list of the paths to run (the JSON line needs the first four)")
     ap.add_argument("--bar-rtol", type=float, default=5e-11,
                     help="rtol of the *_bar paths: the tolerance at which every species' step "
@@ -518,7 +537,7 @@ def main():
         tuned = {"vertices_per_block": best[0], "threads": best[1], "lanes_per_vertex": best[2]}
         log(f"tuned: {tuned}")
     if a.series:
-        run_series(mesh, w, history_dirs=a.history_dirs)
+        run_series(mesh, w, history_dirs=a.history_dirs or 8)
         return
     if a.coupled_only:
         print(json.dumps({"static": run_coupled(mesh, w, a.steps, a.warmup, w["rtol"]),
@@ -527,21 +546,27 @@ def main():
         return
     res, finals = {}, {}
     paths = a.paths.split(",")
-    hd = a.history_dirs
+    hd = a.history_dirs or (6 if a.low_modes > 0 else 8)
     PATHS = (("plain", "csr64", "previous", rtol), ("fp32", "mf32_ir64", "previous", rtol),
              ("chosen", "mf32_ir64", "extrapolate", rtol),
+             # the same Galerkin start without the low-mode correction
+             ("chosen_no_low_modes", "mf32_ir64", "extrapolate", rtol),
              ("learned", "mf32_ir64", "learned", rtol),
              # the learned path's increments without its MLP direction: the network's own gain
-             ("extrapolate_h7", "mf32_ir64", "extrapolate", rtol),
+             ("extrapolate_no_mlp", "mf32_ir64", "extrapolate", rtol),
              # the tolerance at which every species' step solution meets 1e-8 (see error legs)
              ("plain_bar", "csr64", "previous", a.bar_rtol),
              ("chosen_bar", "mf32_ir64", "extrapolate", a.bar_rtol))
     for name, method, warm, rt in PATHS:
         if name not in paths:
             continue
-        h = min(hd, 7) if name in ("learned", "extrapolate_h7") else hd
-        r, st = run_path(mesh, w, method, a.steps, a.warmup, rt, warm=warm, history_dirs=h)
-        r.update(method=method, warm=warm, rtol=rt, history_dirs=h)
+        h = min(hd, 7) if name in ("learned", "extrapolate_no_mlp") else \
+            ((a.history_dirs or 8) if name == "chosen_no_low_modes" else hd)
+        lm = None if name == "chosen_no_low_modes" else \
+            (a.low_modes, a.low_mode_degree, a.low_mode_passes)
+        r, st = run_path(mesh, w, method, a.steps, a.warmup, rt, warm=warm, history_dirs=h,
+                         low_modes=lm)
+        r.update(method=method, warm=warm, rtol=rt, history_dirs=h, low_modes_arg=lm)
         res[name] = r
         finals[name] = [u.clone() for u in st.u]
         log(f"{name} ({method}, {warm} start, rtol {rt:g}): {r['steps_per_s']:.2f} steps/s, "
@@ -561,7 +586,8 @@ def main():
         r = res[name]
         t = time.perf_counter()
         errors[name] = error_replay(mesh, w, r["method"], a.steps, a.warmup, r["rtol"], r["warm"],
-                                    r["history_dirs"], finals[name], every=a.error_every)
+                                    r["history_dirs"], finals[name], every=a.error_every,
+                                    low_modes=r["low_modes_arg"])
         errors[name]["rtol"] = r["rtol"]
         log(f"error replay {name}: max per species {errors[name]['max_per_species']}, "
             f"bit-identical replay {errors[name]['replay_bit_identical']} "
@@ -618,19 +644,33 @@ def main():
                "rtol": r["rtol"]}
         if name in errors:
             out["max_error_vs_fp64"] = errors[name]["max_per_species"]
+        if r.get("low_modes"):
+            out["low_modes"] = r["low_modes"]
         return out
 
     mlp = None
-    if "extrapolate_h7" in res:
-        e7 = res["extrapolate_h7"]
-        mlp = {"compared": "learned (7 increments + the MLP direction) vs the same 7 increments "
-                           "alone, each on its own trajectory",
+    if "extrapolate_no_mlp" in res:
+        e7 = res["extrapolate_no_mlp"]
+        nh = min(hd, 7)
+        mlp = {"compared": f"learned ({nh} increments + the MLP direction) vs the same {nh} "
+                           "increments alone, each on its own trajectory (both with the "
+                           "low-mode correction)",
                "steps_per_s_with_mlp": round(learned["steps_per_s"], 3),
                "steps_per_s_without": round(e7["steps_per_s"], 3),
                "cg_iterations_per_step_with_mlp": its(learned),
                "cg_iterations_per_step_without": its(e7),
                "cg_iterations_saved_by_mlp_pct": round(100 * (1 - its(learned) / its(e7)), 1),
                "steps_per_s_ratio": round(learned["steps_per_s"] / e7["steps_per_s"], 3)}
+    lm_gain = None
+    if "chosen_no_low_modes" in res:
+        c0 = res["chosen_no_low_modes"]
+        lm_gain = {"compared": f"chosen ({hd} increments + the low-mode correction) vs the "
+                               "Galerkin start over 8 increments alone (its best), each on its "
+                               "own trajectory",
+                   "steps_per_s_without": round(c0["steps_per_s"], 3),
+                   "cg_iterations_per_step_without": its(c0),
+                   "cg_iterations_saved_pct": round(100 * (1 - its(chosen) / its(c0)), 1),
+                   "steps_per_s_ratio": round(chosen["steps_per_s"] / c0["steps_per_s"], 3)}
     bar = None
     if "chosen_bar" in res and "plain_bar" in res:
         bar = {"rtol": a.bar_rtol, "chosen": leg("chosen_bar"), "plain_fp64": leg("plain_bar"),
@@ -654,8 +694,11 @@ def main():
                    "method": "FP32 fused 3-species Jacobi-PCG (Chronopoulos-Gear, one kernel per "
                              "iteration, stored FP32 geometry) in FP64 refinement, FP64 "
                              f"Galerkin warm start over the last {hd} solution "
-                             "increments (previous solution as fallback), species narrowing, "
-                             "8-iteration polled chunks of direct launches",
+                             "increments (previous solution as fallback), then the Galerkin "
+                             f"correction over {a.low_modes} low modes per species of D^-1 A "
+                             "(built once per operator before the warm-up, Chebyshev-filtered "
+                             "subspace iteration), species narrowing, 8-iteration polled chunks "
+                             "of direct launches",
                    "step": "the three species batched in one solve with static coefficients "
                            "(f_OH from the initial O2, no O2 under-relaxation)",
                    "parallelism": "1 GPU", "launch_shape": tuned},
@@ -671,6 +714,7 @@ def main():
                                    taken_last_step=learned["taken_per_step"][-1]
                                    if learned["taken_per_step"] else None),
         "mlp_direction_gain": mlp,
+        "low_mode_gain": lm_gain,
         "chosen_vs_plain": round(chosen["steps_per_s"] / plain["steps_per_s"], 3),
         "error_vs_fp64": {
             "definition": "per timed step and species, max_i |u_i - u*_i| / max_i |u*_i| with u* "

@@ -166,6 +166,7 @@ struct dsv_mesh {
     int stat_narrowings = 0;
     int stat_dir_reused = 0;
     int stat_gram_reused = 0;
+    int stat_low_modes = 0;  // species whose start the low-mode correction moved
     unsigned op_epoch = 0;   // bumped by every dsv_set_operator_reuse call
     // Dirichlet lists (address, length) range-checked in epoch bc_ok_epoch (operator reuse)
     mutable std::vector<std::pair<const int32_t*, int32_t>> bc_ok;

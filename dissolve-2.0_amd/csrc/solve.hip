@@ -37,13 +37,17 @@
 // CSR CG (method CSR64, the plain FP64 path): standard Jacobi-PCG with two kernels per
 // iteration, the CSR apply (A_k) and the vector update (U_k), with the same fence-free
 // reductions.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
+#include <numeric>
 #include <atomic>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "dsv_internal.h"
 #include "kernels_common.h"
@@ -1637,6 +1641,16 @@ struct Workspace {
     int graph_chunk = 0;
     int graph_valid = 0;
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
+    // low-mode subspace (dsv_build_low_modes): lm_n FP32 vectors per species (lm_n blocks of V
+    // species records, internal order, 0 at Dirichlet vertices), the inverse of W^T A W per
+    // species under the FP64 operator, the operator key they belong to, and per-solve scratch
+    int lm_n = 0;
+    float* lm_w = nullptr;
+    double* lm_ginv = nullptr;  // [S][lm_n][lm_n]
+    double* lm_part = nullptr;  // dot partials
+    double* lm_c = nullptr;     // coefficients [lm_n][S]
+    double* lm_y = nullptr;     // the correction W c (V FP64 species records)
+    OpKey lm_key;
 };
 
 static size_t rec_bytes(size_t t, int S) { return t * S; }
@@ -1677,6 +1691,11 @@ static void free_ws(Workspace* w) {
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < MAXS; ++q) dfree(w->fw[a][q]);
     for (auto& k : w->kvs) dfree(k);
+    dfree(w->lm_w);
+    dfree(w->lm_ginv);
+    dfree(w->lm_part);
+    dfree(w->lm_c);
+    dfree(w->lm_y);
     dfree(w->bcbits);
     for (int i = 0; i < VB_COUNT; ++i) dfree(w->vb[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -2620,10 +2639,15 @@ struct WarmArgs {
     double* coef = nullptr;
     double* r_before = nullptr;
     double* r_after = nullptr;
+    // set by the solve: the low-mode correction follows the Galerkin start (its rtol decides
+    // which species already meet the tolerance)
+    bool low_modes = false;
+    double rtol = 0.0;
 };
 template <int S>
 static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
                        int nres, double* rr, double* bb, hipStream_t st);
+static bool low_modes_valid(const dsv_mesh* m, const Workspace* w);
 
 // The solution arrays a solve is about to write: the warm-start directions of another
 // workspace may have been formed from them, and this workspace's next call must not take a
@@ -2653,6 +2677,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
     m->stat_narrowings = 0;
     m->stat_dir_reused = 0;
     m->stat_gram_reused = 0;
+    m->stat_low_modes = 0;
     const bool internal = method != DSV_METHOD_CSR64;
     const int32_t* perm = internal ? m->perm : nullptr;
     const unsigned gv = cdiv(m->V, 256);
@@ -2721,7 +2746,10 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         if (warm0) {
             // x64 and r64 move to the Galerkin start (rr: its residual); should that already
             // meet the tolerance, the true residual confirms it
-            fused_warm<S>(m, w, sp, *warm, residual64_launch<S>(m, w, st), rr, bb, st);
+            WarmArgs wa = *warm;
+            wa.low_modes = method == DSV_METHOD_MF32_IR64 && low_modes_valid(m, w);
+            wa.rtol = rtol;
+            fused_warm<S>(m, w, sp, wa, residual64_launch<S>(m, w, st), rr, bb, st);
             bool met = true;
             for (int s = 0; s < S; ++s) met &= std::sqrt(rr[s]) <= rtol * std::sqrt(bb[s]);
             if (met) residual64<S>(m, w, st, rr, bb);
@@ -2810,6 +2838,7 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         stats->narrowings = m->stat_narrowings;
         stats->dirs_reused = m->stat_dir_reused;
         stats->gram_reused = m->stat_gram_reused;
+        stats->low_modes = m->stat_low_modes;
     }
 }
 
@@ -3332,6 +3361,311 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
 // Galerkin warm start of dsv_solve_warm after the residual kernel of the solve's first round
 // (its partials in w->pR, nres per quantity): rr and bb come back as that residual's norms,
 // rr updated to the start taken; one host round trip.
+// ---------------------------------------------------------------------------------------
+// Low-mode subspace of the MF32_IR64 operator (dsv_build_low_modes).  After the warm start's
+// history directions, the error of a start lies mostly in the smooth modes that Jacobi-PCG
+// resolves last.  A subspace W approximating the eigenvectors of the lowest eigenvalues of
+// D^-1 A (D = diag A; vectors 0 at Dirichlet vertices) takes them out once per solve by the
+// Galerkin correction x += W c, (W^T A W) c = W^T r, which minimises the A-norm of the start's
+// error over start + span W.  W is built once per operator by Chebyshev-filtered subspace
+// iteration: a seeded random block; passes of a degree-m Chebyshev polynomial in P = D^-1 A
+// that damps [a, b] (b a Gershgorin bound of P, a the block's largest Ritz value of the pass
+// before, b / 64 at the first pass) and so raises the low end of the spectrum against it;
+// Rayleigh-Ritz in the D inner product after each pass.
+constexpr int LM_MAX = 64;        // modes per species at most (a multiple of 4)
+constexpr int LM_GUARD = 16;      // block vectors beyond the modes kept
+constexpr int LM_BLOCK_MAX = LM_MAX + LM_GUARD;
+constexpr int LM_THREADS = 256;
+constexpr int LM_CHUNKS = 240;    // vertex chunks (workgroups along x) of the dot kernels
+constexpr int LM_TILE = 8;        // modes per workgroup row of the correction's dot kernel
+constexpr int LM_COEF_THREADS = 192;
+static_assert(LM_MAX * MAXS <= LM_COEF_THREADS, "one thread per coefficient");
+
+// uniform in (-1, 1) from a 64-bit key: the splitmix64 finaliser (Steele, Lea and Flood 2014)
+__device__ __forceinline__ float lm_uniform(uint64_t key) {
+    uint64_t z = key + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (float)((double)(z >> 40) * (2.0 / 16777216.0) - 1.0);
+}
+
+template <int S>
+__global__ void k_lm_random(int V, int nb, uint64_t seed, const uint8_t* __restrict__ bcbits,
+                            SV<float, S>* __restrict__ X) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)V * nb) return;
+    const int v = (int)(t % V);
+    const unsigned bcb = bcbits ? (unsigned)bcbits[v] : 0u;
+    SV<float, S> x;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        x.v[s] = ((bcb >> s) & 1u) ? 0.0f
+                                   : lm_uniform(seed * 0x100000001B3ull + (uint64_t)t * S + s);
+    X[t] = x;
+}
+
+// part[((i nb + j) S + s) gridDim.x + blockIdx.x]: the chunk's sum of X_i Y_j per species
+// (divided by dinv with `dinv`: the D inner product) over the 4 x 4 tile blockIdx.y of (i, j)
+template <int S, class TY>
+__global__ __launch_bounds__(LM_THREADS) void k_lm_block_dots(int V, int nb,
+                                                              const SV<float, S>* __restrict__ X,
+                                                              const SV<TY, S>* __restrict__ Y,
+                                                              const SV<float, S>* __restrict__ dinv,
+                                                              double* __restrict__ part) {
+    const int nt = nb / 4;
+    const int i0 = (blockIdx.y / nt) * 4, j0 = (blockIdx.y % nt) * 4;
+    double acc[4][4][S];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[a][b][s] = 0.0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        SV<float, S> xi[4];
+        SV<TY, S> yj[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            xi[q] = X[(size_t)(i0 + q) * V + v];
+            yj[q] = Y[(size_t)(j0 + q) * V + v];
+        }
+        double wgt[S];
+        if (dinv) {
+            const SV<float, S> d = dinv[v];
+#pragma unroll
+            for (int s = 0; s < S; ++s) wgt[s] = 1.0 / (double)d.v[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s) wgt[s] = 1.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const double xa = (double)xi[a].v[s] * wgt[s];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b][s] += xa * (double)yj[b].v[s];
+            }
+    }
+    __shared__ double red[16 * MAXS][LM_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 16 * S; ++q) {
+        const double t = wave_sum(acc[q / (4 * S)][(q / S) % 4][q % S]);
+        if (lane == 0) red[q][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 * S) {
+        const int q = threadIdx.x;
+        double t = 0.0;
+        for (int w = 0; w < LM_THREADS / 64; ++w) t += red[q][w];
+        const int a = q / (4 * S), b = (q / S) % 4, sp = q % S;
+        part[(((size_t)(i0 + a) * nb + (j0 + b)) * S + sp) * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// out[o] = sum over the chunks of part[o nchunk + i], in chunk order
+__global__ void k_lm_sum(int n_out, int nchunk, const double* __restrict__ part,
+                         double* __restrict__ out) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n_out) return;
+    double t = 0.0;
+    for (int i = 0; i < nchunk; ++i) t += part[(size_t)o * nchunk + i];
+    out[o] = t;
+}
+
+// Xout_j = sum_i C_s[i][j] X_i per species (C: [S][nb][nb] FP32), j < nout
+template <int S>
+__global__ __launch_bounds__(LM_THREADS) void k_lm_rotate(int V, int nb, int nout,
+                                                          const SV<float, S>* __restrict__ X,
+                                                          const float* __restrict__ C,
+                                                          SV<float, S>* __restrict__ Xout) {
+    __shared__ float cs[LM_BLOCK_MAX * LM_BLOCK_MAX];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int s = 0; s < S; ++s) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb * nout; i += blockDim.x)
+            cs[i] = C[((size_t)s * nb + i / nout) * nb + i % nout];
+        __syncthreads();
+        if (v < V) {
+            float xin[LM_BLOCK_MAX];
+#pragma unroll
+            for (int i = 0; i < LM_BLOCK_MAX; ++i) xin[i] = i < nb ? X[(size_t)i * V + v].v[s] : 0.0f;
+            for (int j = 0; j < nout; ++j) {
+                float o = 0.0f;
+#pragma unroll
+                for (int i = 0; i < LM_BLOCK_MAX; ++i)
+                    if (i < nb) o += xin[i] * cs[i * nout + j];
+                Xout[(size_t)j * V + v].v[s] = o;
+            }
+        }
+    }
+}
+
+template <int S>
+__global__ void k_lm_to_double(int V, const SV<float, S>* __restrict__ x,
+                               SV<double, S>* __restrict__ y) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const SV<float, S> a = x[v];
+    SV<double, S> b;
+#pragma unroll
+    for (int s = 0; s < S; ++s) b.v[s] = (double)a.v[s];
+    y[v] = b;
+}
+
+// per-block maximum over the rows of sum_j |a_ij| / a_ii (the Gershgorin bound of D^-1 A)
+__global__ __launch_bounds__(256) void k_lm_gershgorin(int V, const int32_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ diag_pos,
+                                                      const double* __restrict__ val,
+                                                      double* __restrict__ part) {
+    double mx = 0.0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < V; r += gridDim.x * blockDim.x) {
+        double t = 0.0;
+        for (int32_t p = indptr[r]; p < indptr[r + 1]; ++p) t += fabs(val[p]);
+        mx = fmax(mx, t / fabs(val[diag_pos[r]]));
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// the correction's dots: part[(j S + s) gridDim.x + blockIdx.x] = the chunk's w_j . r, for the
+// modes [j0, j0 + LM_TILE) of blockIdx.y
+template <int S>
+__global__ __launch_bounds__(LM_THREADS) void k_lm_dots(int V, int n,
+                                                        const SV<float, S>* __restrict__ W,
+                                                        const SV<double, S>* __restrict__ r,
+                                                        double* __restrict__ part) {
+    const int j0 = blockIdx.y * LM_TILE;
+    double acc[LM_TILE][S];
+#pragma unroll
+    for (int j = 0; j < LM_TILE; ++j)
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[j][s] = 0.0;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const SV<double, S> rv = r[v];
+#pragma unroll
+        for (int j = 0; j < LM_TILE; ++j)
+            if (j0 + j < n) {
+                const SV<float, S> wv = W[(size_t)(j0 + j) * V + v];
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[j][s] += (double)wv.v[s] * rv.v[s];
+            }
+    }
+    __shared__ double red[LM_TILE * MAXS][LM_THREADS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < LM_TILE * S; ++q) {
+        const double t = wave_sum(acc[q / S][q % S]);
+        if (lane == 0) red[q][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < LM_TILE * S && j0 + (int)threadIdx.x / S < n) {
+        const int q = threadIdx.x;
+        double t = 0.0;
+        for (int w = 0; w < LM_THREADS / 64; ++w) t += red[q][w];
+        part[((size_t)j0 * S + q) * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// one workgroup: t = W^T r per species, c_s = Ginv_s t_s; a species whose start already meets
+// the tolerance (||r||^2 <= rtol^2 ||b||^2 from the warm start's outputs) or whose coefficients
+// are not finite keeps c = 0
+template <int S>
+__global__ __launch_bounds__(LM_COEF_THREADS) void k_lm_coef(int n, int nchunk,
+                                                             const double* __restrict__ part,
+                                                             const double* __restrict__ ginv,
+                                                             const WarmOut* __restrict__ wout,
+                                                             double rtol2, double* __restrict__ c) {
+    __shared__ double t[LM_MAX * MAXS];
+    __shared__ int ok[MAXS];
+    const int q = threadIdx.x;  // q = j S + s
+    if (q < MAXS) ok[q] = 1;
+    if (q < n * S) {
+        double a = 0.0;
+        for (int i = 0; i < nchunk; ++i) a += part[(size_t)q * nchunk + i];
+        t[q] = a;
+    }
+    __syncthreads();
+    double cq = 0.0;
+    if (q < n * S) {
+        const int j = q / S, s = q % S;
+        for (int i = 0; i < n; ++i) cq += ginv[((size_t)s * n + j) * n + i] * t[i * S + s];
+        if (!isfinite(cq)) ok[s] = 0;
+    }
+    __syncthreads();
+    if (q < n * S) {
+        const int s = q % S;
+        const bool run = ok[s] && wout->rr1[s] > rtol2 * wout->bb[s];
+        c[q] = run ? cq : 0.0;
+    }
+}
+
+// y = W c (FP64), x += y
+template <int S>
+__global__ void k_lm_update(int V, int n, const SV<float, S>* __restrict__ W,
+                            const double* __restrict__ c, SV<double, S>* __restrict__ x,
+                            SV<double, S>* __restrict__ y) {
+    __shared__ double cs[LM_MAX * MAXS];
+    for (int i = threadIdx.x; i < n * S; i += blockDim.x) cs[i] = c[i];
+    __syncthreads();
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const SV<float, S> wv = W[(size_t)j * V + v];
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[s] += cs[j * S + s] * (double)wv.v[s];
+    }
+    SV<double, S> yv, xv = x[v];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        yv.v[s] = acc[s];
+        xv.v[s] += acc[s];
+    }
+    y[v] = yv;
+    x[v] = xv;
+}
+
+static bool low_modes_valid(const dsv_mesh* m, const Workspace* w) {
+    return w->lm_n > 0 && m->op_reuse && w->method == DSV_METHOD_MF32_IR64 &&
+           same_key(w->key[1], w->lm_key);
+}
+
+// the correction of x64 / r64 and the publication of the new residual norms (ResidPub)
+template <int S>
+static void launch_low_mode_correction(dsv_mesh* m, Workspace* w, double rtol, hipStream_t st) {
+    const int n = w->lm_n;
+    const auto* W = (const SV<float, S>*)w->lm_w;
+    auto* r64 = (SV<double, S>*)w->vb[VB_R64];
+    auto* x64 = (SV<double, S>*)w->vb[VB_X64];
+    hipLaunchKernelGGL((k_lm_dots<S>), dim3(LM_CHUNKS, cdiv(n, LM_TILE)), dim3(LM_THREADS), 0, st,
+                       m->V, n, W, (const SV<double, S>*)r64, w->lm_part);
+    hipLaunchKernelGGL((k_lm_coef<S>), dim3(1), dim3(LM_COEF_THREADS), 0, st, n, LM_CHUNKS,
+                       (const double*)w->lm_part, (const double*)w->lm_ginv,
+                       (const WarmOut*)w->wout, rtol * rtol, w->lm_c);
+    hipLaunchKernelGGL((k_lm_update<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V, n, W,
+                       (const double*)w->lm_c, x64, (SV<double, S>*)w->lm_y);
+    // r64 -= A y (the residual kernel with b = r64 read and written in place by each vertex)
+    MFVec<double, S> v{};
+    v.x = (const SV<double, S>*)w->lm_y;
+    v.y = r64;
+    v.b = r64;
+    launch_mf<double, S, MF_RESID>(m, m->part, w->prm2, v, w->pR, st);
+    hipLaunchKernelGGL((k_resid_publish<S>), dim3(1), dim3(64 * 2 * S), 0, st, w->pR,
+                       mf_grid(m->part), (const CGState*)nullptr, w->d_rpub);
+    DSV_HIP(hipGetLastError());
+}
+
 template <int S, int ND>
 static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
                           int nres, double* rr, double* bb, hipStream_t st) {
@@ -3481,13 +3815,25 @@ static void fused_warm_nd(dsv_mesh* m, Workspace* w, const dsv_species* sp, cons
     hipLaunchKernelGGL((k_proj_accept_update<S, ND>), dim3(PROJ_GRID_F), dim3(PROJ_THREADS), 0, st,
                        m->V, pv, (SV<double, S>*)w->vb[VB_R64], w->pT, w->wout, w->d_wout_pub);
     DSV_HIP(hipGetLastError());
-    wait_published(seq, st, "the warm start");
+    if (wa.low_modes) {
+        // the low-mode correction of the start taken; its residual norms arrive after the
+        // warm start's outputs, so one wait covers both
+        volatile int* rseq = &w->h_rpub->seq;
+        *rseq = 0;
+        launch_low_mode_correction<S>(m, w, wa.rtol, st);
+        wait_published(rseq, st, "the low-mode correction");
+    } else {
+        wait_published(seq, st, "the warm start");
+    }
     const WarmOut& o = *w->h_wout;
+    m->stat_low_modes = 0;
     for (int s = 0; s < S; ++s) {
-        rr[s] = o.rr1[s];
+        const double r1 = wa.low_modes ? w->h_rpub->rr[s] : o.rr1[s];
+        if (wa.low_modes && r1 != o.rr1[s]) ++m->stat_low_modes;
+        rr[s] = r1;
         bb[s] = o.bb[s];
         if (wa.r_before) wa.r_before[s] = std::sqrt(o.rr0[s]);
-        if (wa.r_after) wa.r_after[s] = std::sqrt(o.rr1[s]);
+        if (wa.r_after) wa.r_after[s] = std::sqrt(r1);
         if (wa.coef)
             for (int k = 0; k < ND; ++k) wa.coef[api[k] * S + s] = o.take[s] ? o.c[k][s] : 0.0;
     }
@@ -3615,6 +3961,347 @@ static double time_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int l
 
 using namespace dsv;
 
+// ---- the low-mode subspace's build (host side)
+
+// eigen-decomposition of a symmetric n x n matrix (row-major, overwritten) by cyclic Jacobi
+// rotations: eigenvalues in ev, eigenvectors the columns of Q (row-major n x n)
+static void lm_sym_eig(int n, std::vector<double>& A, std::vector<double>& ev,
+                       std::vector<double>& Q) {
+    Q.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double a = A[(size_t)i * n + j];
+                tot += a * a;
+                if (j > i) off += a * a;
+            }
+        if (!(off > 1e-30 * tot)) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2.0 * apq);
+                const double t =
+                    (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * akp - sn * akq;
+                    A[(size_t)k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * apk - sn * aqk;
+                    A[(size_t)q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double qkp = Q[(size_t)k * n + p], qkq = Q[(size_t)k * n + q];
+                    Q[(size_t)k * n + p] = c * qkp - sn * qkq;
+                    Q[(size_t)k * n + q] = sn * qkp + c * qkq;
+                }
+            }
+    }
+    ev.resize(n);
+    for (int i = 0; i < n; ++i) ev[i] = A[(size_t)i * n + i];
+}
+
+// Rayleigh-Ritz of one species' block: G c = lambda M c (nb x nb Gram matrices of A and D).
+// M = U diag(sig) U^T, B = U sig^-1/2 over the directions with sig > 1e-12 max sig, H = B^T G B
+// = Q diag(lambda) Q^T; C = B Q (nb x nb row-major, columns in ascending lambda, columns past
+// the rank 0 with lambda = +inf)
+static void lm_rayleigh_ritz(int nb, const double* G, const double* M, std::vector<double>& C,
+                             std::vector<double>& lam) {
+    std::vector<double> Ms((size_t)nb * nb), sig, U;
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j)
+            Ms[(size_t)i * nb + j] = 0.5 * (M[(size_t)i * nb + j] + M[(size_t)j * nb + i]);
+    lm_sym_eig(nb, Ms, sig, U);
+    double smax = 0.0;
+    for (double x : sig) smax = std::max(smax, x);
+    std::vector<int> keep;
+    for (int k = 0; k < nb; ++k)
+        if (sig[k] > 1e-12 * smax) keep.push_back(k);
+    const int r = (int)keep.size();
+    std::vector<double> B((size_t)nb * r);
+    for (int i = 0; i < nb; ++i)
+        for (int k = 0; k < r; ++k)
+            B[(size_t)i * r + k] = U[(size_t)i * nb + keep[k]] / std::sqrt(sig[keep[k]]);
+    std::vector<double> GB((size_t)nb * r, 0.0), H((size_t)r * r, 0.0);
+    for (int i = 0; i < nb; ++i)
+        for (int k = 0; k < r; ++k) {
+            double a = 0.0;
+            for (int j = 0; j < nb; ++j) a += 0.5 * (G[(size_t)i * nb + j] + G[(size_t)j * nb + i]) * B[(size_t)j * r + k];
+            GB[(size_t)i * r + k] = a;
+        }
+    for (int k = 0; k < r; ++k)
+        for (int l = 0; l < r; ++l) {
+            double a = 0.0;
+            for (int i = 0; i < nb; ++i) a += B[(size_t)i * r + k] * GB[(size_t)i * r + l];
+            H[(size_t)k * r + l] = a;
+        }
+    for (int k = 0; k < r; ++k)
+        for (int l = k + 1; l < r; ++l) {
+            const double a = 0.5 * (H[(size_t)k * r + l] + H[(size_t)l * r + k]);
+            H[(size_t)k * r + l] = H[(size_t)l * r + k] = a;
+        }
+    std::vector<double> hv, Qh;
+    lm_sym_eig(r, H, hv, Qh);
+    std::vector<int> order(r);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return hv[a] < hv[b]; });
+    C.assign((size_t)nb * nb, 0.0);
+    lam.assign(nb, std::numeric_limits<double>::infinity());
+    for (int jj = 0; jj < r; ++jj) {
+        const int j = order[jj];
+        lam[jj] = hv[j];
+        for (int i = 0; i < nb; ++i) {
+            double a = 0.0;
+            for (int k = 0; k < r; ++k) a += B[(size_t)i * r + k] * Qh[(size_t)k * r + j];
+            C[(size_t)i * nb + jj] = a;
+        }
+    }
+}
+
+// one Chebyshev step on P y = dinv (A y), AY = A Y, per species s with centre c[s] and half
+// width e[s]: first, out = (P Y - c Y) / e; otherwise out = 2 (P Y - c Y) / e - out (the
+// iterate before, overwritten)
+struct LmCheb {
+    float c[MAXS], e[MAXS];
+};
+
+template <int S>
+__global__ void k_lm_cheb_s(int V, const SV<float, S>* __restrict__ dinv,
+                            const SV<float, S>* __restrict__ AY, const SV<float, S>* __restrict__ Y,
+                            SV<float, S>* __restrict__ out, LmCheb ce, int first) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const SV<float, S> d = dinv[v], a = AY[v], y = Y[v];
+    SV<float, S> o;
+    if (first) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) o.v[s] = (d.v[s] * a.v[s] - ce.c[s] * y.v[s]) / ce.e[s];
+    } else {
+        const SV<float, S> p = out[v];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            o.v[s] = 2.0f * (d.v[s] * a.v[s] - ce.c[s] * y.v[s]) / ce.e[s] - p.v[s];
+    }
+    out[v] = o;
+}
+
+// Gram matrices of a block: out[(i nb + j) S + s] (host), from the block-dot partials
+template <int S, class TY>
+static void lm_gram(dsv_mesh* m, int nb, const SV<float, S>* X, const SV<TY, S>* Y,
+                    const SV<float, S>* dinv, double* part, double* dtot, std::vector<double>& out,
+                    hipStream_t st) {
+    const int nt = nb / 4;
+    hipLaunchKernelGGL((k_lm_block_dots<S, TY>), dim3(LM_CHUNKS, nt * nt), dim3(LM_THREADS), 0, st,
+                       m->V, nb, X, Y, dinv, part);
+    const int n_out = nb * nb * S;
+    hipLaunchKernelGGL(k_lm_sum, dim3(cdiv(n_out, 256)), dim3(256), 0, st, n_out, LM_CHUNKS,
+                       (const double*)part, dtot);
+    DSV_HIP(hipGetLastError());
+    out.resize(n_out);
+    DSV_HIP(hipMemcpyAsync(out.data(), dtot, n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+    DSV_HIP(hipStreamSynchronize(st));
+}
+
+template <int S>
+static void build_low_modes(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lumped, int n,
+                            int degree, int passes, uint64_t seed, double* ritz,
+                            hipStream_t caller) {
+    Workspace* w = get_ws(m, DSV_METHOD_MF32_IR64, S);
+    hipStream_t st = w->stream;
+    DSV_HIP(hipEventRecord(w->ev_io[0], caller));
+    DSV_HIP(hipStreamWaitEvent(st, w->ev_io[0], 0));
+    w->lm_n = 0;
+    dfree(w->lm_w);
+    dfree(w->lm_ginv);
+    dfree(w->lm_part);
+    dfree(w->lm_c);
+    dfree(w->lm_y);
+    if (n == 0) return;
+    const size_t V = m->V;
+    // the operator, set up as a MF32_IR64 solve sets it up
+    prepare_bc(m, w, S, sp, st);
+    const bool k_ready = prepare_op<float, S>(m, w, m->part, sp, inv_dt, lumped, st, 0);
+    prepare_op<double, S>(m, w, m->part, sp, inv_dt, lumped, st, 1, k_ready);
+    if (!w->dinv_valid) {
+        MFVec<float, S> v{};
+        v.y = (SV<float, S>*)w->vb[VB_DINV];
+        launch_mf<float, S, MF_DIAG>(m, m->part, w->prm, v, nullptr, st);
+        w->dinv_valid = m->op_reuse;
+    }
+    const auto* dinv = (const SV<float, S>*)w->vb[VB_DINV];
+    const int nb = n + LM_GUARD;
+    std::vector<void*> tmp;
+    auto take = [&](size_t bytes) {
+        void* p = nullptr;
+        DSV_HIP(hipMalloc(&p, std::max<size_t>(bytes, 1)));
+        tmp.push_back(p);
+        return p;
+    };
+    try {
+        // Gershgorin bound of D^-1 A per species from its assembled FP64 matrix
+        double bmax[MAXS], a_cut[MAXS];
+        {
+            auto* vals = (double*)take(m->nnz * sizeof(double));
+            auto* gp = (double*)take(256 * sizeof(double));
+            std::vector<double> h(256);
+            for (int s = 0; s < S; ++s) {
+                assemble_into(m, sp[s], inv_dt, lumped, vals, st);
+                hipLaunchKernelGGL(k_lm_gershgorin, dim3(256), dim3(256), 0, st, m->V, m->indptr,
+                                   m->diag_pos, (const double*)vals, gp);
+                DSV_HIP(hipGetLastError());
+                DSV_HIP(hipMemcpyAsync(h.data(), gp, 256 * sizeof(double), hipMemcpyDeviceToHost, st));
+                DSV_HIP(hipStreamSynchronize(st));
+                double b = 0.0;
+                for (double x : h) b = std::max(b, x);
+                if (!(b > 0.0) || !std::isfinite(b)) throw ArgError{"dsv_build_low_modes: non-finite operator"};
+                bmax[s] = 1.02 * b;
+                a_cut[s] = bmax[s] / 64.0;
+            }
+        }
+        const size_t blk = (size_t)nb * V;
+        auto* X = (SV<float, S>*)take(blk * sizeof(SV<float, S>));
+        auto* X2 = (SV<float, S>*)take(blk * sizeof(SV<float, S>));
+        auto* AX = (SV<float, S>*)take(blk * sizeof(SV<float, S>));
+        auto* Yb = (SV<float, S>*)take(V * sizeof(SV<float, S>));
+        auto* AY = (SV<float, S>*)take(V * sizeof(SV<float, S>));
+        auto* part = (double*)take((size_t)nb * nb * S * LM_CHUNKS * sizeof(double));
+        auto* dtot = (double*)take((size_t)nb * nb * S * sizeof(double));
+        auto* dC = (float*)take((size_t)S * nb * nb * sizeof(float));
+        hipLaunchKernelGGL((k_lm_random<S>), dim3(cdiv((int64_t)V * nb, 256)), dim3(256), 0, st,
+                           m->V, nb, seed, (const uint8_t*)w->bcbits, X);
+        DSV_HIP(hipGetLastError());
+        auto apply32 = [&](const SV<float, S>* x, SV<float, S>* y) {
+            MFVec<float, S> v{};
+            v.x = x;
+            v.y = y;
+            launch_mf<float, S, MF_APPLY>(m, m->part, w->prm, v, nullptr, st);
+        };
+        std::vector<double> Gh, Mh, lamv[MAXS];
+        for (int pass = 0; pass < passes; ++pass) {
+            LmCheb ce{};
+            // the filter raises an eigenvalue near 0 by T_m(1 + 2 a / (b - a)) against those in
+            // [a, b]; m is capped so that this stays near 1e4, which leaves the block's weaker
+            // low-mode components above FP32 rounding until the Rayleigh-Ritz step
+            int m_pass = degree;
+            for (int s = 0; s < S; ++s) {
+                ce.c[s] = (float)(0.5 * (bmax[s] + a_cut[s]));
+                ce.e[s] = (float)(0.5 * (bmax[s] - a_cut[s]));
+                const double g = std::acosh(1.0 + 2.0 * a_cut[s] / (bmax[s] - a_cut[s]));
+                if (g > 0.0) m_pass = std::min(m_pass, std::max(4, (int)(9.0 / g)));
+            }
+            for (int j = 0; j < nb; ++j) {
+                SV<float, S>* y0 = X + (size_t)j * V;
+                SV<float, S>* cur = Yb;
+                SV<float, S>* prev = y0;
+                apply32(y0, AY);
+                hipLaunchKernelGGL((k_lm_cheb_s<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
+                                   dinv, (const SV<float, S>*)AY, (const SV<float, S>*)y0, Yb, ce, 1);
+                for (int k = 2; k <= m_pass; ++k) {
+                    apply32(cur, AY);
+                    hipLaunchKernelGGL((k_lm_cheb_s<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st,
+                                       m->V, dinv, (const SV<float, S>*)AY,
+                                       (const SV<float, S>*)cur, prev, ce, 0);
+                    std::swap(cur, prev);
+                }
+                if (cur != y0)
+                    DSV_HIP(hipMemcpyAsync(y0, cur, V * sizeof(SV<float, S>),
+                                           hipMemcpyDeviceToDevice, st));
+                DSV_HIP(hipGetLastError());
+            }
+            // Rayleigh-Ritz in the D inner product
+            for (int j = 0; j < nb; ++j) apply32(X + (size_t)j * V, AX + (size_t)j * V);
+            lm_gram<S, float>(m, nb, X, X, dinv, part, dtot, Mh, st);
+            lm_gram<S, float>(m, nb, X, AX, nullptr, part, dtot, Gh, st);
+            std::vector<float> Ch((size_t)S * nb * nb);
+            for (int s = 0; s < S; ++s) {
+                std::vector<double> Gs((size_t)nb * nb), Ms((size_t)nb * nb), C;
+                for (int i = 0; i < nb; ++i)
+                    for (int j = 0; j < nb; ++j) {
+                        Gs[(size_t)i * nb + j] = Gh[((size_t)i * nb + j) * S + s];
+                        Ms[(size_t)i * nb + j] = Mh[((size_t)i * nb + j) * S + s];
+                    }
+                lm_rayleigh_ritz(nb, Gs.data(), Ms.data(), C, lamv[s]);
+                for (size_t q = 0; q < C.size(); ++q) Ch[(size_t)s * nb * nb + q] = (float)C[q];
+                const double a = lamv[s][nb - 1];
+                if (std::isfinite(a) && a > 0.0 && a < 0.5 * bmax[s]) a_cut[s] = a;
+            }
+            DSV_HIP(hipMemcpyAsync(dC, Ch.data(), Ch.size() * sizeof(float), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL((k_lm_rotate<S>), dim3(cdiv(m->V, LM_THREADS)), dim3(LM_THREADS), 0,
+                               st, m->V, nb, nb, (const SV<float, S>*)X, (const float*)dC, X2);
+            DSV_HIP(hipGetLastError());
+            std::swap(X, X2);
+        }
+        // W: the first n vectors (ascending Ritz values), and W^T A W under the FP64 operator
+        w->lm_w = dalloc<float>((size_t)n * V * S);
+        DSV_HIP(hipMemcpyAsync(w->lm_w, X, (size_t)n * V * sizeof(SV<float, S>),
+                               hipMemcpyDeviceToDevice, st));
+        auto* Wd = (SV<double, S>*)take(V * sizeof(SV<double, S>));
+        auto* AWd = (SV<double, S>*)take((size_t)n * V * sizeof(SV<double, S>));
+        const auto* W = (const SV<float, S>*)w->lm_w;
+        for (int j = 0; j < n; ++j) {
+            hipLaunchKernelGGL((k_lm_to_double<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
+                               W + (size_t)j * V, Wd);
+            MFVec<double, S> v{};
+            v.x = Wd;
+            v.y = AWd + (size_t)j * V;
+            launch_mf<double, S, MF_APPLY>(m, m->part, w->prm2, v, nullptr, st);
+        }
+        std::vector<double> Gw;
+        lm_gram<S, double>(m, n, W, AWd, nullptr, part, dtot, Gw, st);
+        // (W^T A W)^-1 per species, by its eigen-decomposition (directions with eigenvalues
+        // below 1e-12 of the largest get no weight)
+        std::vector<double> ginv((size_t)S * n * n, 0.0);
+        for (int s = 0; s < S; ++s) {
+            std::vector<double> Gs((size_t)n * n), ev, Q;
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                    Gs[(size_t)i * n + j] = 0.5 * (Gw[((size_t)i * n + j) * S + s] +
+                                                   Gw[((size_t)j * n + i) * S + s]);
+            lm_sym_eig(n, Gs, ev, Q);
+            double emax = 0.0;
+            for (double x : ev) emax = std::max(emax, x);
+            for (int k = 0; k < n; ++k) {
+                if (!(ev[k] > 1e-12 * emax)) continue;
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j < n; ++j)
+                        ginv[((size_t)s * n + i) * n + j] +=
+                            Q[(size_t)i * n + k] * Q[(size_t)j * n + k] / ev[k];
+            }
+            if (ritz)
+                for (int k = 0; k < n; ++k) ritz[(size_t)s * n + k] = lamv[s].empty() ? 0.0 : lamv[s][k];
+        }
+        w->lm_ginv = dalloc<double>(ginv.size());
+        DSV_HIP(hipMemcpyAsync(w->lm_ginv, ginv.data(), ginv.size() * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        w->lm_part = dalloc<double>((size_t)n * S * LM_CHUNKS);
+        w->lm_c = dalloc<double>((size_t)LM_MAX * MAXS);
+        w->lm_y = dalloc<double>(V * S);
+        DSV_HIP(hipStreamSynchronize(st));
+        for (void* p : tmp) (void)hipFree(p);
+        tmp.clear();
+        w->lm_key = w->key[1];
+        w->lm_n = n;
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        for (void* p : tmp) (void)hipFree(p);
+        dfree(w->lm_w);
+        dfree(w->lm_ginv);
+        dfree(w->lm_part);
+        dfree(w->lm_c);
+        dfree(w->lm_y);
+        w->lm_n = 0;
+        throw;
+    }
+    DSV_HIP(hipEventRecord(w->ev_io[1], st));
+    DSV_HIP(hipStreamWaitEvent(caller, w->ev_io[1], 0));
+}
+
 extern "C" {
 
 int dsv_apply(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
@@ -3728,6 +4415,29 @@ int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp
             case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
             case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
             default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+        }
+    });
+}
+
+int dsv_build_low_modes(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
+                        int32_t lumped, int32_t n_modes, int32_t degree, int32_t passes,
+                        uint64_t seed, double* ritz, void* stream) {
+    return guard([&] {
+        check_solve_args(m, n_species, sp, false);
+        if (n_modes < 0 || n_modes > LM_MAX || n_modes % 4 != 0)
+            throw ArgError{"dsv_build_low_modes: n_modes must be a multiple of 4 in [0, 64]"};
+        if (n_modes > 0 && (degree < 1 || degree > 1000 || passes < 1 || passes > 50))
+            throw ArgError{"dsv_build_low_modes: degree must be in [1, 1000] and passes in [1, 50]"};
+        if (n_modes > 0 && !m->op_reuse)
+            throw ArgError{"dsv_build_low_modes: operator reuse is off (dsv_set_operator_reuse): "
+                           "the next solve would not keep the operator the subspace belongs to"};
+        if (n_modes > 0 && !(inv_dt >= 0.0)) throw ArgError{"inv_dt must be >= 0"};
+        DSV_HIP(hipSetDevice(m->device));
+        const hipStream_t cs = (hipStream_t)stream;
+        switch (n_species) {
+            case 1: build_low_modes<1>(m, sp, inv_dt, lumped, n_modes, degree, passes, seed, ritz, cs); break;
+            case 2: build_low_modes<2>(m, sp, inv_dt, lumped, n_modes, degree, passes, seed, ritz, cs); break;
+            default: build_low_modes<3>(m, sp, inv_dt, lumped, n_modes, degree, passes, seed, ritz, cs); break;
         }
     });
 }

@@ -32,6 +32,7 @@ EXPORTS = (
     "dsv_set_species_narrowing",
     "dsv_set_operator_reuse", "dsv_time_apply",
     "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_solve_warm_history",
+    "dsv_build_low_modes",
     "dsv_mlp_features",
     "dsv_mlp_forward", "dsv_mlp_directions",
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_grad_noalloc", "dsv_mlp_adam", "dsv_nodal_gradient",
@@ -85,6 +86,7 @@ class Stats(ctypes.Structure):
         ("initial_rel_residual", ctypes.c_double * 3), ("solve_ms", ctypes.c_double),
         ("cg_launches", ctypes.c_int32), ("narrowings", ctypes.c_int32),
         ("dirs_reused", ctypes.c_int32), ("gram_reused", ctypes.c_int32),
+        ("low_modes", ctypes.c_int32),
     ]
 
     def as_dict(self, n):
@@ -95,7 +97,7 @@ class Stats(ctypes.Structure):
             "initial_rel_residual": list(self.initial_rel_residual[:n]),
             "solve_ms": self.solve_ms, "cg_launches": self.cg_launches,
             "narrowings": self.narrowings, "dirs_reused": self.dirs_reused,
-            "gram_reused": self.gram_reused,
+            "gram_reused": self.gram_reused, "low_modes": self.low_modes,
         }
 
 
@@ -153,6 +155,8 @@ def load():
                                                   ctypes.POINTER(P), P, ctypes.POINTER(Stats),
                                                   ctypes.POINTER(D), ctypes.POINTER(D),
                                                   ctypes.POINTER(D)]),
+        "dsv_build_low_modes": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, I32,
+                                               I32, ctypes.c_uint64, ctypes.POINTER(D), P]),
         "dsv_nodal_gradient": (ctypes.c_int, [P, P, P, P, P]),
         "dsv_transport_coefficients": (ctypes.c_int, [P, P, P, P, P, P, ctypes.POINTER(CoefParams),
                                                       P, P, P, P, P]),

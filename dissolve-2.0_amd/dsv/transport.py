@@ -206,6 +206,25 @@ class Stepper:
                               X=per[k][0], sigma=per[k][1])
         return X, sig, per
 
+    def build_low_modes(self, n_modes=32, degree=80, passes=4, seed=0):
+        """Low-mode subspace of the current operator (libdsv dsv_build_low_modes): n_modes
+        vectors per species approximating the eigenvectors of the lowest eigenvalues of D^-1 A,
+        which every later Galerkin-started step applies as a correction of its start while the
+        coefficients stay bound (set_coefficients / refresh_coefficients drop it).  Needs the
+        mf32_ir64 method and operator reuse.  Returns {"seconds", "ritz" [species][mode]}."""
+        import torch
+        if self.method != "mf32_ir64" or not self.reuse_operator:
+            raise ValueError("the low-mode subspace needs method mf32_ir64 with operator reuse")
+        ritz = (ctypes.c_double * max(1, self.S * int(n_modes)))()
+        torch.cuda.synchronize(self.mesh.device)
+        t = time.perf_counter()
+        _lib.call("dsv_build_low_modes", self.mesh.handle, self.S, self._arr, 1.0 / self.dt,
+                  self.lumped, int(n_modes), int(degree), int(passes), int(seed), ritz,
+                  stream_ptr(self.mesh.device))
+        torch.cuda.synchronize(self.mesh.device)
+        return {"seconds": time.perf_counter() - t,
+                "ritz": [[ritz[s * n_modes + k] for k in range(n_modes)] for s in range(self.S)]}
+
     def residual_norms(self):
         """||b - A x|| per species for the current initial guesses self.x (FP64)."""
         rn = (ctypes.c_double * self.S)()

@@ -80,6 +80,8 @@ typedef struct dsv_stats {
                                     previous call (history shifted by one step) */
     int32_t gram_reused;         /* dsv_solve_warm: 1 when the Gram entries of those
                                     directions came from the previous call too */
+    int32_t low_modes;           /* dsv_solve_warm: species whose start the low-mode
+                                    correction (dsv_build_low_modes) moved */
 } dsv_stats;
 
 const char* dsv_last_error(void);
@@ -215,6 +217,25 @@ int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp
                            int32_t n_hist, const double* const* hist, int32_t n_extra,
                            const double* const* extra, void* stream, dsv_stats* stats,
                            double* coef, double* r_before, double* r_after);
+
+/* Low-mode subspace of an operator, for the Galerkin-started MF32_IR64 solves (no counterpart in
+ * the reference, whose solves start from the previous solution; for callers that step one
+ * operator many times).  For the operator of `species` (the arrays, inv_dt and lumping a later
+ * dsv_solve_warm / dsv_solve_warm_history with DSV_METHOD_MF32_IR64 passes), builds n_modes
+ * vectors per species (a multiple of 4, at most 64; 0 drops the subspace) approximating the
+ * eigenvectors of the lowest eigenvalues of D^-1 A (D = diag A) on the non-Dirichlet vertices:
+ * Chebyshev-filtered subspace iteration from a block of n_modes + 16 random vectors (`seed`),
+ * `passes` passes of a degree-`degree` filter, Rayleigh-Ritz after each.  While operator reuse
+ * keeps that operator (dsv_set_operator_reuse on, no epoch change, the same arrays), every such
+ * solve then corrects the start its Galerkin warm start took by the Galerkin projection on the
+ * subspace, x += W c with (W^T A W) c = W^T r, which minimises the A-norm of the start's error
+ * over start + span W; a species whose start already meets rtol is left as it is, and the true
+ * FP64 residual still decides convergence.  `ritz` (n_species x n_modes, nullable) receives the
+ * last Rayleigh-Ritz values.  Device memory kept: 4 n_modes V n_species bytes; about three
+ * times (n_modes + 16) V n_species * 4 bytes more during the build.  Needs operator reuse on. */
+int dsv_build_low_modes(dsv_mesh* m, int32_t n_species, const dsv_species* species, double inv_dt,
+                        int32_t lumped, int32_t n_modes, int32_t degree, int32_t passes,
+                        uint64_t seed, double* ritz, void* stream);
 
 /* Inner-solve controls of DSV_METHOD_MF32_IR64 (relative inner tolerance, max refinement
  * sweeps), the number of CG iterations per polled chunk, and whether chunks are replayed from
