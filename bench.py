@@ -150,6 +150,23 @@ def run_path(mesh, w, method, steps, warmup, rtol, warm="previous", epochs=30, h
     }, st
 
 
+def pick_low_modes(mesh, w, rtol, low_modes, history_dirs=None):
+    """Whether the Galerkin paths take the low-mode correction, by a short trial from the
+    initial state: 2 untimed and 5 timed steps with it (6 increments) and without it (8), the
+    faster wins.  Its per-step cost (two passes over the subspace and one FP64 apply) pays
+    where the warm-started solves still spend their iterations in the few smoothest modes (the
+    disc); on a mesh whose slow spectrum those modes do not cover (the gyroid lattice) it does
+    not."""
+    on, _ = run_path(mesh, w, "mf32_ir64", 5, 2, rtol, warm="extrapolate",
+                     history_dirs=history_dirs or 6, low_modes=low_modes)
+    off, _ = run_path(mesh, w, "mf32_ir64", 5, 2, rtol, warm="extrapolate",
+                      history_dirs=history_dirs or 8)
+    trial = {"steps": "3-7 of the path, each way", "with_steps_per_s": round(on["steps_per_s"], 2),
+             "without_steps_per_s": round(off["steps_per_s"], 2),
+             "setup_seconds": on["low_modes"]["setup_seconds"] if on["low_modes"] else None}
+    return on["steps_per_s"] > off["steps_per_s"], trial
+
+
 def error_replay(mesh, w, method, steps, warmup, rtol, warm, history_dirs, final_u, every=1,
                  epochs=30, low_modes=None):
     """Replays a path untimed and solves each checked step's three systems again with the plain
@@ -546,7 +563,12 @@ def main():
         return
     res, finals = {}, {}
     paths = a.paths.split(",")
-    hd = a.history_dirs or (6 if a.low_modes > 0 else 8)
+    lm_arg = (a.low_modes, a.low_mode_degree, a.low_mode_passes)
+    use_lm, lm_trial = False, None
+    if a.low_modes > 0:
+        use_lm, lm_trial = pick_low_modes(mesh, w, rtol, lm_arg, a.history_dirs)
+        log(f"low-mode trial: {lm_trial} -> {'with' if use_lm else 'without'}")
+    hd = a.history_dirs or (6 if use_lm else 8)
     PATHS = (("plain", "csr64", "previous", rtol), ("fp32", "mf32_ir64", "previous", rtol),
              ("chosen", "mf32_ir64", "extrapolate", rtol),
              # the same Galerkin start without the low-mode correction
@@ -562,8 +584,7 @@ def main():
             continue
         h = min(hd, 7) if name in ("learned", "extrapolate_no_mlp") else \
             ((a.history_dirs or 8) if name == "chosen_no_low_modes" else hd)
-        lm = None if name == "chosen_no_low_modes" else \
-            (a.low_modes, a.low_mode_degree, a.low_mode_passes)
+        lm = lm_arg if use_lm and name != "chosen_no_low_modes" else None
         r, st = run_path(mesh, w, method, a.steps, a.warmup, rt, warm=warm, history_dirs=h,
                          low_modes=lm)
         r.update(method=method, warm=warm, rtol=rt, history_dirs=h, low_modes_arg=lm)
@@ -664,8 +685,9 @@ def main():
     lm_gain = None
     if "chosen_no_low_modes" in res:
         c0 = res["chosen_no_low_modes"]
-        lm_gain = {"compared": f"chosen ({hd} increments + the low-mode correction) vs the "
-                               "Galerkin start over 8 increments alone (its best), each on its "
+        lm_gain = {"compared": f"chosen ({hd} increments"
+                               + (" + the low-mode correction" if use_lm else "") +
+                               ") vs the Galerkin start over 8 increments alone, each on its "
                                "own trajectory",
                    "steps_per_s_without": round(c0["steps_per_s"], 3),
                    "cg_iterations_per_step_without": its(c0),
@@ -694,11 +716,13 @@ def main():
                    "method": "FP32 fused 3-species Jacobi-PCG (Chronopoulos-Gear, one kernel per "
                              "iteration, stored FP32 geometry) in FP64 refinement, FP64 "
                              f"Galerkin warm start over the last {hd} solution "
-                             "increments (previous solution as fallback), then the Galerkin "
-                             f"correction over {a.low_modes} low modes per species of D^-1 A "
-                             "(built once per operator before the warm-up, Chebyshev-filtered "
-                             "subspace iteration), species narrowing, 8-iteration polled chunks "
-                             "of direct launches",
+                             "increments (previous solution as fallback), "
+                             + (f"then the Galerkin correction over {a.low_modes} low modes per "
+                                "species of D^-1 A (built once per operator before the warm-up, "
+                                "Chebyshev-filtered subspace iteration), " if use_lm else
+                                "no low-mode correction (the trial found it slower here), ")
+                             + "species narrowing, 8-iteration polled chunks of direct launches",
+                   "low_mode_trial": lm_trial,
                    "step": "the three species batched in one solve with static coefficients "
                            "(f_OH from the initial O2, no O2 under-relaxation)",
                    "parallelism": "1 GPU", "launch_shape": tuned},
