@@ -1648,7 +1648,7 @@ struct Workspace {
     float* lm_w = nullptr;
     double* lm_ginv = nullptr;  // [S][lm_n][lm_n]
     double* lm_part = nullptr;  // dot partials
-    double* lm_c = nullptr;     // coefficients [lm_n][S]
+    double* lm_c = nullptr;     // coefficients [lm_n][S], then their dot totals
     double* lm_y = nullptr;     // the correction W c (V FP64 species records)
     OpKey lm_key;
 };
@@ -3376,10 +3376,9 @@ constexpr int LM_MAX = 64;        // modes per species at most (a multiple of 4)
 constexpr int LM_GUARD = 16;      // block vectors beyond the modes kept
 constexpr int LM_BLOCK_MAX = LM_MAX + LM_GUARD;
 constexpr int LM_THREADS = 256;
-constexpr int LM_CHUNKS = 240;    // vertex chunks (workgroups along x) of the dot kernels
+constexpr int LM_CHUNKS = 240;    // vertex chunks (workgroups along x) of the build's dot kernels
+constexpr int LM_DOT_CHUNKS = 240;  // the same for the correction's dots
 constexpr int LM_TILE = 8;        // modes per workgroup row of the correction's dot kernel
-constexpr int LM_COEF_THREADS = 192;
-static_assert(LM_MAX * MAXS <= LM_COEF_THREADS, "one thread per coefficient");
 
 // uniform in (-1, 1) from a 64-bit key: the splitmix64 finaliser (Steele, Lea and Flood 2014)
 __device__ __forceinline__ float lm_uniform(uint64_t key) {
@@ -3575,47 +3574,49 @@ __global__ __launch_bounds__(LM_THREADS) void k_lm_dots(int V, int n,
     }
 }
 
-// one workgroup: t = W^T r per species, c_s = Ginv_s t_s; a species whose start already meets
-// the tolerance (||r||^2 <= rtol^2 ||b||^2 from the warm start's outputs) or whose coefficients
-// are not finite keeps c = 0
-template <int S>
-__global__ __launch_bounds__(LM_COEF_THREADS) void k_lm_coef(int n, int nchunk,
-                                                             const double* __restrict__ part,
-                                                             const double* __restrict__ ginv,
-                                                             const WarmOut* __restrict__ wout,
-                                                             double rtol2, double* __restrict__ c) {
-    __shared__ double t[LM_MAX * MAXS];
-    __shared__ int ok[MAXS];
-    const int q = threadIdx.x;  // q = j S + s
-    if (q < MAXS) ok[q] = 1;
-    if (q < n * S) {
-        double a = 0.0;
-        for (int i = 0; i < nchunk; ++i) a += part[(size_t)q * nchunk + i];
-        t[q] = a;
-    }
-    __syncthreads();
-    double cq = 0.0;
-    if (q < n * S) {
-        const int j = q / S, s = q % S;
-        for (int i = 0; i < n; ++i) cq += ginv[((size_t)s * n + j) * n + i] * t[i * S + s];
-        if (!isfinite(cq)) ok[s] = 0;
-    }
-    __syncthreads();
-    if (q < n * S) {
-        const int s = q % S;
-        const bool run = ok[s] && wout->rr1[s] > rtol2 * wout->bb[s];
-        c[q] = run ? cq : 0.0;
-    }
+// t[o] = the sum of the dot partials of coefficient o = j S + s, one wave per coefficient
+__global__ __launch_bounds__(64) void k_lm_tot(int nchunk, const double* __restrict__ part,
+                                               double* __restrict__ t) {
+    const int o = blockIdx.x, lane = threadIdx.x;
+    double a = 0.0;
+    for (int i = lane; i < nchunk; i += 64) a += part[(size_t)o * nchunk + i];
+    a = wave_sum(a);
+    if (lane == 0) t[o] = a;
 }
 
-// y = W c (FP64), x += y
+// c[o] = (Ginv_s t_s)_j for o = j S + s, one wave per coefficient (lane i takes term i)
+template <int S>
+__global__ __launch_bounds__(64) void k_lm_solve(int n, const double* __restrict__ ginv,
+                                                 const double* __restrict__ t,
+                                                 double* __restrict__ c) {
+    const int o = blockIdx.x, lane = threadIdx.x;
+    const int j = o / S, s = o % S;
+    double a = lane < n ? ginv[((size_t)s * n + j) * n + lane] * t[lane * S + s] : 0.0;
+    a = wave_sum(a);
+    if (lane == 0) c[o] = a;
+}
+
+// y = W c (FP64), x += y.  A species whose start already meets the tolerance (||r||^2 <=
+// rtol^2 ||b||^2 from the warm start's outputs) or whose coefficients are not finite takes
+// c = 0: every workgroup decides it from the same values.
 template <int S>
 __global__ void k_lm_update(int V, int n, const SV<float, S>* __restrict__ W,
-                            const double* __restrict__ c, SV<double, S>* __restrict__ x,
+                            const double* __restrict__ c, const WarmOut* __restrict__ wout,
+                            double rtol2, SV<double, S>* __restrict__ x,
                             SV<double, S>* __restrict__ y) {
     __shared__ double cs[LM_MAX * MAXS];
-    for (int i = threadIdx.x; i < n * S; i += blockDim.x) cs[i] = c[i];
+    __shared__ int ok[MAXS];
+    if (threadIdx.x < MAXS) ok[threadIdx.x] = 1;
     __syncthreads();
+    for (int i = threadIdx.x; i < n * S; i += blockDim.x) {
+        const double v = c[i];
+        cs[i] = v;
+        if (!isfinite(v)) ok[i % S] = 0;
+    }
+    __syncthreads();
+    bool run[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) run[s] = ok[s] && wout->rr1[s] > rtol2 * wout->bb[s];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     double acc[S];
@@ -3629,8 +3630,8 @@ __global__ void k_lm_update(int V, int n, const SV<float, S>* __restrict__ W,
     SV<double, S> yv, xv = x[v];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        yv.v[s] = acc[s];
-        xv.v[s] += acc[s];
+        yv.v[s] = run[s] ? acc[s] : 0.0;
+        xv.v[s] += yv.v[s];
     }
     y[v] = yv;
     x[v] = xv;
@@ -3648,13 +3649,16 @@ static void launch_low_mode_correction(dsv_mesh* m, Workspace* w, double rtol, h
     const auto* W = (const SV<float, S>*)w->lm_w;
     auto* r64 = (SV<double, S>*)w->vb[VB_R64];
     auto* x64 = (SV<double, S>*)w->vb[VB_X64];
-    hipLaunchKernelGGL((k_lm_dots<S>), dim3(LM_CHUNKS, cdiv(n, LM_TILE)), dim3(LM_THREADS), 0, st,
-                       m->V, n, W, (const SV<double, S>*)r64, w->lm_part);
-    hipLaunchKernelGGL((k_lm_coef<S>), dim3(1), dim3(LM_COEF_THREADS), 0, st, n, LM_CHUNKS,
-                       (const double*)w->lm_part, (const double*)w->lm_ginv,
-                       (const WarmOut*)w->wout, rtol * rtol, w->lm_c);
+    hipLaunchKernelGGL((k_lm_dots<S>), dim3(LM_DOT_CHUNKS, cdiv(n, LM_TILE)), dim3(LM_THREADS), 0,
+                       st, m->V, n, W, (const SV<double, S>*)r64, w->lm_part);
+    double* tot = w->lm_c + LM_MAX * MAXS;
+    hipLaunchKernelGGL(k_lm_tot, dim3(n * S), dim3(64), 0, st, LM_DOT_CHUNKS,
+                       (const double*)w->lm_part, tot);
+    hipLaunchKernelGGL((k_lm_solve<S>), dim3(n * S), dim3(64), 0, st, n,
+                       (const double*)w->lm_ginv, (const double*)tot, w->lm_c);
     hipLaunchKernelGGL((k_lm_update<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V, n, W,
-                       (const double*)w->lm_c, x64, (SV<double, S>*)w->lm_y);
+                       (const double*)w->lm_c, (const WarmOut*)w->wout, rtol * rtol, x64,
+                       (SV<double, S>*)w->lm_y);
     // r64 -= A y (the residual kernel with b = r64 read and written in place by each vertex)
     MFVec<double, S> v{};
     v.x = (const SV<double, S>*)w->lm_y;
@@ -4280,7 +4284,7 @@ static void build_low_modes(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
         DSV_HIP(hipMemcpyAsync(w->lm_ginv, ginv.data(), ginv.size() * sizeof(double),
                                hipMemcpyHostToDevice, st));
         w->lm_part = dalloc<double>((size_t)n * S * LM_CHUNKS);
-        w->lm_c = dalloc<double>((size_t)LM_MAX * MAXS);
+        w->lm_c = dalloc<double>((size_t)2 * LM_MAX * MAXS);  // coefficients, then totals
         w->lm_y = dalloc<double>(V * S);
         DSV_HIP(hipStreamSynchronize(st));
         for (void* p : tmp) (void)hipFree(p);
