@@ -1641,11 +1641,13 @@ struct Workspace {
     int graph_chunk = 0;
     int graph_valid = 0;
     unsigned char graph_prm[sizeof(OpParams<double>)] = {};
-    // low-mode subspace (dsv_build_low_modes): lm_n FP32 vectors per species (lm_n blocks of V
-    // species records, internal order, 0 at Dirichlet vertices), the inverse of W^T A W per
-    // species under the FP64 operator, the operator key they belong to, and per-solve scratch
+    // low-mode subspace (dsv_build_low_modes): lm_n FP16 vectors per species (element (j, s, v)
+    // at lm_index; internal order, 0 at Dirichlet vertices and in the padding to lm_vp), the
+    // inverse of W^T A W per species under the FP64 operator, the operator key they belong
+    // to, and per-solve scratch
     int lm_n = 0;
-    float* lm_w = nullptr;
+    int lm_vp = 0;              // V rounded up to a multiple of LM_GROUP
+    _Float16* lm_w = nullptr;
     double* lm_ginv = nullptr;  // [S][lm_n][lm_n]
     double* lm_part = nullptr;  // dot partials
     double* lm_c = nullptr;     // coefficients [lm_n][S], then their dot totals
@@ -3371,14 +3373,25 @@ static void warm_start_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
 // iteration: a seeded random block; passes of a degree-m Chebyshev polynomial in P = D^-1 A
 // that damps [a, b] (b a Gershgorin bound of P, a the block's largest Ritz value of the pass
 // before, b / 64 at the first pass) and so raises the low end of the spectrum against it;
-// Rayleigh-Ritz in the D inner product after each pass.
+// Rayleigh-Ritz in the D inner product after each pass.  The solves read W in FP16, each vector
+// scaled by a power of two to a largest entry in [1/2, 1); W^T A W is formed from the rounded
+// vectors, so the correction is the exact Galerkin step of the subspace they span.
 constexpr int LM_MAX = 64;        // modes per species at most (a multiple of 4)
 constexpr int LM_GUARD = 16;      // block vectors beyond the modes kept
 constexpr int LM_BLOCK_MAX = LM_MAX + LM_GUARD;
 constexpr int LM_THREADS = 256;
 constexpr int LM_CHUNKS = 240;    // vertex chunks (workgroups along x) of the build's dot kernels
-constexpr int LM_DOT_CHUNKS = 240;  // the same for the correction's dots
-constexpr int LM_TILE = 8;        // modes per workgroup row of the correction's dot kernel
+constexpr int LM_DOT_CHUNKS = 1024;  // workgroups of the correction's dots at most
+// The FP16 subspace is stored in groups of LM_GROUP vertices: all modes and species of a group
+// are contiguous (vector j, species s of the group in LM_GROUP consecutive halves), so that
+// the workgroup that takes a group streams one contiguous block.
+constexpr int LM_GROUP = 256;
+__host__ __device__ __forceinline__ size_t lm_index(int n, int S, int j, int s, int v) {
+    return (((size_t)(v / LM_GROUP) * n + j) * S + s) * LM_GROUP + (v % LM_GROUP);
+}
+// workgroups of the correction's dot kernel: one per vertex group, up to LM_DOT_CHUNKS
+static int lm_dot_chunks(int Vp) { return std::max(1, std::min(LM_DOT_CHUNKS, Vp / LM_GROUP)); }
+typedef _Float16 lm_half4 __attribute__((ext_vector_type(4)));
 
 // uniform in (-1, 1) from a 64-bit key: the splitmix64 finaliser (Steele, Lea and Flood 2014)
 __device__ __forceinline__ float lm_uniform(uint64_t key) {
@@ -3514,6 +3527,57 @@ __global__ void k_lm_to_double(int V, const SV<float, S>* __restrict__ x,
     y[v] = b;
 }
 
+// mx[j S + s] = max_v |X_j(v)_s| over the vectors j of blockIdx.x
+template <int S>
+__global__ __launch_bounds__(256) void k_lm_colmax(int V, const SV<float, S>* __restrict__ X,
+                                                  float* __restrict__ mx) {
+    const int j = blockIdx.x;
+    float a[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) a[s] = 0.0f;
+    for (int v = threadIdx.x; v < V; v += blockDim.x) {
+        const SV<float, S> t = X[(size_t)j * V + v];
+#pragma unroll
+        for (int s = 0; s < S; ++s) a[s] = fmaxf(a[s], fabsf(t.v[s]));
+    }
+    __shared__ float red[MAXS][256];
+#pragma unroll
+    for (int s = 0; s < S; ++s) red[s][threadIdx.x] = a[s];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                red[s][threadIdx.x] = fmaxf(red[s][threadIdx.x], red[s][threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x < S) mx[j * S + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The FP16 subspace: W[lm_index(j, s, v)] = fp16(X_j(v)_s sc[j S + s]) for v < V, 0 in the
+// padding up to Vp; X_j(v) is replaced by the same rounded values in FP32 (the vectors the
+// Gram matrix W^T A W is formed from).  One thread per (j, v < Vp).
+template <int S>
+__global__ void k_lm_to_half(int V, int Vp, int n, const float* __restrict__ sc,
+                             SV<float, S>* __restrict__ X, _Float16* __restrict__ W) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * Vp) return;
+    const int j = (int)(t / Vp), v = (int)(t % Vp);
+    if (v >= V) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) W[lm_index(n, S, j, s, v)] = (_Float16)0.0f;
+        return;
+    }
+    SV<float, S> x = X[(size_t)j * V + v];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const _Float16 h = (_Float16)(x.v[s] * sc[j * S + s]);
+        W[lm_index(n, S, j, s, v)] = h;
+        x.v[s] = (float)h;
+    }
+    X[(size_t)j * V + v] = x;
+}
+
 // per-block maximum over the rows of sum_j |a_ij| / a_ii (the Gershgorin bound of D^-1 A)
 __global__ __launch_bounds__(256) void k_lm_gershgorin(int V, const int32_t* __restrict__ indptr,
                                                       const int32_t* __restrict__ diag_pos,
@@ -3535,42 +3599,61 @@ __global__ __launch_bounds__(256) void k_lm_gershgorin(int V, const int32_t* __r
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// the correction's dots: part[(j S + s) gridDim.x + blockIdx.x] = the chunk's w_j . r, for the
-// modes [j0, j0 + LM_TILE) of blockIdx.y
+// The per-solve kernels of the correction: a workgroup of LM_WAVES wavefronts takes one vertex
+// group (64 quads) per pass, wavefront w the modes [LM_JW w, LM_JW (w + 1)), a lane one quad
+// (one 8-byte FP16 load per mode and species), so that each vertex's r (or the quad's update)
+// is handled by one workgroup for every mode.
+constexpr int LM_WAVES = 4;
+constexpr int LM_JW = LM_MAX / LM_WAVES;
+constexpr int LM_DOT_WAVES = 8;  // the dot kernel: more, narrower wavefronts (occupancy)
+constexpr int LM_DOT_JW = LM_MAX / LM_DOT_WAVES;
+
+// the correction's dots: part[(j S + s) gridDim.x + blockIdx.x] = the workgroup's sum of
+// w_j(v) r(v) over its quads
 template <int S>
-__global__ __launch_bounds__(LM_THREADS) void k_lm_dots(int V, int n,
-                                                        const SV<float, S>* __restrict__ W,
-                                                        const SV<double, S>* __restrict__ r,
-                                                        double* __restrict__ part) {
-    const int j0 = blockIdx.y * LM_TILE;
-    double acc[LM_TILE][S];
+__global__ __launch_bounds__(64 * LM_DOT_WAVES) void k_lm_dots(int V, int Vp, int n,
+                                                          const _Float16* __restrict__ W,
+                                                          const SV<double, S>* __restrict__ r,
+                                                          double* __restrict__ part) {
+    const int lane = threadIdx.x & 63, j0 = (threadIdx.x >> 6) * LM_DOT_JW;
+    if (j0 >= n) return;  // wave-uniform (no barrier follows)
+    double acc[LM_DOT_JW][S];
 #pragma unroll
-    for (int j = 0; j < LM_TILE; ++j)
+    for (int j = 0; j < LM_DOT_JW; ++j)
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[j][s] = 0.0;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
-        const SV<double, S> rv = r[v];
+    static_assert(LM_GROUP == 4 * 64, "a lane takes one quad of the group");
+    for (int g = blockIdx.x; g < Vp / LM_GROUP; g += gridDim.x) {
+        const int q = g * 64 + lane;
+        double rv[4][S];
 #pragma unroll
-        for (int j = 0; j < LM_TILE; ++j)
+        for (int k = 0; k < 4; ++k) {
+            const int v = 4 * q + k;
+            if (v < V) {
+                const SV<double, S> t = r[v];
+#pragma unroll
+                for (int s = 0; s < S; ++s) rv[k][s] = t.v[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) rv[k][s] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < LM_DOT_JW; ++j)
             if (j0 + j < n) {
-                const SV<float, S> wv = W[(size_t)(j0 + j) * V + v];
 #pragma unroll
-                for (int s = 0; s < S; ++s) acc[j][s] += (double)wv.v[s] * rv.v[s];
+                for (int s = 0; s < S; ++s) {
+                    const lm_half4 h =
+                        *reinterpret_cast<const lm_half4*>(W + lm_index(n, S, j0 + j, s, 4 * q));
+                    acc[j][s] += (double)h[0] * rv[0][s] + (double)h[1] * rv[1][s] +
+                                 (double)h[2] * rv[2][s] + (double)h[3] * rv[3][s];
+                }
             }
     }
-    __shared__ double red[LM_TILE * MAXS][LM_THREADS / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < LM_TILE * S; ++q) {
-        const double t = wave_sum(acc[q / S][q % S]);
-        if (lane == 0) red[q][wid] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < LM_TILE * S && j0 + (int)threadIdx.x / S < n) {
-        const int q = threadIdx.x;
-        double t = 0.0;
-        for (int w = 0; w < LM_THREADS / 64; ++w) t += red[q][w];
-        part[((size_t)j0 * S + q) * gridDim.x + blockIdx.x] = t;
+    for (int i = 0; i < LM_DOT_JW * S; ++i) {
+        const double t = wave_sum(acc[i / S][i % S]);
+        if (lane == 0 && j0 + i / S < n) part[((size_t)j0 * S + i) * gridDim.x + blockIdx.x] = t;
     }
 }
 
@@ -3596,15 +3679,22 @@ __global__ __launch_bounds__(64) void k_lm_solve(int n, const double* __restrict
     if (lane == 0) c[o] = a;
 }
 
-// y = W c (FP64), x += y.  A species whose start already meets the tolerance (||r||^2 <=
-// rtol^2 ||b||^2 from the warm start's outputs) or whose coefficients are not finite takes
-// c = 0: every workgroup decides it from the same values.
+// y = W c (FP64), x += y.  Each wavefront sums its modes' terms for the workgroup's 64 quads;
+// wavefront k then adds the LM_WAVES parts (in wavefront order) for vertex k of every quad.
+// A species whose start already meets the tolerance (||r||^2 <= rtol^2 ||b||^2 from the warm
+// start's outputs) or whose coefficients are not finite takes c = 0: every workgroup decides
+// it from the same values.
 template <int S>
-__global__ void k_lm_update(int V, int n, const SV<float, S>* __restrict__ W,
-                            const double* __restrict__ c, const WarmOut* __restrict__ wout,
-                            double rtol2, SV<double, S>* __restrict__ x,
-                            SV<double, S>* __restrict__ y) {
+__global__ __launch_bounds__(64 * LM_WAVES) void k_lm_update(int V, int Vp, int n,
+                                                            const _Float16* __restrict__ W,
+                                                            const double* __restrict__ c,
+                                                            const WarmOut* __restrict__ wout,
+                                                            double rtol2,
+                                                            SV<double, S>* __restrict__ x,
+                                                            SV<double, S>* __restrict__ y) {
+    static_assert(LM_WAVES == 4, "wavefront k finishes vertex k of each quad");
     __shared__ double cs[LM_MAX * MAXS];
+    __shared__ double red[LM_WAVES][4 * MAXS][64];
     __shared__ int ok[MAXS];
     if (threadIdx.x < MAXS) ok[threadIdx.x] = 1;
     __syncthreads();
@@ -3614,23 +3704,42 @@ __global__ void k_lm_update(int V, int n, const SV<float, S>* __restrict__ W,
         if (!isfinite(v)) ok[i % S] = 0;
     }
     __syncthreads();
-    bool run[S];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, j0 = wv * LM_JW;
+    const int q = blockIdx.x * 64 + lane;
+    double acc[4][S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) run[s] = ok[s] && wout->rr1[s] > rtol2 * wout->bb[s];
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    double acc[S];
+    for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int s = 0; s < S; ++s) acc[s] = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const SV<float, S> wv = W[(size_t)j * V + v];
+        for (int s = 0; s < S; ++s) acc[k][s] = 0.0;
+    if (q < (Vp >> 2)) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) acc[s] += cs[j * S + s] * (double)wv.v[s];
+        for (int j = 0; j < LM_JW; ++j)
+            if (j0 + j < n) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const lm_half4 h =
+                        *reinterpret_cast<const lm_half4*>(W + lm_index(n, S, j0 + j, s, 4 * q));
+                    const double cj = cs[(j0 + j) * S + s];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) acc[k][s] += cj * (double)h[k];
+                }
+            }
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int s = 0; s < S; ++s) red[wv][k * S + s][lane] = acc[k][s];
+    __syncthreads();
+    const int v = 4 * q + wv;
+    if (v >= V) return;
     SV<double, S> yv, xv = x[v];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        yv.v[s] = run[s] ? acc[s] : 0.0;
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < LM_WAVES; ++w) t += red[w][wv * S + s][lane];
+        const bool run = ok[s] && wout->rr1[s] > rtol2 * wout->bb[s];
+        yv.v[s] = run ? t : 0.0;
         xv.v[s] += yv.v[s];
     }
     y[v] = yv;
@@ -3646,17 +3755,20 @@ static bool low_modes_valid(const dsv_mesh* m, const Workspace* w) {
 template <int S>
 static void launch_low_mode_correction(dsv_mesh* m, Workspace* w, double rtol, hipStream_t st) {
     const int n = w->lm_n;
-    const auto* W = (const SV<float, S>*)w->lm_w;
+    const _Float16* W = w->lm_w;
+    const int Vp = w->lm_vp;
     auto* r64 = (SV<double, S>*)w->vb[VB_R64];
     auto* x64 = (SV<double, S>*)w->vb[VB_X64];
-    hipLaunchKernelGGL((k_lm_dots<S>), dim3(LM_DOT_CHUNKS, cdiv(n, LM_TILE)), dim3(LM_THREADS), 0,
-                       st, m->V, n, W, (const SV<double, S>*)r64, w->lm_part);
+    const int nchunk = lm_dot_chunks(Vp);
+    hipLaunchKernelGGL((k_lm_dots<S>), dim3(nchunk), dim3(64 * LM_DOT_WAVES), 0, st, m->V, Vp, n, W,
+                       (const SV<double, S>*)r64, w->lm_part);
     double* tot = w->lm_c + LM_MAX * MAXS;
-    hipLaunchKernelGGL(k_lm_tot, dim3(n * S), dim3(64), 0, st, LM_DOT_CHUNKS,
-                       (const double*)w->lm_part, tot);
+    hipLaunchKernelGGL(k_lm_tot, dim3(n * S), dim3(64), 0, st, nchunk, (const double*)w->lm_part,
+                       tot);
     hipLaunchKernelGGL((k_lm_solve<S>), dim3(n * S), dim3(64), 0, st, n,
                        (const double*)w->lm_ginv, (const double*)tot, w->lm_c);
-    hipLaunchKernelGGL((k_lm_update<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V, n, W,
+    hipLaunchKernelGGL((k_lm_update<S>), dim3(cdiv(Vp / 4, 64)), dim3(64 * LM_WAVES), 0, st, m->V,
+                       Vp, n, W,
                        (const double*)w->lm_c, (const WarmOut*)w->wout, rtol * rtol, x64,
                        (SV<double, S>*)w->lm_y);
     // r64 -= A y (the residual kernel with b = r64 read and written in place by each vertex)
@@ -4241,13 +4353,34 @@ static void build_low_modes(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
             DSV_HIP(hipGetLastError());
             std::swap(X, X2);
         }
-        // W: the first n vectors (ascending Ritz values), and W^T A W under the FP64 operator
-        w->lm_w = dalloc<float>((size_t)n * V * S);
-        DSV_HIP(hipMemcpyAsync(w->lm_w, X, (size_t)n * V * sizeof(SV<float, S>),
-                               hipMemcpyDeviceToDevice, st));
+        // W: the first n vectors (ascending Ritz values), each scaled by a power of two to a
+        // largest entry in [1/2, 1) and rounded to FP16; W^T A W of the rounded vectors under
+        // the FP64 operator
+        const int Vp = (int)((V + LM_GROUP - 1) / LM_GROUP * LM_GROUP);
+        std::vector<float> sc((size_t)n * S);  // outlives its asynchronous upload
+        {
+            auto* dmx = (float*)take((size_t)2 * n * S * sizeof(float));
+            hipLaunchKernelGGL((k_lm_colmax<S>), dim3(n), dim3(256), 0, st, m->V,
+                               (const SV<float, S>*)X, dmx);
+            DSV_HIP(hipGetLastError());
+            DSV_HIP(hipMemcpyAsync(sc.data(), dmx, sc.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+            DSV_HIP(hipStreamSynchronize(st));
+            for (float& x : sc) {
+                if (!std::isfinite(x)) throw ArgError{"dsv_build_low_modes: non-finite subspace"};
+                int e = 0;
+                (void)std::frexp(x, &e);  // x = f 2^e, f in [1/2, 1)
+                x = x > 0.0f ? std::ldexp(1.0f, -e) : 1.0f;
+            }
+            DSV_HIP(hipMemcpyAsync(dmx + (size_t)n * S, sc.data(), sc.size() * sizeof(float),
+                                   hipMemcpyHostToDevice, st));
+            w->lm_w = dalloc<_Float16>((size_t)n * S * Vp);
+            hipLaunchKernelGGL((k_lm_to_half<S>), dim3(cdiv((int64_t)n * Vp, 256)), dim3(256), 0,
+                               st, m->V, Vp, n, (const float*)(dmx + (size_t)n * S), X, w->lm_w);
+            DSV_HIP(hipGetLastError());
+        }
         auto* Wd = (SV<double, S>*)take(V * sizeof(SV<double, S>));
         auto* AWd = (SV<double, S>*)take((size_t)n * V * sizeof(SV<double, S>));
-        const auto* W = (const SV<float, S>*)w->lm_w;
+        const auto* W = (const SV<float, S>*)X;
         for (int j = 0; j < n; ++j) {
             hipLaunchKernelGGL((k_lm_to_double<S>), dim3(cdiv(m->V, 256)), dim3(256), 0, st, m->V,
                                W + (size_t)j * V, Wd);
@@ -4283,13 +4416,14 @@ static void build_low_modes(dsv_mesh* m, const dsv_species* sp, double inv_dt, i
         w->lm_ginv = dalloc<double>(ginv.size());
         DSV_HIP(hipMemcpyAsync(w->lm_ginv, ginv.data(), ginv.size() * sizeof(double),
                                hipMemcpyHostToDevice, st));
-        w->lm_part = dalloc<double>((size_t)n * S * LM_CHUNKS);
+        w->lm_part = dalloc<double>((size_t)n * S * lm_dot_chunks(Vp));
         w->lm_c = dalloc<double>((size_t)2 * LM_MAX * MAXS);  // coefficients, then totals
         w->lm_y = dalloc<double>(V * S);
         DSV_HIP(hipStreamSynchronize(st));
         for (void* p : tmp) (void)hipFree(p);
         tmp.clear();
         w->lm_key = w->key[1];
+        w->lm_vp = Vp;
         w->lm_n = n;
     } catch (...) {
         (void)hipStreamSynchronize(st);

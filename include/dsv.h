@@ -231,7 +231,7 @@ int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp
  * subspace, x += W c with (W^T A W) c = W^T r, which minimises the A-norm of the start's error
  * over start + span W; a species whose start already meets rtol is left as it is, and the true
  * FP64 residual still decides convergence.  `ritz` (n_species x n_modes, nullable) receives the
- * last Rayleigh-Ritz values.  Device memory kept: 4 n_modes V n_species bytes; about three
+ * last Rayleigh-Ritz values.  Device memory kept: 2 n_modes V n_species bytes (FP16); about three
  * times (n_modes + 16) V n_species * 4 bytes more during the build.  Needs operator reuse on. */
 int dsv_build_low_modes(dsv_mesh* m, int32_t n_species, const dsv_species* species, double inv_dt,
                         int32_t lumped, int32_t n_modes, int32_t degree, int32_t passes,
