@@ -284,25 +284,47 @@ def workload_text(w, C, V, rtol):
             f"rtol={rtol:g}")
 
 
+def assembly_kernel(C, V, s):
+    """The kernel dsv_assemble takes for species s under lumped mass: 'scalar' (scalar D and
+    alpha), 'valpha' (scalar D, per-vertex alpha) or 'rows' (the row assembly)."""
+    a = s.get("alpha")
+    if s.get("D") is None and a is None:
+        return "scalar"
+    if s.get("D") is None and len(a) == V and V != C:
+        return "valpha"
+    return "rows"
+
+
 def assembly_bytes(C, V, nnz, s):
-    """Algorithmic bytes of one FP64 CSR assembly of species s: connectivity 16 C, FP64
-    coordinates 24 V, the values written 8 nnz, and 8 bytes per value of a coefficient field."""
+    """Algorithmic bytes of one lumped FP64 CSR assembly of species s.  'scalar' (D K1 +
+    (inv_dt + alpha) M1): K1 and M1 read, the values written, 24 nnz.  'valpha': K1, the
+    consistent unit mass and the column indices read, the values written (28 nnz), and per
+    vertex the row pointer, diagonal position, lumped mass and alpha (24 V).  'rows' (from the
+    geometry): connectivity 16 C, FP64 coordinates 24 V, the values written 8 nnz, and 8 bytes
+    per value of a coefficient field."""
     def coef(a):
         return 0 if a is None else 8 * len(a)
+    kind = assembly_kernel(C, V, s)
+    if kind == "scalar":
+        return 24 * nnz
+    if kind == "valpha":
+        return 28 * nnz + 24 * V
     return 16 * C + 24 * V + 8 * nnz + coef(s.get("D")) + coef(s.get("alpha"))
 
 
-def time_assembly(mesh, w, reps):
-    """ms per launch of dsv_assemble's kernel for the first species with a reaction field (no
-    Dirichlet rows), HIP events around `reps` graph-replayed launches."""
+def time_assembly(mesh, w, reps, s=None):
+    """ms per launch of dsv_assemble's kernel for species s (default: the first species with a
+    reaction field, else the first; no Dirichlet rows), HIP events around `reps`
+    graph-replayed launches."""
     import ctypes
     import torch
     from dsv import _lib
     from dsv.core import species_array, species_struct
-    s = next((x for x in w["species"] if x.get("alpha") is not None), w["species"][0])
+    if s is None:
+        s = next((x for x in w["species"] if x.get("alpha") is not None), w["species"][0])
     keep = []
     st = species_struct(mesh, D=s.get("D"), D_value=s.get("D_value", 0.0), alpha=s.get("alpha"),
-                        keep=keep)
+                        alpha_value=s.get("alpha_value", 0.0), keep=keep)
     stream = torch.cuda.Stream()
     ms = ctypes.c_double()
     _lib.call("dsv_time_apply", mesh.handle, 1, species_array([st]), 1.0 / w["dt"], 1, 64, 2,
@@ -375,11 +397,19 @@ def profile_run(mesh, w, reps):
     torch.cuda.synchronize()
     log(f"fused CG iteration {ms_iter * 1e3:.1f} us, apply {ms_apply * 1e3:.1f} us "
         f"({reps} launches each plus 3 warm-up)")
-    ms_asm, s_asm = time_assembly(mesh, w, reps)
-    torch.cuda.synchronize()
-    log(f"FP64 CSR assembly ({s_asm['name']}) {ms_asm * 1e3:.1f} us, "
-        f"{assembly_bytes(len(w['conn']), len(w['xyz']), mesh.nnz, s_asm)} algorithmic bytes "
-        f"({reps} launches plus 3 warm-up)")
+    import numpy as np
+    C, V = len(w["conn"]), len(w["xyz"])
+    # the first species with a reaction field (else the first), a species with scalar D and
+    # alpha, and a per-cell D field: the three assembly kernels
+    scalar = [x for x in w["species"] if x.get("D") is None and x.get("alpha") is None][:1]
+    cases = [None] + scalar + [{"name": "per-cell D field (lognormal, seed 7)",
+                                "D": np.random.default_rng(7).lognormal(0.0, 0.5, C)}]
+    for s in cases:
+        ms_asm, s_asm = time_assembly(mesh, w, reps, s)
+        torch.cuda.synchronize()
+        log(f"FP64 CSR assembly ({s_asm['name']}, {assembly_kernel(C, V, s_asm)}) "
+            f"{ms_asm * 1e3:.1f} us, {assembly_bytes(C, V, mesh.nnz, s_asm)} algorithmic bytes "
+            f"({reps} launches plus 3 warm-up)")
 
 
 def kernels_at_scale(shapes=((256, 256, 1), (256, 256, 2))):
@@ -633,6 +663,28 @@ def main():
     asm_bytes = assembly_bytes(C, V, mesh.nnz, s_asm)
     asm_gbs = asm_bytes / (ms_asm * 1e-3) / 1e9
     log(f"FP64 CSR assembly ({s_asm['name']}): {ms_asm * 1e3:.1f} us/launch, {asm_gbs:.0f} GB/s")
+    # the row assembly from the geometry, as a per-cell diffusivity field takes it
+    import numpy as np
+    s_fld = {"name": "per-cell D field (lognormal, seed 7)",
+             "D": np.random.default_rng(7).lognormal(0.0, 0.5, C)}
+    ms_fld, _ = time_assembly(mesh, w, a.apply_reps, s_fld)
+    fld_bytes = assembly_bytes(C, V, mesh.nnz, s_fld)
+    fld_gbs = fld_bytes / (ms_fld * 1e-3) / 1e9
+    log(f"FP64 CSR assembly ({s_fld['name']}): {ms_fld * 1e3:.1f} us/launch, {fld_gbs:.0f} GB/s")
+    # the streaming form, as a species with scalar D and alpha takes it
+    s_sc = next((x for x in w["species"] if x.get("D") is None and x.get("alpha") is None), None)
+    sc_entry = None
+    if s_sc is not None:
+        ms_sc, _ = time_assembly(mesh, w, a.apply_reps, s_sc)
+        sc_bytes = assembly_bytes(C, V, mesh.nnz, s_sc)
+        sc_gbs = sc_bytes / (ms_sc * 1e-3) / 1e9
+        log(f"FP64 CSR assembly ({s_sc['name']}, scalar coefficients): {ms_sc * 1e3:.1f} us/launch, "
+            f"{sc_gbs:.0f} GB/s")
+        sc_entry = {"kernel": "k_assemble_scalar (D K1 + (1/dt + alpha) M1 from the unit stiffness "
+                              "and mass kept on the mesh)",
+                    "species": s_sc["name"], "algorithmic_bytes_per_launch": sc_bytes,
+                    "us_per_launch": round(ms_sc * 1e3, 2), "achieved_gbs": round(sc_gbs, 1),
+                    "frac": round(sc_gbs / HBM_PEAK_GBS, 4)}
 
     scale = None
     if not a.no_scale and a.workload == "disc3":
@@ -763,11 +815,24 @@ def main():
                       "us_per_launch": round(ms_apply * 1e3, 2), "achieved_gbs": round(apply_gbs, 1),
                       "frac": round(apply_gbs / HBM_PEAK_GBS, 4),
                       "copy_of_same_bytes": copy_ref},
-        "assembly_hbm": {"kernel": "k_assemble_seg (FP64 CSR assembly, one species, the plain "
-                                   "path's per-step operator)",
+        "assembly_hbm": {"kernel": {"scalar": "k_assemble_scalar (D K1 + (1/dt + alpha) M1)",
+                                    "valpha": "k_assemble_scalar_valpha (D K1, lumped mass and "
+                                              "the per-vertex reaction by graph edge)",
+                                    "rows": "k_assemble_seg (row assembly from the geometry)"}[
+                                        assembly_kernel(C, V, s_asm)]
+                                   + ": the FP64 CSR assembly of the plain path's per-step "
+                                     "operator, this species",
                          "species": s_asm["name"], "algorithmic_bytes_per_launch": asm_bytes,
                          "us_per_launch": round(ms_asm * 1e3, 2), "achieved_gbs": round(asm_gbs, 1),
-                         "frac": round(asm_gbs / HBM_PEAK_GBS, 4)},
+                         "frac": round(asm_gbs / HBM_PEAK_GBS, 4),
+                         "field": {"kernel": "k_assemble_seg (FP64 CSR row assembly from the "
+                                             "geometry, per-cell D)",
+                                   "coefficients": s_fld["name"],
+                                   "algorithmic_bytes_per_launch": fld_bytes,
+                                   "us_per_launch": round(ms_fld * 1e3, 2),
+                                   "achieved_gbs": round(fld_gbs, 1),
+                                   "frac": round(fld_gbs / HBM_PEAK_GBS, 4)},
+                         "scalar": sc_entry},
         "kernels_at_scale": scale,
         "cpu_baseline": cpu,
     }

@@ -9,6 +9,10 @@
 // LDS values the set-up listed for it (`loc`), in ascending cell order -- and the block's values
 // leave as one coalesced stream.  The sums run in the same order as a per-entry gather over the
 // sorted contribution list, so the result is deterministic.
+//
+// A scalar D needs no geometry per call: the values stream from the unit stiffness K1 and the
+// unit mass matrices kept on the mesh (k_assemble_scalar, and k_assemble_scalar_valpha for a
+// per-vertex reaction field with lumped mass).
 #include "dsv_internal.h"
 #include "kernels_common.h"
 
@@ -181,6 +185,106 @@ static void launch_row_assembly(dsv_mesh* m, const dsv_species& s, double inv_dt
                        m->cptr, m->loc, s.D, coef_mode(s.D, s.D_len, m), s.D_value, s.alpha,
                        coef_mode(s.alpha, s.alpha_len, m), s.alpha ? 0.0 : s.alpha_value, inv_dt,
                        lumped ? 1 : 0, values);
+}
+
+// Scalar coefficients (D and alpha both scalars): A = D K1 + (inv_dt + alpha) M1 entry by
+// entry, K1 = K(D = 1) and M1 the mass matrix (consistent or lumped) being assembled once per
+// mesh by k_assemble_seg; one coalesced pass over three FP64 nnz arrays.
+__global__ __launch_bounds__(256) void k_assemble_scalar(int64_t nnz, const double* __restrict__ K1,
+                                                        const double* __restrict__ M1, double D,
+                                                        double mf, double* __restrict__ values) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) values[p] = D * K1[p] + mf * M1[p];
+}
+
+// Scalar D, a per-vertex reaction field alpha and lumped mass.  The cell's reaction coefficient
+// is the mean of its corners' alpha, so the lumped reaction mass of row i regroups by graph
+// edge: sum over cells c holding i of (|det J_c| / 24) (sum_k alpha_k / 4) = sum_k W_ik alpha_k,
+// W_ik = sum over cells holding i and k of |det J_c| / 96, which is 1.25 M_ik (k != i) and
+// 0.625 M_ii of the consistent unit mass M.  Off the diagonal the entries are D K1; the diagonal
+// is D K1 + inv_dt lm_i + (W alpha)_i.  VA_LANES lanes (one DPP row) share a CSR row, so that
+// most rows take one entry per lane and every load of a lane is independent of the others.
+// Workgroup b runs on XCD b mod 8, and XCD x takes a contiguous run of rows, so that the alpha
+// of a row's neighbours is mostly in that XCD's own L2.
+constexpr int VA_LANES = 16;
+__global__ __launch_bounds__(256) void k_assemble_scalar_valpha(
+    int32_t V, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int32_t* __restrict__ diag_pos, const double* __restrict__ K1,
+    const double* __restrict__ Mc, const double* __restrict__ lm, const double* __restrict__ alpha,
+    double D, double inv_dt, double* __restrict__ values) {
+    const int nb = gridDim.x, x = blockIdx.x & 7, q8 = nb >> 3, r8 = nb & 7;
+    const int64_t blk = x * q8 + min(x, r8) + (int)(blockIdx.x >> 3);
+    const int64_t t = blk * blockDim.x + threadIdx.x;
+    const int32_t r = (int32_t)(t / VA_LANES);
+    const int lane = (int)(t % VA_LANES);
+    if (r >= V) return;  // the whole DPP row (it shares r)
+    double acc = 0.0;
+    const int32_t p1 = indptr[r + 1];
+    for (int32_t p = indptr[r] + lane; p < p1; p += VA_LANES) {
+        const int32_t k = indices[p];
+        acc += ((k == r) ? 0.625 : 1.25) * Mc[p] * alpha[k];
+        if (k != r) values[p] = D * K1[p];
+    }
+    // the sum over the 16 lanes: quads, then half-rows, then the row (every lane gets it)
+    acc = add_lane_xor2(add_lane_xor1(acc));
+    acc += dpp_move<DPP_ROW_HALF_MIRROR>(acc);
+    acc += dpp_move<DPP_ROW_MIRROR>(acc);
+    if (lane == 0) {
+        const int32_t dp = diag_pos[r];
+        values[dp] = D * K1[dp] + inv_dt * lm[r] + acc;
+    }
+}
+
+static void ensure_row_assembly(dsv_mesh* m);
+
+// K1 and the mass matrix of the given lumping, once per mesh (null stream, synchronised)
+static void ensure_unit_operators(dsv_mesh* m, int lumped) {
+    const int l = lumped ? 1 : 0;
+    if (m->k1 && m->m1[l]) return;
+    ensure_row_assembly(m);
+    double* k = nullptr;
+    double* mm = nullptr;
+    try {
+        if (!m->k1) {
+            k = dalloc<double>(m->nnz);
+            dsv_species u{};
+            u.D_value = 1.0;
+            launch_row_assembly(m, u, 0.0, 1, k, 0);
+        }
+        if (!m->m1[l]) {
+            mm = dalloc<double>(m->nnz);
+            dsv_species u{};
+            launch_row_assembly(m, u, 1.0, l, mm, 0);
+        }
+        DSV_HIP(hipGetLastError());
+        DSV_HIP(hipStreamSynchronize(0));
+    } catch (...) {
+        dfree(k);
+        dfree(mm);
+        throw;
+    }
+    if (k) m->k1 = k;
+    if (mm) m->m1[l] = mm;
+}
+
+// The assembly of species s: the streaming forms for a scalar D (with a scalar reaction, or a
+// per-vertex one under lumped mass), the row assembly otherwise
+static void launch_assembly(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped,
+                            double* values, hipStream_t st) {
+    if (!s.D && !s.alpha) {
+        ensure_unit_operators(m, lumped);
+        hipLaunchKernelGGL(k_assemble_scalar, dim3(cdiv(m->nnz, 256)), dim3(256), 0, st, m->nnz,
+                           (const double*)m->k1, (const double*)m->m1[lumped ? 1 : 0], s.D_value,
+                           inv_dt + s.alpha_value, values);
+    } else if (!s.D && lumped && coef_mode(s.alpha, s.alpha_len, m) == COEF_VERT) {
+        ensure_unit_operators(m, 0);  // K1 and the consistent unit mass
+        hipLaunchKernelGGL(k_assemble_scalar_valpha, dim3(cdiv((int64_t)m->V * VA_LANES, 256)),
+                           dim3(256), 0, st, m->V, m->indptr, m->indices, m->diag_pos,
+                           (const double*)m->k1, (const double*)m->m1[0], m->lumped_m, s.alpha,
+                           s.D_value, inv_dt, values);
+    } else {
+        launch_row_assembly(m, s, inv_dt, lumped, values, st);
+    }
 }
 
 // Set-up of the assembly, once per mesh: the incident cells of every vertex (entry
@@ -387,15 +491,16 @@ void assemble_into(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped,
     check_species_coefs(m, s);
     if (!values) throw ArgError{"dsv_assemble: NULL values"};
     ensure_row_assembly(m);
-    launch_row_assembly(m, s, inv_dt, lumped, values, st);
+    launch_assembly(m, s, inv_dt, lumped, values, st);
     if (s.n_bc > 0)
         hipLaunchKernelGGL(k_bc_rows, dim3(cdiv(s.n_bc, 128)), dim3(128), 0, st, s.n_bc,
                            s.bc_dofs, m->indptr, m->indices, values);
     DSV_HIP(hipGetLastError());
 }
 
-// Average device time (ms) of one launch of the row assembly of species s (no Dirichlet rows)
-// into a scratch buffer: n_rep launches captured in one graph and replayed between two events.
+// Average device time (ms) of one launch of the assembly of species s (no Dirichlet rows; the
+// kernel assemble_into takes for its coefficients) into a scratch buffer: n_rep launches
+// captured in one graph and replayed between two events.
 double time_assembly(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumped, int n_rep,
                      hipStream_t stream) {
     check_species_coefs(m, s);
@@ -405,7 +510,7 @@ double time_assembly(dsv_mesh* m, const dsv_species& s, double inv_dt, int lumpe
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms = 0.f;
-    auto launch = [&](hipStream_t st) { launch_row_assembly(m, s, inv_dt, lumped, out, st); };
+    auto launch = [&](hipStream_t st) { launch_assembly(m, s, inv_dt, lumped, out, st); };
     try {
         for (int i = 0; i < 3; ++i) launch(stream);
         DSV_HIP(hipStreamSynchronize(stream));

@@ -141,6 +141,10 @@ struct dsv_mesh {
     double* phys[3] = {};         // V+1 each: redistancing phi0 / ping-pong, scaffold offsets
     double* gsten = nullptr;      // 3 nnz  nodal-gradient edge stencil G_rj (x, y, z) (lazy)
     double* gsw = nullptr;        // V  its cell-volume weights W_r (lazy)
+    // unit operators in CSR order for scalar-coefficient assembly (lazy): K(D = 1) and the mass
+    // matrix M (inv_dt = 1, no reaction), consistent [0] and lumped [1]
+    double* k1 = nullptr;         // nnz
+    double* m1[2] = {};           // nnz each
     double h_avg = 0.0;           // mean longest cell edge (computed once, dsv_mesh_h_avg)
     // internal (Morton) numbering for the matrix-free path
     int32_t* perm = nullptr;      // internal -> caller

@@ -413,6 +413,9 @@ static void destroy_mesh(dsv_mesh* m) {
     dfree(m->JinvT);
     dfree(m->gsten);
     dfree(m->gsw);
+    dfree(m->k1);
+    dfree(m->m1[0]);
+    dfree(m->m1[1]);
     dfree(m->indptr);
     dfree(m->indices);
     dfree(m->diag_pos);

@@ -123,7 +123,10 @@ int dsv_mesh_set_partition(dsv_mesh* m, int32_t vertices_per_block, int32_t bloc
 /* --- assembly: GPUSparseAssembler.assemble (`gpu/gpu_assembler.py:472-532`) ---
  * values (device, nnz, pattern order) = A = (inv_dt + alpha_e) M_e + De_e K_e summed over
  * cells, lumped (M = dJ/24 I) when lumped != 0 else consistent, then Dirichlet rows
- * (row zeroed, diagonal 1; `gpu/gpu_assembler.py:147-165`).  Deterministic (no atomics). */
+ * (row zeroed, diagonal 1; `gpu/gpu_assembler.py:147-165`).  Deterministic (no atomics).
+ * With D and alpha both scalars (no arrays) the values are D K1 + (inv_dt + alpha) M1 entry by
+ * entry, from K(D = 1) and the unit mass matrix of the lumping, kept on the mesh after the
+ * first such call (nnz doubles each). */
 int dsv_assemble(dsv_mesh* m, const dsv_species* sp, double inv_dt, int32_t lumped,
                  double* values, void* stream);
 
