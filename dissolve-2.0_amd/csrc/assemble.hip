@@ -403,7 +403,7 @@ __global__ void k_rhs_lumped_batch(int32_t V, int S, const double* __restrict__ 
     const double mr = lm[r];
     for (int s = 0; s < S; ++s) {
         const double fr = a.f[s] ? a.f[s][r] : 0.0;
-        a.b[s][r] = mr * (inv_dt * a.u[s][r] + fr);
+        a.b[s][r] = lumped_rhs(mr, inv_dt, a.u[s][r], fr);
     }
 }
 

@@ -121,6 +121,12 @@ __device__ __forceinline__ void stencil_gradients(int32_t r, int lane, int64_t n
     }
 }
 
+// Lumped backward-Euler right-hand side of one vertex, b = m (inv_dt u + f): the one expression
+// every kernel that forms it uses, so that they all round alike
+__device__ __forceinline__ double lumped_rhs(double m, double inv_dt, double u, double f) {
+    return m * (inv_dt * u + f);
+}
+
 // numpy's maximum / minimum: NaN if either operand is NaN (fmax / fmin drop a NaN operand)
 __device__ __forceinline__ double np_max(double a, double b) {
     return (a != a || b != b) ? a + b : (a >= b ? a : b);

@@ -1338,6 +1338,40 @@ __global__ void k_pack_bx(int V, const int32_t* __restrict__ perm, const double*
     bout[i] = rb;
     xout[i] = rx;
 }
+// k_pack_bx with the lumped right-hand sides formed on the way (dsv_step_warm_history): per
+// species b = lm (inv_dt u + f) from the start u (caller order), b = bc value on Dirichlet
+// vertices, written to the caller's b and packed; the start packed with its Dirichlet values
+template <int S>
+__global__ void k_pack_rhs_bx(int V, const int32_t* __restrict__ perm,
+                              const double* __restrict__ lm, double inv_dt,
+                              double* __restrict__ b0, double* __restrict__ b1,
+                              double* __restrict__ b2, const double* __restrict__ s0,
+                              const double* __restrict__ s1, const double* __restrict__ s2,
+                              const double* __restrict__ f0, const double* __restrict__ f1,
+                              const double* __restrict__ f2, const uint8_t* __restrict__ bcbits,
+                              BcVals bv, SV<double, S>* __restrict__ bout,
+                              SV<double, S>* __restrict__ xout) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int c = perm[i];
+    const unsigned bits = bcbits[i];
+    double* bdst[3] = {b0, b1, b2};
+    const double* src[3] = {s0, s1, s2};
+    const double* fs[3] = {f0, f1, f2};
+    const double mc = lm[c];
+    SV<double, S> rb, rx;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const double u = src[s][c];
+        const bool d = (bits >> s) & 1u;
+        const double b = d ? bv.v[s] : lumped_rhs(mc, inv_dt, u, fs[s] ? fs[s][c] : 0.0);
+        bdst[s][c] = b;
+        rb.v[s] = b;
+        rx.v[s] = d ? bv.v[s] : u;
+    }
+    bout[i] = rb;
+    xout[i] = rx;
+}
 template <class Ti, int S>
 __global__ void k_unpack(int V, const int32_t* __restrict__ perm, const SV<Ti, S>* __restrict__ in,
                          double* __restrict__ d0, double* __restrict__ d1, double* __restrict__ d2) {
@@ -2645,6 +2679,10 @@ struct WarmArgs {
     // which species already meet the tolerance)
     bool low_modes = false;
     double rtol = 0.0;
+    // dsv_step_warm_history: the lumped right-hand sides are formed from hist_0 (and the
+    // per-vertex sources rhs_f, nullable) into the species' b while the solve packs them
+    bool form_rhs = false;
+    const double* const* rhs_f = nullptr;
 };
 template <int S>
 static void fused_warm(dsv_mesh* m, Workspace* w, const dsv_species* sp, const WarmArgs& wa,
@@ -2694,9 +2732,19 @@ static void solve_impl(dsv_mesh* m, const dsv_species* sp, double inv_dt, int lu
         prepare_bc(m, w, S, sp, st);
         BcVals bv{};
         for (int s = 0; s < S; ++s) bv.v[s] = sp[s].bc_value;
-        hipLaunchKernelGGL((k_pack_bx<S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0], bs[1],
-                           bs[2], xs[0], xs[1], xs[2], (const uint8_t*)w->bcbits, bv,
-                           (SV<double, S>*)w->vb[VB_B64], x64);
+        if (warm && warm->form_rhs) {
+            const double* const* f = warm->rhs_f;
+            hipLaunchKernelGGL((k_pack_rhs_bx<S>), dim3(gv), dim3(256), 0, st, m->V, perm,
+                               (const double*)m->lumped_m, inv_dt, const_cast<double*>(bs[0]),
+                               const_cast<double*>(bs[1]), const_cast<double*>(bs[2]), xs[0],
+                               xs[1], xs[2], f ? f[0] : nullptr, f && S > 1 ? f[1] : nullptr,
+                               f && S > 2 ? f[2] : nullptr, (const uint8_t*)w->bcbits, bv,
+                               (SV<double, S>*)w->vb[VB_B64], x64);
+        } else {
+            hipLaunchKernelGGL((k_pack_bx<S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0],
+                               bs[1], bs[2], xs[0], xs[1], xs[2], (const uint8_t*)w->bcbits, bv,
+                               (SV<double, S>*)w->vb[VB_B64], x64);
+        }
     } else {
         hipLaunchKernelGGL((k_pack<double, S>), dim3(gv), dim3(256), 0, st, m->V, perm, bs[0],
                            bs[1], bs[2], (SV<double, S>*)w->vb[VB_B64]);
@@ -4517,43 +4565,81 @@ int dsv_solve_warm(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double
     });
 }
 
+static void warm_history_impl(dsv_mesh* m, int32_t n_species, const dsv_species* sp,
+                              double inv_dt, int32_t lumped, int32_t method, double rtol,
+                              int32_t max_it, int32_t n_hist, const double* const* hist,
+                              int32_t n_extra, const double* const* extra, void* stream,
+                              dsv_stats* stats, double* coef, double* r_before, double* r_after,
+                              bool form_rhs, const double* const* f) {
+    if (method != DSV_METHOD_MF32_IR64 && method != DSV_METHOD_MF64)
+        throw ArgError{"dsv_solve_warm_history: method must be matrix-free (MF32_IR64 or MF64)"};
+    if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
+    if (max_it < 1) throw ArgError{"max_it must be >= 1"};
+    if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > MAXD)
+        throw ArgError{"dsv_solve_warm_history: n_hist + n_extra must be in [0, 8]"};
+    if (!hist || (n_extra > 0 && !extra))
+        throw ArgError{"dsv_solve_warm_history: NULL hist or extra"};
+    check_solve_args(m, n_species, sp, true);
+    for (int i = 0; i < (n_hist + 1) * n_species; ++i)
+        if (!hist[i]) throw ArgError{"dsv_solve_warm_history: NULL history vector"};
+    for (int s = 0; s < n_species; ++s)
+        for (int i = 0; i < (n_hist + 1) * n_species; ++i)
+            if (hist[i] == sp[s].x)
+                throw ArgError{"dsv_solve_warm_history: a solution array is also in the history"};
+    DSV_HIP(hipSetDevice(m->device));
+    const hipStream_t cs = (hipStream_t)stream;
+    WarmArgs wa;
+    wa.n_hist = n_hist;
+    wa.hist = hist;
+    wa.from_hist = true;
+    wa.n_extra = n_extra;
+    wa.extra = extra;
+    wa.coef = coef;
+    wa.r_before = r_before;
+    wa.r_after = r_after;
+    wa.form_rhs = form_rhs;
+    wa.rhs_f = f;
+    switch (n_species) {
+        case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+        case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+        default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+    }
+}
+
 int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp, double inv_dt,
                            int32_t lumped, int32_t method, double rtol, int32_t max_it,
                            int32_t n_hist, const double* const* hist, int32_t n_extra,
                            const double* const* extra, void* stream, dsv_stats* stats,
                            double* coef, double* r_before, double* r_after) {
     return guard([&] {
-        if (method != DSV_METHOD_MF32_IR64 && method != DSV_METHOD_MF64)
-            throw ArgError{"dsv_solve_warm_history: method must be matrix-free (MF32_IR64 or MF64)"};
-        if (!(rtol > 0.0)) throw ArgError{"rtol must be > 0"};
-        if (max_it < 1) throw ArgError{"max_it must be >= 1"};
-        if (n_hist < 0 || n_extra < 0 || n_hist + n_extra > MAXD)
-            throw ArgError{"dsv_solve_warm_history: n_hist + n_extra must be in [0, 8]"};
-        if (!hist || (n_extra > 0 && !extra))
-            throw ArgError{"dsv_solve_warm_history: NULL hist or extra"};
-        check_solve_args(m, n_species, sp, true);
-        for (int i = 0; i < (n_hist + 1) * n_species; ++i)
-            if (!hist[i]) throw ArgError{"dsv_solve_warm_history: NULL history vector"};
-        for (int s = 0; s < n_species; ++s)
+        warm_history_impl(m, n_species, sp, inv_dt, lumped, method, rtol, max_it, n_hist, hist,
+                          n_extra, extra, stream, stats, coef, r_before, r_after, false, nullptr);
+    });
+}
+
+int dsv_step_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp,
+                          const double* const* f, double inv_dt, int32_t lumped, int32_t method,
+                          double rtol, int32_t max_it, int32_t n_hist, const double* const* hist,
+                          int32_t n_extra, const double* const* extra, void* stream,
+                          dsv_stats* stats, double* coef, double* r_before, double* r_after) {
+    return guard([&] {
+        if (!lumped) throw ArgError{"dsv_step_warm_history: lumped mass only"};
+        if (!m || !sp || !hist) throw ArgError{"dsv_step_warm_history: NULL argument"};
+        if (n_species < 1 || n_species > MAXS) throw ArgError{"n_species must be 1, 2 or 3"};
+        if (n_hist < 0 || n_hist > MAXD)
+            throw ArgError{"dsv_step_warm_history: n_hist must be in [0, 8]"};
+        // b is written: it must be none of the arrays the step reads
+        for (int s = 0; s < n_species; ++s) {
+            if (!sp[s].b) throw ArgError{"dsv_step_warm_history: species b and x must be set"};
             for (int i = 0; i < (n_hist + 1) * n_species; ++i)
-                if (hist[i] == sp[s].x)
-                    throw ArgError{"dsv_solve_warm_history: a solution array is also in the history"};
-        DSV_HIP(hipSetDevice(m->device));
-        const hipStream_t cs = (hipStream_t)stream;
-        WarmArgs wa;
-        wa.n_hist = n_hist;
-        wa.hist = hist;
-        wa.from_hist = true;
-        wa.n_extra = n_extra;
-        wa.extra = extra;
-        wa.coef = coef;
-        wa.r_before = r_before;
-        wa.r_after = r_after;
-        switch (n_species) {
-            case 1: solve_impl<1>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
-            case 2: solve_impl<2>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
-            default: solve_impl<3>(m, sp, inv_dt, lumped, method, rtol, max_it, stats, cs, &wa); break;
+                if (hist[i] == sp[s].b)
+                    throw ArgError{"dsv_step_warm_history: a right-hand side is also in the history"};
+            for (int q = 0; q < n_species; ++q)
+                if (sp[q].x == sp[s].b || (f && f[q] == sp[s].b))
+                    throw ArgError{"dsv_step_warm_history: a right-hand side aliases x or f"};
         }
+        warm_history_impl(m, n_species, sp, inv_dt, lumped, method, rtol, max_it, n_hist, hist,
+                          n_extra, extra, stream, stats, coef, r_before, r_after, true, f);
     });
 }
 

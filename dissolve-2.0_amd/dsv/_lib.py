@@ -32,7 +32,7 @@ EXPORTS = (
     "dsv_set_species_narrowing",
     "dsv_set_operator_reuse", "dsv_time_apply",
     "dsv_residual_norms", "dsv_warm_start", "dsv_solve_warm", "dsv_solve_warm_history",
-    "dsv_build_low_modes",
+    "dsv_step_warm_history", "dsv_build_low_modes",
     "dsv_mlp_features",
     "dsv_mlp_forward", "dsv_mlp_directions",
     "dsv_mlp_grad_workspace", "dsv_mlp_grad", "dsv_mlp_grad_noalloc", "dsv_mlp_adam", "dsv_nodal_gradient",
@@ -155,6 +155,11 @@ def load():
                                                   ctypes.POINTER(P), P, ctypes.POINTER(Stats),
                                                   ctypes.POINTER(D), ctypes.POINTER(D),
                                                   ctypes.POINTER(D)]),
+        "dsv_step_warm_history": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), ctypes.POINTER(P),
+                                                 D, I32, I32, D, I32, I32, ctypes.POINTER(P), I32,
+                                                 ctypes.POINTER(P), P, ctypes.POINTER(Stats),
+                                                 ctypes.POINTER(D), ctypes.POINTER(D),
+                                                 ctypes.POINTER(D)]),
         "dsv_build_low_modes": (ctypes.c_int, [P, I32, ctypes.POINTER(Species), D, I32, I32, I32,
                                                I32, ctypes.c_uint64, ctypes.POINTER(D), P]),
         "dsv_nodal_gradient": (ctypes.c_int, [P, P, P, P, P]),

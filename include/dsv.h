@@ -221,6 +221,19 @@ int dsv_solve_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp
                            const double* const* extra, void* stream, dsv_stats* stats,
                            double* coef, double* r_before, double* r_after);
 
+/* One lumped backward-Euler step in one call (no counterpart in the reference, whose species
+ * blocks form each right-hand side with GPUSparseAssembler.assemble_rhs,
+ * `gpu/gpu_assembler.py:534-603`, then solve): dsv_rhs_batch's right-hand sides from the start
+ * hist_0 and the per-vertex sources f[s] (f or any f[s] NULL: none), written into sp[s].b, then
+ * dsv_solve_warm_history.  The right-hand sides are formed while the solve packs its vectors,
+ * with the same arithmetic, so b and the solution are the same bits as the two calls give.
+ * lumped must be nonzero; sp[s].b must be none of hist, x or f. */
+int dsv_step_warm_history(dsv_mesh* m, int32_t n_species, const dsv_species* sp,
+                          const double* const* f, double inv_dt, int32_t lumped, int32_t method,
+                          double rtol, int32_t max_it, int32_t n_hist, const double* const* hist,
+                          int32_t n_extra, const double* const* extra, void* stream,
+                          dsv_stats* stats, double* coef, double* r_before, double* r_after);
+
 /* Low-mode subspace of an operator, for the Galerkin-started MF32_IR64 solves (no counterpart in
  * the reference, whose solves start from the previous solution; for callers that step one
  * operator many times).  For the operator of `species` (the arrays, inv_dt and lumping a later

@@ -399,3 +399,44 @@ def test_nan_direction_keeps_start_finite(gpu, method):
         xs = x[s].cpu().numpy()
         assert np.all(np.isfinite(xs))
         assert np.abs(xs - exact[s]).max() <= 1e-7 * np.abs(exact[s]).max()
+
+
+def test_fused_step_is_bit_identical(gpu):
+    """dsv_step_warm_history (right-hand sides formed inside the solve call) against
+    dsv_rhs_batch + dsv_solve_warm_history: the same right-hand sides and solutions, bit for
+    bit, over Galerkin steps with a Dirichlet species, a source and a reaction field; and its
+    argument checks."""
+    import ctypes
+    import numpy as np
+    import pytest
+    import torch
+    from dsv import _lib, workloads
+    from dsv.core import Mesh
+    from dsv.transport import Stepper
+    w = workloads.disc3(seed=1, scale=6.0)
+    mesh = Mesh(w["conn"], w["xyz"], vertices_per_block=128)
+    runs = []
+    for fuse in (False, True):
+        st = Stepper(mesh, w["species"], w["dt"], method="mf32_ir64", rtol=1e-8)
+        st.set_warm_start("extrapolate", 6)
+        st.fuse_rhs = fuse
+        out = []
+        for _ in range(5):
+            h = st.step()
+            assert all(h["converged"]), h
+            out.append([t.cpu().numpy().copy() for t in st.b + st.u])
+        runs.append(out)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    # b may not be read by the step: a right-hand side that is also a history array is refused
+    P = ctypes.c_void_p * 3
+    hist = (ctypes.c_void_p * 6)(*[st.b[0].data_ptr()] + [u.data_ptr() for u in st.u] + [0, 0])
+    with pytest.raises(ValueError, match="history"):
+        _lib.call("dsv_step_warm_history", mesh.handle, 3, st._arr, P(None, None, None),
+                  1.0 / w["dt"], 1, 1, 1e-8, 100, 1, hist, 0, P(), None, _lib.Stats(), None,
+                  None, None)
+    with pytest.raises(ValueError, match="lumped"):
+        _lib.call("dsv_step_warm_history", mesh.handle, 3, st._arr, P(None, None, None),
+                  1.0 / w["dt"], 0, 1, 1e-8, 100, 1, hist, 0, P(), None, _lib.Stats(), None,
+                  None, None)
